@@ -1,0 +1,183 @@
+/*
+ * fcm_hip.h -- C ABI of the MI355X-native FCM mobility solver (libfcm_hip.so).
+ *
+ * Drop-in boundary for the reference's hot path: the public methods of
+ * fcm_fluid_solver, fft_solver, gmres_solver and collection_of_rigid_bodies
+ * (timwestwood/TimStochRigidBodiesSim).  Each entry point cites the
+ * reference interface it replaces.  All functions return an int status
+ * (FCM_OK = 0); no C++ exception crosses this boundary.  A handle is used by
+ * one host thread at a time; work is ordered on the handle's HIP stream and
+ * every entry point that returns host data synchronises that stream.
+ *
+ * Units and layouts follow the reference:
+ *   blob vectors    interleaved [x0,y0,z0,x1,...]      (fcm_fluid_solver.cpp:590-594)
+ *   body vectors    [U(3), Omega(3)] per body           (collection_of_rigid_bodies.cpp:715-723)
+ *   saddle vectors  [lambda(3*Nb); V(6*NB)]             (gmres_solver.cpp:18)
+ *   real fields     [nx][ny][pad], pad = 2*(nz/2+1)     (fcm_fluid_solver.cpp:574-586)
+ *   k-space fields  transposed order [ny][nx][nz/2+1] complex (fft_solver.cpp:34)
+ */
+#ifndef FCM_HIP_H
+#define FCM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FCM_OK 0
+#define FCM_ERR_ARG 1        /* invalid argument or unsupported size        */
+#define FCM_ERR_HIP 2        /* HIP runtime error (message in fcm_last_error) */
+#define FCM_ERR_NOMEM 3      /* device allocation failed                    */
+#define FCM_ERR_STATE 4      /* call made before the state it needs is set  */
+#define FCM_ERR_SINGULAR 5   /* single-body mobility matrix not SPD         */
+#define FCM_ERR_CAPACITY 6   /* exceeds the capacities given at creation     */
+
+/* GMRES outcomes (gmres_solver.cpp:256-300 prints these; here they are codes) */
+#define FCM_GMRES_CONVERGED 0
+#define FCM_GMRES_MAX_ITER 1   /* iteration cap reached, iterate returned     */
+#define FCM_GMRES_ZERO_RHS 2   /* ||RHS|| = 0, solution 0 (the reference divides by 0) */
+#define FCM_GMRES_BREAKDOWN 3  /* exact Krylov breakdown, solution exact      */
+
+#define FCM_PREC_FP64 0        /* every grid pass in FP64                    */
+#define FCM_PREC_FP32 1        /* spread/FFT/projection/interp in FP32        */
+
+typedef struct fcm_handle_s* fcm_handle;
+
+/* Runtime replacement of the compile-time parameters of config.hpp. */
+typedef struct {
+  int nx, ny, nz;      /* NUM_FFT_GRID_POINTS_{X,Y,Z} (config.hpp:65-67); even, 2^a 3^b   */
+  int ng;              /* NUM_FCM_GAUSSIAN_POINTS (config.hpp:69), <= 16                 */
+  double dx;           /* DX (config.hpp:88), decoupled from the blob radius             */
+  double a;            /* BLOB_RADIUS (config.hpp:48)                                    */
+  double mu;           /* FLUID_VISCOSITY (config.hpp:49)                                */
+  double kT;           /* ENERGY_SCALE (config.hpp:50)                                   */
+  double dt;           /* DT (config.hpp:84), sets the fluctuating-stress variance       */
+  int max_blobs;       /* capacity: blobs                                                */
+  int max_bodies;      /* capacity: bodies                                               */
+  int max_blobs_per_body;
+  int max_krylov;      /* capacity: GMRES iterations (basis columns - 1), GMRES_MAX_ITER */
+  int fp32;            /* also allocate the FP32 bulk path (FCM_PREC_FP32 solves)       */
+  int random_field;    /* allocate the six stress fields for the fluctuating velocity   */
+} fcm_config;
+
+/* ---- lifetime ------------------------------------------------------------ */
+/* fcm_fluid_solver::initialise, fcm_fluid_solver.cpp:29-129 (+ fft_solver::initialise,
+ * fft_solver.cpp:17-30; gmres_solver::initialise, gmres_solver.cpp:14-34) */
+int fcm_create(const fcm_config* cfg, fcm_handle* out);
+/* fcm_fluid_solver::~fcm_fluid_solver, fcm_fluid_solver.cpp:10-25 */
+int fcm_destroy(fcm_handle h);
+/* message of the last failing call on this thread */
+const char* fcm_last_error(void);
+int fcm_sync(fcm_handle h);
+/* HBM bytes fcm_create would allocate for cfg (serving admission) */
+int fcm_query_footprint(const fcm_config* cfg, size_t* bytes);
+
+/* ---- fcm_fluid_solver ----------------------------------------------------- */
+/* set_up_gaussians(const mat& Y), fcm_fluid_solver.cpp:131-165; Y is nb x 3 (host) */
+int fcm_set_up_gaussians(fcm_handle h, const double* Y, int nb);
+/* set_up_force_distribution(const vec& F), fcm_fluid_solver.cpp:552-634; F host 3*nb */
+int fcm_spread(fcm_handle h, const double* F, int precision);
+/* produce_deterministic_velocity_field(), fcm_fluid_solver.cpp:636-701 */
+int fcm_stokes(fcm_handle h, int precision);
+/* calculate_deterministic_blob_velocities(), fcm_fluid_solver.cpp:363-424; V host 3*nb */
+int fcm_interp(fcm_handle h, double* V, int precision);
+/* spread + Stokes + interpolation: the FCM mobility product V = M F */
+int fcm_mobility(fcm_handle h, const double* F, double* V, int precision);
+/* produce_random_velocity_field(), fcm_fluid_solver.cpp:167-361.  stress6 is
+ * NULL (draw from the seeded stream `seed`) or six host fields sxx syy szz sxy
+ * sxz syz, each nx*ny*nz reals (C order x,y,z), already scaled by the variances */
+int fcm_random_field(fcm_handle h, const double* stress6, uint64_t seed);
+/* calculate_random_blob_velocities(), fcm_fluid_solver.cpp:426-487; V host 3*nb */
+int fcm_random_interp(fcm_handle h, double* V);
+/* the public grid arrays ux..fz (fcm_fluid_solver.hpp:23-28) in the reference
+ * layout [nx][ny][pad]: which = 0,1,2 -> velocity/force grid component x,y,z of
+ * the given precision; 3,4,5 -> fluctuating velocity ux_rand..uz_rand */
+int fcm_get_field(fcm_handle h, int which, int precision, double* out);
+int fcm_set_field(fcm_handle h, int which, int precision, const double* in);
+/* single-body reference matrices, produce_reference_matrices,
+ * fcm_fluid_solver.cpp:842-881 (body at the box centre, identity
+ * orientation; N is a pseudo-inverse so collinear rods are allowed).
+ * Needs rb_set_geometry.  Any output may be NULL.  The results are kept
+ * and used by gmres_apply_Pinv / gmres_solve. */
+int fcm_ref_matrices(fcm_handle h, double* Msingle, double* Minv, double* N);
+/* install ref_Minv / ref_N computed elsewhere (fcm_fluid_solver.hpp:59-60) */
+int fcm_set_ref_matrices(fcm_handle h, const double* Minv, const double* N);
+
+/* ---- fft_solver ------------------------------------------------------------ */
+/* transform(double*), fft_solver.cpp:32-36: in-place R2C of grid component
+ * `which` (0..2); read the spectrum with fcm_get_kspace */
+int fft_transform(fcm_handle h, int which, int precision);
+/* inverse_transform(double*), fft_solver.cpp:38-42 */
+int fft_inverse_transform(fcm_handle h, int which, int precision);
+int fcm_get_kspace(fcm_handle h, int which, int precision, double* out_transposed);
+int fcm_set_kspace(fcm_handle h, int which, int precision, const double* in_transposed);
+
+/* ---- collection_of_rigid_bodies ------------------------------------------ */
+/* blob_ref_pos (collection_of_rigid_bodies.hpp:27): nbpb x 3 body-frame offsets */
+int rb_set_geometry(fcm_handle h, int nbpb, const double* blob_ref);
+/* body state (x, q) of all bodies; computes all_blob_pos (fill_all_blob_pos,
+ * collection_of_rigid_bodies.cpp:155-173) and set_up_gaussians on the device */
+int rb_set_bodies(fcm_handle h, int NB, const double* x, const double* q);
+/* body forces/torques (6 per body) used by the right-hand side */
+int rb_set_body_forces(fcm_handle h, const double* F6);
+int rb_get_blob_positions(fcm_handle h, double* Y);
+/* multiply_by_K_at_midstep, collection_of_rigid_bodies.cpp:707-731 */
+int rb_mul_K(fcm_handle h, const double* in6, double* out3);
+/* multiply_by_KT_at_midstep, collection_of_rigid_bodies.cpp:733-757 */
+int rb_mul_KT(fcm_handle h, const double* in3, double* out6);
+
+/* ---- gmres_solver ----------------------------------------------------------- */
+typedef struct {
+  double tol;          /* GMRES_TOL (config.hpp:62): stop when |beta_j|/||RHS|| < tol */
+  int max_iter;        /* GMRES_MAX_ITER (config.hpp:63)                             */
+  int precision;       /* FCM_PREC_FP64: plain solve; FCM_PREC_FP32: FP32 bulk inner
+                          solves with FP64 iterative refinement on the true residual */
+  double inner_tol;    /* FP32 form: relative tolerance of each inner solve (0: auto) */
+  int max_refine;      /* FP32 form: maximum outer refinements (0: 8)               */
+} gmres_opts;
+
+typedef struct {
+  int status;               /* FCM_GMRES_* */
+  int iterations;           /* total inner GMRES iterations (mat-vecs in the Krylov loop) */
+  int refinements;          /* outer refinement steps (FP32 form), 0 for FP64 */
+  double rel_residual;      /* Givens estimate |beta|/||b|| of the last inner solve (FP64: of the solve) */
+  double true_rel_residual; /* ||b - A x||/||b||, FP64 operator (FP32 form; FP64 form if check_residual) */
+  double seconds;           /* device time of the solve */
+} gmres_report;
+
+/* assemble_RHS, gmres_solver.cpp:58-70: RHS = [-V_rand; -F] with V_rand from
+ * the last fcm_random_interp (zero if none) and F from rb_set_body_forces */
+int gmres_assemble_rhs(fcm_handle h, int use_random);
+/* multiply_by_saddle_point_matrix, gmres_solver.cpp:72-83 (host vectors) */
+int gmres_saddle_matvec(fcm_handle h, const double* in, double* out, int precision);
+/* multiply_by_Pinv, gmres_solver.cpp:85-146 (host vectors) */
+int gmres_apply_Pinv(fcm_handle h, const double* in, double* out);
+/* invert_saddle_point_system, gmres_solver.cpp:148-306.  rhs: host vector or
+ * NULL (use the device RHS from gmres_assemble_rhs).  soln: host output or
+ * NULL (keep it on the device; read with gmres_get_solution).  resid_hist may
+ * be NULL or hold max_iter entries (|beta_j|/||RHS||). */
+int gmres_solve(fcm_handle h, const double* rhs, double* soln, const gmres_opts* opts, gmres_report* rep,
+                double* resid_hist);
+int gmres_get_solution(fcm_handle h, double* soln);
+
+/* ---- one whole fluctuating mobility solve, state resident on the device ---- */
+/* produce_random_velocity_field + calculate_random_blob_velocities +
+ * assemble_RHS + invert_saddle_point_system (main.cpp:231-233,308-310) */
+int fcm_mobility_solve(fcm_handle h, uint64_t seed, const gmres_opts* opts, gmres_report* rep);
+
+/* ---- synthetic inputs (host only; no GPU needed) ---------------------------- */
+/* Rods of nbpb spheres spaced 2a on the body x axis, centres uniform in the box
+ * with rejection of inter-body blob pairs closer than 2a (minimum image), at
+ * most max_attempts per rod (then accepted with overlap), uniform random
+ * quaternions (normalised 4-Gaussian, quaternion.cpp:10-17), N(0,1) forces and
+ * torques with the axial torque removed.  Outputs may not be NULL except F6. */
+int fcm_synth_rods(uint64_t seed, int NB, int nbpb, double a, const double L[3], int max_attempts, double* blob_ref,
+                   double* x, double* q, double* F6, int* n_overlapping);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FCM_HIP_H */

@@ -1,0 +1,321 @@
+// fcm_common.hpp -- shared device helpers for the FCM mobility kernels (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace fcm {
+
+// ---------------------------------------------------------------------------
+// complex helpers (double2 / float2)
+// ---------------------------------------------------------------------------
+template <typename T> struct cplx_t;
+template <> struct cplx_t<double> { using type = double2; };
+template <> struct cplx_t<float> { using type = float2; };
+template <typename T> using c2 = typename cplx_t<T>::type;
+
+template <typename T> __host__ __device__ __forceinline__ c2<T> mk(T x, T y) { c2<T> r; r.x = x; r.y = y; return r; }
+template <typename V> __host__ __device__ __forceinline__ V cadd(V a, V b) { V r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <typename V> __host__ __device__ __forceinline__ V csub(V a, V b) { V r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+template <typename V> __host__ __device__ __forceinline__ V cmul(V a, V b) {
+  V r; r.x = a.x * b.x - a.y * b.y; r.y = a.x * b.y + a.y * b.x; return r;
+}
+// a * conj(b)
+template <typename V> __host__ __device__ __forceinline__ V cmulc(V a, V b) {
+  V r; r.x = a.x * b.x + a.y * b.y; r.y = a.y * b.x - a.x * b.y; return r;
+}
+template <typename V> __host__ __device__ __forceinline__ V cconj(V a) { V r; r.x = a.x; r.y = -a.y; return r; }
+template <typename V, typename S> __host__ __device__ __forceinline__ V cscale(V a, S s) { V r; r.x = a.x * s; r.y = a.y * s; return r; }
+// multiply by s*i, s = +1 or -1
+template <int S, typename V> __host__ __device__ __forceinline__ V cmul_si(V a) {
+  V r;
+  if (S > 0) { r.x = -a.y; r.y = a.x; } else { r.x = a.y; r.y = -a.x; }
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// FFT plans: a length N is split into stages of radix 16, 8, 4, 2, 3 or 5.
+// Each thread of a column holds P points; every radix divides P, so each
+// thread owns P/R butterflies per stage (Stockham auto-sort formulation).
+// ---------------------------------------------------------------------------
+struct FftPlan {
+  int N = 0, P = 0, T = 0, ns = 0;
+  int r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  bool ok = false;
+};
+
+constexpr bool only_235(int n) {
+  while (n % 2 == 0) n /= 2;
+  while (n % 3 == 0) n /= 3;
+  while (n % 5 == 0) n /= 5;
+  return n == 1;
+}
+
+constexpr bool primes_divide(int N, int P) {
+  for (int p : {2, 3, 5})
+    if (N % p == 0 && P % p != 0) return false;
+  return true;
+}
+
+constexpr FftPlan make_plan(int N) {
+  FftPlan pl;
+  pl.N = N;
+  if (N < 2 || !only_235(N)) return pl;
+  int P = 0;
+  for (int d = 16; d >= 2; --d)
+    if (N % d == 0 && primes_divide(N, d)) { P = d; break; }
+  if (P == 0) return pl;
+  pl.P = P;
+  pl.T = N / P;
+  int rem = N;
+  const int cand[6] = {16, 8, 4, 2, 3, 5};
+  while (rem > 1) {
+    int pick = 0;
+    for (int c : cand)
+      if (P % c == 0 && rem % c == 0) { pick = c; break; }
+    if (pick == 0 || pl.ns >= 8) return pl;
+    pl.r[pl.ns++] = pick;
+    rem /= pick;
+  }
+  pl.ok = true;
+  return pl;
+}
+
+// inverse plans use the radices in reverse order, so that the register layout
+// at the end of a forward transform is the layout an inverse starts from
+constexpr FftPlan reverse_plan(FftPlan p) {
+  FftPlan q = p;
+  for (int i = 0; i < p.ns; ++i) q.r[i] = p.r[p.ns - 1 - i];
+  return q;
+}
+
+constexpr int plan_ns_before(const FftPlan& p, int s) {
+  int v = 1;
+  for (int i = 0; i < s; ++i) v *= p.r[i];
+  return v;
+}
+
+// supported transform lengths (x, y and half-z)
+#define FCM_FOR_EACH_N(X) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
+
+inline bool fft_len_supported(int n) {
+  switch (n) {
+#define FCM_CASE(v) case v:
+    FCM_FOR_EACH_N(FCM_CASE)
+#undef FCM_CASE
+    return true;
+    default:
+      return false;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// small-radix DFTs, natural-order output; S = -1 forward, +1 inverse
+// ---------------------------------------------------------------------------
+template <typename T, int R, int S> struct Dft;
+
+template <typename T, int S> struct Dft<T, 2, S> {
+  static __device__ __forceinline__ void run(c2<T>* a) {
+    const c2<T> t = a[0];
+    a[0] = cadd(t, a[1]);
+    a[1] = csub(t, a[1]);
+  }
+};
+
+template <typename T, int S> struct Dft<T, 3, S> {
+  static __device__ __forceinline__ void run(c2<T>* a) {
+    const T h = T(0.8660254037844386467637231707529362);
+    const c2<T> t1 = cadd(a[1], a[2]);
+    const c2<T> t2 = csub(a[1], a[2]);
+    const c2<T> m = csub(a[0], cscale(t1, T(0.5)));
+    const c2<T> n = cscale(cmul_si<S>(t2), h);
+    a[0] = cadd(a[0], t1);
+    a[1] = cadd(m, n);
+    a[2] = csub(m, n);
+  }
+};
+
+template <typename T, int S> struct Dft<T, 4, S> {
+  static __device__ __forceinline__ void run(c2<T>* a) {
+    const c2<T> t0 = cadd(a[0], a[2]);
+    const c2<T> t1 = csub(a[0], a[2]);
+    const c2<T> t2 = cadd(a[1], a[3]);
+    const c2<T> t3 = cmul_si<S>(csub(a[1], a[3]));
+    a[0] = cadd(t0, t2);
+    a[2] = csub(t0, t2);
+    a[1] = cadd(t1, t3);
+    a[3] = csub(t1, t3);
+  }
+};
+
+template <typename T, int S> struct Dft<T, 5, S> {
+  static __device__ __forceinline__ void run(c2<T>* a) {
+    const T c1 = T(0.3090169943749474241022934171828191);
+    const T c2v = T(-0.8090169943749474241022934171828191);
+    const T s1 = T(0.9510565162951535721164393333793821);
+    const T s2 = T(0.5877852522924731291687059546390728);
+    const c2<T> t1 = cadd(a[1], a[4]), t2 = cadd(a[2], a[3]);
+    const c2<T> t3 = csub(a[1], a[4]), t4 = csub(a[2], a[3]);
+    const c2<T> a0 = a[0];
+    const c2<T> m1 = cadd(a0, cadd(cscale(t1, c1), cscale(t2, c2v)));
+    const c2<T> m2 = cadd(a0, cadd(cscale(t1, c2v), cscale(t2, c1)));
+    const c2<T> n1 = cmul_si<S>(cadd(cscale(t3, s1), cscale(t4, s2)));
+    const c2<T> n2 = cmul_si<S>(csub(cscale(t3, s2), cscale(t4, s1)));
+    a[0] = cadd(a0, cadd(t1, t2));
+    a[1] = cadd(m1, n1);
+    a[4] = csub(m1, n1);
+    a[2] = cadd(m2, n2);
+    a[3] = csub(m2, n2);
+  }
+};
+
+// radix 8 = two radix-4 halves (even/odd samples) joined with W8^k
+template <typename T, int S> struct Dft<T, 8, S> {
+  static __device__ __forceinline__ void run(c2<T>* a) {
+    c2<T> e[4] = {a[0], a[2], a[4], a[6]};
+    c2<T> o[4] = {a[1], a[3], a[5], a[7]};
+    Dft<T, 4, S>::run(e);
+    Dft<T, 4, S>::run(o);
+    const T h = T(0.7071067811865475244008443621048490);
+    // W8^1 = h(1 + S i), W8^2 = S i, W8^3 = h(-1 + S i)
+    const c2<T> o1 = cscale(cadd(o[1], cmul_si<S>(o[1])), h);
+    const c2<T> o2 = cmul_si<S>(o[2]);
+    const c2<T> o3 = cscale(csub(cmul_si<S>(o[3]), o[3]), h);
+    a[0] = cadd(e[0], o[0]); a[4] = csub(e[0], o[0]);
+    a[1] = cadd(e[1], o1);   a[5] = csub(e[1], o1);
+    a[2] = cadd(e[2], o2);   a[6] = csub(e[2], o2);
+    a[3] = cadd(e[3], o3);   a[7] = csub(e[3], o3);
+  }
+};
+
+// radix 16 = two radix-8 halves joined with W16^k
+template <typename T, int S> struct Dft<T, 16, S> {
+  static __device__ __forceinline__ void run(c2<T>* a) {
+    c2<T> e[8], o[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) { e[i] = a[2 * i]; o[i] = a[2 * i + 1]; }
+    Dft<T, 8, S>::run(e);
+    Dft<T, 8, S>::run(o);
+    // W16^k = cos(2 pi k/16) + S i sin(2 pi k/16)
+    const T cs[8] = {T(1.0), T(0.9238795325112867561281831893967883), T(0.7071067811865475244008443621048490),
+                     T(0.3826834323650897717284599840303989), T(0.0), T(-0.3826834323650897717284599840303989),
+                     T(-0.7071067811865475244008443621048490), T(-0.9238795325112867561281831893967883)};
+    const T sn[8] = {T(0.0), T(0.3826834323650897717284599840303989), T(0.7071067811865475244008443621048490),
+                     T(0.9238795325112867561281831893967883), T(1.0), T(0.9238795325112867561281831893967883),
+                     T(0.7071067811865475244008443621048490), T(0.3826834323650897717284599840303989)};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const c2<T> w = mk<T>(cs[k], T(S) * sn[k]);
+      const c2<T> t = (k == 0) ? o[0] : cmul(o[k], w);
+      a[k] = cadd(e[k], t);
+      a[k + 8] = csub(e[k], t);
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Stockham column FFT.  `v` holds this thread's P points in the layout of the
+// first radix r0: v[i*r0 + r] = x[(tcol + i*T) + r*N/r0].  On return v holds
+// the outputs in the layout of the last radix.  Between stages the data goes
+// through `lds` (element e of this column at lds[e*ls + col]).  Every thread
+// of the block must call this (it contains barriers).
+// tw[m] = exp(-2 pi i m / N), m < N.
+// ---------------------------------------------------------------------------
+template <typename T, int N, int S, bool REV, int STG>
+struct FftStage {
+  static constexpr FftPlan PL = REV ? reverse_plan(make_plan(N)) : make_plan(N);
+  static constexpr int P = PL.P;
+  static constexpr int TT = PL.T;
+  static constexpr int R = PL.r[STG];
+  static constexpr int NS = plan_ns_before(PL, STG);
+  static constexpr bool LAST = (STG == PL.ns - 1);
+
+  static __device__ __forceinline__ void run(c2<T>* v, c2<T>* lds, int ls, int col, int tcol,
+                                             const c2<T>* __restrict__ tw) {
+#pragma unroll
+    for (int i = 0; i < P / R; ++i) {
+      const int j = tcol + i * TT;
+      c2<T>* a = v + i * R;
+      if (NS > 1) {
+        const int k = j % NS;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const int m = r * k * (N / (NS * R));
+          c2<T> w = tw[m];
+          if (S > 0) w.y = -w.y;
+          a[r] = cmul(a[r], w);
+        }
+      }
+      Dft<T, R, S>::run(a);
+    }
+    if constexpr (!LAST) {
+      constexpr int R2 = PL.r[STG + 1];
+#pragma unroll
+      for (int i = 0; i < P / R; ++i) {
+        const int j = tcol + i * TT;
+        const int k = j % NS;
+        const int base = (j / NS) * NS * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[(base + r * NS) * ls + col] = v[i * R + r];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < P / R2; ++i) {
+        const int j = tcol + i * TT;
+#pragma unroll
+        for (int r = 0; r < R2; ++r) v[i * R2 + r] = lds[(j + r * (N / R2)) * ls + col];
+      }
+      __syncthreads();
+      FftStage<T, N, S, REV, STG + 1>::run(v, lds, ls, col, tcol, tw);
+    }
+  }
+};
+
+template <typename T, int N, int S, bool REV>
+__device__ __forceinline__ void fft_column(c2<T>* v, c2<T>* lds, int ls, int col, int tcol,
+                                           const c2<T>* __restrict__ tw) {
+  FftStage<T, N, S, REV, 0>::run(v, lds, ls, col, tcol, tw);
+}
+
+// element index held in register slot (i, r) for the first radix of a plan
+template <int N, bool REV>
+__device__ __forceinline__ int fft_in_index(int tcol, int slot) {
+  constexpr FftPlan PL = REV ? reverse_plan(make_plan(N)) : make_plan(N);
+  constexpr int R0 = PL.r[0];
+  const int i = slot / R0, r = slot % R0;
+  return tcol + i * PL.T + r * (N / R0);
+}
+
+// element index held in register slot for the last radix of a plan
+template <int N, bool REV>
+__device__ __forceinline__ int fft_out_index(int tcol, int slot) {
+  constexpr FftPlan PL = REV ? reverse_plan(make_plan(N)) : make_plan(N);
+  constexpr int RL = PL.r[PL.ns - 1];
+  const int i = slot / RL, r = slot % RL;
+  return tcol + i * PL.T + r * (N / RL);
+}
+
+// ---------------------------------------------------------------------------
+// misc
+// ---------------------------------------------------------------------------
+__host__ __device__ __forceinline__ int cyc(int d, int n, int omin) {
+  // the unique value congruent to d mod n in [omin, omin + n)
+  int t = (d - omin) % n;
+  if (t < 0) t += n;
+  return t + omin;
+}
+
+__device__ __forceinline__ double warp_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ float warp_sum(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+}  // namespace fcm

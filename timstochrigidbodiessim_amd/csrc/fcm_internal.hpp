@@ -1,0 +1,128 @@
+// fcm_internal.hpp -- declarations shared by the kernel translation units and
+// the host orchestration behind the C ABI.
+#pragma once
+
+#include "fcm_common.hpp"
+
+namespace fcm {
+
+// Grid geometry.  Fields are complex arrays [nx][ny][cpitch]; real-space rows
+// hold nz reals in their first nz real slots.
+struct GridGeom {
+  int nx = 0, ny = 0, nz = 0, nzc = 0, cpitch = 0;
+  double dx = 1.0;
+  size_t field_words() const { return (size_t)nx * ny * cpitch; }  // complex words per field
+};
+
+template <typename T> struct FieldPtrs { c2<T>* p[6]; };
+
+// device twiddle tables: x (nx), y (ny), z (nz/2) and the R2C untangling
+// factors zu[k] = exp(-2 pi i k / nz), k = 0..nz/2
+template <typename T> struct TwiddleSet {
+  const c2<T>* x = nullptr;
+  const c2<T>* y = nullptr;
+  const c2<T>* z = nullptr;
+  const c2<T>* zu = nullptr;
+};
+
+struct ProjParams {
+  double facx, facy, facz;  // 2 pi / L per axis (fcm_fluid_solver.cpp:75,97,118)
+  double inv_muN;           // 1 / (mu nx ny nz)
+  int nx, ny, nz;
+};
+
+template <typename T>
+hipError_t fft3d_forward(const GridGeom& g, const FieldPtrs<T>& fp, int nf, const TwiddleSet<T>& tws, hipStream_t st);
+template <typename T>
+hipError_t fft3d_inverse(const GridGeom& g, const FieldPtrs<T>& fp, int nf, const TwiddleSet<T>& tws, hipStream_t st);
+template <typename T>
+hipError_t stokes_solve(const GridGeom& g, const FieldPtrs<T>& fp, const ProjParams& pp, const TwiddleSet<T>& tws,
+                        hipStream_t st);
+template <typename T>
+hipError_t random_velocity(const GridGeom& g, const FieldPtrs<T>& stress6, bool generate, unsigned long long seed,
+                           const double scale[6], const ProjParams& pp, const TwiddleSet<T>& tws, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// blobs: Gaussian set-up, cell binning, spreading and interpolation
+// ---------------------------------------------------------------------------
+struct BlobBins {
+  int bx = 4, by = 8, bz = 8;       // bin extent (nodes) per axis, divides the grid
+  int nbx = 0, nby = 0, nbz = 0;    // bins per axis
+  int sx = 2, sy = 2, sz = 2;       // spread tile = (sx*bx, sy*by, sz*bz) nodes
+  int nbins() const { return nbx * nby * nbz; }
+};
+
+struct BlobData {
+  int nb = 0;                 // number of blobs
+  int ng = 7;                 // Gaussian points per axis
+  const double* Y = nullptr;  // nb x 3 positions
+  int* bin_of = nullptr;      // nb
+  int* bin_count = nullptr;   // nbins
+  int* bin_start = nullptr;   // nbins + 1
+  int* bin_cursor = nullptr;  // nbins
+  int* sorted = nullptr;      // nb, blob ids in bin order (ascending id within a bin)
+  int4* rec = nullptr;        // nb, sorted order: (anchor x, y, z wrapped, id)
+  int4* anc = nullptr;        // nb, sorted order: unwrapped anchors
+  double* w64 = nullptr;      // nb x 3 x ng, sorted order
+  float* w32 = nullptr;       // nb x 3 x ng, sorted order
+  int* scan_tmp = nullptr;    // scratch for the bin scan
+};
+
+hipError_t setup_gaussians(const GridGeom& g, const BlobBins& bb, BlobData& bd, double a, hipStream_t st);
+
+// constraint term folded into interpolation: out[id] = dx^3 J u - (U + W x d_id)
+struct KTerm {
+  const double* zV = nullptr;  // 6 per body (U, Omega), null: no K term
+  const double* dpos = nullptr;  // 3 per blob, R_n r_m
+  int nbpb = 1;
+};
+
+template <typename T>
+hipError_t spread(const GridGeom& g, const BlobBins& bb, const BlobData& bd, const double* F, const FieldPtrs<T>& fp,
+                  hipStream_t st);
+template <typename T>
+hipError_t interp(const GridGeom& g, const BlobBins& bb, const BlobData& bd, const FieldPtrs<T>& fp, double* V,
+                  const KTerm& kt, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// rigid bodies, preconditioner, Krylov kernels
+// ---------------------------------------------------------------------------
+struct BodyData {
+  int NB = 0, nbpb = 0;
+  const double* ref = nullptr;   // 3 x nbpb body-frame blob offsets
+  const double* x = nullptr;     // 3 NB
+  const double* q = nullptr;     // 4 NB
+  double* R = nullptr;           // 9 NB (row-major)
+  double* dpos = nullptr;        // 3 nb (world-frame offsets R r_m)
+  double* Y = nullptr;           // 3 nb wrapped positions
+  const double* Minv = nullptr;  // (3 nbpb)^2
+  const double* Nref = nullptr;  // 36
+};
+
+hipError_t bodies_update(const GridGeom& g, BodyData& b, hipStream_t st);
+// out = P^{-1} in; if wKT != null also writes wKT = -K^T out_lambda (6 per body)
+hipError_t apply_pinv(const BodyData& b, const double* in, double* out, double* wKT, hipStream_t st);
+hipError_t mul_K(const BodyData& b, const double* in6, double* out3, hipStream_t st);
+hipError_t mul_KT(const BodyData& b, const double* in3, double* out6, hipStream_t st);
+
+struct KrylovWork {
+  int nblk = 0;              // row blocks for the reductions
+  double* partial = nullptr; // nblk x max_cols
+  double* red = nullptr;     // reduced values (max_cols + 8)
+};
+
+int krylov_blocks(size_t n);
+// red[i] = sum_r Q[i][r] w[r], i < ncols   (deterministic two-stage)
+hipError_t dots(const double* Q, size_t ldq, int ncols, const double* w, size_t n, KrylovWork& kw, double* red_out,
+                hipStream_t st);
+// w -= sum_i h[i] Q[i]; if norm_out != null, *norm_out = ||w||^2 after the update
+hipError_t update(const double* Q, size_t ldq, int ncols, const double* h, double* w, size_t n, KrylovWork& kw,
+                  double* norm_out, hipStream_t st);
+// y = sum_i c[i] Q[i]
+hipError_t combine(const double* Q, size_t ldq, int ncols, const double* c, double* y, size_t n, hipStream_t st);
+// y = a*x (a from device pointer, taken as 1/sqrt(*a) if invsqrt)
+hipError_t scale_vec(const double* x, double* y, size_t n, const double* a, bool invsqrt, hipStream_t st);
+hipError_t axpby(double a, const double* x, double b, double* y, size_t n, hipStream_t st);  // y = a x + b y
+hipError_t rhs_assemble(const double* Vrand, const double* F6, double* b, size_t n3, size_t n6, hipStream_t st);
+
+}  // namespace fcm

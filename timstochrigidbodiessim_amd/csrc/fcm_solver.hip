@@ -1,0 +1,395 @@
+// fcm_solver.hip -- rigid-body operators, the block-diagonal preconditioner
+// and the Krylov vector kernels of the saddle-point solve (FP64 throughout).
+#include "fcm_internal.hpp"
+
+namespace fcm {
+
+__device__ __forceinline__ double wrapd(double p, double L) { return p - L * floor(p / L); }
+
+// quaternion.cpp:61-103 (row-major R)
+__device__ __forceinline__ void quat_matrix(const double* q, double* R) {
+  const double s = q[0], v0 = q[1], v1 = q[2], v2 = q[3];
+  double M[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  double t = 2.0 * v0 * v0; M[4] -= t; M[8] -= t;
+  t = 2.0 * v1 * v1; M[0] -= t; M[8] -= t;
+  t = 2.0 * v2 * v2; M[0] -= t; M[4] -= t;
+  t = 2.0 * v0 * v1; M[3] += t; M[1] += t;
+  t = 2.0 * v0 * v2; M[6] += t; M[2] += t;
+  t = 2.0 * v1 * v2; M[5] += t; M[7] += t;
+  t = 2.0 * s * v2; M[3] += t; M[1] -= t;
+  t = 2.0 * s * v1; M[6] -= t; M[2] += t;
+  t = 2.0 * s * v0; M[7] += t; M[5] -= t;
+  for (int i = 0; i < 9; ++i) R[i] = M[i];
+}
+
+// one thread per blob: R of its body, world-frame offset R r_m, wrapped
+// position x + R r_m (collection_of_rigid_bodies.cpp:155-173)
+__global__ void bodies_kernel(BodyData b, double LX, double LY, double LZ) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nb = b.NB * b.nbpb;
+  if (id >= nb) return;
+  const int n = id / b.nbpb, m = id % b.nbpb;
+  double R[9];
+  quat_matrix(b.q + 4 * (size_t)n, R);
+  if (m == 0)
+    for (int i = 0; i < 9; ++i) b.R[9 * (size_t)n + i] = R[i];
+  const double* r = b.ref + 3 * m;
+  const double d0 = R[0] * r[0] + R[1] * r[1] + R[2] * r[2];
+  const double d1 = R[3] * r[0] + R[4] * r[1] + R[5] * r[2];
+  const double d2 = R[6] * r[0] + R[7] * r[1] + R[8] * r[2];
+  b.dpos[3 * (size_t)id] = d0;
+  b.dpos[3 * (size_t)id + 1] = d1;
+  b.dpos[3 * (size_t)id + 2] = d2;
+  const double* x = b.x + 3 * (size_t)n;
+  b.Y[3 * (size_t)id] = wrapd(x[0] + d0, LX);
+  b.Y[3 * (size_t)id + 1] = wrapd(x[1] + d1, LY);
+  b.Y[3 * (size_t)id + 2] = wrapd(x[2] + d2, LZ);
+}
+
+hipError_t bodies_update(const GridGeom& g, BodyData& b, hipStream_t st) {
+  const int nb = b.NB * b.nbpb;
+  if (nb == 0) return hipSuccess;
+  hipLaunchKernelGGL(bodies_kernel, dim3((nb + 255) / 256), dim3(256), 0, st, b, g.dx * g.nx, g.dx * g.ny,
+                     g.dx * g.nz);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// P^{-1}, gmres_solver.cpp:85-146, evaluated in the body frame: with R the
+// body rotation, Minv_n = (I(x)R) Minv_ref (I(x)R)^T and N_n = diag(R,R) N_ref
+// diag(R,R)^T, so every product can act on rotated vectors instead of on
+// rotated matrices:
+//   b' = R^T b_lambda,  t = Minv_ref b',  s' = -diag(R^T,R^T) b_V - K_ref^T t,
+//   V' = N_ref s',  out_V = diag(R,R) V',  out_lambda = R (Minv_ref (K_ref V' + b')).
+// One workgroup per body; optionally also -K^T out_lambda for the mat-vec.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(128) pinv_kernel(BodyData b, const double* __restrict__ in, double* __restrict__ out,
+                                                   double* __restrict__ wKT) {
+  extern __shared__ double sm[];
+  const int n3 = 3 * b.nbpb;
+  double* bp = sm;            // n3
+  double* tt = bp + n3;       // n3
+  double* gg = tt + n3;       // n3
+  double* oo = gg + n3;       // n3
+  double* sv = oo + n3;       // 6
+  double* vv = sv + 6;        // 6
+  double* Rs = vv + 6;        // 9
+  double* ref = Rs + 9;       // n3
+  const int n = blockIdx.x;
+  const int t = threadIdx.x;
+  const size_t nlam = 3 * (size_t)b.NB * b.nbpb;
+  const double* inl = in + (size_t)n3 * n;
+  const double* inv = in + nlam + 6 * (size_t)n;
+  if (t < 9) Rs[t] = b.R[9 * (size_t)n + t];
+  for (int i = t; i < n3; i += blockDim.x) ref[i] = b.ref[i];
+  __syncthreads();
+  for (int i = t; i < n3; i += blockDim.x) {
+    const int m = i / 3, c = i % 3;
+    const double* v = inl + 3 * m;
+    bp[i] = Rs[c] * v[0] + Rs[3 + c] * v[1] + Rs[6 + c] * v[2];  // (R^T v)_c
+  }
+  __syncthreads();
+  for (int i = t; i < n3; i += blockDim.x) {
+    const double* row = b.Minv + (size_t)i * n3;
+    double s = 0.0;
+    for (int k = 0; k < n3; ++k) s += row[k] * bp[k];
+    tt[i] = s;
+  }
+  __syncthreads();
+  if (t < 6) {
+    const int c = t % 3;
+    // (R^T b_V)_c for the translational (t<3) or rotational (t>=3) part
+    const double* v = inv + (t < 3 ? 0 : 3);
+    const double rv = Rs[c] * v[0] + Rs[3 + c] * v[1] + Rs[6 + c] * v[2];
+    double k = 0.0;
+    if (t < 3) {
+      for (int m = 0; m < b.nbpb; ++m) k += tt[3 * m + c];
+    } else {
+      const int c1 = (c + 1) % 3, c2i = (c + 2) % 3;
+      for (int m = 0; m < b.nbpb; ++m) k += ref[3 * m + c1] * tt[3 * m + c2i] - ref[3 * m + c2i] * tt[3 * m + c1];
+    }
+    sv[t] = -rv - k;
+  }
+  __syncthreads();
+  if (t < 6) {
+    double s = 0.0;
+    for (int k = 0; k < 6; ++k) s += b.Nref[t * 6 + k] * sv[k];
+    vv[t] = s;
+  }
+  __syncthreads();
+  for (int i = t; i < n3; i += blockDim.x) {
+    const int m = i / 3, c = i % 3;
+    const int c1 = (c + 1) % 3, c2i = (c + 2) % 3;
+    // K_ref V' = U' + Omega' x r_m
+    gg[i] = vv[c] + (vv[3 + c1] * ref[3 * m + c2i] - vv[3 + c2i] * ref[3 * m + c1]) + bp[i];
+  }
+  __syncthreads();
+  for (int i = t; i < n3; i += blockDim.x) {
+    const double* row = b.Minv + (size_t)i * n3;
+    double s = 0.0;
+    for (int k = 0; k < n3; ++k) s += row[k] * gg[k];
+    oo[i] = s;
+  }
+  __syncthreads();
+  double* outl = out + (size_t)n3 * n;
+  for (int i = t; i < n3; i += blockDim.x) {
+    const int m = i / 3, c = i % 3;
+    const double* o = oo + 3 * m;
+    const double v = Rs[3 * c] * o[0] + Rs[3 * c + 1] * o[1] + Rs[3 * c + 2] * o[2];
+    outl[i] = v;
+    gg[i] = v;  // keep the world-frame result for K^T
+  }
+  if (t < 6) {
+    const int c = t % 3;
+    const double* v = vv + (t < 3 ? 0 : 3);
+    out[nlam + 6 * (size_t)n + t] = Rs[3 * c] * v[0] + Rs[3 * c + 1] * v[1] + Rs[3 * c + 2] * v[2];
+  }
+  if (wKT) {
+    __syncthreads();
+    if (t < 6) {
+      // -K^T lambda: -[sum lambda_m ; sum (R r_m) x lambda_m]   (collection_of_rigid_bodies.cpp:749)
+      const int c = t % 3;
+      double s = 0.0;
+      const double* d = b.dpos + 3 * (size_t)n * b.nbpb;
+      if (t < 3) {
+        for (int m = 0; m < b.nbpb; ++m) s += gg[3 * m + c];
+      } else {
+        const int c1 = (c + 1) % 3, c2i = (c + 2) % 3;
+        for (int m = 0; m < b.nbpb; ++m) s += d[3 * m + c1] * gg[3 * m + c2i] - d[3 * m + c2i] * gg[3 * m + c1];
+      }
+      wKT[6 * (size_t)n + t] = -s;
+    }
+  }
+}
+
+hipError_t apply_pinv(const BodyData& b, const double* in, double* out, double* wKT, hipStream_t st) {
+  if (b.NB == 0) return hipSuccess;
+  const int n3 = 3 * b.nbpb;
+  const size_t lds = sizeof(double) * (5 * (size_t)n3 + 21);
+  hipLaunchKernelGGL(pinv_kernel, dim3(b.NB), dim3(128), lds, st, b, in, out, wKT);
+  return hipGetLastError();
+}
+
+// K [U;W] per blob: U + W x (R r_m)   (collection_of_rigid_bodies.cpp:707-731)
+__global__ void mulK_kernel(BodyData b, const double* __restrict__ in6, double* __restrict__ out3) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= b.NB * b.nbpb) return;
+  const int n = id / b.nbpb;
+  const double* u = in6 + 6 * (size_t)n;
+  const double* d = b.dpos + 3 * (size_t)id;
+  out3[3 * (size_t)id] = u[0] + (u[4] * d[2] - u[5] * d[1]);
+  out3[3 * (size_t)id + 1] = u[1] + (u[5] * d[0] - u[3] * d[2]);
+  out3[3 * (size_t)id + 2] = u[2] + (u[3] * d[1] - u[4] * d[0]);
+}
+
+// K^T lambda per body: [sum lambda_m; sum (R r_m) x lambda_m]   (:733-757)
+__global__ void mulKT_kernel(BodyData b, const double* __restrict__ in3, double* __restrict__ out6) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= b.NB) return;
+  double a[6] = {0, 0, 0, 0, 0, 0};
+  for (int m = 0; m < b.nbpb; ++m) {
+    const size_t id = (size_t)n * b.nbpb + m;
+    const double* l = in3 + 3 * id;
+    const double* d = b.dpos + 3 * id;
+    a[0] += l[0]; a[1] += l[1]; a[2] += l[2];
+    a[3] += d[1] * l[2] - d[2] * l[1];
+    a[4] += d[2] * l[0] - d[0] * l[2];
+    a[5] += d[0] * l[1] - d[1] * l[0];
+  }
+  for (int i = 0; i < 6; ++i) out6[6 * (size_t)n + i] = a[i];
+}
+
+hipError_t mul_K(const BodyData& b, const double* in6, double* out3, hipStream_t st) {
+  const int nb = b.NB * b.nbpb;
+  if (nb == 0) return hipSuccess;
+  hipLaunchKernelGGL(mulK_kernel, dim3((nb + 255) / 256), dim3(256), 0, st, b, in6, out3);
+  return hipGetLastError();
+}
+
+hipError_t mul_KT(const BodyData& b, const double* in3, double* out6, hipStream_t st) {
+  if (b.NB == 0) return hipSuccess;
+  hipLaunchKernelGGL(mulKT_kernel, dim3((b.NB + 255) / 256), dim3(256), 0, st, b, in3, out6);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Krylov kernels.  Rows are split into fixed blocks of KB rows; each block
+// writes one partial per column and a second kernel sums the partials of a
+// column in a fixed tree order, so every dot product is bitwise repeatable.
+// ---------------------------------------------------------------------------
+static constexpr int KT = 256;        // threads per block
+static constexpr int KRPT = 4;        // rows per thread
+static constexpr int KB = KT * KRPT;  // rows per block
+static constexpr int KCOLS = 8;       // columns reduced together
+
+int krylov_blocks(size_t n) { return (int)((n + KB - 1) / KB); }
+
+__device__ __forceinline__ void block_reduce8(double (&v)[KCOLS], double* red_lds /* 4 x KCOLS */) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < KCOLS; ++c) v[c] = warp_sum(v[c]);
+  if (lane == 0)
+#pragma unroll
+    for (int c = 0; c < KCOLS; ++c) red_lds[wv * KCOLS + c] = v[c];
+  __syncthreads();
+  if (threadIdx.x < KCOLS) {
+    const int c = threadIdx.x;
+    v[c] = ((red_lds[c] + red_lds[KCOLS + c]) + (red_lds[2 * KCOLS + c] + red_lds[3 * KCOLS + c]));
+  }
+}
+
+__global__ void __launch_bounds__(KT) dots_kernel(const double* __restrict__ Q, size_t ldq, int ncols,
+                                                  const double* __restrict__ w, size_t n, double* __restrict__ partial,
+                                                  int nblk) {
+  __shared__ double red[4 * KCOLS];
+  const size_t r0 = (size_t)blockIdx.x * KB + threadIdx.x;
+  double wv[KRPT];
+#pragma unroll
+  for (int k = 0; k < KRPT; ++k) {
+    const size_t r = r0 + (size_t)k * KT;
+    wv[k] = r < n ? w[r] : 0.0;
+  }
+  for (int c0 = 0; c0 < ncols; c0 += KCOLS) {
+    double acc[KCOLS];
+#pragma unroll
+    for (int c = 0; c < KCOLS; ++c) {
+      acc[c] = 0.0;
+      if (c0 + c < ncols) {
+        const double* q = Q + (size_t)(c0 + c) * ldq;
+#pragma unroll
+        for (int k = 0; k < KRPT; ++k) {
+          const size_t r = r0 + (size_t)k * KT;
+          if (r < n) acc[c] += q[r] * wv[k];
+        }
+      }
+    }
+    block_reduce8(acc, red);
+    if (threadIdx.x < KCOLS && c0 + (int)threadIdx.x < ncols)
+      partial[(size_t)(c0 + threadIdx.x) * nblk + blockIdx.x] = acc[threadIdx.x];
+    __syncthreads();
+  }
+}
+
+// out[c] = sum over blocks of partial[c][blk], fixed tree order; one block per column
+__global__ void __launch_bounds__(256) reduce_cols_kernel(const double* __restrict__ partial, int nblk,
+                                                          double* __restrict__ out, int sqrt_flag) {
+  __shared__ double s[256];
+  const double* p = partial + (size_t)blockIdx.x * nblk;
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += 256) a += p[i];
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = sqrt_flag ? sqrt(s[0]) : s[0];
+}
+
+__global__ void __launch_bounds__(KT) update_kernel(const double* __restrict__ Q, size_t ldq, int ncols,
+                                                    const double* __restrict__ h, double* __restrict__ w, size_t n,
+                                                    double* __restrict__ partial, int nblk) {
+  __shared__ double hs[1024];
+  __shared__ double red[4 * KCOLS];
+  for (int i = threadIdx.x; i < ncols && i < 1024; i += KT) hs[i] = h[i];
+  __syncthreads();
+  const size_t r0 = (size_t)blockIdx.x * KB + threadIdx.x;
+  double wv[KRPT];
+#pragma unroll
+  for (int k = 0; k < KRPT; ++k) {
+    const size_t r = r0 + (size_t)k * KT;
+    wv[k] = r < n ? w[r] : 0.0;
+  }
+  for (int c = 0; c < ncols; ++c) {
+    const double hc = c < 1024 ? hs[c] : h[c];
+    const double* q = Q + (size_t)c * ldq;
+#pragma unroll
+    for (int k = 0; k < KRPT; ++k) {
+      const size_t r = r0 + (size_t)k * KT;
+      if (r < n) wv[k] -= hc * q[r];
+    }
+  }
+  double acc[KCOLS];
+#pragma unroll
+  for (int c = 0; c < KCOLS; ++c) acc[c] = 0.0;
+#pragma unroll
+  for (int k = 0; k < KRPT; ++k) {
+    const size_t r = r0 + (size_t)k * KT;
+    if (r < n) {
+      w[r] = wv[k];
+      acc[0] += wv[k] * wv[k];
+    }
+  }
+  if (partial) {
+    block_reduce8(acc, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc[0];
+  }
+}
+
+__global__ void __launch_bounds__(KT) combine_kernel(const double* __restrict__ Q, size_t ldq, int ncols,
+                                                     const double* __restrict__ c, double* __restrict__ y, size_t n) {
+  const size_t r = (size_t)blockIdx.x * KT + threadIdx.x;
+  if (r >= n) return;
+  double s = 0.0;
+  for (int i = 0; i < ncols; ++i) s += c[i] * Q[(size_t)i * ldq + r];
+  y[r] = s;
+}
+
+__global__ void scale_kernel(const double* __restrict__ x, double* __restrict__ y, size_t n, const double* a,
+                             int invsqrt) {
+  const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const double s = invsqrt ? 1.0 / sqrt(*a) : *a;
+  y[r] = x[r] * s;
+}
+
+__global__ void axpby_kernel(double a, const double* __restrict__ x, double b, double* __restrict__ y, size_t n) {
+  const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) y[r] = a * x[r] + b * y[r];
+}
+
+__global__ void rhs_kernel(const double* __restrict__ Vrand, const double* __restrict__ F6, double* __restrict__ b,
+                           size_t n3, size_t n6) {
+  const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n3) b[r] = Vrand ? -Vrand[r] : 0.0;
+  else if (r < n3 + n6) b[r] = F6 ? -F6[r - n3] : 0.0;
+}
+
+hipError_t dots(const double* Q, size_t ldq, int ncols, const double* w, size_t n, KrylovWork& kw, double* red_out,
+                hipStream_t st) {
+  if (ncols <= 0) return hipSuccess;
+  hipLaunchKernelGGL(dots_kernel, dim3(kw.nblk), dim3(KT), 0, st, Q, ldq, ncols, w, n, kw.partial, kw.nblk);
+  hipLaunchKernelGGL(reduce_cols_kernel, dim3(ncols), dim3(256), 0, st, kw.partial, kw.nblk, red_out, 0);
+  return hipGetLastError();
+}
+
+hipError_t update(const double* Q, size_t ldq, int ncols, const double* h, double* w, size_t n, KrylovWork& kw,
+                  double* norm_out, hipStream_t st) {
+  hipLaunchKernelGGL(update_kernel, dim3(kw.nblk), dim3(KT), 0, st, Q, ldq, ncols, h, w, n,
+                     norm_out ? kw.partial : (double*)nullptr, kw.nblk);
+  if (norm_out) hipLaunchKernelGGL(reduce_cols_kernel, dim3(1), dim3(256), 0, st, kw.partial, kw.nblk, norm_out, 0);
+  return hipGetLastError();
+}
+
+hipError_t combine(const double* Q, size_t ldq, int ncols, const double* c, double* y, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(combine_kernel, dim3((unsigned)((n + KT - 1) / KT)), dim3(KT), 0, st, Q, ldq, ncols, c, y, n);
+  return hipGetLastError();
+}
+
+hipError_t scale_vec(const double* x, double* y, size_t n, const double* a, bool invsqrt, hipStream_t st) {
+  hipLaunchKernelGGL(scale_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, y, n, a, invsqrt ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t axpby(double a, const double* x, double b, double* y, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(axpby_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, x, b, y, n);
+  return hipGetLastError();
+}
+
+hipError_t rhs_assemble(const double* Vrand, const double* F6, double* b, size_t n3, size_t n6, hipStream_t st) {
+  const size_t n = n3 + n6;
+  hipLaunchKernelGGL(rhs_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, Vrand, F6, b, n3, n6);
+  return hipGetLastError();
+}
+
+}  // namespace fcm
