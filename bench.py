@@ -1,0 +1,245 @@
+#!/usr/bin/env python3
+"""Benchmark: fluctuating FCM mobility solves/s at 256^3 on one MI355X.
+
+One step = one whole mobility solve of the 256^3 workload with 20,000 rods of
+12 spheres (240,000 blobs, a = 1.86, dx = 1, mu = 1, kT = 1): the seeded
+fluctuating-stress field and its interpolation, the right-hand side
+[-V_rand; -F], and the right-preconditioned GMRES saddle-point solve to a
+relative residual of 1e-8 -- timed for the FP32-bulk solve with FP64
+iterative refinement (the headline `value`) and for the plain FP64 solve.
+Inputs (geometry, forces, reference matrices) are resident in HBM before the
+timed region; every step draws a fresh stress field from its own seed.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak, GB/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def percentile(xs, p):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = (len(xs) - 1) * p / 100.0
+    lo, hi = int(np.floor(k)), int(np.ceil(k))
+    return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
+
+
+def pass_bytes(nx, ny, nz, nb, word):
+    """Algorithmic HBM bytes per pass of one saddle mat-vec: S real words per
+    field (S = nx*ny*2*(nz/2+1)); spread writes 3 fields, every FFT pass reads
+    and writes 3 fields, interpolation reads 3 fields; per blob the spread
+    reads position+force+permutation (52 B) and interpolation reads
+    position+permutation and writes the velocity (52 B)."""
+    S = nx * ny * 2 * (nz // 2 + 1)
+    return {
+        "spread": 3 * S * word + 52 * nb,
+        "z_forward": 6 * S * word,
+        "y_forward": 6 * S * word,
+        "x_project": 6 * S * word,
+        "y_inverse": 6 * S * word,
+        "z_inverse": 6 * S * word,
+        "interp": 3 * S * word + 52 * nb,
+    }
+
+
+def cpu_baseline(args, ref, x, q, F6, fp64_iters):
+    """Times the CPU oracle (OpenMP over x slabs) on a bounded sample of the
+    same workload: one fluctuating-stress field and one saddle-point mat-vec
+    at full size, extrapolated to a solve with the GPU FP64 iteration count."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    n = args.grid
+    oc = po.cfg(n, ng=7, dx=1.0, a=1.86, mu=1.0, kT=1.0, dt=args.dt)
+    o = po.Fcm(oc, x.shape[0] * ref.shape[0])
+    Y = po.fill_blob_pos(oc, ref, x, q)
+    o.set_up_gaussians(Y)
+    b = po.Bodies(ref, x, q, np.eye(3 * ref.shape[0]), np.eye(6))
+    t0 = time.perf_counter()
+    o.fill_stress(1)
+    o.random_field()
+    o.random_interp()
+    t_rand = time.perf_counter() - t0
+    v = np.random.default_rng(0).standard_normal(b.n)
+    t0 = time.perf_counter()
+    po.saddle_matvec(o, b, v)
+    t_mv = time.perf_counter() - t0
+    t_solve = t_rand + (fp64_iters + 1) * t_mv
+    o.close()
+    return {
+        "value": 1.0 / t_solve,
+        "unit": "solves/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle (C restatement, FP64, {threads} OpenMP threads) at {n}^3 with {Y.size // 3} blobs: "
+                   f"1 fluctuating field {t_rand:.2f} s + 1 saddle mat-vec {t_mv:.2f} s, extrapolated to "
+                   f"{fp64_iters} GMRES iterations + the final P^-1 product (MGS dot products not counted)"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--rods", type=int, default=20000)
+    ap.add_argument("--spheres", type=int, default=12)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--max-iter", type=int, default=300)
+    ap.add_argument("--dt", type=float, default=np.pi * 1.86 ** 3 / 100.0)
+    ap.add_argument("--place-attempts", type=int, default=20)
+    ap.add_argument("--profile-reps", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--skip-fp64", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: the whole 256^3 grid lives on one MI355X and no collective or "
+              "multi-GPU path exists in this build; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import timstochrigidbodiessim_amd as fa
+
+    n = args.grid
+    L = [float(n)] * 3
+    t0 = time.perf_counter()
+    ref, x, q, F6, nov = fa.synth_rods(2, args.rods, args.spheres, 1.86, L, max_attempts=args.place_attempts)
+    nb = args.rods * args.spheres
+    phi = nb * 4.0 / 3.0 * np.pi * 1.86 ** 3 / n ** 3
+    log(f"[bench] synthetic rods: {args.rods} x {args.spheres}, volume fraction {phi:.3f}, "
+        f"{nov} rods placed with overlap ({time.perf_counter() - t0:.1f} s)")
+    cfg = fa.make_config(n, ng=7, dx=1.0, a=1.86, mu=1.0, kT=1.0, dt=args.dt, max_blobs=nb, max_bodies=args.rods,
+                         max_blobs_per_body=args.spheres, max_krylov=args.max_iter, fp32=True, random_field=True)
+    s = fa.FcmSolver(cfg)
+    s.set_geometry(ref)
+    t0 = time.perf_counter()
+    s.ref_matrices()
+    log(f"[bench] reference matrices: {time.perf_counter() - t0:.2f} s")
+    s.set_bodies(x, q)
+    s.set_body_forces(F6)
+
+    results = {}
+    modes = [("fp32_refine", fa.FCM_PREC_FP32)]
+    if not args.skip_fp64:
+        modes.append(("fp64", fa.FCM_PREC_FP64))
+    for name, prec in modes:
+        for w in range(args.warmup):
+            rep = s.mobility_solve(1000 + w, tol=args.tol, max_iter=args.max_iter, precision=prec)
+            log(f"[bench] warmup {name}: {rep}")
+        s.L.fcm_sync(s.h)
+        lat, reps = [], []
+        t_start = time.perf_counter()
+        for k in range(args.steps):
+            t1 = time.perf_counter()
+            rep = s.mobility_solve(k + 1, tol=args.tol, max_iter=args.max_iter, precision=prec)
+            lat.append(time.perf_counter() - t1)
+            reps.append(rep)
+        s.L.fcm_sync(s.h)
+        total = time.perf_counter() - t_start
+        log(f"[bench] {name}: {args.steps} solves in {total:.3f} s; last {reps[-1]}")
+        results[name] = {
+            "solves_per_s": args.steps / total,
+            "ms_per_step": 1e3 * total / args.steps,
+            "p50_ms": 1e3 * percentile(lat, 50),
+            "p99_ms": 1e3 * percentile(lat, 99),
+            "iterations": [r["iterations"] for r in reps],
+            "refinements": [r["refinements"] for r in reps],
+            "status": sorted(set(r["status"] for r in reps)),
+            "true_rel_residual_max": max((r["true_rel_residual"] for r in reps), default=None),
+        }
+
+    # FP32+refine against the plain FP64 solve of the same stress seed
+    agree = None
+    if not args.skip_fp64:
+        s.mobility_solve(1, tol=args.tol, max_iter=args.max_iter, precision=fa.FCM_PREC_FP64)
+        sol64 = s.solution()
+        s.mobility_solve(1, tol=args.tol, max_iter=args.max_iter, precision=fa.FCM_PREC_FP32)
+        sol32 = s.solution()
+        nl = 3 * nb
+        agree = {
+            "body_velocity_rel_l2": float(np.linalg.norm(sol32[nl:] - sol64[nl:]) / np.linalg.norm(sol64[nl:])),
+            "blob_force_rel_l2": float(np.linalg.norm(sol32[:nl] - sol64[:nl]) / np.linalg.norm(sol64[:nl])),
+        }
+        log(f"[bench] fp32+refine vs fp64: {agree}")
+
+    # roofline of the dominant kernel (per-pass device time, HIP events on the solver's stream)
+    prof = {}
+    for name, prec, word in (("fp32", fa.FCM_PREC_FP32, 4), ("fp64", fa.FCM_PREC_FP64, 8)):
+        ms = s.profile_matvec(prec, args.profile_reps)
+        by = pass_bytes(n, n, n, nb, word)
+        prof[name] = {k: {"ms": ms[k], "GBps": (by[k] / (ms[k] * 1e-3) / 1e9 if k in by and ms[k] > 0 else None)}
+                      for k in ms}
+        log(f"[bench] passes {name}: " + ", ".join(f"{k} {v['ms']:.3f} ms" + (f" {v['GBps']:.0f} GB/s" if v['GBps'] else "")
+                                                 for k, v in prof[name].items()))
+    head = prof["fp32"]
+    dom = max((k for k in head if k != "KT"), key=lambda k: head[k]["ms"])
+    achieved = head[dom]["GBps"]
+    mv_bytes = sum(pass_bytes(n, n, n, nb, 4).values())
+    mv_ms = sum(v["ms"] for v in head.values())
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "matvec_fp32": {"ms": round(mv_ms, 4), "GBps": round(mv_bytes / (mv_ms * 1e-3) / 1e9, 1)}}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            it64 = int(np.median(results["fp64"]["iterations"])) if "fp64" in results else 50
+            t0 = time.perf_counter()
+            cpu = cpu_baseline(args, ref, x, q, F6, it64)
+            log(f"[bench] cpu baseline ({time.perf_counter() - t0:.1f} s): {cpu}")
+        except Exception as e:  # the baseline is reported, never required
+            log(f"[bench] cpu baseline failed: {e!r}")
+
+    head_r = results["fp32_refine"]
+    line = {
+        "metric": "FCM mobility solves/s at 256^3 (FP32+refine vs FP64) at 1e-8; p50/p99 latency",
+        "value": round(head_r["solves_per_s"], 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head_r["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 grid passes + f64 Krylov/refinement",
+        "data": "synthetic (seeded rods, forces and stress stream)",
+        "config": {"workload": f"{n}^3 grid, {args.rods} rods x {args.spheres} spheres ({nb} blobs), a=1.86, dx=1, "
+                               f"mu=1, kT=1, ng=7, GMRES tol {args.tol:g}, volume fraction {phi:.3f}, "
+                               f"{nov} rods overlapping",
+                   "parallelism": "single GPU"},
+        "p50_ms": round(head_r["p50_ms"], 3),
+        "p99_ms": round(head_r["p99_ms"], 3),
+        "fp32_refine": head_r,
+        "fp64": results.get("fp64"),
+        "speedup_fp32_refine_vs_fp64": (round(head_r["solves_per_s"] / results["fp64"]["solves_per_s"], 3)
+                                        if "fp64" in results else None),
+        "fp32_vs_fp64_solution": agree,
+        "passes_ms": {k: {kk: round(vv["ms"], 4) for kk, vv in v.items()} for k, v in prof.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    s.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -167,6 +167,13 @@ int gmres_get_solution(fcm_handle h, double* soln);
  * assemble_RHS + invert_saddle_point_system (main.cpp:231-233,308-310) */
 int fcm_mobility_solve(fcm_handle h, uint64_t seed, const gmres_opts* opts, gmres_report* rep);
 
+/* ---- measurement hook (no counterpart in the reference) ----------------------- */
+/* Runs one saddle-point mat-vec `reps` times on a fixed input with HIP events
+ * between its passes, on the handle's stream; ms_per_pass[0..7] receives the
+ * mean device time of: spread, z forward, y forward, x forward+projection+x
+ * inverse, y inverse, z inverse, interpolation(+K), K^T. */
+int fcm_profile_matvec(fcm_handle h, int precision, int reps, double* ms_per_pass);
+
 /* ---- synthetic inputs (host only; no GPU needed) ---------------------------- */
 /* Rods of nbpb spheres spaced 2a on the body x axis, centres uniform in the box
  * with rejection of inter-body blob pairs closer than 2a (minimum image), at

@@ -27,8 +27,10 @@ EXPORTS = [
     "rb_set_geometry", "rb_set_bodies", "rb_set_body_forces", "rb_get_blob_positions",
     "rb_mul_K", "rb_mul_KT",
     "gmres_assemble_rhs", "gmres_saddle_matvec", "gmres_apply_Pinv", "gmres_solve",
-    "gmres_get_solution", "fcm_mobility_solve", "fcm_synth_rods",
+    "gmres_get_solution", "fcm_mobility_solve", "fcm_profile_matvec", "fcm_synth_rods",
 ]
+
+PASS_NAMES = ["spread", "z_forward", "y_forward", "x_project", "y_inverse", "z_inverse", "interp", "KT"]
 
 
 class FcmConfig(C.Structure):
@@ -102,6 +104,7 @@ def lib():
         L.gmres_solve.argtypes = [H, _dp, _dp, C.POINTER(GmresOpts), C.POINTER(GmresReport), _dp]
         L.gmres_get_solution.argtypes = [H, _dp]
         L.fcm_mobility_solve.argtypes = [H, C.c_uint64, C.POINTER(GmresOpts), C.POINTER(GmresReport)]
+        L.fcm_profile_matvec.argtypes = [H, C.c_int, C.c_int, _dp]
         L.fcm_synth_rods.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_double, _dp, C.c_int, _dp, _dp,
                                      _dp, _dp, C.POINTER(C.c_int)]
         _lib = L
@@ -314,6 +317,11 @@ class FcmSolver:
         r = GmresReport()
         _chk(self.L.fcm_mobility_solve(self.h, seed, C.byref(o), C.byref(r)))
         return r.as_dict()
+
+    def profile_matvec(self, precision=FCM_PREC_FP64, reps=10):
+        ms = np.zeros(8)
+        _chk(self.L.fcm_profile_matvec(self.h, precision, reps, _ptr(ms)))
+        return dict(zip(PASS_NAMES, ms.tolist()))
 
     def solution(self):
         s = np.zeros(self.n)

@@ -1008,6 +1008,61 @@ int gmres_get_solution(fcm_handle h, double* soln) {
   return sync_stream(h);
 }
 
+int fcm_profile_matvec(fcm_handle h, int precision, int reps, double* ms_per_pass) {
+  if (int rc = check(h)) return rc;
+  if (!h->have_bodies || !ms_per_pass || reps < 1) return fail(FCM_ERR_STATE, "bodies must be set; reps >= 1");
+  if (precision == FCM_PREC_FP32 && !h->f32[0]) return fail(FCM_ERR_STATE, "FP32 path not allocated");
+  const size_t n = h->nsys(), n3 = h->n3();
+  std::vector<double> v(n);
+  uint64_t st = 12345;
+  for (size_t i = 0; i < n; ++i) {  // a fixed pseudo-random input vector
+    st = st * 6364136223846793005ULL + 1442695040888963407ULL;
+    v[i] = (double)(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+  }
+  HIPCHK(hipMemcpyAsync(h->xacc, v.data(), n * sizeof(double), hipMemcpyHostToDevice, h->st));
+  hipEvent_t ev[9];
+  for (int i = 0; i < 9; ++i) HIPCHK(hipEventCreate(&ev[i]));
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  KTerm kt;
+  kt.zV = h->xacc + n3;
+  kt.dpos = h->d_dpos;
+  kt.nbpb = h->nbpb;
+  BodyData b = h->body();
+  int rc = FCM_OK;
+  for (int r = 0; r < reps && rc == FCM_OK; ++r) {
+    HIPCHK(hipEventRecord(ev[0], h->st));
+    if (precision == FCM_PREC_FP32) {
+      HIPCHK(spread<float>(h->g, h->bb, h->bd, h->xacc, h->fp32(), h->st));
+      HIPCHK(hipEventRecord(ev[1], h->st));
+      for (int p = 0; p < 5; ++p) {
+        HIPCHK(stokes_pass<float>(h->g, h->fp32(), p, h->pp, h->tw32, h->st));
+        HIPCHK(hipEventRecord(ev[2 + p], h->st));
+      }
+      HIPCHK(interp<float>(h->g, h->bb, h->bd, h->fp32(), h->tmp, kt, h->st));
+    } else {
+      HIPCHK(spread<double>(h->g, h->bb, h->bd, h->xacc, h->fp64(), h->st));
+      HIPCHK(hipEventRecord(ev[1], h->st));
+      for (int p = 0; p < 5; ++p) {
+        HIPCHK(stokes_pass<double>(h->g, h->fp64(), p, h->pp, h->tw64, h->st));
+        HIPCHK(hipEventRecord(ev[2 + p], h->st));
+      }
+      HIPCHK(interp<double>(h->g, h->bb, h->bd, h->fp64(), h->tmp, kt, h->st));
+    }
+    HIPCHK(hipEventRecord(ev[7], h->st));
+    HIPCHK(mul_KT(b, h->xacc, h->tmp + n3, h->st));
+    HIPCHK(hipEventRecord(ev[8], h->st));
+    HIPCHK(hipEventSynchronize(ev[8]));
+    for (int i = 0; i < 8; ++i) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+      acc[i] += ms;
+    }
+  }
+  for (int i = 0; i < 9; ++i) (void)hipEventDestroy(ev[i]);
+  for (int i = 0; i < 8; ++i) ms_per_pass[i] = acc[i] / reps;
+  return rc;
+}
+
 int fcm_mobility_solve(fcm_handle h, uint64_t seed, const gmres_opts* opts, gmres_report* rep) {
   if (int rc = check(h)) return rc;
   if (!h->have_bodies || !h->have_ref) return fail(FCM_ERR_STATE, "bodies and reference matrices must be set");

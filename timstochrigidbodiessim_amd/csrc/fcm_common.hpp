@@ -59,12 +59,13 @@ constexpr bool primes_divide(int N, int P) {
   return true;
 }
 
-constexpr FftPlan make_plan(int N) {
+// PM caps the points held per thread (register budget of the caller)
+constexpr FftPlan make_plan(int N, int PM = 16) {
   FftPlan pl;
   pl.N = N;
   if (N < 2 || !only_235(N)) return pl;
   int P = 0;
-  for (int d = 16; d >= 2; --d)
+  for (int d = PM; d >= 2; --d)
     if (N % d == 0 && primes_divide(N, d)) { P = d; break; }
   if (P == 0) return pl;
   pl.P = P;
@@ -223,9 +224,9 @@ template <typename T, int S> struct Dft<T, 16, S> {
 // of the block must call this (it contains barriers).
 // tw[m] = exp(-2 pi i m / N), m < N.
 // ---------------------------------------------------------------------------
-template <typename T, int N, int S, bool REV, int STG>
+template <typename T, int N, int S, bool REV, int STG, int PM>
 struct FftStage {
-  static constexpr FftPlan PL = REV ? reverse_plan(make_plan(N)) : make_plan(N);
+  static constexpr FftPlan PL = REV ? reverse_plan(make_plan(N, PM)) : make_plan(N, PM);
   static constexpr int P = PL.P;
   static constexpr int TT = PL.T;
   static constexpr int R = PL.r[STG];
@@ -268,30 +269,30 @@ struct FftStage {
         for (int r = 0; r < R2; ++r) v[i * R2 + r] = lds[(j + r * (N / R2)) * ls + col];
       }
       __syncthreads();
-      FftStage<T, N, S, REV, STG + 1>::run(v, lds, ls, col, tcol, tw);
+      FftStage<T, N, S, REV, STG + 1, PM>::run(v, lds, ls, col, tcol, tw);
     }
   }
 };
 
-template <typename T, int N, int S, bool REV>
+template <typename T, int N, int S, bool REV, int PM = 16>
 __device__ __forceinline__ void fft_column(c2<T>* v, c2<T>* lds, int ls, int col, int tcol,
                                            const c2<T>* __restrict__ tw) {
-  FftStage<T, N, S, REV, 0>::run(v, lds, ls, col, tcol, tw);
+  FftStage<T, N, S, REV, 0, PM>::run(v, lds, ls, col, tcol, tw);
 }
 
 // element index held in register slot (i, r) for the first radix of a plan
-template <int N, bool REV>
+template <int N, bool REV, int PM = 16>
 __device__ __forceinline__ int fft_in_index(int tcol, int slot) {
-  constexpr FftPlan PL = REV ? reverse_plan(make_plan(N)) : make_plan(N);
+  constexpr FftPlan PL = REV ? reverse_plan(make_plan(N, PM)) : make_plan(N, PM);
   constexpr int R0 = PL.r[0];
   const int i = slot / R0, r = slot % R0;
   return tcol + i * PL.T + r * (N / R0);
 }
 
 // element index held in register slot for the last radix of a plan
-template <int N, bool REV>
+template <int N, bool REV, int PM = 16>
 __device__ __forceinline__ int fft_out_index(int tcol, int slot) {
-  constexpr FftPlan PL = REV ? reverse_plan(make_plan(N)) : make_plan(N);
+  constexpr FftPlan PL = REV ? reverse_plan(make_plan(N, PM)) : make_plan(N, PM);
   constexpr int RL = PL.r[PL.ns - 1];
   const int i = slot / RL, r = slot % RL;
   return tcol + i * PL.T + r * (N / RL);

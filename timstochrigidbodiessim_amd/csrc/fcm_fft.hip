@@ -20,8 +20,8 @@ namespace fcm {
 // ---------------------------------------------------------------------------
 // block shapes
 // ---------------------------------------------------------------------------
-template <typename T, int N> struct ColShape {
-  static constexpr FftPlan PL = make_plan(N);
+template <typename T, int N, int PM = 16> struct ColShape {
+  static constexpr FftPlan PL = make_plan(N, PM);
   static constexpr int TC = PL.T;
   // columns per block: at most 16, at most 256 threads, at most 64 KiB LDS
   static constexpr int C0 = 256 / TC < 16 ? 256 / TC : 16;
@@ -224,12 +224,17 @@ __device__ __forceinline__ void project_mode(c2<T>& a, c2<T>& b, c2<T>& c, T q1,
 __device__ __forceinline__ int signed_k(int k, int n) { return k <= n / 2 ? k : k - n; }
 
 // forward x FFT of 3 fields -> projection -> inverse x FFT, in place.
+// points per thread of the fused x passes: three (or more) fields stay in
+// registers between the forward and inverse transforms
+template <typename T> struct XPM { static constexpr int value = sizeof(T) == 8 ? 8 : 16; };
+
 template <typename T, int N>
 __global__ void __launch_bounds__(256) xproj_kernel(FieldPtrs<T> fp, int ny, int cpitch, int nzc,
                                                     ProjParams pp, const c2<T>* __restrict__ tw) {
   using V = c2<T>;
-  using SH = ColShape<T, N>;
-  constexpr FftPlan PL = make_plan(N);
+  constexpr int PM = XPM<T>::value;
+  using SH = ColShape<T, N, PM>;
+  constexpr FftPlan PL = make_plan(N, PM);
   constexpr int P = PL.P, TC = SH::TC, C = SH::C;
   __shared__ V lds[N * C];
   const int t = threadIdx.x;
@@ -245,27 +250,27 @@ __global__ void __launch_bounds__(256) xproj_kernel(FieldPtrs<T> fp, int ny, int
     const V* d = fp.p[f] + off;
 #pragma unroll
     for (int s = 0; s < P; ++s) {
-      const int e = fft_in_index<N, false>(tcol, s);
+      const int e = fft_in_index<N, false, PM>(tcol, s);
       v[f][s] = valid ? d[(size_t)e * estride] : mk<T>(T(0), T(0));
     }
-    fft_column<T, N, -1, false>(v[f], lds, C, col, tcol, tw);
+    fft_column<T, N, -1, false, PM>(v[f], lds, C, col, tcol, tw);
   }
   const T q2 = T(signed_k(ky, pp.ny) * pp.facy);
   const T q3 = T(kz * pp.facz);
   const T inv_muN = T(pp.inv_muN);
 #pragma unroll
   for (int s = 0; s < P; ++s) {
-    const int kx = fft_out_index<N, false>(tcol, s);
+    const int kx = fft_out_index<N, false, PM>(tcol, s);
     const T q1 = T(signed_k(kx, pp.nx) * pp.facx);
     project_mode<T>(v[0][s], v[1][s], v[2][s], q1, q2, q3, inv_muN, (kx | ky | kz) == 0);
   }
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
-    fft_column<T, N, +1, true>(v[f], lds, C, col, tcol, tw);
+    fft_column<T, N, +1, true, PM>(v[f], lds, C, col, tcol, tw);
     if (valid) {
       V* d = fp.p[f] + off;
 #pragma unroll
-      for (int s = 0; s < P; ++s) d[(size_t)fft_out_index<N, true>(tcol, s) * estride] = v[f][s];
+      for (int s = 0; s < P; ++s) d[(size_t)fft_out_index<N, true, PM>(tcol, s) * estride] = v[f][s];
     }
   }
 }
@@ -278,8 +283,9 @@ template <typename T, int N>
 __global__ void __launch_bounds__(256) xstress_kernel(FieldPtrs<T> fp, int ny, int cpitch, int nzc,
                                                       ProjParams pp, const c2<T>* __restrict__ tw) {
   using V = c2<T>;
-  using SH = ColShape<T, N>;
-  constexpr FftPlan PL = make_plan(N);
+  constexpr int PM = XPM<T>::value;
+  using SH = ColShape<T, N, PM>;
+  constexpr FftPlan PL = make_plan(N, PM);
   constexpr int P = PL.P, TC = SH::TC, C = SH::C;
   __shared__ V lds[N * C];
   const int t = threadIdx.x;
@@ -299,13 +305,13 @@ __global__ void __launch_bounds__(256) xstress_kernel(FieldPtrs<T> fp, int ny, i
     const V* d = fp.p[f] + off;
 #pragma unroll
     for (int s = 0; s < P; ++s) {
-      const int e = fft_in_index<N, false>(tcol, s);
+      const int e = fft_in_index<N, false, PM>(tcol, s);
       v[s] = valid ? d[(size_t)e * estride] : mk<T>(T(0), T(0));
     }
-    fft_column<T, N, -1, false>(v, lds, C, col, tcol, tw);
+    fft_column<T, N, -1, false, PM>(v, lds, C, col, tcol, tw);
 #pragma unroll
     for (int s = 0; s < P; ++s) {
-      const int kx = fft_out_index<N, false>(tcol, s);
+      const int kx = fft_out_index<N, false, PM>(tcol, s);
       const T q1n = (kx == pp.nx / 2) ? T(0) : T(signed_k(kx, pp.nx) * pp.facx);
       switch (f) {
         case 0: acc[0][s] = cadd(acc[0][s], cscale(v[s], q1n)); break;
@@ -322,7 +328,7 @@ __global__ void __launch_bounds__(256) xstress_kernel(FieldPtrs<T> fp, int ny, i
   const T inv_muN = T(pp.inv_muN);
 #pragma unroll
   for (int s = 0; s < P; ++s) {
-    const int kx = fft_out_index<N, false>(tcol, s);
+    const int kx = fft_out_index<N, false, PM>(tcol, s);
     const T q1 = T(signed_k(kx, pp.nx) * pp.facx);
 #pragma unroll
     for (int c = 0; c < 3; ++c) acc[c][s] = mk<T>(-acc[c][s].y, acc[c][s].x);  // times i
@@ -330,11 +336,11 @@ __global__ void __launch_bounds__(256) xstress_kernel(FieldPtrs<T> fp, int ny, i
   }
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
-    fft_column<T, N, +1, true>(acc[f], lds, C, col, tcol, tw);
+    fft_column<T, N, +1, true, PM>(acc[f], lds, C, col, tcol, tw);
     if (valid) {
       V* d = fp.p[f] + off;
 #pragma unroll
-      for (int s = 0; s < P; ++s) d[(size_t)fft_out_index<N, true>(tcol, s) * estride] = acc[f][s];
+      for (int s = 0; s < P; ++s) d[(size_t)fft_out_index<N, true, PM>(tcol, s) * estride] = acc[f][s];
     }
   }
 }
@@ -406,7 +412,7 @@ static hipError_t launch_xproj(const GridGeom& g, const FieldPtrs<T>& fp, bool s
   switch (g.nx) {
 #define FCM_CASE(v)                                                                                             \
   case v: {                                                                                                     \
-    using SH = ColShape<T, v>;                                                                                  \
+    using SH = ColShape<T, v, XPM<T>::value>;                                                                   \
     dim3 grid((g.nzc + SH::C - 1) / SH::C, g.ny);                                                               \
     if (stress)                                                                                                 \
       hipLaunchKernelGGL((xstress_kernel<T, v>), grid, dim3(SH::THREADS), 0, st, fp, g.ny, g.cpitch, g.nzc, pp, tws.x); \
@@ -450,6 +456,19 @@ hipError_t stokes_solve(const GridGeom& g, const FieldPtrs<T>& fp, const ProjPar
 }
 
 template <typename T>
+hipError_t stokes_pass(const GridGeom& g, const FieldPtrs<T>& fp, int pass, const ProjParams& pp,
+                       const TwiddleSet<T>& tws, hipStream_t st) {
+  switch (pass) {
+    case 0: return launch_z<T>(g, fp, 3, true, tws, st, nullptr);
+    case 1: return launch_cols<T>(g, fp, 3, 1, true, tws, st);
+    case 2: return launch_xproj<T>(g, fp, false, pp, tws, st);
+    case 3: return launch_cols<T>(g, fp, 3, 1, false, tws, st);
+    case 4: return launch_z<T>(g, fp, 3, false, tws, st, nullptr);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename T>
 hipError_t random_velocity(const GridGeom& g, const FieldPtrs<T>& stress6, bool generate, unsigned long long seed,
                            const double scale[6], const ProjParams& pp, const TwiddleSet<T>& tws, hipStream_t st) {
   StressGen sg{};
@@ -470,6 +489,8 @@ template hipError_t fft3d_inverse<double>(const GridGeom&, const FieldPtrs<doubl
 template hipError_t fft3d_inverse<float>(const GridGeom&, const FieldPtrs<float>&, int, const TwiddleSet<float>&, hipStream_t);
 template hipError_t stokes_solve<double>(const GridGeom&, const FieldPtrs<double>&, const ProjParams&, const TwiddleSet<double>&, hipStream_t);
 template hipError_t stokes_solve<float>(const GridGeom&, const FieldPtrs<float>&, const ProjParams&, const TwiddleSet<float>&, hipStream_t);
+template hipError_t stokes_pass<double>(const GridGeom&, const FieldPtrs<double>&, int, const ProjParams&, const TwiddleSet<double>&, hipStream_t);
+template hipError_t stokes_pass<float>(const GridGeom&, const FieldPtrs<float>&, int, const ProjParams&, const TwiddleSet<float>&, hipStream_t);
 template hipError_t random_velocity<double>(const GridGeom&, const FieldPtrs<double>&, bool, unsigned long long, const double*, const ProjParams&, const TwiddleSet<double>&, hipStream_t);
 template hipError_t random_velocity<float>(const GridGeom&, const FieldPtrs<float>&, bool, unsigned long long, const double*, const ProjParams&, const TwiddleSet<float>&, hipStream_t);
 

@@ -38,6 +38,11 @@ hipError_t fft3d_inverse(const GridGeom& g, const FieldPtrs<T>& fp, int nf, cons
 template <typename T>
 hipError_t stokes_solve(const GridGeom& g, const FieldPtrs<T>& fp, const ProjParams& pp, const TwiddleSet<T>& tws,
                         hipStream_t st);
+// one pass of stokes_solve: 0 z forward, 1 y forward, 2 x forward+projection+
+// x inverse, 3 y inverse, 4 z inverse (all three fields)
+template <typename T>
+hipError_t stokes_pass(const GridGeom& g, const FieldPtrs<T>& fp, int pass, const ProjParams& pp,
+                       const TwiddleSet<T>& tws, hipStream_t st);
 template <typename T>
 hipError_t random_velocity(const GridGeom& g, const FieldPtrs<T>& stress6, bool generate, unsigned long long seed,
                            const double scale[6], const ProjParams& pp, const TwiddleSet<T>& tws, hipStream_t st);
