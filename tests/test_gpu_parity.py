@@ -79,7 +79,8 @@ def _blobs(rng, nb, dims, dx):
 
 
 @pytest.mark.parametrize("dims,ng,nb", [((16, 16, 16), 7, 40), ((32, 24, 16), 7, 200), ((64, 64, 64), 7, 1000),
-                                        ((12, 12, 12), 5, 30), ((32, 32, 32), 12, 100)])
+                                        ((12, 12, 12), 5, 30), ((32, 32, 32), 12, 100), ((16, 8, 8), 7, 30),
+                                        ((32, 16, 16), 8, 80), ((24, 24, 48), 5, 100), ((48, 32, 96), 7, 300)])
 def test_spread_and_interp_match_oracle(dims, ng, nb):
     say("spread/interp", dims, ng, nb)
     dx = 0.9

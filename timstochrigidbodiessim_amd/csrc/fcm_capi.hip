@@ -86,7 +86,8 @@ struct fcm_handle_s {
   double* tmp = nullptr;
   double* Vrand = nullptr;
   double* Vtmp = nullptr;
-  double* red = nullptr;      // reductions: [h1 | h2 | norm], 2*(max_krylov+1)+8
+  double* red = nullptr;      // reductions, 2*(max_krylov+2)+8
+  double* coef = nullptr;     // coefficients of the reorthogonalisation pass
   KrylovWork kw;
   bool have_rhs = false, have_sol = false;
   std::vector<void*> allocs;
@@ -182,12 +183,16 @@ Sizes sizes_of(const fcm_config* c) {
   t += 3 * s.field64 + s.twbytes64;
   if (c->fp32) t += 3 * s.field32 + s.twbytes32;
   if (c->random_field) t += 6 * s.field64;
-  t += (size_t)nb * (3 * 8 + 4 + 4 + 16 + 3 * c->ng * 12) + (3 * nbins + 2) * 4;
+  t += (size_t)nb * (3 * 8 + 4 + 4 + 16 + 3 * c->ng * 12 + 8 * 4) + (3 * nbins + 2) * 4;
+  {
+    GridGeom g2 = make_geom(c);
+    t += 3 * (size_t)(spread_tiles(g2) + 1) * 4;
+  }
   const int n3b = 3 * c->max_blobs_per_body;
   t += (size_t)(3 * n3b + n3b * n3b + 36) * 8 + (size_t)c->max_bodies * (3 + 4 + 9 + 6) * 8 + (size_t)nb * 3 * 8;
   t += (size_t)(c->max_krylov + 1) * s.ldq * 8 + 6 * s.ldq * 8 + 2 * (size_t)nb * 3 * 8;
   const size_t nblk = (s.nmax + 1023) / 1024;
-  t += nblk * (size_t)(c->max_krylov + 2) * 8 + (2 * (size_t)c->max_krylov + 16) * 8;
+  t += nblk * (2 * (size_t)c->max_krylov + 6) * 8 + 2 * (2 * (size_t)c->max_krylov + 16) * 8;
   s.total = t;
   return s;
 }
@@ -314,10 +319,76 @@ struct InnerResult {
   double rel = 0;
 };
 
+// Givens least-squares state of gmres_solver.cpp:212-276 (columns are
+// pushed one at a time, unrotated; the rotated column and the rotations are
+// kept).  estimate() evaluates the residual a column would give without
+// committing it.
+struct GivensLS {
+  int M;
+  std::vector<double> R, CS, SN, beta;
+  double nrm;
+  GivensLS(int M_, double nrm_) : M(M_), R((size_t)(M_ + 2) * (M_ + 2), 0.0), CS(M_ + 2, 0.0), SN(M_ + 2, 0.0),
+                                  beta(M_ + 2, 0.0), nrm(nrm_) { beta[0] = nrm_; }
+  // col holds H[0..k+1][k]; returns |beta_{k+1}| / ||b||
+  double apply(int k, const double* col, bool commit) {
+    std::vector<double> h(col, col + k + 2);
+    for (int i = 1; i <= k; ++i) {
+      const double temp = CS[i - 1] * h[i - 1] + SN[i - 1] * h[i];
+      h[i] = CS[i - 1] * h[i] - SN[i - 1] * h[i - 1];
+      h[i - 1] = temp;
+    }
+    double cs, sn;
+    if (h[k + 1] == 0.0) {
+      cs = 1.0;
+      sn = 0.0;
+    } else if (std::fabs(h[k + 1]) > std::fabs(h[k])) {
+      const double temp = h[k] / h[k + 1];
+      sn = (2.0 * double(h[k + 1] > 0) - 1.0) / std::sqrt(1.0 + temp * temp);
+      cs = temp * sn;
+    } else {
+      const double temp = h[k + 1] / h[k];
+      cs = (2.0 * double(h[k] > 0) - 1.0) / std::sqrt(1.0 + temp * temp);
+      sn = temp * cs;
+    }
+    const double b1 = -sn * beta[k];
+    if (commit) {
+      h[k] = cs * h[k] + sn * h[k + 1];
+      for (int i = 0; i <= k; ++i) R[(size_t)i * (M + 2) + k] = h[i];
+      CS[k] = cs;
+      SN[k] = sn;
+      beta[k + 1] = b1;
+      beta[k] = cs * beta[k];
+    }
+    return std::fabs(b1) / nrm;
+  }
+  // back substitution over the first ncols columns (gmres_solver.cpp:260-276)
+  std::vector<double> solve(int ncols) const {
+    std::vector<double> y(ncols, 0.0);
+    auto RR = [&](int i, int j) { return R[(size_t)i * (M + 2) + j]; };
+    y[ncols - 1] = beta[ncols - 1] / RR(ncols - 1, ncols - 1);
+    for (int i = ncols - 2; i >= 0; --i) {
+      y[i] = beta[i];
+      for (int j = ncols - 1; j > i; --j) y[i] -= RR(i, j) * y[j];
+      y[i] /= RR(i, i);
+    }
+    return y;
+  }
+};
+
 // Unrestarted right-preconditioned GMRES, x0 = 0 (gmres_solver.cpp:148-306),
-// on device vectors b -> x.  The Arnoldi step orthogonalises with classical
-// Gram-Schmidt applied twice (two fused passes over the basis per sweep), the
-// Hessenberg/Givens algebra is the reference's, on the host, in FP64.
+// on device vectors b -> x, with the Hessenberg/Givens algebra of the
+// reference on the host in FP64.  The Arnoldi basis is built by classical
+// Gram-Schmidt with delayed reorthogonalisation (DCGS2, Swirydowicz et al.,
+// Numer. Linear Algebra Appl. 2021): each iteration makes one fused pass
+// that projects the new vector and the reorthogonalisation of the previous,
+// still once-orthogonalised vector against the final basis, and one fused
+// pass that finalises the previous vector and updates the new one -- two
+// reads of the basis per iteration instead of the four of CGS2.  The mat-vec
+// is applied to the provisional vector; the Hessenberg column is corrected
+// with the reorthogonalisation coefficients (A P^-1 is linear).  A column is
+// committed to the Givens least squares once its vector is final; when the
+// provisional column already predicts convergence the previous vector is
+// finalised without another mat-vec.
 int gmres_core(fcm_handle h, const double* b, double* x, double tol, int max_iter_in, int precision,
                InnerResult& out, double* hist) {
   const size_t n = h->nsys();
@@ -332,77 +403,134 @@ int gmres_core(fcm_handle h, const double* b, double* x, double tol, int max_ite
     out.status = FCM_GMRES_ZERO_RHS;
     return FCM_OK;
   }
-  const int M = max_iter + 1;
-  std::vector<double> H((size_t)M * M, 0.0), beta(M, 0.0), CS(M, 0.0), SN(M, 0.0), hv(2 * (size_t)M + 1);
-  auto HH = [&](int i, int j) -> double& { return H[(size_t)i * M + j]; };
+  if (max_iter < 1) return fail(FCM_ERR_ARG, "max_iter must be >= 1");
+  const int M = max_iter;
+  GivensLS G(M, nrm);
+  std::vector<double> Hraw((size_t)(M + 2) * (M + 2), 0.0);
+  auto HR = [&](int i, int j) -> double& { return Hraw[(size_t)i * (M + 2) + j]; };
+  std::vector<double> hv(2 * (size_t)M + 8), coef(2 * (size_t)M + 8), hprov, col(M + 2);
+  double eta = 0.0;
   HIPCHK(axpby(1.0 / nrm, b, 0.0, h->Q, n, h->st));
-  beta[0] = nrm;
-  double* red_h1 = h->red;
-  double* red_h2 = h->red + M;
-  double* red_nrm = h->red + 2 * M;
-  bool breakdown = false;
-  for (int iter = 1; iter <= max_iter; ++iter) {
-    double* w = h->Q + (size_t)iter * ldq;
-    rc = arnoldi_product(h, h->Q + (size_t)(iter - 1) * ldq, w, precision);
-    if (rc) return rc;
-    HIPCHK(dots(h->Q, ldq, iter, w, n, h->kw, red_h1, h->st));
-    HIPCHK(update(h->Q, ldq, iter, red_h1, w, n, h->kw, nullptr, h->st));
-    HIPCHK(dots(h->Q, ldq, iter, w, n, h->kw, red_h2, h->st));
-    HIPCHK(update(h->Q, ldq, iter, red_h2, w, n, h->kw, red_nrm, h->st));
-    HIPCHK(hipMemcpyAsync(hv.data(), h->red, sizeof(double) * (2 * (size_t)M + 1), hipMemcpyDeviceToHost, h->st));
+
+  auto fetch = [&](size_t count) -> int {
+    HIPCHK(hipMemcpyAsync(hv.data(), h->red, sizeof(double) * count, hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
-    for (int i = 0; i < iter; ++i) HH(i, iter - 1) = hv[i] + hv[M + i];
-    const double q_norm = std::sqrt(hv[2 * M]);
-    HH(iter, iter - 1) = q_norm;
-    if (q_norm > 0.0) {
-      HIPCHK(axpby(1.0 / q_norm, w, 0.0, w, n, h->st));
-    } else {
-      breakdown = true;
-    }
-    // Givens rotations, gmres_solver.cpp:214-252
-    for (int i = 1; i < iter; ++i) {
-      const double temp = CS[i - 1] * HH(i - 1, iter - 1) + SN[i - 1] * HH(i, iter - 1);
-      HH(i, iter - 1) = CS[i - 1] * HH(i, iter - 1) - SN[i - 1] * HH(i - 1, iter - 1);
-      HH(i - 1, iter - 1) = temp;
-    }
-    if (HH(iter, iter - 1) == 0.0) {
-      CS[iter - 1] = 1.0;
-      SN[iter - 1] = 0.0;
-    } else if (std::fabs(HH(iter, iter - 1)) > std::fabs(HH(iter - 1, iter - 1))) {
-      const double temp = HH(iter - 1, iter - 1) / HH(iter, iter - 1);
-      SN[iter - 1] = (2.0 * double(HH(iter, iter - 1) > 0) - 1.0) / std::sqrt(1.0 + temp * temp);
-      CS[iter - 1] = temp * SN[iter - 1];
-    } else {
-      const double temp = HH(iter, iter - 1) / HH(iter - 1, iter - 1);
-      CS[iter - 1] = (2.0 * double(HH(iter - 1, iter - 1) > 0) - 1.0) / std::sqrt(1.0 + temp * temp);
-      SN[iter - 1] = temp * CS[iter - 1];
-    }
-    HH(iter - 1, iter - 1) = CS[iter - 1] * HH(iter - 1, iter - 1) + SN[iter - 1] * HH(iter, iter - 1);
-    HH(iter, iter - 1) = 0.0;
-    beta[iter] = -SN[iter - 1] * beta[iter - 1];
-    beta[iter - 1] = CS[iter - 1] * beta[iter - 1];
-    const double rel = std::fabs(beta[iter]) / nrm;
-    if (hist) hist[iter - 1] = rel;
-    if (rel < tol || iter == max_iter || breakdown) {
-      std::vector<double> y(iter, 0.0);
-      y[iter - 1] = beta[iter - 1] / HH(iter - 1, iter - 1);
-      for (int i = iter - 2; i >= 0; --i) {
-        y[i] = beta[i];
-        for (int j = iter - 1; j > i; --j) y[i] -= HH(i, j) * y[j];
-        y[i] /= HH(i, i);
+    return FCM_OK;
+  };
+  auto put_coef = [&](size_t count) -> int {
+    HIPCHK(hipMemcpyAsync(h->coef, coef.data(), sizeof(double) * count, hipMemcpyHostToDevice, h->st));
+    return FCM_OK;
+  };
+  auto finish = [&](int ncols, int status, double rel) -> int {
+    std::vector<double> y = G.solve(ncols);
+    HIPCHK(hipMemcpyAsync(h->red, y.data(), sizeof(double) * ncols, hipMemcpyHostToDevice, h->st));
+    HIPCHK(combine(h->Q, ldq, ncols, h->red, h->tmp, n, h->st));
+    BodyData bdy = h->body();
+    HIPCHK(apply_pinv(bdy, h->tmp, x, nullptr, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    out.iters = ncols;
+    out.rel = rel;
+    out.status = status;
+    return FCM_OK;
+  };
+  auto commit = [&](int kc) -> double {
+    for (int i = 0; i <= kc + 1; ++i) col[i] = HR(i, kc);
+    const double rel = G.apply(kc, col.data(), true);
+    if (hist) hist[kc] = rel;
+    return rel;
+  };
+
+  bool prov = false;
+  for (int k = 0;; ++k) {
+    double* qk = h->Q + (size_t)k * ldq;
+    double* w = h->Q + (size_t)(k + 1) * ldq;
+    rc = arnoldi_product(h, qk, w, precision);
+    if (rc) return rc;
+    std::vector<double> hnew(k + 1);
+    double nu = 1.0;
+    if (prov) {
+      // columns 0..k-1 final, column k provisional
+      HIPCHK(dots2(h->Q, ldq, k + 1, qk, w, n, h->kw, h->red, h->st));
+      if ((rc = fetch(2 * (size_t)(k + 1)))) return rc;
+      const double* sv = hv.data();
+      const double alpha = hv[k];
+      const double* tv = hv.data() + (k + 1);
+      const double tau = hv[2 * k + 1];
+      double ss = 0.0, st = 0.0;
+      for (int i = 0; i < k; ++i) {
+        ss += sv[i] * sv[i];
+        st += sv[i] * tv[i];
       }
-      HIPCHK(hipMemcpyAsync(h->red, y.data(), sizeof(double) * iter, hipMemcpyHostToDevice, h->st));
-      HIPCHK(combine(h->Q, ldq, iter, h->red, h->tmp, n, h->st));
-      BodyData bdy = h->body();
-      HIPCHK(apply_pinv(bdy, h->tmp, x, nullptr, h->st));
-      HIPCHK(hipStreamSynchronize(h->st));
-      out.iters = iter;
-      out.rel = rel;
-      out.status = breakdown ? FCM_GMRES_BREAKDOWN : (rel < tol ? FCM_GMRES_CONVERGED : FCM_GMRES_MAX_ITER);
-      return FCM_OK;
+      nu = std::sqrt(std::max(alpha - ss, 0.0));
+      for (int i = 0; i < k; ++i) HR(i, k - 1) = hprov[i] + eta * sv[i];
+      HR(k, k - 1) = eta * nu;
+      const double rel = commit(k - 1);
+      if (rel < tol || k == max_iter) return finish(k, rel < tol ? FCM_GMRES_CONVERGED : FCM_GMRES_MAX_ITER, rel);
+      if (!(nu > 0.0)) return finish(k, FCM_GMRES_BREAKDOWN, rel);
+      // projections of A P^-1 q_k on the final basis q_0..q_k
+      std::vector<double> proj(k + 1);
+      for (int i = 0; i < k; ++i) proj[i] = tv[i];
+      proj[k] = (tau - st) / nu;
+      for (int l = 0; l <= k; ++l) {
+        double c = 0.0;
+        for (int i = 0; i < k; ++i) c += HR(l, i) * sv[i];
+        hnew[l] = (proj[l] - c) / nu;
+      }
+      for (int i = 0; i < k; ++i) coef[i] = sv[i];
+      for (int i = 0; i < k; ++i) coef[k + i] = tv[i];
+      coef[2 * k] = 1.0 / nu;
+      coef[2 * k + 1] = proj[k];
+      if ((rc = put_coef(2 * (size_t)k + 2))) return rc;
+      HIPCHK(update2(h->Q, ldq, k, k, h->coef, qk, w, n, h->kw, h->red, h->st));
+    } else {
+      // columns 0..k final
+      HIPCHK(dots2(h->Q, ldq, k + 1, nullptr, w, n, h->kw, h->red, h->st));
+      if ((rc = fetch(2 * (size_t)(k + 1)))) return rc;
+      for (int i = 0; i <= k; ++i) hnew[i] = hv[k + 1 + i];
+      for (int i = 0; i <= k; ++i) coef[i] = hnew[i];
+      coef[k + 1] = 1.0;
+      coef[k + 2] = 0.0;
+      if ((rc = put_coef((size_t)k + 3))) return rc;
+      HIPCHK(update2(h->Q, ldq, 0, k + 1, h->coef, nullptr, w, n, h->kw, h->red, h->st));
+    }
+    if ((rc = fetch(1))) return rc;
+    const double etar = std::sqrt(hv[0]);
+    eta = etar / nu;
+    hprov = hnew;
+    if (!(etar > 0.0)) {
+      // exact breakdown: the Krylov space is invariant, column k is exact
+      for (int i = 0; i <= k; ++i) HR(i, k) = hprov[i];
+      HR(k + 1, k) = 0.0;
+      const double rel = commit(k);
+      return finish(k + 1, FCM_GMRES_BREAKDOWN, rel);
+    }
+    HIPCHK(axpby(1.0 / etar, w, 0.0, w, n, h->st));
+    prov = true;
+    // predict convergence from the provisional column
+    for (int i = 0; i <= k; ++i) col[i] = hprov[i];
+    col[k + 1] = eta;
+    const double est = G.apply(k, col.data(), false);
+    if (est < tol || k + 1 == max_iter) {
+      // finalise q_{k+1} without a further mat-vec
+      HIPCHK(dots2(h->Q, ldq, k + 2, w, nullptr, n, h->kw, h->red, h->st));
+      if ((rc = fetch((size_t)k + 2))) return rc;
+      double ss = 0.0;
+      for (int i = 0; i <= k; ++i) ss += hv[i] * hv[i];
+      const double nu2 = std::sqrt(std::max(hv[k + 1] - ss, 0.0));
+      for (int i = 0; i <= k; ++i) HR(i, k) = hprov[i] + eta * hv[i];
+      HR(k + 1, k) = eta * nu2;
+      const double rel = commit(k);
+      if (rel < tol || k + 1 == max_iter)
+        return finish(k + 1, rel < tol ? FCM_GMRES_CONVERGED : FCM_GMRES_MAX_ITER, rel);
+      if (!(nu2 > 0.0)) return finish(k + 1, FCM_GMRES_BREAKDOWN, rel);
+      for (int i = 0; i <= k; ++i) coef[i] = hv[i];
+      coef[k + 1] = 1.0 / nu2;
+      coef[k + 2] = 0.0;
+      if ((rc = put_coef((size_t)k + 3))) return rc;
+      HIPCHK(update2(h->Q, ldq, k + 1, 0, h->coef, w, nullptr, n, h->kw, h->red, h->st));
+      prov = false;
     }
   }
-  return fail(FCM_ERR_STATE, "GMRES loop ended without a result");
 }
 
 int solve_dev(fcm_handle h, const gmres_opts* o, gmres_report* rep, double* hist) {
@@ -609,6 +737,13 @@ int fcm_create(const fcm_config* cfg, fcm_handle* out) {
   if (e == hipSuccess) h->bd.rec = dalloc<int4>(h, nb, e);
   if (e == hipSuccess) h->bd.w64 = dalloc<double>(h, 3 * (size_t)nb * cfg->ng, e);
   if (e == hipSuccess) h->bd.w32 = dalloc<float>(h, 3 * (size_t)nb * cfg->ng, e);
+  {
+    const int nt = spread_tiles(h->g);
+    if (e == hipSuccess) h->bd.tile_count = dalloc<int>(h, nt, e);
+    if (e == hipSuccess) h->bd.tile_start = dalloc<int>(h, nt + 1, e);
+    if (e == hipSuccess) h->bd.tile_cursor = dalloc<int>(h, nt, e);
+    if (e == hipSuccess) h->bd.tile_list = dalloc<int>(h, 8 * (size_t)nb, e);
+  }
   h->bd.ng = cfg->ng;
   const int n3b = 3 * cfg->max_blobs_per_body;
   if (e == hipSuccess) h->d_ref = dalloc<double>(h, n3b, e);
@@ -631,8 +766,9 @@ int fcm_create(const fcm_config* cfg, fcm_handle* out) {
   if (e == hipSuccess) h->Vrand = dalloc<double>(h, 3 * (size_t)nb, e);
   if (e == hipSuccess) h->Vtmp = dalloc<double>(h, 3 * (size_t)nb, e);
   h->kw.nblk = krylov_blocks(s.nmax);
-  if (e == hipSuccess) h->kw.partial = dalloc<double>(h, (size_t)h->kw.nblk * (cfg->max_krylov + 2), e);
+  if (e == hipSuccess) h->kw.partial = dalloc<double>(h, (size_t)h->kw.nblk * (2 * (size_t)cfg->max_krylov + 6), e);
   if (e == hipSuccess) h->red = dalloc<double>(h, 2 * (size_t)cfg->max_krylov + 16, e);
+  if (e == hipSuccess) h->coef = dalloc<double>(h, 2 * (size_t)cfg->max_krylov + 16, e);
   if (e != hipSuccess) return cleanup_fail(FCM_ERR_NOMEM, "device allocation failed");
   for (int i = 0; i < 3; ++i)
     if (zero_field64(h, h->f64[i])) return cleanup_fail(FCM_ERR_HIP, "memset");

@@ -308,6 +308,43 @@ __host__ __device__ __forceinline__ int cyc(int d, int n, int omin) {
   return t + omin;
 }
 
+// Wavefront (64-lane) sum with DPP moves: quad butterflies, row shifts by 4
+// and 8, then the row broadcasts 15 and 31 leave the total in lane 63, which
+// is read back into every lane.  The addition order is fixed, so the result
+// is bitwise repeatable.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWMASK, 0xf, false));
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, ROWMASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWMASK, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float wave_sum(float v) {
+  v += dpp_f32<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f32<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f32<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_f32<0x118, 0xf>(v);  // row_shr:8
+  v += dpp_f32<0x142, 0xa>(v);  // row_bcast:15
+  v += dpp_f32<0x143, 0xc>(v);  // row_bcast:31
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+__device__ __forceinline__ double wave_sum(double v) {
+  v += dpp_f64<0xb1, 0xf>(v);
+  v += dpp_f64<0x4e, 0xf>(v);
+  v += dpp_f64<0x114, 0xf>(v);
+  v += dpp_f64<0x118, 0xf>(v);
+  v += dpp_f64<0x142, 0xa>(v);
+  v += dpp_f64<0x143, 0xc>(v);
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), 63);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 __device__ __forceinline__ double warp_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -115,6 +115,8 @@ __global__ void record_kernel(const double* __restrict__ Y, const int* __restric
   rec[s] = make_int4(wa[0], wa[1], wa[2], id);
 }
 
+hipError_t build_tile_lists(const GridGeom& g, const BlobBins& bb, BlobData& bd, hipStream_t st);
+
 hipError_t setup_gaussians(const GridGeom& g, const BlobBins& bb, BlobData& bd, double a, hipStream_t st) {
   const int nb = bd.nb, nbins = bb.nbins();
   hipLaunchKernelGGL(fill_int_kernel, dim3((nbins + 255) / 256), dim3(256), 0, st, bd.bin_count, nbins, 0);
@@ -128,7 +130,9 @@ hipError_t setup_gaussians(const GridGeom& g, const BlobBins& bb, BlobData& bd, 
     hipLaunchKernelGGL(record_kernel, dim3((nb + 255) / 256), dim3(256), 0, st, bd.Y, bd.sorted, nb, g, bd.ng, a,
                        bd.rec, bd.w64, bd.w32);
   }
-  return hipGetLastError();
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return build_tile_lists(g, bb, bd, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -303,11 +307,304 @@ __global__ void __launch_bounds__(256) spread_kernel(SpreadArgs A, FieldPtrs<T> 
   }
 }
 
+// ---------------------------------------------------------------------------
+// spreading, main path (grid sides multiples of 8, nx >= 16, ng <= 8).
+// Tile = 8 (x, registers) x 8 (y) x 8*NW (z) nodes; wavefront w owns the
+// 8 x 8 (y, z) patch z in [8w, 8w+8).  The blobs whose stencil meets a tile
+// are listed once per set of positions (tile_list); a spread stages them in
+// LDS (force, raw weights, stencil start per axis relative to the tile and a
+// mask of the wavefronts they touch).  A wavefront skips blobs that miss its
+// patch, and the x loop is specialised per stencil start so that only
+// stencil nodes are touched.  The tile leaves through LDS as whole z rows.
+// Blocks are mapped to tiles so that each XCD works on a contiguous range.
+// ---------------------------------------------------------------------------
+static constexpr int SF_BATCH = 128;
+
+__device__ __forceinline__ int xcd_tile(int b, int ntiles) {
+  const int per = (ntiles + 7) / 8;
+  return (b % 8) * per + b / 8;
+}
+
+__device__ __forceinline__ int wrap_half(int v, int n) {
+  if (v < -(n / 2)) v += n; else if (v >= n - n / 2) v -= n;
+  return v;
+}
+
+// relevance of a blob (wrapped anchor r) to the tile at (x0, y0, z0)
+__device__ __forceinline__ bool tile_relevant(const int4& r, int x0, int y0, int z0, int TZ, const GridGeom& g,
+                                              int ng, int omin) {
+  const int sx = wrap_half(r.x + omin - x0, g.nx);
+  const int sy = wrap_half(r.y + omin - y0, g.ny);
+  const int sz = wrap_half(r.z + omin - z0, g.nz);
+  const bool rx = (sx <= 7) && (sx + ng - 1 >= 0);
+  const bool ry = g.ny <= 8 || ((sy <= 7) && (sy + ng - 1 >= 0));
+  const bool rz = g.nz <= TZ || ((sz <= TZ - 1) && (sz + ng - 1 >= 0));
+  return rx && ry && rz;
+}
+
+// pass 0: count the relevant blobs of every tile; pass 1: write them, in bin
+// order, from tile_start[tile]
+__global__ void __launch_bounds__(256) tile_list_kernel(GridGeom g, BlobBins bb, int ng, const int* bin_start,
+                                                        const int4* rec, int NW, int pass, int* tile_count,
+                                                        const int* tile_start, int* tile_list) {
+  const int omin = 1 - ng / 2, omax = ng - ng / 2;
+  const int TZ = 8 * NW;
+  const int nty = g.ny / 8, ntz = g.nz / TZ;
+  int tile = blockIdx.x;
+  const int tz = tile % ntz; tile /= ntz;
+  const int ty = tile % nty;
+  const int tx = tile / nty;
+  const int x0 = tx * 8, y0 = ty * 8, z0 = tz * TZ;
+  __shared__ int c_start[SP_MAXC], c_off[SP_MAXC + 1];
+  __shared__ int s_ncand;
+  __shared__ int s_wave_cnt[4];
+  const int t = threadIdx.x;
+  if (t == 0) {
+    int lo[3], cnt[3];
+    const int org[3] = {x0, y0, z0}, ext[3] = {8, 8, TZ}, bsz[3] = {bb.bx, bb.by, bb.bz},
+              nbd[3] = {bb.nbx, bb.nby, bb.nbz};
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = floor_div(org[d] - omax, bsz[d]);
+      const int hi = floor_div(org[d] + ext[d] - 1 - omin, bsz[d]);
+      cnt[d] = hi - lo[d] + 1;
+      if (cnt[d] >= nbd[d]) { lo[d] = 0; cnt[d] = nbd[d]; }
+    }
+    int nc = 0, off = 0;
+    for (int i = 0; i < cnt[0]; ++i)
+      for (int j = 0; j < cnt[1]; ++j) {
+        const int bxy = wrapi(lo[0] + i, bb.nbx) * bb.nby + wrapi(lo[1] + j, bb.nby);
+        int k = 0;
+        while (k < cnt[2] && nc < SP_MAXC) {
+          const int bz0 = wrapi(lo[2] + k, bb.nbz);
+          int run = 1;
+          while (k + run < cnt[2] && bz0 + run < bb.nbz) ++run;
+          const int b0 = bxy * bb.nbz + bz0;
+          c_start[nc] = bin_start[b0];
+          c_off[nc] = off;
+          off += bin_start[b0 + run] - bin_start[b0];
+          ++nc;
+          k += run;
+        }
+      }
+    c_off[nc] = off;
+    s_ncand = nc;
+  }
+  __syncthreads();
+  const int ncand = s_ncand, ntot = c_off[ncand];
+  const int lane = t & 63, wv = t >> 6;
+  int written = 0;
+  for (int base = 0; base < ntot; base += 256) {
+    const int c = base + t;
+    bool rel = false;
+    int s = 0;
+    if (c < ntot) {
+      int lo = 0, hi = ncand - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (c_off[mid] <= c) lo = mid; else hi = mid - 1;
+      }
+      s = c_start[lo] + (c - c_off[lo]);
+      rel = tile_relevant(rec[s], x0, y0, z0, TZ, g, ng, omin);
+    }
+    const unsigned long long m = __ballot(rel);
+    const int below = __popcll(m & ((1ULL << lane) - 1ULL));
+    if (lane == 0) s_wave_cnt[wv] = __popcll(m);
+    __syncthreads();
+    int woff = 0, nrel = 0;
+    for (int i = 0; i < 4; ++i) {
+      if (i < wv) woff += s_wave_cnt[i];
+      nrel += s_wave_cnt[i];
+    }
+    if (pass == 1 && rel) tile_list[tile_start[blockIdx.x] + written + woff + below] = s;
+    written += nrel;
+    __syncthreads();
+  }
+  if (pass == 0 && t == 0) tile_count[blockIdx.x] = written;
+}
+
+template <typename T, int S>
+__device__ __forceinline__ void sf_accum(T (&acc)[8][3], T f0, T f1, T f2, const T* gxw) {
+#pragma unroll
+  for (int xi = 0; xi < 8; ++xi) {
+    if (xi - S >= 0 && xi - S < 8) {
+      const T gx = gxw[xi - S];
+      acc[xi][0] += f0 * gx;
+      acc[xi][1] += f1 * gx;
+      acc[xi][2] += f2 * gx;
+    }
+  }
+}
+
+struct SpreadListArgs {
+  GridGeom g;
+  int ng, NW;
+  const int* tile_start;
+  const int* tile_list;
+  const int4* rec;
+  const double* w64;
+  const float* w32;
+  const double* F;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) spread_fast_kernel(SpreadListArgs A, FieldPtrs<T> fp) {
+  const GridGeom& g = A.g;
+  const int ng = A.ng, omin = 1 - ng / 2, NW = A.NW;
+  const int TZ = 8 * NW;
+  const int nty = g.ny / 8, ntz = g.nz / TZ;
+  const int ntiles = (g.nx / 8) * nty * ntz;
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  if (tile >= ntiles) return;
+  int tt = tile;
+  const int tz = tt % ntz; tt /= ntz;
+  const int ty = tt % nty;
+  const int tx = tt / nty;
+  const int x0 = tx * 8, y0 = ty * 8, z0 = tz * TZ;
+
+  constexpr int REC_WORDS = SF_BATCH * (3 + 8 + 8 + 8);
+  constexpr int STAGE_WORDS = 3 * 8 * 8 * 32;
+  __shared__ T buf[REC_WORDS > STAGE_WORDS ? REC_WORDS : STAGE_WORDS];
+  __shared__ int r_sx[SF_BATCH], r_sy[SF_BATCH], r_sz[SF_BATCH], r_mask[SF_BATCH];
+  T (*r_F)[3] = reinterpret_cast<T(*)[3]>(buf);
+  T (*r_gx)[8] = reinterpret_cast<T(*)[8]>(buf + SF_BATCH * 3);
+  T (*r_gy)[8] = reinterpret_cast<T(*)[8]>(buf + SF_BATCH * 11);
+  T (*r_gz)[8] = reinterpret_cast<T(*)[8]>(buf + SF_BATCH * 19);
+
+  const int t = threadIdx.x;
+  const int lane = t & 63, wv = t >> 6;
+  const int lz = lane & 7, ly = lane >> 3;
+  const int zl = 8 * wv + lz;
+  const int L0 = A.tile_start[tile], L1 = A.tile_start[tile + 1];
+  T acc[8][3];
+#pragma unroll
+  for (int xi = 0; xi < 8; ++xi) acc[xi][0] = acc[xi][1] = acc[xi][2] = T(0);
+  const T* W = wsel<T>(A.w64, A.w32);
+
+  for (int base = L0; base < L1; base += SF_BATCH) {
+    const int cnt = min(SF_BATCH, L1 - base);
+    if (t < cnt) {
+      const int s = A.tile_list[base + t];
+      const int4 r = A.rec[s];
+      const T* ws = W + (size_t)s * 3 * ng;
+      T wx[8], wy[8], wz[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        wx[i] = i < ng ? ws[i] : T(0);
+        wy[i] = i < ng ? ws[ng + i] : T(0);
+        wz[i] = i < ng ? ws[2 * ng + i] : T(0);
+      }
+      const double f0 = A.F[3 * (size_t)r.w], f1 = A.F[3 * (size_t)r.w + 1], f2 = A.F[3 * (size_t)r.w + 2];
+      const int sx = wrap_half(r.x + omin - x0, g.nx);
+      const int sy = wrap_half(r.y + omin - y0, g.ny);
+      const int sz = wrap_half(r.z + omin - z0, g.nz);
+      int mask = 0;
+      for (int i = 0; i < TZ; ++i) {
+        int oz = i - sz;
+        if (oz < 0) oz += g.nz; else if (oz >= g.nz) oz -= g.nz;
+        if (oz < ng) mask |= 1 << (i >> 3);
+      }
+      r_F[t][0] = T(f0);
+      r_F[t][1] = T(f1);
+      r_F[t][2] = T(f2);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        r_gx[t][i] = wx[i];
+        r_gy[t][i] = wy[i];
+        r_gz[t][i] = wz[i];
+      }
+      r_sx[t] = sx;
+      r_sy[t] = sy;
+      r_sz[t] = sz;
+      r_mask[t] = mask;
+    }
+    __syncthreads();
+    if (wv < NW) {
+      for (int p = 0; p < cnt; ++p) {
+        if (!((r_mask[p] >> wv) & 1)) continue;
+        int oy = ly - r_sy[p];
+        if (oy < 0) oy += g.ny; else if (oy >= g.ny) oy -= g.ny;
+        int oz = zl - r_sz[p];
+        if (oz < 0) oz += g.nz; else if (oz >= g.nz) oz -= g.nz;
+        if (oy < ng && oz < ng) {
+          const T wgt = r_gy[p][oy] * r_gz[p][oz];
+          const T f0 = r_F[p][0] * wgt, f1 = r_F[p][1] * wgt, f2 = r_F[p][2] * wgt;
+          const T* gxw = r_gx[p];
+          switch (r_sx[p]) {
+            case -7: sf_accum<T, -7>(acc, f0, f1, f2, gxw); break;
+            case -6: sf_accum<T, -6>(acc, f0, f1, f2, gxw); break;
+            case -5: sf_accum<T, -5>(acc, f0, f1, f2, gxw); break;
+            case -4: sf_accum<T, -4>(acc, f0, f1, f2, gxw); break;
+            case -3: sf_accum<T, -3>(acc, f0, f1, f2, gxw); break;
+            case -2: sf_accum<T, -2>(acc, f0, f1, f2, gxw); break;
+            case -1: sf_accum<T, -1>(acc, f0, f1, f2, gxw); break;
+            case 0: sf_accum<T, 0>(acc, f0, f1, f2, gxw); break;
+            case 1: sf_accum<T, 1>(acc, f0, f1, f2, gxw); break;
+            case 2: sf_accum<T, 2>(acc, f0, f1, f2, gxw); break;
+            case 3: sf_accum<T, 3>(acc, f0, f1, f2, gxw); break;
+            case 4: sf_accum<T, 4>(acc, f0, f1, f2, gxw); break;
+            case 5: sf_accum<T, 5>(acc, f0, f1, f2, gxw); break;
+            case 6: sf_accum<T, 6>(acc, f0, f1, f2, gxw); break;
+            default: sf_accum<T, 7>(acc, f0, f1, f2, gxw); break;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // stage [comp][x][y][z] and write whole z rows
+  T* stage = buf;
+  if (wv < NW) {
+#pragma unroll
+    for (int xi = 0; xi < 8; ++xi)
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) stage[((cc * 8 + xi) * 8 + ly) * TZ + zl] = acc[xi][cc];
+  }
+  __syncthreads();
+  const size_t rstride = 2 * (size_t)g.cpitch;
+  const int total = 3 * 8 * 8 * TZ;
+  for (int e = t; e < total; e += 256) {
+    const int zi = e % TZ;
+    const int row = e / TZ;  // (cc * 8 + xi) * 8 + yi
+    const int yi = row & 7, xi = (row >> 3) & 7, cc = row >> 6;
+    const size_t idx = ((size_t)(x0 + xi) * g.ny + (y0 + yi)) * rstride + z0 + zi;
+    reinterpret_cast<T*>(fp.p[cc])[idx] = stage[e];
+  }
+}
+
+bool spread_fast_ok(const GridGeom& g, const BlobBins& bb, int ng) {
+  return ng <= 8 && g.nx >= 16 && g.nx % 8 == 0 && g.ny % 8 == 0 && g.nz % 8 == 0 && bb.bx == 4 && bb.by == 8 &&
+         bb.bz == 8;
+}
+
+int spread_tiles(const GridGeom& g) {
+  const int NW = g.nz % 32 == 0 ? 4 : (g.nz % 16 == 0 ? 2 : 1);
+  return (g.nx / 8) * (g.ny / 8) * (g.nz / (8 * NW));
+}
+
+hipError_t build_tile_lists(const GridGeom& g, const BlobBins& bb, BlobData& bd, hipStream_t st) {
+  if (!spread_fast_ok(g, bb, bd.ng)) return hipSuccess;
+  const int NW = g.nz % 32 == 0 ? 4 : (g.nz % 16 == 0 ? 2 : 1);
+  const int nt = spread_tiles(g);
+  hipLaunchKernelGGL(tile_list_kernel, dim3(nt), dim3(256), 0, st, g, bb, bd.ng, bd.bin_start, bd.rec, NW, 0,
+                     bd.tile_count, (const int*)nullptr, (int*)nullptr);
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, bd.tile_count, bd.tile_start, bd.tile_cursor, nt);
+  hipLaunchKernelGGL(tile_list_kernel, dim3(nt), dim3(256), 0, st, g, bb, bd.ng, bd.bin_start, bd.rec, NW, 1,
+                     bd.tile_count, (const int*)bd.tile_start, bd.tile_list);
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t spread(const GridGeom& g, const BlobBins& bb, const BlobData& bd, const double* F, const FieldPtrs<T>& fp,
                   hipStream_t st) {
-  if (bd.ng > SP_NGMAX || bb.sx * bb.bx > SP_TXMAX || bb.sy * bb.by * bb.sz * bb.bz > 256) return hipErrorInvalidValue;
+  if (spread_fast_ok(g, bb, bd.ng)) {
+    const int NW = g.nz % 32 == 0 ? 4 : (g.nz % 16 == 0 ? 2 : 1);
+    const int nt = spread_tiles(g);
+    SpreadListArgs A{g, bd.ng, NW, bd.tile_start, bd.tile_list, bd.rec, bd.w64, bd.w32, F};
+    hipLaunchKernelGGL((spread_fast_kernel<T>), dim3(8 * ((nt + 7) / 8)), dim3(256), 0, st, A, fp);
+    return hipGetLastError();
+  }
   SpreadArgs A{g, bb, bd.ng, bd.bin_start, bd.rec, bd.w64, bd.w32, F};
+  if (bd.ng > SP_NGMAX || bb.sx * bb.bx > SP_TXMAX || bb.sy * bb.by * bb.sz * bb.bz > 256) return hipErrorInvalidValue;
   const int ntiles = (g.nx / (bb.sx * bb.bx)) * (g.ny / (bb.sy * bb.by)) * (g.nz / (bb.sz * bb.bz));
   hipLaunchKernelGGL((spread_kernel<T>), dim3(ntiles), dim3(256), 0, st, A, fp);
   return hipGetLastError();
@@ -439,11 +736,165 @@ __global__ void __launch_bounds__(256) interp_gather_kernel(InterpArgs A, FieldP
   if (lane == 0) write_velocity(A, r.w, v0, v1, v2);
 }
 
+// ---------------------------------------------------------------------------
+// interpolation, main path (same grid conditions as the main spreading path).
+// One workgroup per tile of 8 x 8 x 8*IZB nodes (2 x 1 x IZB bins): the node
+// box of the tile plus the stencil halo is staged in LDS (all three
+// components in FP32, one component per pass in FP64), then each wavefront
+// reduces one blob at a time over its ng x ng (y, z) lanes.
+// ---------------------------------------------------------------------------
+template <typename T> struct IfNc { static constexpr int value = sizeof(T) == 4 ? 3 : 1; };
+
+template <typename T, int NG>
+__global__ void __launch_bounds__(256) interp_fast_kernel(InterpArgs A, FieldPtrs<T> fp, int IZB) {
+  constexpr int NC = IfNc<T>::value;
+  constexpr int OMIN = 1 - NG / 2;
+  constexpr int HX = 8 + NG - 1, HY = 8 + NG - 1;
+  extern __shared__ unsigned char smem_raw[];
+  T* box = reinterpret_cast<T*>(smem_raw);
+  __shared__ int s_lo[2], s_hi[2];
+  __shared__ double vres[256][3];
+  const GridGeom& g = A.g;
+  const BlobBins& bb = A.bb;
+  const int TZ = 8 * IZB;
+  const int nty = g.ny / 8, ntz = g.nz / TZ;
+  const int ntiles = (g.nx / 8) * nty * ntz;
+  int tile = xcd_tile(blockIdx.x, ntiles);
+  if (tile >= ntiles) return;
+  const int tz = tile % ntz; tile /= ntz;
+  const int ty = tile % nty;
+  const int tx = tile / nty;
+  const int x0 = tx * 8, y0 = ty * 8, z0 = tz * TZ;
+  const int t = threadIdx.x;
+  if (t < 2) {
+    const int b0 = ((2 * tx + t) * bb.nby + ty) * bb.nbz + tz * IZB;
+    s_lo[t] = A.bin_start[b0];
+    s_hi[t] = A.bin_start[b0 + IZB];
+  }
+  __syncthreads();
+  const int n0 = s_hi[0] - s_lo[0], nblob = n0 + (s_hi[1] - s_lo[1]);
+  if (nblob == 0) return;
+  const int HZ = TZ + NG - 1;
+  const int nbox = HX * HY * HZ;
+  const size_t rstride = 2 * (size_t)g.cpitch;
+  const T* W = wsel<T>(A.w64, A.w32);
+  const int lane = t & 63, wv = t >> 6;
+  const int X0 = x0 + OMIN, Y0 = y0 + OMIN, Z0 = z0 + OMIN;
+  const int zl = t & 31;
+  int zg = Z0 + zl;
+  if (zg < 0) zg += g.nz; else if (zg >= g.nz) zg -= g.nz;
+  const bool zok = zl < HZ;
+  size_t rowoff[2];
+  bool yok[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int hy = (t >> 5) + 8 * j;
+    int yg = Y0 + hy;
+    if (yg < 0) yg += g.ny; else if (yg >= g.ny) yg -= g.ny;
+    yok[j] = hy < HY && zok;
+    rowoff[j] = (size_t)yg * rstride + zg;
+  }
+  const int jy = lane / NG, jz = lane - (lane / NG) * NG;
+  const bool pair_ok = lane < NG * NG;
+
+  for (int chunk = 0; chunk < nblob; chunk += 256) {
+    const int nch = min(256, nblob - chunk);
+    for (int c0 = 0; c0 < 3; c0 += NC) {
+      if (c0 > 0 || chunk > 0) __syncthreads();
+      // stage the node box: all loads of half the x rows are issued before
+      // their LDS stores
+      constexpr int HALF = (HX + 1) / 2;
+#pragma unroll
+      for (int h2 = 0; h2 < 2; ++h2) {
+        T v[HALF][2][NC];
+#pragma unroll
+        for (int a = 0; a < HALF; ++a) {
+          const int hx = h2 * HALF + a;
+          int xg = X0 + hx;
+          if (xg < 0) xg += g.nx; else if (xg >= g.nx) xg -= g.nx;
+          const size_t xoff = (size_t)xg * g.ny * rstride;
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            const bool ok = hx < HX && yok[j];
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc)
+              v[a][j][cc] = ok ? reinterpret_cast<const T*>(fp.p[c0 + cc])[xoff + rowoff[j]] : T(0);
+          }
+        }
+#pragma unroll
+        for (int a = 0; a < HALF; ++a) {
+          const int hx = h2 * HALF + a;
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            const int hy = (t >> 5) + 8 * j;
+            if (hx < HX && yok[j])
+#pragma unroll
+              for (int cc = 0; cc < NC; ++cc) box[cc * nbox + (hx * HY + hy) * HZ + zl] = v[a][j][cc];
+          }
+        }
+      }
+      __syncthreads();
+      for (int k = wv; k < nch; k += 4) {
+        const int kk = chunk + k;
+        const int sidx = __builtin_amdgcn_readfirstlane(kk < n0 ? s_lo[0] + kk : s_lo[1] + (kk - n0));
+        const int4 r = A.rec[sidx];
+        const T* ws = W + (size_t)sidx * 3 * NG;
+        const int lx = r.x - x0, ly = r.y - y0, lz = r.z - z0;
+        T a[NC];
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc) a[cc] = T(0);
+        if (pair_ok) {
+          const int hyz = (ly + jy) * HZ + (lz + jz);
+          T sv[NC];
+#pragma unroll
+          for (int cc = 0; cc < NC; ++cc) sv[cc] = T(0);
+#pragma unroll
+          for (int jx = 0; jx < NG; ++jx) {
+            const int i = (lx + jx) * HY * HZ + hyz;
+            const T gx = ws[jx];
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) sv[cc] += gx * box[cc * nbox + i];
+          }
+          const T wyz = ws[NG + jy] * ws[2 * NG + jz];
+#pragma unroll
+          for (int cc = 0; cc < NC; ++cc) a[cc] = wyz * sv[cc];
+        }
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc) {
+          const T v = wave_sum(a[cc]);
+          if (lane == 0) vres[k][c0 + cc] = (double)v;
+        }
+      }
+    }
+    __syncthreads();
+    for (int k = t; k < nch; k += 256) {
+      const int kk = chunk + k;
+      const int sidx = kk < n0 ? s_lo[0] + kk : s_lo[1] + (kk - n0);
+      write_velocity(A, A.rec[sidx].w, vres[k][0], vres[k][1], vres[k][2]);
+    }
+  }
+}
+
+static bool interp_fast_ok(const GridGeom& g, const BlobBins& bb, int ng) {
+  return spread_fast_ok(g, bb, ng) && bb.nbx % 2 == 0;
+}
+
 template <typename T>
 hipError_t interp(const GridGeom& g, const BlobBins& bb, const BlobData& bd, const FieldPtrs<T>& fp, double* V,
                   const KTerm& kt, hipStream_t st) {
   InterpArgs A{g, bb, bd.ng, bd.bin_start, bd.rec, bd.w64, bd.w32, V, kt, g.dx * g.dx * g.dx};
   if (bd.nb == 0) return hipSuccess;
+  if (interp_fast_ok(g, bb, bd.ng) && (bd.ng == 7 || bd.ng == 8)) {
+    const int IZB = bb.nbz % 2 == 0 ? 2 : 1;
+    const int TZ = 8 * IZB;
+    const size_t lds = IfNc<T>::value * sizeof(T) * (size_t)(8 + bd.ng - 1) * (8 + bd.ng - 1) * (TZ + bd.ng - 1);
+    const int ntiles = (g.nx / 8) * (g.ny / 8) * (g.nz / TZ);
+    if (bd.ng == 7)
+      hipLaunchKernelGGL((interp_fast_kernel<T, 7>), dim3(8 * ((ntiles + 7) / 8)), dim3(256), lds, st, A, fp, IZB);
+    else
+      hipLaunchKernelGGL((interp_fast_kernel<T, 8>), dim3(8 * ((ntiles + 7) / 8)), dim3(256), lds, st, A, fp, IZB);
+    return hipGetLastError();
+  }
   const size_t lds = 3 * sizeof(T) * (size_t)(bb.bx + bd.ng - 1) * (bb.by + bd.ng - 1) * (bb.bz + bd.ng - 1);
   if (lds <= 64 * 1024) {
     hipLaunchKernelGGL((interp_box_kernel<T>), dim3(bb.nbins()), dim3(256), lds, st, A, fp);

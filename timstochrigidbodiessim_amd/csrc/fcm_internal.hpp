@@ -70,8 +70,15 @@ struct BlobData {
   int4* anc = nullptr;        // nb, sorted order: unwrapped anchors
   double* w64 = nullptr;      // nb x 3 x ng, sorted order
   float* w32 = nullptr;       // nb x 3 x ng, sorted order
-  int* scan_tmp = nullptr;    // scratch for the bin scan
+  int* tile_count = nullptr;  // spreading tiles: relevant blobs per tile
+  int* tile_start = nullptr;  // ntiles + 1
+  int* tile_cursor = nullptr; // ntiles (scan scratch)
+  int* tile_list = nullptr;   // sorted slots of the blobs relevant to each tile (<= 8 per blob)
 };
+
+// spreading tiles of the main path (0 if the grid does not qualify)
+bool spread_fast_ok(const GridGeom& g, const BlobBins& bb, int ng);
+int spread_tiles(const GridGeom& g);
 
 hipError_t setup_gaussians(const GridGeom& g, const BlobBins& bb, BlobData& bd, double a, hipStream_t st);
 
@@ -123,6 +130,14 @@ hipError_t dots(const double* Q, size_t ldq, int ncols, const double* w, size_t 
 // w -= sum_i h[i] Q[i]; if norm_out != null, *norm_out = ||w||^2 after the update
 hipError_t update(const double* Q, size_t ldq, int ncols, const double* h, double* w, size_t n, KrylovWork& kw,
                   double* norm_out, hipStream_t st);
+// red[0..ncols) = Q[c].v1, red[ncols..2 ncols) = Q[c].v2 in one pass (either vector may be null)
+hipError_t dots2(const double* Q, size_t ldq, int ncols, const double* v1, const double* v2, size_t n, KrylovWork& kw,
+                 double* red_out, hipStream_t st);
+// lagged reorthogonalisation pass, coef = [s(m) | t(mt) | 1/nu | pk]:
+//   qt <- (qt - sum_{i<m} s_i Q_i)/nu (if qt), w <- w - sum_{i<mt} t_i Q_i - pk qt (if w),
+//   *norm_out = ||w||^2 (or ||qt||^2 when w is null)
+hipError_t update2(const double* Q, size_t ldq, int m, int mt, const double* coef, double* qt, double* w, size_t n,
+                   KrylovWork& kw, double* norm_out, hipStream_t st);
 // y = sum_i c[i] Q[i]
 hipError_t combine(const double* Q, size_t ldq, int ncols, const double* c, double* y, size_t n, hipStream_t st);
 // y = a*x (a from device pointer, taken as 1/sqrt(*a) if invsqrt)

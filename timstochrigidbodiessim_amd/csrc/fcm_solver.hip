@@ -227,7 +227,7 @@ int krylov_blocks(size_t n) { return (int)((n + KB - 1) / KB); }
 __device__ __forceinline__ void block_reduce8(double (&v)[KCOLS], double* red_lds /* 4 x KCOLS */) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-  for (int c = 0; c < KCOLS; ++c) v[c] = warp_sum(v[c]);
+  for (int c = 0; c < KCOLS; ++c) v[c] = wave_sum(v[c]);
   if (lane == 0)
 #pragma unroll
     for (int c = 0; c < KCOLS; ++c) red_lds[wv * KCOLS + c] = v[c];
@@ -326,6 +326,111 @@ __global__ void __launch_bounds__(KT) update_kernel(const double* __restrict__ Q
   }
 }
 
+// Two dot products per basis column in one pass over the basis:
+// partial[c][blk] = Q[c].v1 (c < ncols, skipped if v1 is null) and
+// partial[ncols + c][blk] = Q[c].v2 (skipped if v2 is null).
+__global__ void __launch_bounds__(KT) dots2_kernel(const double* __restrict__ Q, size_t ldq, int ncols,
+                                                   const double* __restrict__ v1, const double* __restrict__ v2,
+                                                   size_t n, double* __restrict__ partial, int nblk) {
+  __shared__ double red[4 * KCOLS];
+  const size_t r0 = (size_t)blockIdx.x * KB + threadIdx.x;
+  double a[KRPT], b[KRPT];
+#pragma unroll
+  for (int k = 0; k < KRPT; ++k) {
+    const size_t r = r0 + (size_t)k * KT;
+    a[k] = (v1 && r < n) ? v1[r] : 0.0;
+    b[k] = (v2 && r < n) ? v2[r] : 0.0;
+  }
+  constexpr int H = KCOLS / 2;
+  for (int c0 = 0; c0 < ncols; c0 += H) {
+    double acc[KCOLS];
+#pragma unroll
+    for (int c = 0; c < H; ++c) {
+      acc[c] = 0.0;
+      acc[H + c] = 0.0;
+      if (c0 + c < ncols) {
+        const double* q = Q + (size_t)(c0 + c) * ldq;
+#pragma unroll
+        for (int k = 0; k < KRPT; ++k) {
+          const size_t r = r0 + (size_t)k * KT;
+          if (r < n) {
+            const double qv = q[r];
+            acc[c] += qv * a[k];
+            acc[H + c] += qv * b[k];
+          }
+        }
+      }
+    }
+    block_reduce8(acc, red);
+    if (threadIdx.x < KCOLS) {
+      const int c = threadIdx.x % H, which = threadIdx.x / H;
+      if (c0 + c < ncols && (which == 0 ? v1 != nullptr : v2 != nullptr))
+        partial[((size_t)which * ncols + c0 + c) * nblk + blockIdx.x] = acc[threadIdx.x];
+    }
+    __syncthreads();
+  }
+}
+
+// One pass of the lagged (delayed) reorthogonalisation:
+//   if qt: q = (qt - sum_{i<m} s_i Q_i) * inv_nu, written back over qt
+//   if w:  w' = w - sum_{i<mt} t_i Q_i - pk * q     (q term only if qt)
+// and the per-block partial of ||w'||^2 (or of ||q||^2 when w is null).
+// coef = [s(m) | t(mt) | inv_nu | pk]
+__global__ void __launch_bounds__(KT) update2_kernel(const double* __restrict__ Q, size_t ldq, int m, int mt,
+                                                     const double* __restrict__ coef, double* __restrict__ qt,
+                                                     double* __restrict__ w, size_t n, double* __restrict__ partial) {
+  __shared__ double cs[2048 + 4];
+  __shared__ double red[4 * KCOLS];
+  const int ncoef = m + mt + 2;
+  for (int i = threadIdx.x; i < ncoef && i < 2048 + 4; i += KT) cs[i] = coef[i];
+  __syncthreads();
+  const double* sc = coef;  // fallbacks for very long bases read global memory
+  auto S = [&](int i) { return i < 2048 + 4 ? cs[i] : sc[i]; };
+  const double inv_nu = S(m + mt), pk = S(m + mt + 1);
+  const size_t r0 = (size_t)blockIdx.x * KB + threadIdx.x;
+  double as[KRPT], at[KRPT];
+#pragma unroll
+  for (int k = 0; k < KRPT; ++k) as[k] = at[k] = 0.0;
+  const int mm = m > mt ? m : mt;
+  for (int c = 0; c < mm; ++c) {
+    const double sv = (qt && c < m) ? S(c) : 0.0;
+    const double tv = (w && c < mt) ? S(m + c) : 0.0;
+    const double* q = Q + (size_t)c * ldq;
+#pragma unroll
+    for (int k = 0; k < KRPT; ++k) {
+      const size_t r = r0 + (size_t)k * KT;
+      if (r < n) {
+        const double qv = q[r];
+        as[k] += sv * qv;
+        at[k] += tv * qv;
+      }
+    }
+  }
+  double acc[KCOLS];
+#pragma unroll
+  for (int c = 0; c < KCOLS; ++c) acc[c] = 0.0;
+#pragma unroll
+  for (int k = 0; k < KRPT; ++k) {
+    const size_t r = r0 + (size_t)k * KT;
+    if (r < n) {
+      double qk = 0.0;
+      if (qt) {
+        qk = (qt[r] - as[k]) * inv_nu;
+        qt[r] = qk;
+      }
+      if (w) {
+        const double v = w[r] - at[k] - pk * qk;
+        w[r] = v;
+        acc[0] += v * v;
+      } else {
+        acc[0] += qk * qk;
+      }
+    }
+  }
+  block_reduce8(acc, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc[0];
+}
+
 __global__ void __launch_bounds__(KT) combine_kernel(const double* __restrict__ Q, size_t ldq, int ncols,
                                                      const double* __restrict__ c, double* __restrict__ y, size_t n) {
   const size_t r = (size_t)blockIdx.x * KT + threadIdx.x;
@@ -368,6 +473,25 @@ hipError_t update(const double* Q, size_t ldq, int ncols, const double* h, doubl
   hipLaunchKernelGGL(update_kernel, dim3(kw.nblk), dim3(KT), 0, st, Q, ldq, ncols, h, w, n,
                      norm_out ? kw.partial : (double*)nullptr, kw.nblk);
   if (norm_out) hipLaunchKernelGGL(reduce_cols_kernel, dim3(1), dim3(256), 0, st, kw.partial, kw.nblk, norm_out, 0);
+  return hipGetLastError();
+}
+
+hipError_t dots2(const double* Q, size_t ldq, int ncols, const double* v1, const double* v2, size_t n, KrylovWork& kw,
+                 double* red_out, hipStream_t st) {
+  if (ncols <= 0) return hipSuccess;
+  hipLaunchKernelGGL(dots2_kernel, dim3(kw.nblk), dim3(KT), 0, st, Q, ldq, ncols, v1, v2, n, kw.partial, kw.nblk);
+  // red_out[0..ncols) = Q.v1, red_out[ncols..2 ncols) = Q.v2 (absent vectors leave zeros untouched)
+  if (v1) hipLaunchKernelGGL(reduce_cols_kernel, dim3(ncols), dim3(256), 0, st, kw.partial, kw.nblk, red_out, 0);
+  if (v2)
+    hipLaunchKernelGGL(reduce_cols_kernel, dim3(ncols), dim3(256), 0, st, kw.partial + (size_t)ncols * kw.nblk, kw.nblk,
+                       red_out + ncols, 0);
+  return hipGetLastError();
+}
+
+hipError_t update2(const double* Q, size_t ldq, int m, int mt, const double* coef, double* qt, double* w, size_t n,
+                   KrylovWork& kw, double* norm_out, hipStream_t st) {
+  hipLaunchKernelGGL(update2_kernel, dim3(kw.nblk), dim3(KT), 0, st, Q, ldq, m, mt, coef, qt, w, n, kw.partial);
+  hipLaunchKernelGGL(reduce_cols_kernel, dim3(1), dim3(256), 0, st, kw.partial, kw.nblk, norm_out, 0);
   return hipGetLastError();
 }
 
