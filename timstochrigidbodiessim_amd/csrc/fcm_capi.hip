@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -89,6 +90,16 @@ struct fcm_handle_s {
   double* red = nullptr;      // reductions, 2*(max_krylov+2)+8
   double* coef = nullptr;     // coefficients of the reorthogonalisation pass
   KrylovWork kw;
+  // device-controlled GMRES
+  GmDev gm;
+  double* gm_ctl = nullptr;   // 16 doubles: dctl[0..8) then ictl (int[16])
+  double* w = nullptr;        // A P^-1 q_k of the current step (fixed address for graph capture)
+  double* hctl = nullptr;     // pinned host snapshots, 3 x 16 doubles
+  hipEvent_t pev[2] = {nullptr, nullptr};
+  struct StepGraph {
+    hipGraphExec_t exec = nullptr;
+    long long key[3] = {-1, -1, -1};
+  } graph[2];
   bool have_rhs = false, have_sol = false;
   std::vector<void*> allocs;
 
@@ -151,6 +162,10 @@ bool valid_cfg(const fcm_config* c, std::string& why) {
     why = "capacities must be positive";
     return false;
   }
+  if (c->max_krylov > 2048) {
+    why = "max_krylov must be <= 2048";
+    return false;
+  }
   return true;
 }
 
@@ -193,6 +208,8 @@ Sizes sizes_of(const fcm_config* c) {
   t += (size_t)(c->max_krylov + 1) * s.ldq * 8 + 6 * s.ldq * 8 + 2 * (size_t)nb * 3 * 8;
   const size_t nblk = (s.nmax + 1023) / 1024;
   t += nblk * (2 * (size_t)c->max_krylov + 6) * 8 + 2 * (2 * (size_t)c->max_krylov + 16) * 8;
+  const size_t m2 = (size_t)(c->max_krylov + 2) * (c->max_krylov + 2);
+  t += (2 * m2 + 8 * (size_t)(c->max_krylov + 2) + 16) * 8 + s.ldq * 8;
   s.total = t;
   return s;
 }
@@ -260,17 +277,22 @@ int sync_stream(fcm_handle h) {
 // ---------------------------------------------------------------------------
 // mobility product on device vectors: Vout = dx^3 J S Fin (- K zV if given)
 // ---------------------------------------------------------------------------
-int mobility_dev(fcm_handle h, const double* Fin, double* Vout, int precision, const KTerm& kt) {
+int mobility_dev(fcm_handle h, const double* Fin, double* Vout, int precision, const KTerm& kt,
+                 const int* gate = nullptr) {
   if (!h->have_blobs) return fail(FCM_ERR_STATE, "no blob positions set");
   if (precision == FCM_PREC_FP32) {
     if (!h->f32[0]) return fail(FCM_ERR_STATE, "FP32 path not allocated (fcm_config.fp32 = 0)");
-    HIPCHK(spread<float>(h->g, h->bb, h->bd, Fin, h->fp32(), h->st));
-    HIPCHK(stokes_solve<float>(h->g, h->fp32(), h->pp, h->tw32, h->st));
-    HIPCHK(interp<float>(h->g, h->bb, h->bd, h->fp32(), Vout, kt, h->st));
+    FieldPtrs<float> fp = h->fp32();
+    fp.gate = gate;
+    HIPCHK(spread<float>(h->g, h->bb, h->bd, Fin, fp, h->st));
+    HIPCHK(stokes_solve<float>(h->g, fp, h->pp, h->tw32, h->st));
+    HIPCHK(interp<float>(h->g, h->bb, h->bd, fp, Vout, kt, h->st));
   } else {
-    HIPCHK(spread<double>(h->g, h->bb, h->bd, Fin, h->fp64(), h->st));
-    HIPCHK(stokes_solve<double>(h->g, h->fp64(), h->pp, h->tw64, h->st));
-    HIPCHK(interp<double>(h->g, h->bb, h->bd, h->fp64(), Vout, kt, h->st));
+    FieldPtrs<double> fp = h->fp64();
+    fp.gate = gate;
+    HIPCHK(spread<double>(h->g, h->bb, h->bd, Fin, fp, h->st));
+    HIPCHK(stokes_solve<double>(h->g, fp, h->pp, h->tw64, h->st));
+    HIPCHK(interp<double>(h->g, h->bb, h->bd, fp, Vout, kt, h->st));
   }
   return FCM_OK;
 }
@@ -319,218 +341,118 @@ struct InnerResult {
   double rel = 0;
 };
 
-// Givens least-squares state of gmres_solver.cpp:212-276 (columns are
-// pushed one at a time, unrotated; the rotated column and the rotations are
-// kept).  estimate() evaluates the residual a column would give without
-// committing it.
-struct GivensLS {
-  int M;
-  std::vector<double> R, CS, SN, beta;
-  double nrm;
-  GivensLS(int M_, double nrm_) : M(M_), R((size_t)(M_ + 2) * (M_ + 2), 0.0), CS(M_ + 2, 0.0), SN(M_ + 2, 0.0),
-                                  beta(M_ + 2, 0.0), nrm(nrm_) { beta[0] = nrm_; }
-  // col holds H[0..k+1][k]; returns |beta_{k+1}| / ||b||
-  double apply(int k, const double* col, bool commit) {
-    std::vector<double> h(col, col + k + 2);
-    for (int i = 1; i <= k; ++i) {
-      const double temp = CS[i - 1] * h[i - 1] + SN[i - 1] * h[i];
-      h[i] = CS[i - 1] * h[i] - SN[i - 1] * h[i - 1];
-      h[i - 1] = temp;
-    }
-    double cs, sn;
-    if (h[k + 1] == 0.0) {
-      cs = 1.0;
-      sn = 0.0;
-    } else if (std::fabs(h[k + 1]) > std::fabs(h[k])) {
-      const double temp = h[k] / h[k + 1];
-      sn = (2.0 * double(h[k + 1] > 0) - 1.0) / std::sqrt(1.0 + temp * temp);
-      cs = temp * sn;
-    } else {
-      const double temp = h[k + 1] / h[k];
-      cs = (2.0 * double(h[k] > 0) - 1.0) / std::sqrt(1.0 + temp * temp);
-      sn = temp * cs;
-    }
-    const double b1 = -sn * beta[k];
-    if (commit) {
-      h[k] = cs * h[k] + sn * h[k + 1];
-      for (int i = 0; i <= k; ++i) R[(size_t)i * (M + 2) + k] = h[i];
-      CS[k] = cs;
-      SN[k] = sn;
-      beta[k + 1] = b1;
-      beta[k] = cs * beta[k];
-    }
-    return std::fabs(b1) / nrm;
-  }
-  // back substitution over the first ncols columns (gmres_solver.cpp:260-276)
-  std::vector<double> solve(int ncols) const {
-    std::vector<double> y(ncols, 0.0);
-    auto RR = [&](int i, int j) { return R[(size_t)i * (M + 2) + j]; };
-    y[ncols - 1] = beta[ncols - 1] / RR(ncols - 1, ncols - 1);
-    for (int i = ncols - 2; i >= 0; --i) {
-      y[i] = beta[i];
-      for (int j = ncols - 1; j > i; --j) y[i] -= RR(i, j) * y[j];
-      y[i] /= RR(i, i);
-    }
-    return y;
-  }
-};
+// One Arnoldi step under device control: w = A P^-1 q_k with k read on the
+// device, then the DCGS2 basis update and the Givens step.  Every kernel is
+// gated on the status word, so steps queued after convergence do nothing.
+int launch_arnoldi_step(fcm_handle h, int precision) {
+  const size_t n = h->nsys(), n3 = h->n3();
+  const int* gate = h->gm.ictl + GM_STATUS;
+  BodyData b = h->body();
+  b.gate = gate;
+  b.kcol = h->gm.ictl + GM_K;
+  b.ldq = h->ldq;
+  HIPCHK(apply_pinv(b, h->Q, h->z, h->w + n3, h->st));
+  KTerm kt;
+  kt.zV = h->z + n3;
+  kt.dpos = h->d_dpos;
+  kt.nbpb = h->nbpb;
+  if (int rc = mobility_dev(h, h->z, h->w, precision, kt, gate)) return rc;
+  HIPCHK(gm_launch_basis(h->gm, h->Q, h->ldq, h->w, n, h->kw.partial, gm_blocks(n), h->st));
+  return FCM_OK;
+}
 
-// Unrestarted right-preconditioned GMRES, x0 = 0 (gmres_solver.cpp:148-306),
-// on device vectors b -> x, with the Hessenberg/Givens algebra of the
-// reference on the host in FP64.  The Arnoldi basis is built by classical
-// Gram-Schmidt with delayed reorthogonalisation (DCGS2, Swirydowicz et al.,
-// Numer. Linear Algebra Appl. 2021): each iteration makes one fused pass
-// that projects the new vector and the reorthogonalisation of the previous,
-// still once-orthogonalised vector against the final basis, and one fused
-// pass that finalises the previous vector and updates the new one -- two
-// reads of the basis per iteration instead of the four of CGS2.  The mat-vec
-// is applied to the provisional vector; the Hessenberg column is corrected
-// with the reorthogonalisation coefficients (A P^-1 is linear).  A column is
-// committed to the Givens least squares once its vector is final; when the
-// provisional column already predicts convergence the previous vector is
-// finalised without another mat-vec.
-int gmres_core(fcm_handle h, const double* b, double* x, double tol, int max_iter_in, int precision,
-               InnerResult& out, double* hist) {
+bool graphs_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("FCM_HIP_GRAPHS");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+// The Arnoldi step captured once into a hipGraph per precision; recaptured
+// when the blob or body counts baked into its kernel arguments change.
+int step_graph(fcm_handle h, int precision, hipGraphExec_t* out) {
+  auto& gc = h->graph[precision == FCM_PREC_FP32 ? 1 : 0];
+  const long long key[3] = {h->bd.nb, h->NB, h->nbpb};
+  if (gc.exec && std::memcmp(gc.key, key, sizeof key) == 0) {
+    *out = gc.exec;
+    return FCM_OK;
+  }
+  if (gc.exec) {
+    (void)hipGraphExecDestroy(gc.exec);
+    gc.exec = nullptr;
+  }
+  HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+  const int rc = launch_arnoldi_step(h, precision);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(h->st, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  HIPCHK(ec);
+  const hipError_t ei = hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HIPCHK(ei);
+  std::memcpy(gc.key, key, sizeof key);
+  *out = gc.exec;
+  return FCM_OK;
+}
+
+// Unrestarted right-preconditioned GMRES, x0 = 0 (gmres_solver.cpp:148-306)
+// on device vectors b -> x.  The Arnoldi loop, the Givens least squares of
+// the reference and the stopping test run on the device (fcm_gmres.hip); the
+// host queues steps in chunks and reads the status word of the chunk before
+// the one it has just queued, so the device never waits for the host.
+int gmres_device(fcm_handle h, const double* b, double* x, double tol, int max_iter_in, int precision,
+                 InnerResult& out, double* hist) {
   const size_t n = h->nsys();
-  const size_t ldq = h->ldq;
   const int max_iter = (int)std::min<size_t>((size_t)std::min(max_iter_in, h->cfg.max_krylov), n);
-  int rc;
-  const double nrm = dev_norm(h, b, n, rc);
-  if (rc) return rc;
-  out = InnerResult{};
-  if (nrm == 0.0) {
-    HIPCHK(axpby(0.0, b, 0.0, x, n, h->st));
-    out.status = FCM_GMRES_ZERO_RHS;
-    return FCM_OK;
-  }
   if (max_iter < 1) return fail(FCM_ERR_ARG, "max_iter must be >= 1");
-  const int M = max_iter;
-  GivensLS G(M, nrm);
-  std::vector<double> Hraw((size_t)(M + 2) * (M + 2), 0.0);
-  auto HR = [&](int i, int j) -> double& { return Hraw[(size_t)i * (M + 2) + j]; };
-  std::vector<double> hv(2 * (size_t)M + 8), coef(2 * (size_t)M + 8), hprov, col(M + 2);
-  double eta = 0.0;
-  HIPCHK(axpby(1.0 / nrm, b, 0.0, h->Q, n, h->st));
-
-  auto fetch = [&](size_t count) -> int {
-    HIPCHK(hipMemcpyAsync(hv.data(), h->red, sizeof(double) * count, hipMemcpyDeviceToHost, h->st));
-    HIPCHK(hipStreamSynchronize(h->st));
-    return FCM_OK;
-  };
-  auto put_coef = [&](size_t count) -> int {
-    HIPCHK(hipMemcpyAsync(h->coef, coef.data(), sizeof(double) * count, hipMemcpyHostToDevice, h->st));
-    return FCM_OK;
-  };
-  auto finish = [&](int ncols, int status, double rel) -> int {
-    std::vector<double> y = G.solve(ncols);
-    HIPCHK(hipMemcpyAsync(h->red, y.data(), sizeof(double) * ncols, hipMemcpyHostToDevice, h->st));
-    HIPCHK(combine(h->Q, ldq, ncols, h->red, h->tmp, n, h->st));
-    BodyData bdy = h->body();
-    HIPCHK(apply_pinv(bdy, h->tmp, x, nullptr, h->st));
-    HIPCHK(hipStreamSynchronize(h->st));
-    out.iters = ncols;
-    out.rel = rel;
-    out.status = status;
-    return FCM_OK;
-  };
-  auto commit = [&](int kc) -> double {
-    for (int i = 0; i <= kc + 1; ++i) col[i] = HR(i, kc);
-    const double rel = G.apply(kc, col.data(), true);
-    if (hist) hist[kc] = rel;
-    return rel;
-  };
-
-  bool prov = false;
-  for (int k = 0;; ++k) {
-    double* qk = h->Q + (size_t)k * ldq;
-    double* w = h->Q + (size_t)(k + 1) * ldq;
-    rc = arnoldi_product(h, qk, w, precision);
-    if (rc) return rc;
-    std::vector<double> hnew(k + 1);
-    double nu = 1.0;
-    if (prov) {
-      // columns 0..k-1 final, column k provisional
-      HIPCHK(dots2(h->Q, ldq, k + 1, qk, w, n, h->kw, h->red, h->st));
-      if ((rc = fetch(2 * (size_t)(k + 1)))) return rc;
-      const double* sv = hv.data();
-      const double alpha = hv[k];
-      const double* tv = hv.data() + (k + 1);
-      const double tau = hv[2 * k + 1];
-      double ss = 0.0, st = 0.0;
-      for (int i = 0; i < k; ++i) {
-        ss += sv[i] * sv[i];
-        st += sv[i] * tv[i];
+  if (precision == FCM_PREC_FP32 && !h->f32[0]) return fail(FCM_ERR_STATE, "FP32 path not allocated");
+  out = InnerResult{};
+  HIPCHK(dots(b, 0, 1, b, n, h->kw, h->gm.red, h->st));
+  HIPCHK(gm_launch_start(h->gm, tol, max_iter, b, h->Q, n, h->st));
+  hipGraphExec_t ge = nullptr;
+  if (graphs_enabled())
+    if (int rc = step_graph(h, precision, &ge)) return rc;
+  constexpr int CHUNK = 4;
+  constexpr size_t CTL = 16;  // doubles per control snapshot
+  int launched = 0, slot = 0, pending = -1;
+  while (launched < max_iter) {
+    const int c = std::min(CHUNK, max_iter - launched);
+    for (int i = 0; i < c; ++i) {
+      if (ge) {
+        HIPCHK(hipGraphLaunch(ge, h->st));
+      } else if (int rc = launch_arnoldi_step(h, precision)) {
+        return rc;
       }
-      nu = std::sqrt(std::max(alpha - ss, 0.0));
-      for (int i = 0; i < k; ++i) HR(i, k - 1) = hprov[i] + eta * sv[i];
-      HR(k, k - 1) = eta * nu;
-      const double rel = commit(k - 1);
-      if (rel < tol || k == max_iter) return finish(k, rel < tol ? FCM_GMRES_CONVERGED : FCM_GMRES_MAX_ITER, rel);
-      if (!(nu > 0.0)) return finish(k, FCM_GMRES_BREAKDOWN, rel);
-      // projections of A P^-1 q_k on the final basis q_0..q_k
-      std::vector<double> proj(k + 1);
-      for (int i = 0; i < k; ++i) proj[i] = tv[i];
-      proj[k] = (tau - st) / nu;
-      for (int l = 0; l <= k; ++l) {
-        double c = 0.0;
-        for (int i = 0; i < k; ++i) c += HR(l, i) * sv[i];
-        hnew[l] = (proj[l] - c) / nu;
-      }
-      for (int i = 0; i < k; ++i) coef[i] = sv[i];
-      for (int i = 0; i < k; ++i) coef[k + i] = tv[i];
-      coef[2 * k] = 1.0 / nu;
-      coef[2 * k + 1] = proj[k];
-      if ((rc = put_coef(2 * (size_t)k + 2))) return rc;
-      HIPCHK(update2(h->Q, ldq, k, k, h->coef, qk, w, n, h->kw, h->red, h->st));
-    } else {
-      // columns 0..k final
-      HIPCHK(dots2(h->Q, ldq, k + 1, nullptr, w, n, h->kw, h->red, h->st));
-      if ((rc = fetch(2 * (size_t)(k + 1)))) return rc;
-      for (int i = 0; i <= k; ++i) hnew[i] = hv[k + 1 + i];
-      for (int i = 0; i <= k; ++i) coef[i] = hnew[i];
-      coef[k + 1] = 1.0;
-      coef[k + 2] = 0.0;
-      if ((rc = put_coef((size_t)k + 3))) return rc;
-      HIPCHK(update2(h->Q, ldq, 0, k + 1, h->coef, nullptr, w, n, h->kw, h->red, h->st));
     }
-    if ((rc = fetch(1))) return rc;
-    const double etar = std::sqrt(hv[0]);
-    eta = etar / nu;
-    hprov = hnew;
-    if (!(etar > 0.0)) {
-      // exact breakdown: the Krylov space is invariant, column k is exact
-      for (int i = 0; i <= k; ++i) HR(i, k) = hprov[i];
-      HR(k + 1, k) = 0.0;
-      const double rel = commit(k);
-      return finish(k + 1, FCM_GMRES_BREAKDOWN, rel);
+    launched += c;
+    HIPCHK(hipMemcpyAsync(h->hctl + slot * CTL, h->gm_ctl, CTL * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipEventRecord(h->pev[slot], h->st));
+    if (pending >= 0) {
+      HIPCHK(hipEventSynchronize(h->pev[pending]));
+      const int* ic = reinterpret_cast<const int*>(h->hctl + pending * CTL + 8);
+      if (ic[GM_STATUS] != 0) break;
     }
-    HIPCHK(axpby(1.0 / etar, w, 0.0, w, n, h->st));
-    prov = true;
-    // predict convergence from the provisional column
-    for (int i = 0; i <= k; ++i) col[i] = hprov[i];
-    col[k + 1] = eta;
-    const double est = G.apply(k, col.data(), false);
-    if (est < tol || k + 1 == max_iter) {
-      // finalise q_{k+1} without a further mat-vec
-      HIPCHK(dots2(h->Q, ldq, k + 2, w, nullptr, n, h->kw, h->red, h->st));
-      if ((rc = fetch((size_t)k + 2))) return rc;
-      double ss = 0.0;
-      for (int i = 0; i <= k; ++i) ss += hv[i] * hv[i];
-      const double nu2 = std::sqrt(std::max(hv[k + 1] - ss, 0.0));
-      for (int i = 0; i <= k; ++i) HR(i, k) = hprov[i] + eta * hv[i];
-      HR(k + 1, k) = eta * nu2;
-      const double rel = commit(k);
-      if (rel < tol || k + 1 == max_iter)
-        return finish(k + 1, rel < tol ? FCM_GMRES_CONVERGED : FCM_GMRES_MAX_ITER, rel);
-      if (!(nu2 > 0.0)) return finish(k + 1, FCM_GMRES_BREAKDOWN, rel);
-      for (int i = 0; i <= k; ++i) coef[i] = hv[i];
-      coef[k + 1] = 1.0 / nu2;
-      coef[k + 2] = 0.0;
-      if ((rc = put_coef((size_t)k + 3))) return rc;
-      HIPCHK(update2(h->Q, ldq, k + 1, 0, h->coef, w, nullptr, n, h->kw, h->red, h->st));
-      prov = false;
-    }
+    pending = slot;
+    slot ^= 1;
   }
+  HIPCHK(gm_launch_finish(h->gm, h->Q, h->ldq, h->tmp, n, h->st));
+  BodyData bdy = h->body();
+  HIPCHK(apply_pinv(bdy, h->tmp, x, nullptr, h->st));
+  double* snap = h->hctl + 2 * CTL;
+  HIPCHK(hipMemcpyAsync(snap, h->gm_ctl, CTL * sizeof(double), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  const int* ic = reinterpret_cast<const int*>(snap + 8);
+  if (ic[GM_STATUS] == 0) return fail(FCM_ERR_STATE, "GMRES control did not terminate");
+  out.status = ic[GM_STATUS] - 1;
+  out.iters = out.status == FCM_GMRES_ZERO_RHS ? 0 : ic[GM_NCOLS];
+  out.rel = out.status == FCM_GMRES_ZERO_RHS ? 0.0 : snap[4];
+  if (hist && out.iters > 0)
+    HIPCHK(hipMemcpy(hist, h->gm.hist, (size_t)out.iters * sizeof(double), hipMemcpyDeviceToHost));
+  return FCM_OK;
 }
 
 int solve_dev(fcm_handle h, const gmres_opts* o, gmres_report* rep, double* hist) {
@@ -544,7 +466,7 @@ int solve_dev(fcm_handle h, const gmres_opts* o, gmres_report* rep, double* hist
   InnerResult ir;
   int rc;
   if (prec != FCM_PREC_FP32) {
-    rc = gmres_core(h, h->rhs, h->sol, tol, max_iter, FCM_PREC_FP64, ir, hist);
+    rc = gmres_device(h, h->rhs, h->sol, tol, max_iter, FCM_PREC_FP64, ir, hist);
     if (rc) return rc;
     r.status = ir.status;
     r.iterations = ir.iters;
@@ -566,7 +488,7 @@ int solve_dev(fcm_handle h, const gmres_opts* o, gmres_report* rep, double* hist
       for (int k = 0; k < max_ref; ++k) {
         double itol = (o && o->inner_tol > 0) ? o->inner_tol : std::max(0.5 * tol * bn / rn, 1e-5);
         itol = std::min(itol, 0.5);
-        rc = gmres_core(h, h->res, h->sol, itol, max_iter - r.iterations, FCM_PREC_FP32, ir,
+        rc = gmres_device(h, h->res, h->sol, itol, max_iter - r.iterations, FCM_PREC_FP32, ir,
                         hist ? hist + r.iterations : nullptr);
         if (rc) return rc;
         r.iterations += ir.iters;
@@ -769,7 +691,35 @@ int fcm_create(const fcm_config* cfg, fcm_handle* out) {
   if (e == hipSuccess) h->kw.partial = dalloc<double>(h, (size_t)h->kw.nblk * (2 * (size_t)cfg->max_krylov + 6), e);
   if (e == hipSuccess) h->red = dalloc<double>(h, 2 * (size_t)cfg->max_krylov + 16, e);
   if (e == hipSuccess) h->coef = dalloc<double>(h, 2 * (size_t)cfg->max_krylov + 16, e);
+  {
+    const int M = cfg->max_krylov;
+    const size_t m2 = (size_t)(M + 2) * (M + 2);
+    GmDev& G = h->gm;
+    G.M = M;
+    if (e == hipSuccess) h->gm_ctl = dalloc<double>(h, 16, e);
+    G.dctl = h->gm_ctl;
+    G.ictl = reinterpret_cast<int*>(h->gm_ctl + 8);
+    if (e == hipSuccess) G.Hraw = dalloc<double>(h, m2, e);
+    if (e == hipSuccess) G.R = dalloc<double>(h, m2, e);
+    if (e == hipSuccess) G.CS = dalloc<double>(h, 8 * (size_t)(M + 2), e);
+    if (e == hipSuccess) {
+      G.SN = G.CS + (M + 2);
+      G.beta = G.SN + (M + 2);
+      G.hprov = G.beta + (M + 2);
+      G.hnew = G.hprov + (M + 2);
+      G.hist = G.hnew + (M + 2);
+      G.y = G.hist + (M + 2);
+    }
+    G.coef = h->coef;
+    G.red = h->red;
+    if (e == hipSuccess) h->w = dalloc<double>(h, s.ldq, e);
+  }
   if (e != hipSuccess) return cleanup_fail(FCM_ERR_NOMEM, "device allocation failed");
+  if (hipHostMalloc((void**)&h->hctl, 3 * 16 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+    return cleanup_fail(FCM_ERR_NOMEM, "pinned host allocation failed");
+  if (hipEventCreateWithFlags(&h->pev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->pev[1], hipEventDisableTiming) != hipSuccess)
+    return cleanup_fail(FCM_ERR_HIP, "event");
   for (int i = 0; i < 3; ++i)
     if (zero_field64(h, h->f64[i])) return cleanup_fail(FCM_ERR_HIP, "memset");
   if (hipMemsetAsync(h->Vrand, 0, 3 * (size_t)nb * sizeof(double), h->st) != hipSuccess ||
@@ -783,7 +733,12 @@ int fcm_create(const fcm_config* cfg, fcm_handle* out) {
 int fcm_destroy(fcm_handle h) {
   if (!h) return FCM_OK;
   if (h->st) (void)hipStreamSynchronize(h->st);
+  for (auto& gc : h->graph)
+    if (gc.exec) (void)hipGraphExecDestroy(gc.exec);
   for (void* p : h->allocs) (void)hipFree(p);
+  if (h->hctl) (void)hipHostFree(h->hctl);
+  for (auto& ev : h->pev)
+    if (ev) (void)hipEventDestroy(ev);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->st) (void)hipStreamDestroy(h->st);

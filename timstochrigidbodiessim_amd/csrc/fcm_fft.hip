@@ -78,6 +78,7 @@ template <typename T, int M, bool GEN>
 __global__ void __launch_bounds__(256) zfwd_kernel(FieldPtrs<T> fp, int rows, int cpitch,
                                                    const c2<T>* __restrict__ tw, const c2<T>* __restrict__ twz,
                                                    StressGen sg, int nz_total_nodes_hi, int nz_total_nodes_lo) {
+  if (fp.gate && *fp.gate) return;
   using V = c2<T>;
   using SH = RowShape<T, M>;
   constexpr FftPlan PL = make_plan(M);
@@ -130,6 +131,7 @@ __global__ void __launch_bounds__(256) zfwd_kernel(FieldPtrs<T> fp, int rows, in
 template <typename T, int M>
 __global__ void __launch_bounds__(256) zinv_kernel(FieldPtrs<T> fp, int rows, int cpitch,
                                                    const c2<T>* __restrict__ tw, const c2<T>* __restrict__ twz) {
+  if (fp.gate && *fp.gate) return;
   using V = c2<T>;
   using SH = RowShape<T, M>;
   constexpr FftPlan PL = make_plan(M);
@@ -174,6 +176,7 @@ __global__ void __launch_bounds__(256) zinv_kernel(FieldPtrs<T> fp, int rows, in
 template <typename T, int N, int S>
 __global__ void __launch_bounds__(256) cols_kernel(FieldPtrs<T> fp, size_t plane_stride, size_t estride, int nzc,
                                                    const c2<T>* __restrict__ tw) {
+  if (fp.gate && *fp.gate) return;
   using V = c2<T>;
   using SH = ColShape<T, N>;
   constexpr FftPlan PL = make_plan(N);
@@ -231,6 +234,7 @@ template <typename T> struct XPM { static constexpr int value = sizeof(T) == 8 ?
 template <typename T, int N>
 __global__ void __launch_bounds__(256) xproj_kernel(FieldPtrs<T> fp, int ny, int cpitch, int nzc,
                                                     ProjParams pp, const c2<T>* __restrict__ tw) {
+  if (fp.gate && *fp.gate) return;
   using V = c2<T>;
   constexpr int PM = XPM<T>::value;
   using SH = ColShape<T, N, PM>;
@@ -282,6 +286,7 @@ __global__ void __launch_bounds__(256) xproj_kernel(FieldPtrs<T> fp, int ny, int
 template <typename T, int N>
 __global__ void __launch_bounds__(256) xstress_kernel(FieldPtrs<T> fp, int ny, int cpitch, int nzc,
                                                       ProjParams pp, const c2<T>* __restrict__ tw) {
+  if (fp.gate && *fp.gate) return;
   using V = c2<T>;
   constexpr int PM = XPM<T>::value;
   using SH = ColShape<T, N, PM>;

@@ -1,0 +1,490 @@
+// fcm_gmres.hip -- device-resident control of the GMRES saddle-point solve.
+//
+// gmres_solver.cpp:148-306 runs the Arnoldi loop, the Givens rotations and
+// the stopping test on the host.  Here every scalar of that loop lives in
+// device memory (GmDev) and is updated by small single-workgroup kernels, so
+// one Arnoldi iteration is a fixed sequence of launches that reads nothing
+// back: it can be captured once into a hipGraph and replayed.  Once the
+// stopping test fires, the status word is set and every later kernel of the
+// sequence returns at once, so the host only polls the status every few
+// iterations.
+//
+// The basis is built by classical Gram-Schmidt with delayed
+// reorthogonalisation (DCGS2, Swirydowicz et al., Numer. Linear Algebra
+// Appl. 2021): per iteration one pass over the basis computes the
+// projections of the new vector and of the previous, once-orthogonalised
+// vector; a second pass finalises the previous vector and updates the new
+// one.  The Hessenberg column of the previous step is corrected with the
+// reorthogonalisation coefficients and only then committed to the Givens
+// least squares, whose algebra (including the sign convention of the
+// rotations) is the reference's.
+#include "fcm_internal.hpp"
+
+namespace fcm {
+
+static constexpr int GT = 256;   // threads of the control kernels
+static constexpr int KRT = 256;  // threads of the basis kernels
+static constexpr int KRP = 4;    // rows per thread
+static constexpr int KRB = KRT * KRP;
+
+__device__ __forceinline__ double* Hp(const GmDev& G, int i, int j) { return G.Hraw + (size_t)i * (G.M + 2) + j; }
+__device__ __forceinline__ double* Rp(const GmDev& G, int i, int j) { return G.R + (size_t)i * (G.M + 2) + j; }
+
+// block sum over GT threads (fixed order)
+__device__ double block_sum(double v, double* sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wv] = v;
+  __syncthreads();
+  const double r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  __syncthreads();
+  return r;
+}
+
+// Apply the previous rotations and a new one to the unrotated column
+// col[0..k+1] (gmres_solver.cpp:214-252).  commit: store the rotated column,
+// the rotation and beta.  Returns |beta_{k+1}| / ||b||.  One thread.
+__device__ double givens_column(const GmDev& G, int k, double* col, bool commit) {
+  for (int i = 1; i <= k; ++i) {
+    const double temp = G.CS[i - 1] * col[i - 1] + G.SN[i - 1] * col[i];
+    col[i] = G.CS[i - 1] * col[i] - G.SN[i - 1] * col[i - 1];
+    col[i - 1] = temp;
+  }
+  double cs, sn;
+  if (col[k + 1] == 0.0) {
+    cs = 1.0;
+    sn = 0.0;
+  } else if (fabs(col[k + 1]) > fabs(col[k])) {
+    const double temp = col[k] / col[k + 1];
+    sn = (2.0 * double(col[k + 1] > 0) - 1.0) / sqrt(1.0 + temp * temp);
+    cs = temp * sn;
+  } else {
+    const double temp = col[k + 1] / col[k];
+    cs = (2.0 * double(col[k] > 0) - 1.0) / sqrt(1.0 + temp * temp);
+    sn = temp * cs;
+  }
+  const double b1 = -sn * G.beta[k];
+  if (commit) {
+    col[k] = cs * col[k] + sn * col[k + 1];
+    for (int i = 0; i <= k; ++i) *Rp(G, i, k) = col[i];
+    G.CS[k] = cs;
+    G.SN[k] = sn;
+    G.beta[k + 1] = b1;
+    G.beta[k] = cs * G.beta[k];
+  }
+  return fabs(b1) / G.dctl[1];
+}
+
+__device__ __forceinline__ void set_done(const GmDev& G, int status, int ncols, double rel) {
+  G.ictl[GM_STATUS] = 1 + status;
+  G.ictl[GM_NCOLS] = ncols;
+  G.dctl[4] = rel;
+}
+
+// ---- set-up -----------------------------------------------------------------
+// red[0] = ||b||^2 from a preceding reduction
+__global__ void __launch_bounds__(GT) gm_start_kernel(GmDev G, double tol, int max_iter) {
+  const size_t nh = (size_t)(G.M + 2) * (G.M + 2);
+  for (size_t i = threadIdx.x; i < nh; i += GT) G.Hraw[i] = 0.0;
+  if (threadIdx.x == 0) {
+    const double nrm = sqrt(G.red[0]);
+    G.dctl[0] = tol;
+    G.dctl[1] = nrm;
+    G.dctl[2] = 0.0;
+    G.dctl[3] = 1.0;
+    G.dctl[4] = 1.0;
+    G.ictl[GM_K] = 0;
+    G.ictl[GM_PROV] = 0;
+    G.ictl[GM_FIN] = 0;
+    G.ictl[GM_MAXIT] = max_iter;
+    G.ictl[GM_NCOLS] = 0;
+    G.ictl[GM_STATUS] = nrm == 0.0 ? 1 + 2 /* zero rhs */ : 0;
+    G.beta[0] = nrm;
+  }
+}
+
+// Q col 0 = b / ||b||
+__global__ void gm_q0_kernel(GmDev G, const double* __restrict__ b, double* __restrict__ Q, size_t n) {
+  if (G.ictl[GM_STATUS] != 0) return;
+  const double s = 1.0 / G.dctl[1];
+  for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (size_t)gridDim.x * blockDim.x)
+    Q[r] = b[r] * s;
+}
+
+// ---- basis passes ------------------------------------------------------------
+// mode 0 (main): columns 0..k against v1 = Q col k (if provisional) and v2 = w
+// mode 1 (finalise): columns 0..k+1 against v1 = Q col k+1 (only if fin)
+__global__ void __launch_bounds__(KRT) gm_dots_kernel(GmDev G, const double* __restrict__ Q, size_t ldq,
+                                                      const double* __restrict__ wbuf, size_t n, int mode,
+                                                      double* __restrict__ partial, int nblk) {
+  const int status = G.ictl[GM_STATUS];
+  if (status != 0 || (mode == 1 && G.ictl[GM_FIN] == 0)) return;
+  const int k = G.ictl[GM_K];
+  const bool prov = G.ictl[GM_PROV] != 0;
+  const int ncols = mode == 0 ? k + 1 : k + 2;
+  const double* v1 = mode == 0 ? (prov ? Q + (size_t)k * ldq : nullptr) : Q + (size_t)(k + 1) * ldq;
+  const double* v2 = mode == 0 ? wbuf : nullptr;
+  __shared__ double red[4 * 8];
+  const size_t r0 = (size_t)blockIdx.x * KRB + threadIdx.x;
+  double a[KRP], b[KRP];
+#pragma unroll
+  for (int q = 0; q < KRP; ++q) {
+    const size_t r = r0 + (size_t)q * KRT;
+    a[q] = (v1 && r < n) ? v1[r] : 0.0;
+    b[q] = (v2 && r < n) ? v2[r] : 0.0;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int c0 = 0; c0 < ncols; c0 += 4) {
+    double acc[8];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      acc[c] = 0.0;
+      acc[4 + c] = 0.0;
+      if (c0 + c < ncols) {
+        const double* q = Q + (size_t)(c0 + c) * ldq;
+#pragma unroll
+        for (int qq = 0; qq < KRP; ++qq) {
+          const size_t r = r0 + (size_t)qq * KRT;
+          if (r < n) {
+            const double qv = q[r];
+            acc[c] += qv * a[qq];
+            acc[4 + c] += qv * b[qq];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[c] = wave_sum(acc[c]);
+    if (lane == 0)
+#pragma unroll
+      for (int c = 0; c < 8; ++c) red[wv * 8 + c] = acc[c];
+    __syncthreads();
+    if (threadIdx.x < 8) {
+      const int c = threadIdx.x & 3, which = threadIdx.x >> 2;
+      const double v = (red[threadIdx.x] + red[8 + threadIdx.x]) + (red[16 + threadIdx.x] + red[24 + threadIdx.x]);
+      if (c0 + c < ncols) partial[((size_t)which * ncols + c0 + c) * nblk + blockIdx.x] = v;
+    }
+    __syncthreads();
+  }
+}
+
+// red[c] = sum_blk partial[c][blk] for the rows the dots pass wrote; fixed order
+__global__ void __launch_bounds__(256) gm_reduce_kernel(GmDev G, const double* __restrict__ partial, int nblk,
+                                                        int mode) {
+  const int status = G.ictl[GM_STATUS];
+  if (status != 0) return;
+  if (mode == 1 && G.ictl[GM_FIN] == 0) return;
+  const int k = G.ictl[GM_K];
+  // mode 0: 2 (k+1) rows; mode 1: k+2 rows; mode 2: the single norm row
+  const int rows = mode == 0 ? 2 * (k + 1) : (mode == 1 ? k + 2 : 1);
+  __shared__ double s[256];
+  for (int c = blockIdx.x; c < rows; c += gridDim.x) {
+    const double* p = partial + (size_t)c * nblk;
+    double a = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += 256) a += p[i];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+      if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) G.red[c] = s[0];
+    __syncthreads();
+  }
+}
+
+// mode 0: q = (Q col k - sum_{i<m} s_i Q_i) / nu written back (if m >= 0 and
+// provisional), w <- w - sum_{i<mt} t_i Q_i - pk q, partial ||w||^2.
+// mode 1: Q col k+1 <- (Q col k+1 - sum_{i<=k} s_i Q_i) / nu (if fin).
+__global__ void __launch_bounds__(KRT) gm_update_kernel(GmDev G, double* __restrict__ Q, size_t ldq,
+                                                        double* __restrict__ wbuf, size_t n, int mode,
+                                                        double* __restrict__ partial) {
+  const int status = G.ictl[GM_STATUS];
+  if (status != 0 || (mode == 1 && G.ictl[GM_FIN] == 0)) return;
+  const int k = G.ictl[GM_K];
+  const int m = G.ictl[GM_M], mt = G.ictl[GM_MT];
+  double* qt = mode == 0 ? (m > 0 ? Q + (size_t)k * ldq : nullptr) : Q + (size_t)(k + 1) * ldq;
+  double* w = mode == 0 ? wbuf : nullptr;
+  __shared__ double cs[2048 + 4];
+  __shared__ double red[4];
+  const int ncoef = m + mt + 2;
+  for (int i = threadIdx.x; i < ncoef && i < 2048 + 4; i += KRT) cs[i] = G.coef[i];
+  __syncthreads();
+  auto S = [&](int i) { return i < 2048 + 4 ? cs[i] : G.coef[i]; };
+  const double inv_nu = S(m + mt), pk = S(m + mt + 1);
+  const size_t r0 = (size_t)blockIdx.x * KRB + threadIdx.x;
+  double as[KRP], at[KRP];
+#pragma unroll
+  for (int q = 0; q < KRP; ++q) as[q] = at[q] = 0.0;
+  const int mm = m > mt ? m : mt;
+  for (int c = 0; c < mm; ++c) {
+    const double sv = (qt && c < m) ? S(c) : 0.0;
+    const double tv = (w && c < mt) ? S(m + c) : 0.0;
+    const double* q = Q + (size_t)c * ldq;
+#pragma unroll
+    for (int qq = 0; qq < KRP; ++qq) {
+      const size_t r = r0 + (size_t)qq * KRT;
+      if (r < n) {
+        const double qv = q[r];
+        as[qq] += sv * qv;
+        at[qq] += tv * qv;
+      }
+    }
+  }
+  double acc = 0.0;
+#pragma unroll
+  for (int qq = 0; qq < KRP; ++qq) {
+    const size_t r = r0 + (size_t)qq * KRT;
+    if (r < n) {
+      double qk = 0.0;
+      if (qt) {
+        qk = (qt[r] - as[qq]) * inv_nu;
+        qt[r] = qk;
+      }
+      if (w) {
+        const double v = w[r] - at[qq] - pk * qk;
+        w[r] = v;
+        acc += v * v;
+      }
+    }
+  }
+  if (mode == 0) {
+    acc = wave_sum(acc);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) red[wv] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+  }
+}
+
+// ---- control kernels (one workgroup) --------------------------------------------
+// after the main dots pass
+__global__ void __launch_bounds__(GT) gm_ctl1_kernel(GmDev G) {
+  if (G.ictl[GM_STATUS] != 0) return;
+  __shared__ double sh[8];
+  __shared__ double col[2048 + 4];
+  __shared__ int s_done;
+  const int k = G.ictl[GM_K];
+  const bool prov = G.ictl[GM_PROV] != 0;
+  const int ncols = k + 1;
+  const double* s = G.red;           // Q.v1, k+1 values (s_0..s_{k-1}, alpha)
+  const double* t = G.red + ncols;   // Q.v2, k+1 values
+  if (prov) {
+    double ss = 0.0, st = 0.0;
+    for (int i = threadIdx.x; i < k; i += GT) {
+      ss += s[i] * s[i];
+      st += s[i] * t[i];
+    }
+    ss = block_sum(ss, sh);
+    st = block_sum(st, sh);
+    const double alpha = s[k], tau = t[k];
+    const double nu = sqrt(fmax(alpha - ss, 0.0));
+    const double eta = G.dctl[2];
+    // finalise column k-1 of H
+    for (int i = threadIdx.x; i < k; i += GT) *Hp(G, i, k - 1) = G.hprov[i] + eta * s[i];
+    if (threadIdx.x == 0) *Hp(G, k, k - 1) = eta * nu;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int i = 0; i <= k; ++i) col[i] = *Hp(G, i, k - 1);
+      const double rel = givens_column(G, k - 1, col, true);
+      G.hist[k - 1] = rel;
+      s_done = 0;
+      if (rel < G.dctl[0] || k == G.ictl[GM_MAXIT]) {
+        set_done(G, rel < G.dctl[0] ? 0 : 1, k, rel);
+        s_done = 1;
+      } else if (!(nu > 0.0)) {
+        set_done(G, 3, k, rel);
+        s_done = 1;
+      }
+      G.dctl[3] = nu;
+    }
+    __syncthreads();
+    if (s_done) return;
+    // projections of A P^-1 q_k on q_0..q_k, corrected for the provisional input
+    for (int l = threadIdx.x; l <= k; l += GT) {
+      const double proj = l < k ? t[l] : (tau - st) / nu;
+      double c = 0.0;
+      for (int i = (l > 0 ? l - 1 : 0); i < k; ++i) c += *Hp(G, l, i) * s[i];
+      G.hnew[l] = (proj - c) / nu;
+      if (l < k) {
+        G.coef[l] = s[l];
+        G.coef[k + l] = t[l];
+      } else {
+        G.coef[2 * k] = 1.0 / nu;
+        G.coef[2 * k + 1] = proj;
+      }
+    }
+    if (threadIdx.x == 0) {
+      G.ictl[GM_M] = k;
+      G.ictl[GM_MT] = k;
+    }
+  } else {
+    for (int l = threadIdx.x; l <= k; l += GT) {
+      G.hnew[l] = t[l];
+      G.coef[l] = t[l];
+    }
+    if (threadIdx.x == 0) {
+      G.coef[k + 1] = 1.0;
+      G.coef[k + 2] = 0.0;
+      G.ictl[GM_M] = 0;
+      G.ictl[GM_MT] = k + 1;
+      G.dctl[3] = 1.0;
+    }
+  }
+}
+
+// after the main update: normalisation, breakdown, predicted convergence
+__global__ void __launch_bounds__(GT) gm_ctl2_kernel(GmDev G) {
+  if (G.ictl[GM_STATUS] != 0) return;
+  __shared__ double col[2048 + 4];
+  const int k = G.ictl[GM_K];
+  const double etar = sqrt(G.red[0]);
+  const double eta = etar / G.dctl[3];
+  for (int l = threadIdx.x; l <= k; l += GT) G.hprov[l] = G.hnew[l];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    G.dctl[2] = eta;
+    if (!(etar > 0.0)) {
+      for (int i = 0; i <= k; ++i) *Hp(G, i, k) = G.hprov[i];
+      *Hp(G, k + 1, k) = 0.0;
+      for (int i = 0; i <= k + 1; ++i) col[i] = *Hp(G, i, k);
+      const double rel = givens_column(G, k, col, true);
+      G.hist[k] = rel;
+      set_done(G, 3, k + 1, rel);
+      return;
+    }
+    G.dctl[5] = 1.0 / etar;
+    for (int i = 0; i <= k; ++i) col[i] = G.hprov[i];
+    col[k + 1] = eta;
+    const double est = givens_column(G, k, col, false);
+    G.ictl[GM_FIN] = (est < G.dctl[0] || k + 1 == G.ictl[GM_MAXIT]) ? 1 : 0;
+  }
+}
+
+// Q col k+1 = w / ||w||
+__global__ void gm_scale_kernel(GmDev G, const double* __restrict__ wbuf, double* __restrict__ Q, size_t ldq,
+                                size_t n) {
+  if (G.ictl[GM_STATUS] != 0) return;
+  const int k = G.ictl[GM_K];
+  const double s = G.dctl[5];
+  double* q = Q + (size_t)(k + 1) * ldq;
+  for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (size_t)gridDim.x * blockDim.x)
+    q[r] = wbuf[r] * s;
+}
+
+// after the finalising dots pass
+__global__ void __launch_bounds__(GT) gm_ctl3_kernel(GmDev G) {
+  if (G.ictl[GM_STATUS] != 0 || G.ictl[GM_FIN] == 0) return;
+  __shared__ double sh[8];
+  __shared__ double col[2048 + 4];
+  __shared__ int s_done;
+  const int k = G.ictl[GM_K];
+  const double* s = G.red;  // k+2 values: s_0..s_k, alpha
+  double ss = 0.0;
+  for (int i = threadIdx.x; i <= k; i += GT) ss += s[i] * s[i];
+  ss = block_sum(ss, sh);
+  const double nu2 = sqrt(fmax(s[k + 1] - ss, 0.0));
+  const double eta = G.dctl[2];
+  for (int i = threadIdx.x; i <= k; i += GT) *Hp(G, i, k) = G.hprov[i] + eta * s[i];
+  if (threadIdx.x == 0) *Hp(G, k + 1, k) = eta * nu2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i <= k + 1; ++i) col[i] = *Hp(G, i, k);
+    const double rel = givens_column(G, k, col, true);
+    G.hist[k] = rel;
+    s_done = 0;
+    if (rel < G.dctl[0] || k + 1 == G.ictl[GM_MAXIT]) {
+      set_done(G, rel < G.dctl[0] ? 0 : 1, k + 1, rel);
+      s_done = 1;
+    } else if (!(nu2 > 0.0)) {
+      set_done(G, 3, k + 1, rel);
+      s_done = 1;
+    }
+  }
+  __syncthreads();
+  if (s_done) return;
+  for (int i = threadIdx.x; i <= k; i += GT) G.coef[i] = s[i];
+  if (threadIdx.x == 0) {
+    G.coef[k + 1] = 1.0 / nu2;
+    G.coef[k + 2] = 0.0;
+    G.ictl[GM_M] = k + 1;
+    G.ictl[GM_MT] = 0;
+  }
+}
+
+// end of an iteration
+__global__ void gm_end_kernel(GmDev G) {
+  if (G.ictl[GM_STATUS] != 0) return;
+  if (threadIdx.x == 0) {
+    G.ictl[GM_PROV] = G.ictl[GM_FIN] ? 0 : 1;
+    G.ictl[GM_FIN] = 0;
+    G.ictl[GM_K] += 1;
+  }
+}
+
+// back substitution (gmres_solver.cpp:260-276) into G.y; one wavefront
+__global__ void __launch_bounds__(64) gm_backsub_kernel(GmDev G) {
+  const int st = G.ictl[GM_STATUS] - 1;
+  const int nc = G.ictl[GM_NCOLS];
+  if (st == 2 || nc <= 0) return;  // zero right-hand side: nothing to solve
+  __shared__ double ys[2048 + 4];
+  const int lane = threadIdx.x;
+  for (int i = nc - 1; i >= 0; --i) {
+    double a = 0.0;
+    for (int j = i + 1 + lane; j < nc; j += 64) a += *Rp(G, i, j) * ys[j];
+    a = wave_sum(a);
+    if (lane == 0) ys[i] = (G.beta[i] - a) / *Rp(G, i, i);
+    __syncthreads();
+  }
+  for (int j = lane; j < nc; j += 64) G.y[j] = ys[j];
+}
+
+// x = sum_{i < ncols} y_i Q_i (zero for a zero right-hand side)
+__global__ void gm_combine_kernel(GmDev G, const double* __restrict__ Q, size_t ldq, double* __restrict__ x,
+                                  size_t n) {
+  const int st = G.ictl[GM_STATUS] - 1;
+  const int nc = st == 2 ? 0 : G.ictl[GM_NCOLS];
+  for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (size_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int i = 0; i < nc; ++i) s += G.y[i] * Q[(size_t)i * ldq + r];
+    x[r] = s;
+  }
+}
+
+// ---- host-side launch helpers -------------------------------------------------------
+int gm_blocks(size_t n) { return (int)((n + KRB - 1) / KRB); }
+
+hipError_t gm_launch_start(const GmDev& G, double tol, int max_iter, const double* b, double* Q, size_t n,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(gm_start_kernel, dim3(1), dim3(GT), 0, st, G, tol, max_iter);
+  hipLaunchKernelGGL(gm_q0_kernel, dim3(1024), dim3(256), 0, st, G, b, Q, n);
+  return hipGetLastError();
+}
+
+hipError_t gm_launch_basis(const GmDev& G, double* Q, size_t ldq, double* wbuf, size_t n, double* partial,
+                           int nblk, hipStream_t st) {
+  // main dots, reduce, control 1, update, norm reduce, control 2, scale
+  hipLaunchKernelGGL(gm_dots_kernel, dim3(nblk), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 0, partial, nblk);
+  hipLaunchKernelGGL(gm_reduce_kernel, dim3(256), dim3(256), 0, st, G, partial, nblk, 0);
+  hipLaunchKernelGGL(gm_ctl1_kernel, dim3(1), dim3(GT), 0, st, G);
+  hipLaunchKernelGGL(gm_update_kernel, dim3(nblk), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 0, partial);
+  hipLaunchKernelGGL(gm_reduce_kernel, dim3(1), dim3(256), 0, st, G, partial, nblk, 2);
+  hipLaunchKernelGGL(gm_ctl2_kernel, dim3(1), dim3(GT), 0, st, G);
+  hipLaunchKernelGGL(gm_scale_kernel, dim3(1024), dim3(256), 0, st, G, wbuf, Q, ldq, n);
+  // finalisation of the new vector when convergence is predicted
+  hipLaunchKernelGGL(gm_dots_kernel, dim3(nblk), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 1, partial, nblk);
+  hipLaunchKernelGGL(gm_reduce_kernel, dim3(256), dim3(256), 0, st, G, partial, nblk, 1);
+  hipLaunchKernelGGL(gm_ctl3_kernel, dim3(1), dim3(GT), 0, st, G);
+  hipLaunchKernelGGL(gm_update_kernel, dim3(nblk), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 1, partial);
+  hipLaunchKernelGGL(gm_end_kernel, dim3(1), dim3(64), 0, st, G);
+  return hipGetLastError();
+}
+
+hipError_t gm_launch_finish(const GmDev& G, const double* Q, size_t ldq, double* x, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(gm_backsub_kernel, dim3(1), dim3(64), 0, st, G);
+  hipLaunchKernelGGL(gm_combine_kernel, dim3(2048), dim3(256), 0, st, G, Q, ldq, x, n);
+  return hipGetLastError();
+}
+
+}  // namespace fcm

@@ -162,6 +162,7 @@ struct SpreadArgs {
 
 template <typename T>
 __global__ void __launch_bounds__(256) spread_kernel(SpreadArgs A, FieldPtrs<T> fp) {
+  if (fp.gate && *fp.gate) return;
   const GridGeom& g = A.g;
   const BlobBins& bb = A.bb;
   const int ng = A.ng, omin = 1 - ng / 2, omax = ng - ng / 2;
@@ -448,6 +449,7 @@ struct SpreadListArgs {
 
 template <typename T>
 __global__ void __launch_bounds__(256) spread_fast_kernel(SpreadListArgs A, FieldPtrs<T> fp) {
+  if (fp.gate && *fp.gate) return;
   const GridGeom& g = A.g;
   const int ng = A.ng, omin = 1 - ng / 2, NW = A.NW;
   const int TZ = 8 * NW;
@@ -645,6 +647,7 @@ __device__ __forceinline__ void write_velocity(const InterpArgs& A, int id, doub
 // one bin per workgroup; node box of (bin + ng - 1) per axis staged in LDS
 template <typename T>
 __global__ void __launch_bounds__(256) interp_box_kernel(InterpArgs A, FieldPtrs<T> fp) {
+  if (fp.gate && *fp.gate) return;
   extern __shared__ unsigned char smem_raw[];
   T* box = reinterpret_cast<T*>(smem_raw);
   const GridGeom& g = A.g;
@@ -703,6 +706,7 @@ __global__ void __launch_bounds__(256) interp_box_kernel(InterpArgs A, FieldPtrs
 // blob gathers straight from the grid
 template <typename T>
 __global__ void __launch_bounds__(256) interp_gather_kernel(InterpArgs A, FieldPtrs<T> fp, int nb) {
+  if (fp.gate && *fp.gate) return;
   const GridGeom& g = A.g;
   const int ng = A.ng, omin = 1 - ng / 2;
   const int s = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -747,6 +751,7 @@ template <typename T> struct IfNc { static constexpr int value = sizeof(T) == 4 
 
 template <typename T, int NG>
 __global__ void __launch_bounds__(256) interp_fast_kernel(InterpArgs A, FieldPtrs<T> fp, int IZB) {
+  if (fp.gate && *fp.gate) return;
   constexpr int NC = IfNc<T>::value;
   constexpr int OMIN = 1 - NG / 2;
   constexpr int HX = 8 + NG - 1, HY = 8 + NG - 1;

@@ -14,7 +14,12 @@ struct GridGeom {
   size_t field_words() const { return (size_t)nx * ny * cpitch; }  // complex words per field
 };
 
-template <typename T> struct FieldPtrs { c2<T>* p[6]; };
+// grid fields handed to a kernel; if gate is set and *gate != 0 the kernel
+// returns at once (device-controlled iterations after convergence)
+template <typename T> struct FieldPtrs {
+  c2<T>* p[6];
+  const int* gate;
+};
 
 // device twiddle tables: x (nx), y (ny), z (nz/2) and the R2C untangling
 // factors zu[k] = exp(-2 pi i k / nz), k = 0..nz/2
@@ -109,6 +114,9 @@ struct BodyData {
   double* Y = nullptr;           // 3 nb wrapped positions
   const double* Minv = nullptr;  // (3 nbpb)^2
   const double* Nref = nullptr;  // 36
+  const int* gate = nullptr;     // P^-1 kernel returns when *gate != 0
+  const int* kcol = nullptr;     // if set, the P^-1 input is column *kcol of a basis with stride ldq
+  size_t ldq = 0;
 };
 
 hipError_t bodies_update(const GridGeom& g, BodyData& b, hipStream_t st);
@@ -144,5 +152,30 @@ hipError_t combine(const double* Q, size_t ldq, int ncols, const double* c, doub
 hipError_t scale_vec(const double* x, double* y, size_t n, const double* a, bool invsqrt, hipStream_t st);
 hipError_t axpby(double a, const double* x, double b, double* y, size_t n, hipStream_t st);  // y = a x + b y
 hipError_t rhs_assemble(const double* Vrand, const double* F6, double* b, size_t n3, size_t n6, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// device-resident GMRES control (fcm_gmres.hip)
+// ---------------------------------------------------------------------------
+enum { GM_K = 0, GM_PROV = 1, GM_STATUS = 2, GM_NCOLS = 3, GM_FIN = 4, GM_MAXIT = 5, GM_M = 6, GM_MT = 7, GM_NINT = 8 };
+
+struct GmDev {
+  int* ictl = nullptr;     // GM_* words; status 0 = running, 1 + FCM_GMRES_* when done
+  double* dctl = nullptr;  // 0 tol, 1 ||b||, 2 eta, 3 nu, 4 last residual, 5 1/||w||
+  double* Hraw = nullptr;  // (M+2)^2 unrotated Hessenberg
+  double* R = nullptr;     // (M+2)^2 rotated
+  double *CS = nullptr, *SN = nullptr, *beta = nullptr, *hprov = nullptr, *hnew = nullptr;  // M+2
+  double* coef = nullptr;  // 2M+8
+  double* red = nullptr;   // 2M+8
+  double* hist = nullptr;  // M+2
+  double* y = nullptr;     // M+2
+  int M = 0;
+};
+
+int gm_blocks(size_t n);
+hipError_t gm_launch_start(const GmDev& G, double tol, int max_iter, const double* b, double* Q, size_t n,
+                           hipStream_t st);
+hipError_t gm_launch_basis(const GmDev& G, double* Q, size_t ldq, double* wbuf, size_t n, double* partial,
+                           int nblk, hipStream_t st);
+hipError_t gm_launch_finish(const GmDev& G, const double* Q, size_t ldq, double* x, size_t n, hipStream_t st);
 
 }  // namespace fcm

@@ -75,6 +75,8 @@ __global__ void __launch_bounds__(128) pinv_kernel(BodyData b, const double* __r
   double* vv = sv + 6;        // 6
   double* Rs = vv + 6;        // 9
   double* ref = Rs + 9;       // n3
+  if (b.gate && *b.gate) return;
+  if (b.kcol) in += (size_t)(*b.kcol) * b.ldq;
   const int n = blockIdx.x;
   const int t = threadIdx.x;
   const size_t nlam = 3 * (size_t)b.NB * b.nbpb;
