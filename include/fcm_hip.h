@@ -166,6 +166,34 @@ int gmres_get_solution(fcm_handle h, double* soln);
 /* produce_random_velocity_field + calculate_random_blob_velocities +
  * assemble_RHS + invert_saddle_point_system (main.cpp:231-233,308-310) */
 int fcm_mobility_solve(fcm_handle h, uint64_t seed, const gmres_opts* opts, gmres_report* rep);
+/* The right-hand side half of fcm_mobility_solve (main.cpp:231-233 then
+ * gmres_solver.cpp:58-70): the seeded fluctuating field, its interpolation
+ * and the assembled RHS, left on the device and NOT waited for, so that it
+ * overlaps with other work; a following fcm_batch_submit is ordered after it. */
+int fcm_mobility_prepare(fcm_handle h, uint64_t seed);
+
+/* ---- batches of independent solves (no counterpart in the reference) ------ */
+/* The reference solves one system per process (main.cpp:308-310).  A batch
+ * advances many handles' solves together: every fcm_batch_step launches one
+ * round of `steps_per_round` Arnoldi steps of every iterating job (one hipGraph
+ * launch whose branches run on `worker_streams` streams; 0 runs them on the
+ * batch's own stream), then reads the control words of the round before and
+ * enqueues the finished jobs' next stage.  A job's result is bitwise the same
+ * as gmres_solve on its own.  Jobs may be submitted at any time (continuous
+ * batching); a handle may hold one job at a time and must not be changed while
+ * its job runs.  Not thread-safe: one host thread per batch. */
+typedef struct fcm_batch_s* fcm_batch;
+int fcm_batch_create(int max_jobs, int worker_streams, int steps_per_round, fcm_batch* out);
+int fcm_batch_destroy(fcm_batch b);
+/* queue a solve of the handle's assembled RHS (gmres_assemble_rhs or
+ * fcm_mobility_prepare); the solution stays on the device (gmres_get_solution) */
+int fcm_batch_submit(fcm_batch b, fcm_handle h, const gmres_opts* opts);
+/* one round; *unfinished = jobs submitted and not yet finished */
+int fcm_batch_step(fcm_batch b, int* unfinished);
+/* move up to max finished jobs out: their handles and reports, in finishing order */
+int fcm_batch_take(fcm_batch b, fcm_handle* handles, gmres_report* reps, int max, int* count);
+/* step until every submitted job has finished */
+int fcm_batch_run(fcm_batch b);
 
 /* ---- measurement hook (no counterpart in the reference) ----------------------- */
 /* Runs one saddle-point mat-vec `reps` times on a fixed input with HIP events

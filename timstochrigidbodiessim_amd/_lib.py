@@ -27,7 +27,10 @@ EXPORTS = [
     "rb_set_geometry", "rb_set_bodies", "rb_set_body_forces", "rb_get_blob_positions",
     "rb_mul_K", "rb_mul_KT",
     "gmres_assemble_rhs", "gmres_saddle_matvec", "gmres_apply_Pinv", "gmres_solve",
-    "gmres_get_solution", "fcm_mobility_solve", "fcm_profile_matvec", "fcm_synth_rods",
+    "gmres_get_solution", "fcm_mobility_solve", "fcm_mobility_prepare",
+    "fcm_batch_create", "fcm_batch_destroy", "fcm_batch_submit", "fcm_batch_step", "fcm_batch_take",
+    "fcm_batch_run",
+    "fcm_profile_matvec", "fcm_synth_rods",
 ]
 
 PASS_NAMES = ["spread", "z_forward", "y_forward", "x_project", "y_inverse", "z_inverse", "interp", "KT"]
@@ -104,6 +107,13 @@ def lib():
         L.gmres_solve.argtypes = [H, _dp, _dp, C.POINTER(GmresOpts), C.POINTER(GmresReport), _dp]
         L.gmres_get_solution.argtypes = [H, _dp]
         L.fcm_mobility_solve.argtypes = [H, C.c_uint64, C.POINTER(GmresOpts), C.POINTER(GmresReport)]
+        L.fcm_mobility_prepare.argtypes = [H, C.c_uint64]
+        L.fcm_batch_create.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(H)]
+        L.fcm_batch_destroy.argtypes = [H]
+        L.fcm_batch_submit.argtypes = [H, H, C.POINTER(GmresOpts)]
+        L.fcm_batch_step.argtypes = [H, C.POINTER(C.c_int)]
+        L.fcm_batch_take.argtypes = [H, C.POINTER(H), C.POINTER(GmresReport), C.c_int, C.POINTER(C.c_int)]
+        L.fcm_batch_run.argtypes = [H]
         L.fcm_profile_matvec.argtypes = [H, C.c_int, C.c_int, _dp]
         L.fcm_synth_rods.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_double, _dp, C.c_int, _dp, _dp,
                                      _dp, _dp, C.POINTER(C.c_int)]
@@ -318,6 +328,10 @@ class FcmSolver:
         _chk(self.L.fcm_mobility_solve(self.h, seed, C.byref(o), C.byref(r)))
         return r.as_dict()
 
+    def mobility_prepare(self, seed):
+        """Seeded fluctuating field + RHS on the device, not waited for."""
+        _chk(self.L.fcm_mobility_prepare(self.h, seed))
+
     def profile_matvec(self, precision=FCM_PREC_FP64, reps=10):
         ms = np.zeros(8)
         _chk(self.L.fcm_profile_matvec(self.h, precision, reps, _ptr(ms)))
@@ -327,3 +341,58 @@ class FcmSolver:
         s = np.zeros(self.n)
         _chk(self.L.gmres_get_solution(self.h, _ptr(s)))
         return s
+
+
+class FcmBatch:
+    """Many independent solves advanced together (fcm_batch_* in the C ABI).
+
+    submit() queues the solve of a solver's assembled RHS; step() launches one
+    round and returns the number of unfinished jobs; take() returns finished
+    (solver, report) pairs in finishing order."""
+
+    def __init__(self, max_jobs=64, worker_streams=4, steps_per_round=4):
+        self.L = lib()
+        self.b = C.c_void_p()
+        _chk(self.L.fcm_batch_create(max_jobs, worker_streams, steps_per_round, C.byref(self.b)))
+        self.max_jobs = max_jobs
+        self._owner = {}
+
+    def close(self):
+        if self.b:
+            self.L.fcm_batch_destroy(self.b)
+            self.b = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, solver, tol=1e-8, max_iter=200, precision=FCM_PREC_FP64, inner_tol=0.0, max_refine=0):
+        o = GmresOpts(tol, max_iter, precision, inner_tol, max_refine)
+        _chk(self.L.fcm_batch_submit(self.b, solver.h, C.byref(o)))
+        self._owner[solver.h.value] = solver
+
+    def step(self):
+        n = C.c_int()
+        _chk(self.L.fcm_batch_step(self.b, C.byref(n)))
+        return n.value
+
+    def take(self):
+        hs = (C.c_void_p * self.max_jobs)()
+        rs = (GmresReport * self.max_jobs)()
+        n = C.c_int()
+        _chk(self.L.fcm_batch_take(self.b, hs, rs, self.max_jobs, C.byref(n)))
+        out = []
+        for i in range(n.value):
+            out.append((self._owner.pop(hs[i], None), rs[i].as_dict()))
+        return out
+
+    def run(self):
+        _chk(self.L.fcm_batch_run(self.b))

@@ -4,8 +4,7 @@
 // a HIP stream (default priority).  Host code here only sequences kernels,
 // moves the small per-iteration scalars of GMRES and runs the Givens/least-
 // squares algebra of gmres_solver.cpp:212-284 on those scalars.
-#include "fcm_internal.hpp"
-#include "../../include/fcm_hip.h"
+#include "fcm_handle.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -20,7 +19,7 @@
 
 using namespace fcm;
 
-namespace {
+namespace fcm_host {
 
 thread_local std::string g_err;
 
@@ -29,11 +28,11 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
-#define HIPCHK(expr)                                                                        \
-  do {                                                                                      \
-    hipError_t e_ = (expr);                                                                 \
-    if (e_ != hipSuccess) return fail(FCM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
+}  // namespace fcm_host
+
+using namespace fcm_host;
+
+namespace {
 
 constexpr double PI = 3.14159265358979323846264338327950288;
 
@@ -46,100 +45,6 @@ int largest_divisor_le(int n, int cap) {
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
 }  // namespace
-
-struct fcm_handle_s {
-  fcm_config cfg{};
-  GridGeom g;
-  BlobBins bb;
-  ProjParams pp{};
-  hipStream_t st = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-
-  c2<double>* f64[3] = {nullptr, nullptr, nullptr};
-  c2<float>* f32[3] = {nullptr, nullptr, nullptr};
-  c2<double>* stress[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-  bool have_urand = false;
-  void* twbuf64 = nullptr;
-  void* twbuf32 = nullptr;
-  TwiddleSet<double> tw64;
-  TwiddleSet<float> tw32;
-
-  // blobs
-  BlobData bd;
-  double* dY = nullptr;
-  bool have_blobs = false;
-
-  // bodies
-  int nbpb = 0, NB = 0;
-  double *d_ref = nullptr, *d_x = nullptr, *d_q = nullptr, *d_R = nullptr, *d_dpos = nullptr;
-  double *d_Minv = nullptr, *d_N = nullptr, *d_F6 = nullptr;
-  bool have_geometry = false, have_bodies = false, have_ref = false, have_forces = false;
-  std::vector<double> h_ref;
-
-  // vectors
-  size_t nmax = 0, ldq = 0;
-  double* Q = nullptr;        // (max_krylov + 1) columns
-  double* z = nullptr;        // preconditioned vector
-  double* rhs = nullptr;
-  double* sol = nullptr;
-  double* xacc = nullptr;
-  double* res = nullptr;
-  double* tmp = nullptr;
-  double* Vrand = nullptr;
-  double* Vtmp = nullptr;
-  double* red = nullptr;      // reductions, 2*(max_krylov+2)+8
-  double* coef = nullptr;     // coefficients of the reorthogonalisation pass
-  KrylovWork kw;
-  // device-controlled GMRES
-  GmDev gm;
-  double* gm_ctl = nullptr;   // 16 doubles: dctl[0..8) then ictl (int[16])
-  double* w = nullptr;        // A P^-1 q_k of the current step (fixed address for graph capture)
-  double* hctl = nullptr;     // pinned host snapshots, 3 x 16 doubles
-  hipEvent_t pev[2] = {nullptr, nullptr};
-  struct StepGraph {
-    hipGraphExec_t exec = nullptr;
-    long long key[3] = {-1, -1, -1};
-  } graph[2];
-  bool have_rhs = false, have_sol = false;
-  std::vector<void*> allocs;
-
-  BodyData body() const {
-    BodyData b;
-    b.NB = NB;
-    b.nbpb = nbpb;
-    b.ref = d_ref;
-    b.x = d_x;
-    b.q = d_q;
-    b.R = d_R;
-    b.dpos = d_dpos;
-    b.Y = dY;
-    b.Minv = d_Minv;
-    b.Nref = d_N;
-    return b;
-  }
-  size_t n3() const { return 3 * (size_t)bd.nb; }
-  size_t nsys() const { return 3 * (size_t)NB * nbpb + 6 * (size_t)NB; }
-  FieldPtrs<double> fp64() const {
-    FieldPtrs<double> p{};
-    for (int i = 0; i < 3; ++i) p.p[i] = f64[i];
-    return p;
-  }
-  FieldPtrs<float> fp32() const {
-    FieldPtrs<float> p{};
-    for (int i = 0; i < 3; ++i) p.p[i] = f32[i];
-    return p;
-  }
-  FieldPtrs<double> fpstress() const {
-    FieldPtrs<double> p{};
-    for (int i = 0; i < 6; ++i) p.p[i] = stress[i];
-    return p;
-  }
-  FieldPtrs<double> fprand() const {
-    FieldPtrs<double> p{};
-    for (int i = 0; i < 3; ++i) p.p[i] = stress[i];
-    return p;
-  }
-};
 
 namespace {
 
@@ -209,7 +114,7 @@ Sizes sizes_of(const fcm_config* c) {
   const size_t nblk = (s.nmax + 1023) / 1024;
   t += nblk * (2 * (size_t)c->max_krylov + 6) * 8 + 2 * (2 * (size_t)c->max_krylov + 16) * 8;
   const size_t m2 = (size_t)(c->max_krylov + 2) * (c->max_krylov + 2);
-  t += (2 * m2 + 8 * (size_t)(c->max_krylov + 2) + 16) * 8 + s.ldq * 8;
+  t += (2 * m2 + 9 * (size_t)(c->max_krylov + 2) + 16) * 8 + s.ldq * 8;
   s.total = t;
   return s;
 }
@@ -274,253 +179,73 @@ int sync_stream(fcm_handle h) {
   return FCM_OK;
 }
 
-// ---------------------------------------------------------------------------
-// mobility product on device vectors: Vout = dx^3 J S Fin (- K zV if given)
-// ---------------------------------------------------------------------------
-int mobility_dev(fcm_handle h, const double* Fin, double* Vout, int precision, const KTerm& kt,
-                 const int* gate = nullptr) {
+}  // namespace
+
+namespace fcm_host {
+
+int mobility_dev(fcm_handle h, const double* Fin, double* Vout, int precision, const KTerm& kt, hipStream_t st,
+                 const int* gate) {
   if (!h->have_blobs) return fail(FCM_ERR_STATE, "no blob positions set");
   if (precision == FCM_PREC_FP32) {
     if (!h->f32[0]) return fail(FCM_ERR_STATE, "FP32 path not allocated (fcm_config.fp32 = 0)");
     FieldPtrs<float> fp = h->fp32();
     fp.gate = gate;
-    HIPCHK(spread<float>(h->g, h->bb, h->bd, Fin, fp, h->st));
-    HIPCHK(stokes_solve<float>(h->g, fp, h->pp, h->tw32, h->st));
-    HIPCHK(interp<float>(h->g, h->bb, h->bd, fp, Vout, kt, h->st));
+    HIPCHK(spread<float>(h->g, h->bb, h->bd, Fin, fp, st));
+    HIPCHK(stokes_solve<float>(h->g, fp, h->pp, h->tw32, st));
+    HIPCHK(interp<float>(h->g, h->bb, h->bd, fp, Vout, kt, st));
   } else {
     FieldPtrs<double> fp = h->fp64();
     fp.gate = gate;
-    HIPCHK(spread<double>(h->g, h->bb, h->bd, Fin, fp, h->st));
-    HIPCHK(stokes_solve<double>(h->g, fp, h->pp, h->tw64, h->st));
-    HIPCHK(interp<double>(h->g, h->bb, h->bd, fp, Vout, kt, h->st));
+    HIPCHK(spread<double>(h->g, h->bb, h->bd, Fin, fp, st));
+    HIPCHK(stokes_solve<double>(h->g, fp, h->pp, h->tw64, st));
+    HIPCHK(interp<double>(h->g, h->bb, h->bd, fp, Vout, kt, st));
   }
   return FCM_OK;
 }
 
-// saddle-point product (gmres_solver.cpp:72-83) on device vectors
-int saddle_dev(fcm_handle h, const double* in, double* out, int precision) {
+int saddle_dev(fcm_handle h, const double* in, double* out, int precision, hipStream_t st) {
   const size_t n3 = h->n3();
   KTerm kt;
   kt.zV = in + n3;
   kt.dpos = h->d_dpos;
   kt.nbpb = h->nbpb;
-  int rc = mobility_dev(h, in, out, precision, kt);
+  int rc = mobility_dev(h, in, out, precision, kt, st);
   if (rc) return rc;
   BodyData b = h->body();
-  HIPCHK(mul_KT(b, in, out + n3, h->st));
-  HIPCHK(axpby(0.0, out + n3, -1.0, out + n3, 6 * (size_t)h->NB, h->st));
+  HIPCHK(mul_KT(b, in, out + n3, st));
+  HIPCHK(axpby(0.0, out + n3, -1.0, out + n3, 6 * (size_t)h->NB, st));
   return FCM_OK;
 }
 
-// w = A P^{-1} q  (one Arnoldi product; the K^T term comes out of the P^{-1} kernel)
-int arnoldi_product(fcm_handle h, const double* q, double* w, int precision) {
-  const size_t n3 = h->n3();
-  BodyData b = h->body();
-  HIPCHK(apply_pinv(b, q, h->z, w + n3, h->st));
-  KTerm kt;
-  kt.zV = h->z + n3;
-  kt.dpos = h->d_dpos;
-  kt.nbpb = h->nbpb;
-  return mobility_dev(h, h->z, w, precision, kt);
-}
-
-double dev_norm(fcm_handle h, const double* v, size_t n, int& rc) {
-  rc = FCM_OK;
-  hipError_t e = dots(v, 0, 1, v, n, h->kw, h->red, h->st);
-  if (e != hipSuccess) { rc = fail(FCM_ERR_HIP, hipGetErrorString(e)); return 0; }
-  double s = 0;
-  e = hipMemcpyAsync(&s, h->red, sizeof(double), hipMemcpyDeviceToHost, h->st);
-  if (e == hipSuccess) e = hipStreamSynchronize(h->st);
-  if (e != hipSuccess) { rc = fail(FCM_ERR_HIP, hipGetErrorString(e)); return 0; }
-  return std::sqrt(s);
-}
-
-struct InnerResult {
-  int status = FCM_GMRES_MAX_ITER;
-  int iters = 0;
-  double rel = 0;
-};
-
 // One Arnoldi step under device control: w = A P^-1 q_k with k read on the
-// device, then the DCGS2 basis update and the Givens step.  Every kernel is
-// gated on the status word, so steps queued after convergence do nothing.
-int launch_arnoldi_step(fcm_handle h, int precision) {
+// device (the K^T term comes out of the P^-1 kernel), then the DCGS2 basis
+// update and the Givens step.  Every kernel is gated on the status word, so
+// steps queued after convergence do nothing.
+int launch_arnoldi_step(fcm_handle h, int precision, hipStream_t st) {
   const size_t n = h->nsys(), n3 = h->n3();
   const int* gate = h->gm.ictl + GM_STATUS;
   BodyData b = h->body();
   b.gate = gate;
   b.kcol = h->gm.ictl + GM_K;
   b.ldq = h->ldq;
-  HIPCHK(apply_pinv(b, h->Q, h->z, h->w + n3, h->st));
+  HIPCHK(apply_pinv(b, h->Q, h->z, h->w + n3, st));
   KTerm kt;
   kt.zV = h->z + n3;
   kt.dpos = h->d_dpos;
   kt.nbpb = h->nbpb;
-  if (int rc = mobility_dev(h, h->z, h->w, precision, kt, gate)) return rc;
-  HIPCHK(gm_launch_basis(h->gm, h->Q, h->ldq, h->w, n, h->kw.partial, gm_blocks(n), h->st));
+  if (int rc = mobility_dev(h, h->z, h->w, precision, kt, st, gate)) return rc;
+  HIPCHK(gm_launch_basis(h->gm, h->Q, h->ldq, h->w, n, h->kw.partial, gm_blocks(n), st));
   return FCM_OK;
 }
 
-bool graphs_enabled() {
-  static const bool on = [] {
-    const char* e = std::getenv("FCM_HIP_GRAPHS");
-    return !(e && e[0] == '0');
-  }();
-  return on;
-}
+}  // namespace fcm_host
 
-// The Arnoldi step captured once into a hipGraph per precision; recaptured
-// when the blob or body counts baked into its kernel arguments change.
-int step_graph(fcm_handle h, int precision, hipGraphExec_t* out) {
-  auto& gc = h->graph[precision == FCM_PREC_FP32 ? 1 : 0];
-  const long long key[3] = {h->bd.nb, h->NB, h->nbpb};
-  if (gc.exec && std::memcmp(gc.key, key, sizeof key) == 0) {
-    *out = gc.exec;
-    return FCM_OK;
-  }
-  if (gc.exec) {
-    (void)hipGraphExecDestroy(gc.exec);
-    gc.exec = nullptr;
-  }
-  HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
-  const int rc = launch_arnoldi_step(h, precision);
-  hipGraph_t graph = nullptr;
-  const hipError_t ec = hipStreamEndCapture(h->st, &graph);
-  if (rc) {
-    if (graph) (void)hipGraphDestroy(graph);
-    return rc;
-  }
-  HIPCHK(ec);
-  const hipError_t ei = hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(graph);
-  HIPCHK(ei);
-  std::memcpy(gc.key, key, sizeof key);
-  *out = gc.exec;
-  return FCM_OK;
-}
-
-// Unrestarted right-preconditioned GMRES, x0 = 0 (gmres_solver.cpp:148-306)
-// on device vectors b -> x.  The Arnoldi loop, the Givens least squares of
-// the reference and the stopping test run on the device (fcm_gmres.hip); the
-// host queues steps in chunks and reads the status word of the chunk before
-// the one it has just queued, so the device never waits for the host.
-int gmres_device(fcm_handle h, const double* b, double* x, double tol, int max_iter_in, int precision,
-                 InnerResult& out, double* hist) {
-  const size_t n = h->nsys();
-  const int max_iter = (int)std::min<size_t>((size_t)std::min(max_iter_in, h->cfg.max_krylov), n);
-  if (max_iter < 1) return fail(FCM_ERR_ARG, "max_iter must be >= 1");
-  if (precision == FCM_PREC_FP32 && !h->f32[0]) return fail(FCM_ERR_STATE, "FP32 path not allocated");
-  out = InnerResult{};
-  HIPCHK(dots(b, 0, 1, b, n, h->kw, h->gm.red, h->st));
-  HIPCHK(gm_launch_start(h->gm, tol, max_iter, b, h->Q, n, h->st));
-  hipGraphExec_t ge = nullptr;
-  if (graphs_enabled())
-    if (int rc = step_graph(h, precision, &ge)) return rc;
-  constexpr int CHUNK = 4;
-  constexpr size_t CTL = 16;  // doubles per control snapshot
-  int launched = 0, slot = 0, pending = -1;
-  while (launched < max_iter) {
-    const int c = std::min(CHUNK, max_iter - launched);
-    for (int i = 0; i < c; ++i) {
-      if (ge) {
-        HIPCHK(hipGraphLaunch(ge, h->st));
-      } else if (int rc = launch_arnoldi_step(h, precision)) {
-        return rc;
-      }
-    }
-    launched += c;
-    HIPCHK(hipMemcpyAsync(h->hctl + slot * CTL, h->gm_ctl, CTL * sizeof(double), hipMemcpyDeviceToHost, h->st));
-    HIPCHK(hipEventRecord(h->pev[slot], h->st));
-    if (pending >= 0) {
-      HIPCHK(hipEventSynchronize(h->pev[pending]));
-      const int* ic = reinterpret_cast<const int*>(h->hctl + pending * CTL + 8);
-      if (ic[GM_STATUS] != 0) break;
-    }
-    pending = slot;
-    slot ^= 1;
-  }
-  HIPCHK(gm_launch_finish(h->gm, h->Q, h->ldq, h->tmp, n, h->st));
-  BodyData bdy = h->body();
-  HIPCHK(apply_pinv(bdy, h->tmp, x, nullptr, h->st));
-  double* snap = h->hctl + 2 * CTL;
-  HIPCHK(hipMemcpyAsync(snap, h->gm_ctl, CTL * sizeof(double), hipMemcpyDeviceToHost, h->st));
-  HIPCHK(hipStreamSynchronize(h->st));
-  const int* ic = reinterpret_cast<const int*>(snap + 8);
-  if (ic[GM_STATUS] == 0) return fail(FCM_ERR_STATE, "GMRES control did not terminate");
-  out.status = ic[GM_STATUS] - 1;
-  out.iters = out.status == FCM_GMRES_ZERO_RHS ? 0 : ic[GM_NCOLS];
-  out.rel = out.status == FCM_GMRES_ZERO_RHS ? 0.0 : snap[4];
-  if (hist && out.iters > 0)
-    HIPCHK(hipMemcpy(hist, h->gm.hist, (size_t)out.iters * sizeof(double), hipMemcpyDeviceToHost));
-  return FCM_OK;
-}
-
-int solve_dev(fcm_handle h, const gmres_opts* o, gmres_report* rep, double* hist) {
-  if (!h->have_bodies || !h->have_ref) return fail(FCM_ERR_STATE, "bodies and reference matrices must be set");
-  const size_t n = h->nsys();
-  gmres_report r{};
-  HIPCHK(hipEventRecord(h->ev0, h->st));
-  const double tol = o ? o->tol : 1e-8;
-  const int max_iter = o ? o->max_iter : h->cfg.max_krylov;
-  const int prec = o ? o->precision : FCM_PREC_FP64;
-  InnerResult ir;
-  int rc;
-  if (prec != FCM_PREC_FP32) {
-    rc = gmres_device(h, h->rhs, h->sol, tol, max_iter, FCM_PREC_FP64, ir, hist);
-    if (rc) return rc;
-    r.status = ir.status;
-    r.iterations = ir.iters;
-    r.rel_residual = ir.rel;
-    r.true_rel_residual = -1.0;
-  } else {
-    // FP32-bulk inner solves, FP64 outer residual (iterative refinement)
-    const double bn = dev_norm(h, h->rhs, n, rc);
-    if (rc) return rc;
-    if (bn == 0.0) {
-      HIPCHK(axpby(0.0, h->rhs, 0.0, h->sol, n, h->st));
-      r.status = FCM_GMRES_ZERO_RHS;
-    } else {
-      const int max_ref = (o && o->max_refine > 0) ? o->max_refine : 8;
-      HIPCHK(axpby(0.0, h->rhs, 0.0, h->xacc, n, h->st));
-      HIPCHK(axpby(1.0, h->rhs, 0.0, h->res, n, h->st));
-      double rn = bn;
-      r.status = FCM_GMRES_MAX_ITER;
-      for (int k = 0; k < max_ref; ++k) {
-        double itol = (o && o->inner_tol > 0) ? o->inner_tol : std::max(0.5 * tol * bn / rn, 1e-5);
-        itol = std::min(itol, 0.5);
-        rc = gmres_device(h, h->res, h->sol, itol, max_iter - r.iterations, FCM_PREC_FP32, ir,
-                        hist ? hist + r.iterations : nullptr);
-        if (rc) return rc;
-        r.iterations += ir.iters;
-        r.refinements = k + 1;
-        r.rel_residual = ir.rel;
-        HIPCHK(axpby(1.0, h->sol, 1.0, h->xacc, n, h->st));
-        rc = saddle_dev(h, h->xacc, h->tmp, FCM_PREC_FP64);
-        if (rc) return rc;
-        HIPCHK(axpby(1.0, h->rhs, 0.0, h->res, n, h->st));
-        HIPCHK(axpby(-1.0, h->tmp, 1.0, h->res, n, h->st));
-        rn = dev_norm(h, h->res, n, rc);
-        if (rc) return rc;
-        r.true_rel_residual = rn / bn;
-        if (rn / bn < tol) { r.status = FCM_GMRES_CONVERGED; break; }
-        if (r.iterations >= max_iter) break;
-      }
-      HIPCHK(axpby(1.0, h->xacc, 0.0, h->sol, n, h->st));
-    }
-  }
-  HIPCHK(hipEventRecord(h->ev1, h->st));
-  HIPCHK(hipEventSynchronize(h->ev1));
-  float ms = 0;
-  HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-  r.seconds = ms * 1e-3;
-  h->have_sol = true;
-  if (rep) *rep = r;
-  return FCM_OK;
-}
+namespace {
 
 int setup_blobs_device(fcm_handle h, int nb) {
   h->bd.nb = nb;
   h->bd.Y = h->dY;
+  ++h->epoch;
   HIPCHK(setup_gaussians(h->g, h->bb, h->bd, h->cfg.a, h->st));
   h->have_blobs = true;
   return FCM_OK;
@@ -713,13 +438,10 @@ int fcm_create(const fcm_config* cfg, fcm_handle* out) {
     G.coef = h->coef;
     G.red = h->red;
     if (e == hipSuccess) h->w = dalloc<double>(h, s.ldq, e);
+    if (e == hipSuccess) h->hist_all = dalloc<double>(h, (size_t)M + 2, e);
   }
   if (e != hipSuccess) return cleanup_fail(FCM_ERR_NOMEM, "device allocation failed");
-  if (hipHostMalloc((void**)&h->hctl, 3 * 16 * sizeof(double), hipHostMallocDefault) != hipSuccess)
-    return cleanup_fail(FCM_ERR_NOMEM, "pinned host allocation failed");
-  if (hipEventCreateWithFlags(&h->pev[0], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&h->pev[1], hipEventDisableTiming) != hipSuccess)
-    return cleanup_fail(FCM_ERR_HIP, "event");
+
   for (int i = 0; i < 3; ++i)
     if (zero_field64(h, h->f64[i])) return cleanup_fail(FCM_ERR_HIP, "memset");
   if (hipMemsetAsync(h->Vrand, 0, 3 * (size_t)nb * sizeof(double), h->st) != hipSuccess ||
@@ -733,12 +455,9 @@ int fcm_create(const fcm_config* cfg, fcm_handle* out) {
 int fcm_destroy(fcm_handle h) {
   if (!h) return FCM_OK;
   if (h->st) (void)hipStreamSynchronize(h->st);
-  for (auto& gc : h->graph)
-    if (gc.exec) (void)hipGraphExecDestroy(gc.exec);
+  batch_forget(h);
+  batch_destroy_solo(h);
   for (void* p : h->allocs) (void)hipFree(p);
-  if (h->hctl) (void)hipHostFree(h->hctl);
-  for (auto& ev : h->pev)
-    if (ev) (void)hipEventDestroy(ev);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->st) (void)hipStreamDestroy(h->st);
@@ -804,7 +523,7 @@ int fcm_mobility(fcm_handle h, const double* F, double* V, int precision) {
   if (!h->have_blobs) return fail(FCM_ERR_STATE, "no blob positions set");
   const size_t n3 = h->n3();
   HIPCHK(hipMemcpyAsync(h->tmp, F, n3 * sizeof(double), hipMemcpyHostToDevice, h->st));
-  int rc = mobility_dev(h, h->tmp, h->Vtmp, precision, KTerm{});
+  int rc = mobility_dev(h, h->tmp, h->Vtmp, precision, KTerm{}, h->st);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(V, h->Vtmp, n3 * sizeof(double), hipMemcpyDeviceToHost, h->st));
   return sync_stream(h);
@@ -925,6 +644,7 @@ int rb_set_geometry(fcm_handle h, int nbpb, const double* blob_ref) {
   h->h_ref.assign(blob_ref, blob_ref + 3 * (size_t)nbpb);
   HIPCHK(hipMemcpyAsync(h->d_ref, blob_ref, 3 * (size_t)nbpb * sizeof(double), hipMemcpyHostToDevice, h->st));
   h->have_geometry = true;
+  ++h->epoch;
   h->have_bodies = false;
   h->have_ref = false;
   return sync_stream(h);
@@ -1017,7 +737,7 @@ int fcm_ref_matrices(fcm_handle h, double* Msingle, double* Minv, double* N) {
     std::fill(e.begin(), e.end(), 0.0);
     e[n] = 1.0;
     HIPCHK(hipMemcpyAsync(h->tmp, e.data(), n3 * sizeof(double), hipMemcpyHostToDevice, h->st));
-    rc = mobility_dev(h, h->tmp, h->Vtmp, FCM_PREC_FP64, KTerm{});
+    rc = mobility_dev(h, h->tmp, h->Vtmp, FCM_PREC_FP64, KTerm{}, h->st);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(col.data(), h->Vtmp, n3 * sizeof(double), hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
@@ -1056,7 +776,7 @@ int gmres_saddle_matvec(fcm_handle h, const double* in, double* out, int precisi
   if (!h->have_bodies) return fail(FCM_ERR_STATE, "rb_set_bodies first");
   const size_t n = h->nsys();
   HIPCHK(hipMemcpyAsync(h->xacc, in, n * sizeof(double), hipMemcpyHostToDevice, h->st));
-  int rc = saddle_dev(h, h->xacc, h->tmp, precision);
+  int rc = saddle_dev(h, h->xacc, h->tmp, precision, h->st);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(out, h->tmp, n * sizeof(double), hipMemcpyDeviceToHost, h->st));
   return sync_stream(h);
@@ -1083,7 +803,7 @@ int gmres_solve(fcm_handle h, const double* rhs, double* soln, const gmres_opts*
     h->have_rhs = true;
   }
   if (!h->have_rhs) return fail(FCM_ERR_STATE, "no right-hand side");
-  int rc = solve_dev(h, opts, rep, resid_hist);
+  int rc = solve_single(h, opts, rep, resid_hist);
   if (rc) return rc;
   if (soln) {
     HIPCHK(hipMemcpyAsync(soln, h->sol, n * sizeof(double), hipMemcpyDeviceToHost, h->st));
@@ -1155,6 +875,11 @@ int fcm_profile_matvec(fcm_handle h, int precision, int reps, double* ms_per_pas
 }
 
 int fcm_mobility_solve(fcm_handle h, uint64_t seed, const gmres_opts* opts, gmres_report* rep) {
+  if (int rc = fcm_mobility_prepare(h, seed)) return rc;
+  return solve_single(h, opts, rep, nullptr);
+}
+
+int fcm_mobility_prepare(fcm_handle h, uint64_t seed) {
   if (int rc = check(h)) return rc;
   if (!h->have_bodies || !h->have_ref) return fail(FCM_ERR_STATE, "bodies and reference matrices must be set");
   const bool fluct = h->cfg.kT > 0.0 && h->stress[0] != nullptr;
@@ -1170,7 +895,7 @@ int fcm_mobility_solve(fcm_handle h, uint64_t seed, const gmres_opts* opts, gmre
   HIPCHK(rhs_assemble(fluct ? h->Vrand : nullptr, h->have_forces ? h->d_F6 : nullptr, h->rhs, h->n3(),
                       6 * (size_t)h->NB, h->st));
   h->have_rhs = true;
-  return solve_dev(h, opts, rep, nullptr);
+  return FCM_OK;
 }
 
 }  // extern "C"

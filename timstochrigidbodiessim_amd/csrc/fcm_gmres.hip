@@ -85,8 +85,8 @@ __device__ __forceinline__ void set_done(const GmDev& G, int status, int ncols, 
 // ---- set-up -----------------------------------------------------------------
 // red[0] = ||b||^2 from a preceding reduction
 __global__ void __launch_bounds__(GT) gm_start_kernel(GmDev G, double tol, int max_iter) {
-  const size_t nh = (size_t)(G.M + 2) * (G.M + 2);
-  for (size_t i = threadIdx.x; i < nh; i += GT) G.Hraw[i] = 0.0;
+  // Hraw and R need no clearing: every entry a step reads was written earlier
+  // in the same solve (column i holds rows 0..i+1 once it is final).
   if (threadIdx.x == 0) {
     const double nrm = sqrt(G.red[0]);
     G.dctl[0] = tol;
@@ -450,6 +450,31 @@ __global__ void gm_combine_kernel(GmDev G, const double* __restrict__ Q, size_t 
     for (int i = 0; i < nc; ++i) s += G.y[i] * Q[(size_t)i * ldq + r];
     x[r] = s;
   }
+}
+
+// ---- control snapshots ----------------------------------------------------------------
+// dst[16 j + e] = src[j][e] for the control blocks of a batch's jobs (null: zeros)
+__global__ void gm_gather_kernel(const double* const* __restrict__ src, double* __restrict__ dst, int n) {
+  const int j = blockIdx.x * 16 + (threadIdx.x >> 4), e = threadIdx.x & 15;
+  if (j < n) {
+    const double* p = src[j];
+    dst[(size_t)j * 16 + e] = p ? p[e] : 0.0;
+  }
+}
+
+__global__ void gm_set_ptr_kernel(const double** table, int idx, const double* p) {
+  if (threadIdx.x == 0) table[idx] = p;
+}
+
+hipError_t gm_gather_ctl(const double* const* src, double* dst, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gm_gather_kernel, dim3((n + 15) / 16), dim3(256), 0, st, src, dst, n);
+  return hipGetLastError();
+}
+
+hipError_t gm_set_ptr(const double** table, int idx, const double* p, hipStream_t st) {
+  hipLaunchKernelGGL(gm_set_ptr_kernel, dim3(1), dim3(64), 0, st, table, idx, p);
+  return hipGetLastError();
 }
 
 // ---- host-side launch helpers -------------------------------------------------------

@@ -177,5 +177,9 @@ hipError_t gm_launch_start(const GmDev& G, double tol, int max_iter, const doubl
 hipError_t gm_launch_basis(const GmDev& G, double* Q, size_t ldq, double* wbuf, size_t n, double* partial,
                            int nblk, hipStream_t st);
 hipError_t gm_launch_finish(const GmDev& G, const double* Q, size_t ldq, double* x, size_t n, hipStream_t st);
+// copy the 16-double control blocks src[0..n) into dst (device memory)
+hipError_t gm_gather_ctl(const double* const* src, double* dst, int n, hipStream_t st);
+// table[idx] = p, stream-ordered
+hipError_t gm_set_ptr(const double** table, int idx, const double* p, hipStream_t st);
 
 }  // namespace fcm

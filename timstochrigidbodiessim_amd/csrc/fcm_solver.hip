@@ -450,9 +450,10 @@ __global__ void scale_kernel(const double* __restrict__ x, double* __restrict__ 
   y[r] = x[r] * s;
 }
 
-__global__ void axpby_kernel(double a, const double* __restrict__ x, double b, double* __restrict__ y, size_t n) {
+__global__ void axpby_kernel(double a, const double* x, double b, double* y, size_t n) {  // x may alias y
   const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < n) y[r] = a * x[r] + b * y[r];
+  // a zero coefficient drops its operand, so stale non-finite data is never read into y
+  if (r < n) y[r] = (a == 0.0 ? 0.0 : a * x[r]) + (b == 0.0 ? 0.0 : b * y[r]);
 }
 
 __global__ void rhs_kernel(const double* __restrict__ Vrand, const double* __restrict__ F6, double* __restrict__ b,
