@@ -93,8 +93,121 @@ def cpu_baseline(args, ref, x, q, F6, fp64_iters):
     }
 
 
+def bench_batch(args):
+    """64 independent uneven instances (grids 32..128^3, 50-2000 rods of 6-12
+    spheres, log-uniform) solved as one batch, against the same solves one at
+    a time on the same GPU."""
+    import math
+    import timstochrigidbodiessim_amd as fa
+    rng = np.random.default_rng(3)
+    insts = []
+    t0 = time.perf_counter()
+    for i in range(args.instances):
+        n = int(rng.choice([32, 48, 64, 96, 128]))
+        NB = int(round(math.exp(rng.uniform(math.log(50), math.log(2000)))))
+        nbpb = int(rng.integers(6, 13))
+        cfg = fa.make_config(n, ng=7, dx=1.0, a=1.86, mu=1.0, kT=0.0, dt=args.dt, max_blobs=NB * nbpb,
+                             max_bodies=NB, max_blobs_per_body=nbpb, max_krylov=args.max_iter, fp32=True,
+                             random_field=False)
+        ref, x, q, F6, nov = fa.synth_rods(100 + i, NB, nbpb, 1.86, [float(n)] * 3, max_attempts=5)
+        s = fa.FcmSolver(cfg)
+        s.set_geometry(ref)
+        s.ref_matrices()
+        s.set_bodies(x, q)
+        s.set_body_forces(F6)
+        s.assemble_rhs(use_random=False)
+        insts.append((s, n, NB, nbpb))
+    log(f"[bench] batch set-up {time.perf_counter() - t0:.1f} s: {len(insts)} instances, "
+        f"{sum(i[2] * i[3] for i in insts)} blobs in total")
+    prec = fa.FCM_PREC_FP32
+    # one at a time
+    t0 = time.perf_counter()
+    alone = []
+    for s, *_ in insts:
+        _, r = s.solve(None, tol=args.tol, max_iter=args.max_iter, precision=prec, want_solution=False)
+        alone.append(r)
+    t_alone = time.perf_counter() - t0
+    times = []
+    for k in range(args.warmup + args.steps):
+        with fa.FcmBatch(max_jobs=len(insts), worker_streams=args.workers, steps_per_round=args.steps_per_round) as b:
+            t0 = time.perf_counter()
+            for s, *_ in insts:
+                b.submit(s, tol=args.tol, max_iter=args.max_iter, precision=prec)
+            b.run()
+            dt = time.perf_counter() - t0
+            reps = [r for _, r in b.take()]
+        if k >= args.warmup:
+            times.append(dt)
+        log(f"[bench] batch round {k}: {dt * 1e3:.1f} ms")
+    t_batch = statistics.median(times)
+    conv = sum(1 for r in reps if r["status"] == "converged")
+    line = {
+        "metric": "batched FCM mobility solves/s (64 uneven instances, FP32+refine, tol 1e-8)",
+        "value": round(len(insts) / t_batch, 3), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(t_batch * 1e3, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 grid passes + f64 Krylov/refinement",
+        "data": "synthetic (seeded rods and forces)",
+        "config": {"workload": f"{len(insts)} instances, grids 32..128^3, 50-2000 rods x 6-12 spheres, kT=0",
+                   "parallelism": "single GPU", "worker_streams": args.workers,
+                   "steps_per_round": args.steps_per_round},
+        "one_at_a_time_solves_per_s": round(len(insts) / t_alone, 3),
+        "batch_speedup_vs_one_at_a_time": round(t_alone / t_batch, 3),
+        "converged": conv, "iterations_total": sum(r["iterations"] for r in reps),
+    }
+    for s, *_ in insts:
+        s.close()
+    print(json.dumps(line), flush=True)
+
+
+def bench_serve(args):
+    """Poisson stream of mobility-solve requests (grids 64..256^3, 100-20,000
+    rods of 10 spheres, kT = 1) served with continuous batching, FP32+refine
+    and plain FP64."""
+    from timstochrigidbodiessim_amd import serve
+    import timstochrigidbodiessim_amd as fa
+    out = {}
+    for name, prec in (("fp32_refine", fa.FCM_PREC_FP32), ("fp64", fa.FCM_PREC_FP64)):
+        reqs = serve.poisson_requests(rate=args.rate, count=args.requests, seed=11,
+                                      grids=(64, 96, 128, 192, 256), rods=(100, 20000), nbpb=10,
+                                      precision=prec, tol=args.tol, max_iter=args.max_iter)
+        t0 = time.perf_counter()
+        for r in reqs:
+            r.make_geometry()
+        log(f"[bench] {len(reqs)} requests generated ({time.perf_counter() - t0:.1f} s)")
+        with serve.MobilityServer(max_jobs=args.max_jobs, worker_streams=args.workers,
+                                  steps_per_round=args.steps_per_round, max_krylov=args.max_iter) as srv:
+            srv.prepare_refs(reqs)
+            st = srv.run(reqs)
+            st["handles_created"] = srv.created
+            st["max_active"] = srv.max_active
+        log(f"[bench] serve {name}: {st}")
+        out[name] = st
+    h = out["fp32_refine"]
+    line = {
+        "metric": "served FCM mobility solves/s (Poisson arrivals, 64..256^3, FP32+refine vs FP64); p50/p99 latency",
+        "value": round(h["solves_per_s"], 3), "unit": "solves/s", "n_gpus": 1, "steps": args.requests,
+        "warmup": 0, "ms_per_step": round(1e3 * h["wall_s"] / max(1, h["requests"]), 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 grid passes + f64 Krylov/refinement",
+        "data": "synthetic (seeded Poisson stream, rods, forces and stress)",
+        "config": {"workload": f"{args.requests} requests at {args.rate}/s, grids 64..256^3, 100-20000 rods x 10, "
+                               f"kT=1, tol {args.tol:g}", "parallelism": "single GPU"},
+        "p50_ms": round(1e3 * h["p50_latency_s"], 2), "p99_ms": round(1e3 * h["p99_latency_s"], 2),
+        "fp32_refine": out["fp32_refine"], "fp64": out["fp64"],
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--workload", choices=["solve256", "batch", "serve"], default="solve256",
+                    help="solve256: the headline single-instance solve; batch: 64 uneven instances in one "
+                         "batch; serve: a Poisson stream of requests with continuous batching")
+    ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--steps-per-round", type=int, default=4)
+    ap.add_argument("--requests", type=int, default=40)
+    ap.add_argument("--rate", type=float, default=4.0)
+    ap.add_argument("--max-jobs", type=int, default=16)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
@@ -115,6 +228,11 @@ def main():
         print("bench.py measures one GPU: the whole 256^3 grid lives on one MI355X and no collective or "
               "multi-GPU path exists in this build; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     import timstochrigidbodiessim_amd as fa
 

@@ -73,6 +73,8 @@ const char* fcm_last_error(void);
 int fcm_sync(fcm_handle h);
 /* HBM bytes fcm_create would allocate for cfg (serving admission) */
 int fcm_query_footprint(const fcm_config* cfg, size_t* bytes);
+/* free and total device memory of the current device (admission by footprint) */
+int fcm_device_memory(size_t* free_bytes, size_t* total_bytes);
 
 /* ---- fcm_fluid_solver ----------------------------------------------------- */
 /* set_up_gaussians(const mat& Y), fcm_fluid_solver.cpp:131-165; Y is nb x 3 (host) */

@@ -19,7 +19,7 @@ GMRES_STATUS = {0: "converged", 1: "max_iter", 2: "zero_rhs", 3: "breakdown"}
 
 # every symbol include/fcm_hip.h declares
 EXPORTS = [
-    "fcm_create", "fcm_destroy", "fcm_last_error", "fcm_sync", "fcm_query_footprint",
+    "fcm_create", "fcm_destroy", "fcm_last_error", "fcm_sync", "fcm_query_footprint", "fcm_device_memory",
     "fcm_set_up_gaussians", "fcm_spread", "fcm_stokes", "fcm_interp", "fcm_mobility",
     "fcm_random_field", "fcm_random_interp", "fcm_get_field", "fcm_set_field",
     "fcm_ref_matrices", "fcm_set_ref_matrices",
@@ -80,6 +80,7 @@ def lib():
         L.fcm_destroy.argtypes = [H]
         L.fcm_sync.argtypes = [H]
         L.fcm_query_footprint.argtypes = [C.POINTER(FcmConfig), C.POINTER(C.c_size_t)]
+        L.fcm_device_memory.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
         L.fcm_set_up_gaussians.argtypes = [H, _dp, C.c_int]
         L.fcm_spread.argtypes = [H, _dp, C.c_int]
         L.fcm_stokes.argtypes = [H, C.c_int]
@@ -150,6 +151,13 @@ def synth_rods(seed, NB, nbpb, a, L, max_attempts=50, forces=True):
     _chk(lib().fcm_synth_rods(seed, NB, nbpb, a, _ptr(Lh), max_attempts, _ptr(ref), _ptr(x), _ptr(q),
                               _ptr(F6), C.byref(nov)))
     return ref.reshape(nbpb, 3), x.reshape(NB, 3), q.reshape(NB, 4), (F6.reshape(NB, 6) if forces else None), nov.value
+
+
+def device_memory():
+    """(free, total) bytes of device memory."""
+    f, t = C.c_size_t(), C.c_size_t()
+    _chk(lib().fcm_device_memory(C.byref(f), C.byref(t)))
+    return f.value, t.value
 
 
 def make_config(nx, ny=None, nz=None, ng=7, dx=1.0, a=1.86, mu=1.0, kT=0.0, dt=1.0, max_blobs=1,

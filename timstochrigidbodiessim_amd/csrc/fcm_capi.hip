@@ -328,6 +328,12 @@ int fcm_query_footprint(const fcm_config* cfg, size_t* bytes) {
   return FCM_OK;
 }
 
+int fcm_device_memory(size_t* free_bytes, size_t* total_bytes) {
+  if (!free_bytes || !total_bytes) return fail(FCM_ERR_ARG, "null argument");
+  HIPCHK(hipMemGetInfo(free_bytes, total_bytes));
+  return FCM_OK;
+}
+
 int fcm_create(const fcm_config* cfg, fcm_handle* out) {
   std::string why;
   if (!cfg || !out) return fail(FCM_ERR_ARG, "null argument");
