@@ -130,6 +130,25 @@ int rb_mul_K(fcm_handle h, const double* in6, double* out3);
 /* multiply_by_KT_at_midstep, collection_of_rigid_bodies.cpp:733-757 */
 int rb_mul_KT(fcm_handle h, const double* in3, double* out6);
 
+/* ---- linked_list and the time step (config 5) ------------------------------ */
+/* linked_list::populate + apply_barrier_forces (linked_list.cpp:80-102,192-281):
+ * soft-sphere repulsion between blobs of different bodies (interaction_magnitude,
+ * linked_list.cpp:362-382, SOFT_SPHERE: U0 = 20 kT, b = a/2) along the nearest-
+ * image separation, for pairs closer than `cutoff`, on a device cell list with
+ * cells at least `cutoff` wide.  The forces stay on the device as the blob
+ * forces of the next RHS (assemble_RHS adds -K^T of them, gmres_solver.cpp:58-70);
+ * F3 (3 Nb, blob order) receives a copy unless NULL.  Coincident pairs are
+ * skipped (the reference divides by r = 0). */
+int ll_barrier_forces(fcm_handle h, double cutoff, double* F3);
+/* end_of_step_update (collection_of_rigid_bodies.cpp:759-790): every body moves
+ * by nu*DT*[U; Omega] with particle::update (particle.cpp:45-56): x += ..., q =
+ * lie_exp(...) * q (quaternion.cpp:128-145), x wrapped into the box; then the
+ * blob positions and Gaussians are recomputed.  V6: host [U; Omega] per body, or
+ * NULL for the body part of the last solution on the device.  DT = fcm_config.dt. */
+int rb_end_of_step_update(fcm_handle h, const double* V6, double nu);
+/* current body positions (3 NB) and quaternions (4 NB); either may be NULL */
+int rb_get_bodies(fcm_handle h, double* x, double* q);
+
 /* ---- gmres_solver ----------------------------------------------------------- */
 typedef struct {
   double tol;          /* GMRES_TOL (config.hpp:62): stop when |beta_j|/||RHS|| < tol */
@@ -149,8 +168,9 @@ typedef struct {
   double seconds;           /* device time of the solve */
 } gmres_report;
 
-/* assemble_RHS, gmres_solver.cpp:58-70: RHS = [-V_rand; -F] with V_rand from
- * the last fcm_random_interp (zero if none) and F from rb_set_body_forces */
+/* assemble_RHS, gmres_solver.cpp:58-70: RHS = [-V_rand; -(F + K^T f)] with V_rand
+ * from the last fcm_random_interp (zero if none), F from rb_set_body_forces and
+ * f the blob forces of ll_barrier_forces (each zero if not set) */
 int gmres_assemble_rhs(fcm_handle h, int use_random);
 /* multiply_by_saddle_point_matrix, gmres_solver.cpp:72-83 (host vectors) */
 int gmres_saddle_matvec(fcm_handle h, const double* in, double* out, int precision);
@@ -173,6 +193,11 @@ int fcm_mobility_solve(fcm_handle h, uint64_t seed, const gmres_opts* opts, gmre
  * and the assembled RHS, left on the device and NOT waited for, so that it
  * overlaps with other work; a following fcm_batch_submit is ordered after it. */
 int fcm_mobility_prepare(fcm_handle h, uint64_t seed);
+/* one Euler-Maruyama step (main.cpp:222-312 with EULER_MARUYAMA, nu = 1):
+ * barrier forces (cutoff > 0), the seeded fluctuating field and its
+ * interpolation, the RHS, the saddle-point solve and the body update, all on
+ * the device. */
+int fcm_em_step(fcm_handle h, uint64_t seed, double cutoff, const gmres_opts* opts, gmres_report* rep);
 
 /* ---- batches of independent solves (no counterpart in the reference) ------ */
 /* The reference solves one system per process (main.cpp:308-310).  A batch

@@ -705,6 +705,69 @@ void or_mul_KT(int NB, int nbpb, const double* blob_ref, const double* q, const 
 
 /* Inverse of an SPD matrix by Cholesky (lower triangle read), result exactly
  * symmetric.  Returns 0 on success, 1 if not positive definite. */
+/* ------------------------------------------------------------------------ */
+/* time-step pieces                                                          */
+/* ------------------------------------------------------------------------ */
+void or_barrier_forces(const or_cfg* c, int Nb, int nbpb, const double* Y, double cutoff, double* F) {
+  const double L[3] = {c->dx * c->nx, c->dx * c->ny, c->dx * c->nz};
+  const double U0 = 20.0 * c->kT, b = 0.5 * c->a, two_a = 2.0 * c->a;  /* linked_list.cpp:367-368 */
+  const double cut2 = cutoff * cutoff;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < Nb; ++i) {
+    double f[3] = {0.0, 0.0, 0.0};
+    const int bi = i / nbpb;
+    for (int j = 0; j < Nb; ++j) {
+      if (j / nbpb == bi) continue;  /* blobs of one rigid body do not repel (:215) */
+      double d[3];
+      for (int k = 0; k < 3; ++k) {
+        const double dd = Y[3 * i + k] - Y[3 * j + k];
+        d[k] = dd - L[k] * trunc(2.0 * dd / L[k]);
+      }
+      const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (!(r2 < cut2) || r2 == 0.0) continue;
+      const double r = sqrt(r2);
+      const double mag = r < two_a ? U0 / b : U0 * exp((two_a - r) / b) / b;  /* :377 */
+      for (int k = 0; k < 3; ++k) f[k] += mag * (d[k] / r);
+    }
+    for (int k = 0; k < 3; ++k) F[3 * i + k] = f[k];
+  }
+}
+
+void or_lie_exp(const double u[3], double q[4]) {
+  const double theta = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  if (theta < 1e-10) {
+    q[0] = 1.0; q[1] = q[2] = q[3] = 0.0;
+    return;
+  }
+  const double cs = cos(0.5 * theta), sn = sin(0.5 * theta);
+  q[0] = cs;
+  for (int k = 0; k < 3; ++k) q[1 + k] = sn * u[k] / theta;
+}
+
+void or_quat_mul(const double p[4], const double q[4], double out[4]) {
+  const double s = p[0] * q[0] - (p[1] * q[1] + p[2] * q[2] + p[3] * q[3]);
+  double v[3], cr[3];
+  cross3(p + 1, q + 1, cr);
+  for (int k = 0; k < 3; ++k) v[k] = p[0] * q[1 + k] + q[0] * p[1 + k] + cr[k];
+  out[0] = s;
+  for (int k = 0; k < 3; ++k) out[1 + k] = v[k];
+}
+
+void or_end_of_step_update(const or_cfg* c, int NB, const double* V6, double s, double* x, double* q) {
+  const double L[3] = {c->dx * c->nx, c->dx * c->ny, c->dx * c->nz};
+  for (int n = 0; n < NB; ++n) {
+    double w[3], e[4], qn[4];
+    for (int k = 0; k < 3; ++k) {
+      double v = x[3 * n + k] + s * V6[6 * n + k];
+      x[3 * n + k] = v - L[k] * floor(v / L[k]);
+      w[k] = s * V6[6 * n + 3 + k];
+    }
+    or_lie_exp(w, e);
+    or_quat_mul(e, q + 4 * n, qn);
+    for (int k = 0; k < 4; ++k) q[4 * n + k] = qn[k];
+  }
+}
+
 int or_inv_sympd(int n, const double* A, double* Ainv) {
   double* L = (double*)calloc((size_t)n * n, sizeof(double));
   for (int j = 0; j < n; ++j) {

@@ -73,6 +73,24 @@ void or_mul_K(int NB, int nbpb, const double* blob_ref, const double* q,
 void or_mul_KT(int NB, int nbpb, const double* blob_ref, const double* q,
                const double* in3, double* out6);
 
+/* ---- time-step pieces (config 5) --------------------------------------- */
+/* Soft-sphere barrier forces between blobs of different bodies
+ * (linked_list.cpp:192-281, magnitude interaction_magnitude :362-382 with
+ * SOFT_SPHERE: U0 = 20 kT, b = a/2, |F| = U0/b for r < 2a, else
+ * U0 exp((2a - r)/b)/b), acting along the minimum-image separation
+ * x1 - x2 - L trunc(2 (x1 - x2)/L) (:255-257), for pairs closer than
+ * `cutoff` (the reference's cell traversal bounds the range instead).
+ * Brute force over all pairs, in blob order; coincident pairs are skipped. */
+void or_barrier_forces(const or_cfg* c, int Nb, int nbpb, const double* Y, double cutoff, double* F);
+/* quaternion.cpp:128-145: exp map of a rotation vector (identity below 1e-10) */
+void or_lie_exp(const double u[3], double q[4]);
+/* p * q (quaternion.hpp operator*) */
+void or_quat_mul(const double p[4], const double q[4], double out[4]);
+/* end_of_step_update (collection_of_rigid_bodies.cpp:759-790) with
+ * particle::update (particle.cpp:45-56): x += s U, q = lie_exp(s Omega) q,
+ * x wrapped into [0, L); V6 holds [U; Omega] per body, s = nu DT. */
+void or_end_of_step_update(const or_cfg* c, int NB, const double* V6, double s, double* x, double* q);
+
 /* ---- small dense algebra ------------------------------------------------ */
 int  or_inv_sympd(int n, const double* A, double* Ainv);
 int  or_pinv_sym(int n, const double* A, double* Ainv, double rtol);

@@ -75,6 +75,10 @@ def lib():
         L.or_fill_blob_pos.argtypes = [C.POINTER(OrCfg), C.c_int, C.c_int, _dp, _dp, _dp, _dp]
         L.or_mul_K.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp]
         L.or_mul_KT.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+        L.or_barrier_forces.argtypes = [C.POINTER(OrCfg), C.c_int, C.c_int, _dp, C.c_double, _dp]
+        L.or_lie_exp.argtypes = [_dp, _dp]
+        L.or_quat_mul.argtypes = [_dp, _dp, _dp]
+        L.or_end_of_step_update.argtypes = [C.POINTER(OrCfg), C.c_int, _dp, C.c_double, _dp, _dp]
         L.or_inv_sympd.argtypes = [C.c_int, _dp, _dp]
         L.or_pinv_sym.argtypes = [C.c_int, _dp, _dp, C.c_double]
         L.or_solve_dense.argtypes = [C.c_int, _dp, _dp, _dp]
@@ -193,6 +197,34 @@ def mul_KT(blob_ref, q, in3):
     out = np.zeros(6 * NB)
     lib().or_mul_KT(NB, nbpb, _f64(blob_ref).ravel(), _f64(q).ravel(), _f64(in3).ravel(), out)
     return out
+
+
+def barrier_forces(c, Y, nbpb, cutoff):
+    """Soft-sphere barrier forces (linked_list.cpp:192-281) by brute force."""
+    Y = _f64(Y).ravel()
+    F = np.zeros_like(Y)
+    lib().or_barrier_forces(C.byref(c), Y.size // 3, nbpb, Y, cutoff, F)
+    return F
+
+
+def lie_exp(u):
+    q = np.zeros(4)
+    lib().or_lie_exp(_f64(u).ravel(), q)
+    return q
+
+
+def quat_mul(p, q):
+    out = np.zeros(4)
+    lib().or_quat_mul(_f64(p).ravel(), _f64(q).ravel(), out)
+    return out
+
+
+def end_of_step_update(c, V6, s, x, q):
+    """Returns the updated (x, q) copies (collection_of_rigid_bodies.cpp:759-790)."""
+    x = _f64(x).ravel().copy()
+    q = _f64(q).ravel().copy()
+    lib().or_end_of_step_update(C.byref(c), x.size // 3, _f64(V6).ravel(), s, x, q)
+    return x, q
 
 
 def inv_sympd(A):

@@ -1,0 +1,92 @@
+"""Config-5 pieces on the GPU against the oracle: barrier forces on the device
+cell list (linked_list.cpp:192-281), the end-of-step body update
+(collection_of_rigid_bodies.cpp:759-790) and whole Euler-Maruyama steps
+(main.cpp:222-312): body positions and quaternions after each step agree with
+the oracle's step to 1e-9."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+import timstochrigidbodiessim_amd as fa
+from _common import pair_cfg, random_quats, rel, rod
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("n,NB,nbpb,a,cutoff", [(32, 60, 6, 1.86, 2.2 * 1.86), (16, 20, 3, 1.0, 6.0),
+                                                (48, 200, 8, 1.86, 2.2 * 1.86), (24, 40, 4, 1.5, 5.0)])
+def test_barrier_forces_match_oracle(n, NB, nbpb, a, cutoff):
+    say("barrier", n, NB, nbpb, a, cutoff)
+    gc, oc = pair_cfg(n, a=a, kT=1.0, max_blobs=NB * nbpb, max_bodies=NB, max_blobs_per_body=nbpb)
+    rng = np.random.default_rng(n + NB)
+    ref = rod(nbpb, a)
+    x = rng.uniform(0, n, (NB, 3))
+    q = random_quats(rng, NB)
+    with fa.FcmSolver(gc) as s:
+        s.set_geometry(ref)
+        s.set_bodies(x, q)
+        F = s.barrier_forces(cutoff)
+        Fo = po.barrier_forces(oc, po.fill_blob_pos(oc, ref, x, q), nbpb, cutoff).reshape(-1, 3)
+        assert np.abs(Fo).max() > 0
+        assert rel(F, Fo) < 1e-12, rel(F, Fo)
+        assert np.array_equal(F, s.barrier_forces(cutoff))
+
+
+def test_end_of_step_update_matches_oracle():
+    gc, oc = pair_cfg(32, dt=0.2, max_blobs=50, max_bodies=10, max_blobs_per_body=5)
+    rng = np.random.default_rng(9)
+    ref = rod(5, 1.86)
+    x = rng.uniform(0, 32, (10, 3))
+    q = random_quats(rng, 10)
+    V6 = rng.standard_normal((10, 6)) * 30
+    with fa.FcmSolver(gc) as s:
+        s.set_geometry(ref)
+        s.set_bodies(x, q)
+        s.end_of_step_update(V6, nu=0.8)
+        xg, qg = s.bodies()
+        xo, qo = po.end_of_step_update(oc, V6, 0.8 * 0.2, x, q)
+        assert np.allclose(xg.ravel(), xo, rtol=0, atol=1e-13)
+        assert np.allclose(qg.ravel(), qo, rtol=0, atol=1e-14)
+        assert rel(s.blob_positions(), po.fill_blob_pos(oc, ref, xo, qo)) < 1e-14
+
+
+def test_euler_maruyama_steps_match_oracle():
+    n, NB, nbpb, dt, cutoff, tol = 32, 12, 6, 0.20216, 2.2 * 1.86, 1e-10
+    gc, oc = pair_cfg(n, kT=1.0, dt=dt, max_blobs=NB * nbpb, max_bodies=NB, max_blobs_per_body=nbpb, max_krylov=150)
+    ref, x, q, _, _ = fa.synth_rods(21, NB, nbpb, 1.86, [float(n)] * 3, max_attempts=20)
+    o = po.Fcm(oc, NB * nbpb)
+    st, M, Minv, N = o.ref_matrices(ref)
+    xo, qo = x.ravel().copy(), q.ravel().copy()
+    with fa.FcmSolver(gc) as s:
+        s.set_geometry(ref)
+        s.set_bodies(x, q)
+        s.set_ref_matrices(Minv, N)
+        s.set_body_forces(np.zeros((NB, 6)))
+        for step in range(3):
+            seed = 500 + step
+            Y = po.fill_blob_pos(oc, ref, xo, qo)
+            o.set_up_gaussians(Y)
+            o.fill_stress(seed)
+            o.random_field()
+            Vr = o.random_interp()
+            f = po.barrier_forces(oc, Y, nbpb, cutoff)
+            F6 = po.mul_KT(ref, qo, f)
+            rhs = np.concatenate([-Vr, -F6])
+            b = po.Bodies(ref, xo, qo, Minv, N)
+            st, sol, it, _ = po.gmres(o, b, rhs, tol=tol, max_iter=150)
+            assert st == 0
+            xo, qo = po.end_of_step_update(oc, sol[3 * NB * nbpb:], dt, xo, qo)
+            rep = s.em_step(seed, cutoff, tol=tol, max_iter=150)
+            xg, qg = s.bodies()
+            dx = np.abs(xg.ravel() - xo)
+            dx = np.minimum(dx, n - dx)
+            say("step", step, rep, "oracle iterations", it, "max |dx|", dx.max(), "max |dq|",
+                np.abs(qg.ravel() - qo).max())
+            assert rep["status"] == "converged"
+            assert dx.max() < 1e-9
+            assert np.abs(qg.ravel() - qo).max() < 1e-9
+    o.close()

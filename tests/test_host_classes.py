@@ -1,0 +1,95 @@
+"""The reference-named C++ host classes (timstochrigidbodiessim_amd/host):
+host-only math on the CPU; on the GPU, time steps driven through
+fcm_fluid_solver / collection_of_rigid_bodies / linked_list / gmres_solver in
+main.cpp's order, checked against the oracle (first solve <= 1e-10) and
+bitwise against the device-resident fcm_em_step path."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+import timstochrigidbodiessim_amd as fa
+
+EXE = os.path.join(os.path.dirname(fa.__file__), "fcm_host_check")
+
+
+def _need_exe():
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} is not built (python -m timstochrigidbodiessim_amd.build)")
+
+
+def test_host_math_classes():
+    _need_exe()
+    r = subprocess.run([EXE, "math"], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "FAIL" not in r.stdout
+
+
+def _write_problem(path, n, NB, nbpb, steps, prec, seed, a, kT, dt, tol, cutoff, ref, x, q, F6):
+    with open(path, "wb") as f:
+        f.write(struct.pack("<5i", n, NB, nbpb, steps, prec))
+        f.write(struct.pack("<Q", seed))
+        f.write(struct.pack("<5d", a, kT, dt, tol, cutoff))
+        for arr in (ref, x, q, F6):
+            f.write(np.ascontiguousarray(arr, dtype="<f8").tobytes())
+
+
+def _read_result(path, nsys, NB, steps):
+    raw = open(path, "rb").read()
+    off, sols, its = 0, [], []
+    for _ in range(steps):
+        sols.append(np.frombuffer(raw, "<f8", nsys, off).copy())
+        off += 8 * nsys
+        its.append(struct.unpack_from("<i", raw, off)[0])
+        off += 4
+    x = np.frombuffer(raw, "<f8", 3 * NB, off).reshape(NB, 3)
+    off += 24 * NB
+    q = np.frombuffer(raw, "<f8", 4 * NB, off).reshape(NB, 4)
+    return sols, its, x, q
+
+
+@pytest.mark.gpu
+def test_host_classes_time_steps(tmp_path):
+    _need_exe()
+    n, NB, nbpb, steps, seed, a, kT, dt, tol, cutoff = 32, 10, 6, 3, 77, 1.86, 1.0, 0.20216, 1e-10, 2.2 * 1.86
+    ref, x, q, F6, _ = fa.synth_rods(5, NB, nbpb, a, [float(n)] * 3, max_attempts=20)
+    inp, out = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    _write_problem(inp, n, NB, nbpb, steps, fa.FCM_PREC_FP64, seed, a, kT, dt, tol, cutoff, ref, x, q, F6)
+    r = subprocess.run([EXE, "steps", inp, out], capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    nb = NB * nbpb
+    sols, its, xh, qh = _read_result(out, 3 * nb + 6 * NB, NB, steps)
+    # first step against the oracle
+    oc = po.cfg(n, ng=7, dx=1.0, a=a, mu=1.0, kT=kT, dt=dt)
+    o = po.Fcm(oc, nb)
+    st, M, Minv, N = o.ref_matrices(ref)
+    Y = po.fill_blob_pos(oc, ref, x, q)
+    o.set_up_gaussians(Y)
+    o.fill_stress(seed)
+    o.random_field()
+    Vr = o.random_interp()
+    f = po.barrier_forces(oc, Y, nbpb, cutoff)
+    rhs = np.concatenate([-Vr, -(po.mul_KT(ref, q, f) + F6.ravel())])
+    st, sol, it, _ = po.gmres(o, po.Bodies(ref, x, q, Minv, N), rhs, tol=tol, max_iter=200)
+    o.close()
+    err = np.linalg.norm(sols[0] - sol) / np.linalg.norm(sol)
+    print("host-class first solve vs oracle", err, "iterations", its[0], it)
+    assert err < 1e-10
+    # every step bitwise equal to the device-resident Euler-Maruyama step
+    cfg = fa.make_config(n, ng=7, dx=1.0, a=a, mu=1.0, kT=kT, dt=dt, max_blobs=nb, max_bodies=NB,
+                         max_blobs_per_body=nbpb, max_krylov=200, fp32=False, random_field=True)
+    with fa.FcmSolver(cfg) as s:
+        s.set_geometry(ref)
+        s.ref_matrices()
+        s.set_bodies(x, q)
+        s.set_body_forces(F6)
+        for k in range(steps):
+            rep = s.em_step(seed + k, cutoff, tol=tol, max_iter=200)
+            assert rep["iterations"] == its[k]
+        xd, qd = s.bodies()
+    assert np.array_equal(xd, xh) and np.array_equal(qd, qh)

@@ -25,7 +25,7 @@ EXPORTS = [
     "fcm_ref_matrices", "fcm_set_ref_matrices",
     "fft_transform", "fft_inverse_transform", "fcm_get_kspace", "fcm_set_kspace",
     "rb_set_geometry", "rb_set_bodies", "rb_set_body_forces", "rb_get_blob_positions",
-    "rb_mul_K", "rb_mul_KT",
+    "rb_mul_K", "rb_mul_KT", "ll_barrier_forces", "rb_end_of_step_update", "rb_get_bodies", "fcm_em_step",
     "gmres_assemble_rhs", "gmres_saddle_matvec", "gmres_apply_Pinv", "gmres_solve",
     "gmres_get_solution", "fcm_mobility_solve", "fcm_mobility_prepare",
     "fcm_batch_create", "fcm_batch_destroy", "fcm_batch_submit", "fcm_batch_step", "fcm_batch_take",
@@ -102,6 +102,10 @@ def lib():
         L.rb_get_blob_positions.argtypes = [H, _dp]
         L.rb_mul_K.argtypes = [H, _dp, _dp]
         L.rb_mul_KT.argtypes = [H, _dp, _dp]
+        L.ll_barrier_forces.argtypes = [H, C.c_double, _dp]
+        L.rb_end_of_step_update.argtypes = [H, _dp, C.c_double]
+        L.rb_get_bodies.argtypes = [H, _dp, _dp]
+        L.fcm_em_step.argtypes = [H, C.c_uint64, C.c_double, C.POINTER(GmresOpts), C.POINTER(GmresReport)]
         L.gmres_assemble_rhs.argtypes = [H, C.c_int]
         L.gmres_saddle_matvec.argtypes = [H, _dp, _dp, C.c_int]
         L.gmres_apply_Pinv.argtypes = [H, _dp, _dp]
@@ -304,6 +308,29 @@ class FcmSolver:
         return out
 
     # gmres_solver
+    def barrier_forces(self, cutoff):
+        """Soft-sphere barrier forces (3 Nb), kept on the device for the next RHS."""
+        F = np.zeros(3 * self.NB * self.nbpb)
+        _chk(self.L.ll_barrier_forces(self.h, cutoff, _ptr(F)))
+        return F.reshape(-1, 3)
+
+    def end_of_step_update(self, V6=None, nu=1.0):
+        """Move the bodies by nu*dt*V6 (host [U; Omega] per body, or the device solution)."""
+        _chk(self.L.rb_end_of_step_update(self.h, None if V6 is None else _ptr(_f64(V6).ravel()), nu))
+
+    def bodies(self):
+        x = np.zeros(3 * self.NB)
+        q = np.zeros(4 * self.NB)
+        _chk(self.L.rb_get_bodies(self.h, _ptr(x), _ptr(q)))
+        return x.reshape(-1, 3), q.reshape(-1, 4)
+
+    def em_step(self, seed, cutoff, tol=1e-8, max_iter=200, precision=FCM_PREC_FP64, inner_tol=0.0, max_refine=0):
+        """One Euler-Maruyama step on the device; returns the solve report."""
+        o = GmresOpts(tol, max_iter, precision, inner_tol, max_refine)
+        r = GmresReport()
+        _chk(self.L.fcm_em_step(self.h, seed, cutoff, C.byref(o), C.byref(r)))
+        return r.as_dict()
+
     def assemble_rhs(self, use_random=True):
         _chk(self.L.gmres_assemble_rhs(self.h, int(use_random)))
 

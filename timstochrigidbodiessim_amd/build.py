@@ -18,7 +18,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfcm_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["fcm_fft.hip", "fcm_grid.hip", "fcm_solver.hip", "fcm_gmres.hip", "fcm_capi.hip", "fcm_batch.hip", "fcm_synth.cpp"]
+SOURCES = ["fcm_fft.hip", "fcm_grid.hip", "fcm_solver.hip", "fcm_gmres.hip", "fcm_capi.hip", "fcm_batch.hip", "fcm_steric.hip", "fcm_synth.cpp"]
 ARCH = os.environ.get("FCM_OFFLOAD_ARCH", "gfx950")
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result", "-Wno-unused-function"]
@@ -55,7 +55,29 @@ def build(jobs=None, verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_host()
     return LIB
+
+
+HOST = os.path.join(HERE, "host")
+HOST_LIB = os.path.join(HERE, "libfcm_host.so")
+HOST_CHECK = os.path.join(HERE, "fcm_host_check")
+
+
+def build_host():
+    """The reference-named C++ host classes (host/fcm_classes.*) as
+    libfcm_host.so over libfcm_hip.so, and the fcm_host_check driver."""
+    cxx = os.environ.get("CXX", "g++")
+    rpath = "-Wl,-rpath,$ORIGIN"
+    lib_cmd = [cxx, "-O2", "-std=c++17", "-fPIC", "-Wall", "-shared", "-o", HOST_LIB + ".tmp",
+               os.path.join(HOST, "fcm_classes.cpp"), f"-L{HERE}", "-lfcm_hip", rpath]
+    exe_cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-o", HOST_CHECK + ".tmp", os.path.join(HOST, "fcm_host_check.cpp"),
+               f"-L{HERE}", "-lfcm_host", "-lfcm_hip", rpath]
+    for cmd, out in ((lib_cmd, HOST_LIB), (exe_cmd, HOST_CHECK)):
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"host build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(out + ".tmp", out)
 
 
 if __name__ == "__main__":

@@ -113,6 +113,7 @@ Sizes sizes_of(const fcm_config* c) {
   t += (size_t)(c->max_krylov + 1) * s.ldq * 8 + 6 * s.ldq * 8 + 2 * (size_t)nb * 3 * 8;
   const size_t nblk = (s.nmax + 1023) / 1024;
   t += nblk * (2 * (size_t)c->max_krylov + 6) * 8 + 2 * (2 * (size_t)c->max_krylov + 16) * 8;
+  t += (size_t)c->max_bodies * 12 * 8 + (size_t)nb * (3 * 8 + 2 * 4);  // time-step buffers
   const size_t m2 = (size_t)(c->max_krylov + 2) * (c->max_krylov + 2);
   t += (2 * m2 + 9 * (size_t)(c->max_krylov + 2) + 16) * 8 + s.ldq * 8;
   s.total = t;
@@ -241,6 +242,18 @@ int launch_arnoldi_step(fcm_handle h, int precision, hipStream_t st) {
 }  // namespace fcm_host
 
 namespace {
+
+// body forces for the RHS: F6 (rb_set_body_forces) plus K^T of the blob forces
+// (ll_barrier_forces), as assemble_RHS forms -K^T blob_forces (gmres_solver.cpp:58-70)
+int body_forces_dev(fcm_handle h, const double** out) {
+  *out = h->have_forces ? h->d_F6 : nullptr;
+  if (!h->have_blob_forces) return FCM_OK;
+  BodyData b = h->body();
+  HIPCHK(mul_KT(b, h->d_fblob, h->d_F6tot, h->st));
+  if (h->have_forces) HIPCHK(axpby(1.0, h->d_F6, 1.0, h->d_F6tot, 6 * (size_t)h->NB, h->st));
+  *out = h->d_F6tot;
+  return FCM_OK;
+}
 
 int setup_blobs_device(fcm_handle h, int nb) {
   h->bd.nb = nb;
@@ -406,6 +419,11 @@ int fcm_create(const fcm_config* cfg, fcm_handle* out) {
   if (e == hipSuccess) h->d_q = dalloc<double>(h, 4 * (size_t)cfg->max_bodies, e);
   if (e == hipSuccess) h->d_R = dalloc<double>(h, 9 * (size_t)cfg->max_bodies, e);
   if (e == hipSuccess) h->d_F6 = dalloc<double>(h, 6 * (size_t)cfg->max_bodies, e);
+  if (e == hipSuccess) h->d_F6tot = dalloc<double>(h, 6 * (size_t)cfg->max_bodies, e);
+  if (e == hipSuccess) h->d_V6 = dalloc<double>(h, 6 * (size_t)cfg->max_bodies, e);
+  if (e == hipSuccess) h->d_fblob = dalloc<double>(h, 3 * (size_t)nb, e);
+  if (e == hipSuccess) h->sw.cell_of = dalloc<int>(h, nb, e);
+  if (e == hipSuccess) h->sw.sorted = dalloc<int>(h, nb, e);
   if (e == hipSuccess) h->d_dpos = dalloc<double>(h, 3 * (size_t)nb, e);
   h->nmax = s.nmax;
   h->ldq = s.ldq;
@@ -464,6 +482,7 @@ int fcm_destroy(fcm_handle h) {
   batch_forget(h);
   batch_destroy_solo(h);
   for (void* p : h->allocs) (void)hipFree(p);
+  for (void* p : h->lazy) (void)hipFree(p);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->st) (void)hipStreamDestroy(h->st);
@@ -671,6 +690,7 @@ int rb_set_bodies(fcm_handle h, int NB, const double* x, const double* q) {
   if (rc) return rc;
   h->have_bodies = true;
   h->have_rhs = false;
+  h->have_blob_forces = false;
   return sync_stream(h);
 }
 
@@ -771,7 +791,9 @@ int fcm_ref_matrices(fcm_handle h, double* Msingle, double* Minv, double* N) {
 int gmres_assemble_rhs(fcm_handle h, int use_random) {
   if (int rc = check(h)) return rc;
   if (!h->have_bodies) return fail(FCM_ERR_STATE, "rb_set_bodies first");
-  HIPCHK(rhs_assemble(use_random ? h->Vrand : nullptr, h->have_forces ? h->d_F6 : nullptr, h->rhs, h->n3(),
+  const double* F6 = nullptr;
+  if (int rc = body_forces_dev(h, &F6)) return rc;
+  HIPCHK(rhs_assemble(use_random ? h->Vrand : nullptr, F6, h->rhs, h->n3(),
                       6 * (size_t)h->NB, h->st));
   h->have_rhs = true;
   return sync_stream(h);
@@ -898,10 +920,87 @@ int fcm_mobility_prepare(fcm_handle h, uint64_t seed) {
     h->have_urand = true;
     HIPCHK(interp<double>(h->g, h->bb, h->bd, h->fprand(), h->Vrand, KTerm{}, h->st));
   }
-  HIPCHK(rhs_assemble(fluct ? h->Vrand : nullptr, h->have_forces ? h->d_F6 : nullptr, h->rhs, h->n3(),
+  const double* F6 = nullptr;
+  if (int rc = body_forces_dev(h, &F6)) return rc;
+  HIPCHK(rhs_assemble(fluct ? h->Vrand : nullptr, F6, h->rhs, h->n3(),
                       6 * (size_t)h->NB, h->st));
   h->have_rhs = true;
   return FCM_OK;
+}
+
+int ll_barrier_forces(fcm_handle h, double cutoff, double* F3) {
+  if (int rc = check(h)) return rc;
+  if (!h->have_bodies) return fail(FCM_ERR_STATE, "rb_set_bodies first");
+  if (!(cutoff > 0.0)) return fail(FCM_ERR_ARG, "cutoff must be positive");
+  const fcm_config& c = h->cfg;
+  StericCells cg;
+  const int n[3] = {c.nx, c.ny, c.nz};
+  size_t ncells = 1;
+  for (int d = 0; d < 3; ++d) {
+    cg.L[d] = c.dx * n[d];
+    cg.nc[d] = std::max(1, std::min(1024, (int)std::floor(cg.L[d] / cutoff)));
+    ncells *= (size_t)cg.nc[d];
+  }
+  if ((int)ncells > h->sw.cap_cells) {
+    HIPCHK(hipStreamSynchronize(h->st));
+    for (void* p : h->lazy) (void)hipFree(p);
+    h->lazy.clear();
+    h->sw.cap_cells = 0;
+    void *a = nullptr, *b = nullptr, *d = nullptr;
+    HIPCHK(hipMalloc(&a, ncells * sizeof(int)));
+    h->lazy.push_back(a);
+    HIPCHK(hipMalloc(&b, (ncells + 1) * sizeof(int)));
+    h->lazy.push_back(b);
+    HIPCHK(hipMalloc(&d, ncells * sizeof(int)));
+    h->lazy.push_back(d);
+    h->sw.count = static_cast<int*>(a);
+    h->sw.start = static_cast<int*>(b);
+    h->sw.cursor = static_cast<int*>(d);
+    h->sw.cap_cells = (int)ncells;
+  }
+  const int nb = h->NB * h->nbpb;
+  HIPCHK(barrier_forces(h->dY, nb, h->nbpb, cg, cutoff, c.kT, c.a, h->sw, h->d_fblob, h->st));
+  h->have_blob_forces = true;
+  h->have_rhs = false;
+  if (F3) HIPCHK(hipMemcpyAsync(F3, h->d_fblob, 3 * (size_t)nb * sizeof(double), hipMemcpyDeviceToHost, h->st));
+  return sync_stream(h);
+}
+
+int rb_end_of_step_update(fcm_handle h, const double* V6, double nu) {
+  if (int rc = check(h)) return rc;
+  if (!h->have_bodies) return fail(FCM_ERR_STATE, "rb_set_bodies first");
+  const double* v = nullptr;
+  if (V6) {
+    HIPCHK(hipMemcpyAsync(h->d_V6, V6, 6 * (size_t)h->NB * sizeof(double), hipMemcpyHostToDevice, h->st));
+    v = h->d_V6;
+  } else {
+    if (!h->have_sol) return fail(FCM_ERR_STATE, "no solution to take the body velocities from");
+    v = h->sol + h->n3();
+  }
+  HIPCHK(bodies_step(h->g, h->d_x, h->d_q, h->NB, v, nu * h->cfg.dt, h->st));
+  BodyData b = h->body();
+  HIPCHK(bodies_update(h->g, b, h->st));
+  if (int rc = setup_blobs_device(h, h->NB * h->nbpb)) return rc;
+  h->have_blob_forces = false;
+  h->have_rhs = false;
+  return sync_stream(h);
+}
+
+int rb_get_bodies(fcm_handle h, double* x, double* q) {
+  if (int rc = check(h)) return rc;
+  if (!h->have_bodies) return fail(FCM_ERR_STATE, "rb_set_bodies first");
+  if (x) HIPCHK(hipMemcpyAsync(x, h->d_x, 3 * (size_t)h->NB * sizeof(double), hipMemcpyDeviceToHost, h->st));
+  if (q) HIPCHK(hipMemcpyAsync(q, h->d_q, 4 * (size_t)h->NB * sizeof(double), hipMemcpyDeviceToHost, h->st));
+  return sync_stream(h);
+}
+
+int fcm_em_step(fcm_handle h, uint64_t seed, double cutoff, const gmres_opts* opts, gmres_report* rep) {
+  if (int rc = check(h)) return rc;
+  if (cutoff > 0.0)
+    if (int rc = ll_barrier_forces(h, cutoff, nullptr)) return rc;
+  if (int rc = fcm_mobility_prepare(h, seed)) return rc;
+  if (int rc = solve_single(h, opts, rep, nullptr)) return rc;
+  return rb_end_of_step_update(h, nullptr, 1.0);
 }
 
 }  // extern "C"

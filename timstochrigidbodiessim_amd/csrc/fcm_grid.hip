@@ -117,6 +117,21 @@ __global__ void record_kernel(const double* __restrict__ Y, const int* __restric
 
 hipError_t build_tile_lists(const GridGeom& g, const BlobBins& bb, BlobData& bd, hipStream_t st);
 
+hipError_t sort_by_key(const int* key_of, int n, int nkeys, const int* count, int* start, int* cursor, int* sorted,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, count, start, cursor, nkeys);
+  if (n > 0) {
+    hipLaunchKernelGGL(scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, st, key_of, n, start, cursor, sorted);
+    hipLaunchKernelGGL(bin_sort_kernel, dim3((nkeys + 255) / 256), dim3(256), 0, st, start, nkeys, sorted);
+  }
+  return hipGetLastError();
+}
+
+hipError_t fill_ints(int* p, int n, int v, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(fill_int_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+
 hipError_t setup_gaussians(const GridGeom& g, const BlobBins& bb, BlobData& bd, double a, hipStream_t st) {
   const int nb = bd.nb, nbins = bb.nbins();
   hipLaunchKernelGGL(fill_int_kernel, dim3((nbins + 255) / 256), dim3(256), 0, st, bd.bin_count, nbins, 0);

@@ -51,6 +51,12 @@ struct fcm_handle_s {
   int nbpb = 0, NB = 0;
   double *d_ref = nullptr, *d_x = nullptr, *d_q = nullptr, *d_R = nullptr, *d_dpos = nullptr;
   double *d_Minv = nullptr, *d_N = nullptr, *d_F6 = nullptr;
+  double* d_fblob = nullptr;    // 3 Nb blob forces (barrier forces), blob order
+  double* d_F6tot = nullptr;    // 6 NB body forces + K^T blob forces, for the RHS
+  double* d_V6 = nullptr;       // 6 NB body velocities for the end-of-step update
+  StericWork sw;                // cell list of the barrier forces
+  std::vector<void*> lazy;      // cell arrays, grown on demand
+  bool have_blob_forces = false;
   bool have_geometry = false, have_bodies = false, have_ref = false, have_forces = false;
   std::vector<double> h_ref;
 

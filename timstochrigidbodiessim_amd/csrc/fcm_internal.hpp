@@ -120,6 +120,32 @@ struct BodyData {
 };
 
 hipError_t bodies_update(const GridGeom& g, BodyData& b, hipStream_t st);
+// end_of_step_update (collection_of_rigid_bodies.cpp:759-790): body n moves by
+// s [U; Omega] (V6 with stride vstride doubles between bodies' 6-vectors)
+hipError_t bodies_step(const GridGeom& g, double* x, double* q, int NB, const double* V6, double s, hipStream_t st);
+
+// counting sort of n items by key (count[] already holds the key histogram);
+// items keep ascending index order inside a key
+hipError_t sort_by_key(const int* key_of, int n, int nkeys, const int* count, int* start, int* cursor, int* sorted,
+                       hipStream_t st);
+hipError_t fill_ints(int* p, int n, int v, hipStream_t st);
+
+// soft-sphere barrier forces between blobs of different bodies within a cutoff
+// (linked_list.cpp:80-102,192-281,362-382) on a cell list of cells >= cutoff
+struct StericCells {
+  int nc[3];
+  double L[3];
+};
+struct StericWork {
+  int* cell_of = nullptr;  // nb
+  int* sorted = nullptr;   // nb
+  int* count = nullptr;    // ncells
+  int* start = nullptr;    // ncells + 1
+  int* cursor = nullptr;   // ncells
+  int cap_cells = 0;
+};
+hipError_t barrier_forces(const double* Y, int nb, int nbpb, const StericCells& cg, double cutoff, double kT, double a,
+                          StericWork& w, double* F, hipStream_t st);
 // out = P^{-1} in; if wKT != null also writes wKT = -K^T out_lambda (6 per body)
 hipError_t apply_pinv(const BodyData& b, const double* in, double* out, double* wKT, hipStream_t st);
 hipError_t mul_K(const BodyData& b, const double* in6, double* out3, hipStream_t st);

@@ -1,0 +1,191 @@
+// fcm_classes.hpp -- the reference's host classes, kept by name and meaning,
+// over the C ABI of libfcm_hip.so (include/fcm_hip.h).
+//
+// A program written against the reference (fcm_fluid_solver.hpp,
+// fft_solver.hpp, gmres_solver.hpp, collection_of_rigid_bodies.hpp,
+// linked_list.hpp, particle.hpp, quaternion.hpp, parallel_random_numbers.hpp)
+// keeps its call sequence: the same classes, methods and public members, with
+// std::vector<double> in place of Armadillo vectors (blob positions Nb x 3
+// row-major, blob vectors interleaved [x0,y0,z0,...], body vectors [U; Omega]
+// per body, GMRES vectors [lambda; V]).  What differs:
+//   * compile-time config.hpp becomes a runtime fcm_config passed to
+//     fcm_fluid_solver::initialise;
+//   * one process drives the whole grid on one GPU (no MPI ranks), and all grid
+//     state lives on the device: the solver owns an fcm_handle that the other
+//     classes share;
+//   * produce_random_velocity_field takes the seed of the stress stream (the
+//     reference draws from an unseeded SPRNG stream);
+//   * errors are exceptions (fcm_error) carrying the library's status code.
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/fcm_hip.h"
+
+namespace tsrb {
+
+struct fcm_error : std::runtime_error {
+  int code;
+  fcm_error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// throws fcm_error unless rc == FCM_OK
+void check(int rc);
+
+// ---- quaternion.hpp / quaternion.cpp ------------------------------------------
+class quaternion {
+ public:
+  double scalar_part = 1.0;
+  double vector_part[3] = {0.0, 0.0, 0.0};
+
+  quaternion() = default;
+  quaternion(double q0, double q1, double q2, double q3);
+  double norm() const;
+  void normalise_in_place();
+  quaternion conj() const;
+  // quaternion::matrix (quaternion.cpp:61-103), row-major
+  void matrix(double R[9]) const;
+  void rotate(const double v[3], double out[3]) const;
+  quaternion operator*(const quaternion& p) const;
+  quaternion operator*(double s) const;
+  quaternion operator+(const quaternion& p) const;
+};
+
+// quaternion.cpp:128-145
+quaternion lie_exp(const double u[3]);
+
+// ---- particle.hpp ---------------------------------------------------------------
+class particle {
+ public:
+  double x[3] = {0.0, 0.0, 0.0};
+  quaternion q;
+  // particle::update(const vec& grand_update) (particle.cpp:45-56) in a box L
+  void update(const double grand_update[6], const double L[3]);
+};
+
+// ---- parallel_random_numbers.hpp -------------------------------------------------
+// gaussian_random (parallel_random_numbers.cpp:8-19): Marsaglia's polar method,
+// here over a seeded SplitMix64 stream instead of SPRNG.
+void seed_gaussian_random(uint64_t seed);
+double gaussian_random();
+
+// ---- fft_solver.hpp ---------------------------------------------------------------
+// In-place unnormalised R2C / C2R of a host field in the reference layout
+// ([nx][ny][pad] reals in, transposed-order complex out), on the device.
+class fft_solver {
+ public:
+  int local_nx = 0, local_x_start = 0, local_ny_after_transpose = 0, local_y_start_after_transpose = 0;
+  int total_local_size = 0, pad = 0;
+  void initialise(fcm_handle h, const fcm_config& cfg);
+  void transform(double* in);
+  void inverse_transform(double* in);
+
+ private:
+  fcm_handle h_ = nullptr;
+};
+
+class collection_of_rigid_bodies;
+
+// ---- fcm_fluid_solver.hpp ---------------------------------------------------------
+class fcm_fluid_solver {
+ public:
+  std::vector<double> V;         // blob velocities of the last interpolation (3 Nb)
+  std::vector<double> ref_Minv;  // (3 n_b)^2, row-major
+  std::vector<double> ref_N;     // 6 x 6
+  fft_solver fft;
+
+  fcm_fluid_solver() = default;
+  ~fcm_fluid_solver();
+  fcm_fluid_solver(const fcm_fluid_solver&) = delete;
+  fcm_fluid_solver& operator=(const fcm_fluid_solver&) = delete;
+
+  // initialise (fcm_fluid_solver.cpp:29-129); total_num_blobs <= cfg.max_blobs
+  void initialise(const fcm_config& cfg, int total_num_blobs);
+  void set_up_gaussians(const std::vector<double>& Y);
+  void set_up_force_distribution(const std::vector<double>& F);
+  void produce_deterministic_velocity_field();
+  void calculate_deterministic_blob_velocities();
+  void produce_random_velocity_field(uint64_t seed);
+  void calculate_random_blob_velocities();
+  void produce_reference_matrices(collection_of_rigid_bodies& all_bodies);
+
+  // real-space field `which` (0..2 ux,uy,uz / fx,fy,fz after a spread; 3..5 the
+  // random velocity) in the reference layout [nx][ny][pad]
+  std::vector<double> field(int which) const;
+  fcm_handle handle() const { return h_; }
+  const fcm_config& config() const { return cfg_; }
+
+ private:
+  fcm_handle h_ = nullptr;
+  fcm_config cfg_{};
+  int num_blobs_ = 0;
+};
+
+// ---- linked_list.hpp ---------------------------------------------------------------
+// The cell list lives on the device; apply_barrier_forces adds the soft-sphere
+// forces (linked_list.cpp:192-281) of the current blob positions to blob_forces.
+class linked_list {
+ public:
+  double cutoff = 0.0;
+  void initialise(fcm_handle h, double cutoff_radius);
+  void apply_barrier_forces(std::vector<double>& blob_forces) const;
+
+ private:
+  fcm_handle h_ = nullptr;
+};
+
+// ---- collection_of_rigid_bodies.hpp -------------------------------------------------
+class collection_of_rigid_bodies {
+ public:
+  int blobs_per_body = 0;
+  int total_num_blobs = 0;
+  std::vector<double> blob_ref_pos;  // n_b x 3 body-frame offsets
+  std::vector<double> all_blob_pos;  // Nb x 3
+  std::vector<particle> bodies;
+  std::vector<double> blob_forces;   // 3 Nb
+  std::vector<double> body_forces;   // 6 NB external forces and torques
+  linked_list list;
+
+  // geometry and initial state; shares fcm's device state
+  void initialise(fcm_fluid_solver& fcm, const std::vector<double>& ref_pos, const std::vector<particle>& init,
+                  double steric_cutoff);
+  void fill_all_blob_pos();
+  void calculate_forces();
+  std::vector<double> multiply_by_K_at_midstep(const std::vector<double>& in) const;
+  std::vector<double> multiply_by_KT_at_midstep(const std::vector<double>& in) const;
+  // collection_of_rigid_bodies.cpp:759-790; returns whether the step must be
+  // redone for an overlap (never: SKIP_ON_OVERLAP is false, no channel walls)
+  bool end_of_step_update(const std::vector<double>& V, double nu);
+
+ private:
+  fcm_handle h_ = nullptr;
+  double L_[3] = {0.0, 0.0, 0.0};
+  void upload() const;
+};
+
+// ---- gmres_solver.hpp ----------------------------------------------------------------
+class gmres_solver {
+ public:
+  int num_blobs = 0;
+  int system_size = 0;
+  std::vector<double> RHS;
+  gmres_opts opts{1e-8, 200, FCM_PREC_FP64, 0.0, 0};
+  gmres_report report{};
+  std::vector<double> residual_history;
+
+  void initialise(int total_num_blobs, int num_bodies, const gmres_opts& o);
+  // RHS = [-fcm.V; -(K^T blob_forces + body_forces)] (gmres_solver.cpp:58-70)
+  void assemble_RHS(const collection_of_rigid_bodies& all_bodies, const fcm_fluid_solver& fcm);
+  std::vector<double> multiply_by_Pinv(const std::vector<double>& in, const collection_of_rigid_bodies& all_bodies,
+                                       const fcm_fluid_solver& fcm);
+  std::vector<double> multiply_by_saddle_point_matrix(const std::vector<double>& in,
+                                                      const collection_of_rigid_bodies& all_bodies,
+                                                      fcm_fluid_solver& fcm);
+  std::vector<double> invert_saddle_point_system(const collection_of_rigid_bodies& all_bodies, fcm_fluid_solver& fcm,
+                                                 int myrank, double t);
+};
+
+}  // namespace tsrb
