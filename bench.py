@@ -197,11 +197,59 @@ def bench_serve(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_timestep(args):
+    """Config-5 time steps: 512^3 grid, 150,000 rods of 12 spheres (1.8M
+    blobs), kT = 1, steric cutoff 2.2a, Euler-Maruyama steps each with a full
+    fluctuating mobility solve (FP32+refine, tol 1e-8) and the body update."""
+    import timstochrigidbodiessim_amd as fa
+    n, NB, nbpb, a = args.ts_grid, args.ts_rods, 12, 1.86
+    t0 = time.perf_counter()
+    ref, x, q, _, nov = fa.synth_rods(5, NB, nbpb, a, [float(n)] * 3, max_attempts=args.place_attempts)
+    nb = NB * nbpb
+    phi = nb * 4.0 / 3.0 * np.pi * a ** 3 / n ** 3
+    log(f"[bench] timestep geometry {time.perf_counter() - t0:.1f} s: {NB} rods, phi {phi:.3f}, {nov} overlapping")
+    cfg = fa.make_config(n, ng=7, dx=1.0, a=a, mu=1.0, kT=1.0, dt=args.dt, max_blobs=nb, max_bodies=NB,
+                         max_blobs_per_body=nbpb, max_krylov=args.max_iter, fp32=True, random_field=True)
+    log(f"[bench] footprint {fa.footprint(cfg) / 1e9:.1f} GB")
+    s = fa.FcmSolver(cfg)
+    s.set_geometry(ref)
+    s.ref_matrices()
+    s.set_bodies(x, q)
+    s.set_body_forces(np.zeros((NB, 6)))
+    cutoff = 2.2 * a
+    times, reps = [], []
+    for k in range(args.warmup + args.steps):
+        t1 = time.perf_counter()
+        rep = s.em_step(1 + k, cutoff, tol=args.tol, max_iter=args.max_iter, precision=fa.FCM_PREC_FP32)
+        dt = time.perf_counter() - t1
+        log(f"[bench] step {k}: {dt:.3f} s {rep}")
+        if k >= args.warmup:
+            times.append(dt)
+            reps.append(rep)
+    s.close()
+    ms = 1e3 * statistics.mean(times)
+    line = {
+        "metric": "Brownian time steps/s at 512^3 with 1.8M spheres (FP32+refine GMRES at 1e-8, sterics, EM)",
+        "value": round(1e3 / ms, 4), "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 1), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 grid passes + f64 Krylov/refinement/constraints/quaternions",
+        "data": "synthetic (seeded rods and stress stream)",
+        "config": {"workload": f"{n}^3 grid, {NB} rods x {nbpb} spheres ({nb} blobs), a=1.86, kT=1, steric "
+                               f"cutoff 2.2a, Euler-Maruyama, dt {args.dt:.5f}, volume fraction {phi:.3f}",
+                   "parallelism": "single GPU"},
+        "iterations": [r["iterations"] for r in reps], "status": sorted(set(r["status"] for r in reps)),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
-    ap.add_argument("--workload", choices=["solve256", "batch", "serve"], default="solve256",
+    ap.add_argument("--workload", choices=["solve256", "batch", "serve", "timestep"], default="solve256",
                     help="solve256: the headline single-instance solve; batch: 64 uneven instances in one "
-                         "batch; serve: a Poisson stream of requests with continuous batching")
+                         "batch; serve: a Poisson stream of requests with continuous batching; timestep: "
+                         "Euler-Maruyama steps at 512^3 with 1.8M spheres")
+    ap.add_argument("--ts-grid", type=int, default=512)
+    ap.add_argument("--ts-rods", type=int, default=150000)
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--steps-per-round", type=int, default=4)
@@ -233,6 +281,8 @@ def main():
         return bench_batch(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "timestep":
+        return bench_timestep(args)
 
     import timstochrigidbodiessim_amd as fa
 
