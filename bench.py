@@ -269,6 +269,7 @@ def main():
     ap.add_argument("--profile-reps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -303,6 +304,12 @@ def main():
     log(f"[bench] reference matrices: {time.perf_counter() - t0:.2f} s")
     s.set_bodies(x, q)
     s.set_body_forces(F6)
+    tuning = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        s.autotune(fa.FCM_PREC_FP32, reps=5)
+        tuning = s.autotune(fa.FCM_PREC_FP64, reps=5)
+        log(f"[bench] autotuned launch shapes {tuning} ({time.perf_counter() - t0:.2f} s)")
 
     results = {}
     modes = [("fp32_refine", fa.FCM_PREC_FP32)]
@@ -403,6 +410,7 @@ def main():
         "fp32_vs_fp64_solution": agree,
         "passes_ms": {k: {kk: round(vv["ms"], 4) for kk, vv in v.items()} for k, v in prof.items()},
         "roofline": roofline,
+        "launch_tuning": tuning,
         "cpu_baseline": cpu,
     }
     s.close()

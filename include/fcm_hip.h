@@ -222,6 +222,23 @@ int fcm_batch_take(fcm_batch b, fcm_handle* handles, gmres_report* reps, int max
 /* step until every submitted job has finished */
 int fcm_batch_run(fcm_batch b);
 
+/* ---- launch shapes (no counterpart in the reference) ---------------------------- */
+/* Tunable launch shapes of a handle's mat-vec, per precision ([FCM_PREC_FP64],
+ * [FCM_PREC_FP32]): the spreading kernel (0: stencil-start specialised x loop,
+ * 1: branch-free tile-shifted weight tables) and the interpolation tile depth
+ * (1 or 2 bins of 8 z nodes; 0 picks 2 when it divides).  Results do not depend
+ * on the choice beyond rounding; FP64 parity holds for all. */
+typedef struct {
+  int spread_variant[2];
+  int interp_izb[2];
+} fcm_tuning;
+int fcm_get_tuning(fcm_handle h, fcm_tuning* t);
+int fcm_set_tuning(fcm_handle h, const fcm_tuning* t);
+/* Times every candidate of one precision on the handle's current blobs (`reps`
+ * launches each, HIP events on the handle's stream), keeps the fastest for each
+ * pass and returns the handle's tuning. */
+int fcm_autotune(fcm_handle h, int precision, int reps, fcm_tuning* chosen);
+
 /* ---- measurement hook (no counterpart in the reference) ----------------------- */
 /* Runs one saddle-point mat-vec `reps` times on a fixed input with HIP events
  * between its passes, on the handle's stream; ms_per_pass[0..7] receives the

@@ -206,3 +206,41 @@ def test_ref_matrices_and_Pinv_match_oracle():
         assert rel(s.apply_Pinv(v), po.apply_Pinv(b, v)) < 1e-12
         o.set_up_gaussians(po.fill_blob_pos(oc, ref, x, q))
         assert rel(s.saddle_matvec(v), po.saddle_matvec(o, b, v)) < FP64_TOL
+
+
+@pytest.mark.parametrize("dims,ng,nb", [((32, 24, 16), 7, 200), ((64, 64, 64), 7, 1000), ((16, 8, 8), 7, 30),
+                                        ((48, 32, 96), 7, 300), ((32, 16, 16), 8, 80), ((24, 24, 48), 5, 100)])
+def test_spread_interp_launch_variants_match_oracle(dims, ng, nb):
+    """Every tunable launch shape (spreading kernel, interpolation tile depth)
+    keeps the parity bar and bitwise repeatability."""
+    say("variants", dims, ng, nb)
+    dx = 0.9
+    gc, oc = pair_cfg(*dims, ng=ng, dx=dx, a=1.5, max_blobs=nb)
+    rng = np.random.default_rng(14)
+    Y = _blobs(rng, nb, dims, dx)
+    F = rng.standard_normal(3 * nb)
+    o = po.Fcm(oc, nb)
+    o.set_up_gaussians(Y)
+    o.spread(F)
+    ref_f = np.stack([po.from_padded(oc, o.field(n)) for n in ("fx", "fy", "fz")])
+    u = rng.standard_normal((3,) + dims)
+    for c, name in enumerate(("ux", "uy", "uz")):
+        o.field(name)[:] = po.to_padded(oc, u[c])
+    ref_V = o.interp()
+    with fa.FcmSolver(gc) as s:
+        s.set_up_gaussians(Y)
+        for sv, iz in ((0, 1), (1, 1), (1, 2), (0, 2)):
+            s.set_tuning((sv, sv), (iz, iz))
+            for prec, tol in ((fa.FCM_PREC_FP64, FP64_TOL), (fa.FCM_PREC_FP32, FP32_TOL)):
+                s.spread(F, prec)
+                got = np.stack([po.from_padded(oc, s.get_field(c, prec)) for c in range(3)])
+                assert rel(got, ref_f) < tol, ("spread", sv, prec, rel(got, ref_f))
+                s.spread(F, prec)
+                assert np.array_equal(got, np.stack([po.from_padded(oc, s.get_field(c, prec)) for c in range(3)]))
+                for c in range(3):
+                    s.set_field(c, po.to_padded(oc, u[c]), prec)
+                V = s.interp(prec)
+                assert rel(V, ref_V) < tol, ("interp", iz, prec, rel(V, ref_V))
+        t = s.autotune(fa.FCM_PREC_FP32, reps=2)
+        say("autotuned", t)
+        assert t["spread_variant"][1] in (0, 1) and t["interp_izb"][1] in (1, 2)

@@ -30,10 +30,18 @@ EXPORTS = [
     "gmres_get_solution", "fcm_mobility_solve", "fcm_mobility_prepare",
     "fcm_batch_create", "fcm_batch_destroy", "fcm_batch_submit", "fcm_batch_step", "fcm_batch_take",
     "fcm_batch_run",
+    "fcm_get_tuning", "fcm_set_tuning", "fcm_autotune",
     "fcm_profile_matvec", "fcm_synth_rods",
 ]
 
 PASS_NAMES = ["spread", "z_forward", "y_forward", "x_project", "y_inverse", "z_inverse", "interp", "KT"]
+
+
+class FcmTuning(C.Structure):
+    _fields_ = [("spread_variant", C.c_int * 2), ("interp_izb", C.c_int * 2)]
+
+    def as_dict(self):
+        return {"spread_variant": list(self.spread_variant), "interp_izb": list(self.interp_izb)}
 
 
 class FcmConfig(C.Structure):
@@ -119,6 +127,9 @@ def lib():
         L.fcm_batch_step.argtypes = [H, C.POINTER(C.c_int)]
         L.fcm_batch_take.argtypes = [H, C.POINTER(H), C.POINTER(GmresReport), C.c_int, C.POINTER(C.c_int)]
         L.fcm_batch_run.argtypes = [H]
+        L.fcm_get_tuning.argtypes = [H, C.POINTER(FcmTuning)]
+        L.fcm_set_tuning.argtypes = [H, C.POINTER(FcmTuning)]
+        L.fcm_autotune.argtypes = [H, C.c_int, C.c_int, C.POINTER(FcmTuning)]
         L.fcm_profile_matvec.argtypes = [H, C.c_int, C.c_int, _dp]
         L.fcm_synth_rods.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_double, _dp, C.c_int, _dp, _dp,
                                      _dp, _dp, C.POINTER(C.c_int)]
@@ -362,6 +373,21 @@ class FcmSolver:
         r = GmresReport()
         _chk(self.L.fcm_mobility_solve(self.h, seed, C.byref(o), C.byref(r)))
         return r.as_dict()
+
+    def tuning(self):
+        """{'spread_variant': [fp64, fp32], 'interp_izb': [fp64, fp32]}"""
+        t = FcmTuning()
+        _chk(self.L.fcm_get_tuning(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, spread_variant=(0, 0), interp_izb=(0, 0)):
+        t = FcmTuning((C.c_int * 2)(*spread_variant), (C.c_int * 2)(*interp_izb))
+        _chk(self.L.fcm_set_tuning(self.h, C.byref(t)))
+
+    def autotune(self, precision=FCM_PREC_FP32, reps=5):
+        t = FcmTuning()
+        _chk(self.L.fcm_autotune(self.h, precision, reps, C.byref(t)))
+        return t.as_dict()
 
     def mobility_prepare(self, seed):
         """Seeded fluctuating field + RHS on the device, not waited for."""

@@ -902,6 +902,97 @@ int fcm_profile_matvec(fcm_handle h, int precision, int reps, double* ms_per_pas
   return rc;
 }
 
+int fcm_get_tuning(fcm_handle h, fcm_tuning* t) {
+  if (int rc = check(h)) return rc;
+  if (!t) return fail(FCM_ERR_ARG, "null tuning");
+  for (int p = 0; p < 2; ++p) {
+    t->spread_variant[p] = h->bd.spread_variant[p];
+    t->interp_izb[p] = h->bd.interp_izb[p];
+  }
+  return FCM_OK;
+}
+
+int fcm_set_tuning(fcm_handle h, const fcm_tuning* t) {
+  if (int rc = check(h)) return rc;
+  if (!t) return fail(FCM_ERR_ARG, "null tuning");
+  for (int p = 0; p < 2; ++p)
+    if (t->spread_variant[p] < 0 || t->spread_variant[p] > 1 || t->interp_izb[p] < 0 || t->interp_izb[p] > 2)
+      return fail(FCM_ERR_ARG, "bad tuning");
+  HIPCHK(hipStreamSynchronize(h->st));
+  for (int p = 0; p < 2; ++p) {
+    h->bd.spread_variant[p] = t->spread_variant[p];
+    h->bd.interp_izb[p] = t->interp_izb[p];
+  }
+  ++h->epoch;
+  return FCM_OK;
+}
+
+int fcm_autotune(fcm_handle h, int precision, int reps, fcm_tuning* chosen) {
+  if (int rc = check(h)) return rc;
+  if (!h->have_blobs) return fail(FCM_ERR_STATE, "no blob positions set");
+  if (precision == FCM_PREC_FP32 && !h->f32[0]) return fail(FCM_ERR_STATE, "FP32 path not allocated");
+  reps = std::max(1, reps);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const size_t n3 = h->n3();
+  HIPCHK(axpby(0.0, h->tmp, 0.0, h->tmp, n3, h->st));  // spread input: zeros (same work)
+  auto time_it = [&](auto&& launch, float& ms) -> int {
+    if (int rc = launch()) return rc;  // warm-up
+    HIPCHK(hipEventRecord(e0, h->st));
+    for (int r = 0; r < reps; ++r)
+      if (int rc = launch()) return rc;
+    HIPCHK(hipEventRecord(e1, h->st));
+    HIPCHK(hipEventSynchronize(e1));
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    return FCM_OK;
+  };
+  const KTerm kt{};
+  auto spread_once = [&]() -> int {
+    if (precision == FCM_PREC_FP32) HIPCHK(spread<float>(h->g, h->bb, h->bd, h->tmp, h->fp32(), h->st));
+    else HIPCHK(spread<double>(h->g, h->bb, h->bd, h->tmp, h->fp64(), h->st));
+    return FCM_OK;
+  };
+  auto interp_once = [&]() -> int {
+    if (precision == FCM_PREC_FP32) HIPCHK(interp<float>(h->g, h->bb, h->bd, h->fp32(), h->Vtmp, kt, h->st));
+    else HIPCHK(interp<double>(h->g, h->bb, h->bd, h->fp64(), h->Vtmp, kt, h->st));
+    return FCM_OK;
+  };
+  const int pi = precision == FCM_PREC_FP32 ? 1 : 0;
+  int rc = FCM_OK;
+  float best = 0;
+  int best_v = h->bd.spread_variant[pi];
+  for (int v = 0; v <= 1 && !rc; ++v) {
+    h->bd.spread_variant[pi] = v;
+    float ms = 0;
+    rc = time_it(spread_once, ms);
+    if (!rc && (v == 0 || ms < best)) {
+      best = ms;
+      best_v = v;
+    }
+  }
+  h->bd.spread_variant[pi] = best_v;
+  int best_z = h->bd.interp_izb[pi];
+  float best_i = 0;
+  for (int z = 1; z <= 2 && !rc; ++z) {
+    h->bd.interp_izb[pi] = z;
+    float ms = 0;
+    rc = time_it(interp_once, ms);
+    if (!rc && (z == 1 || ms < best_i)) {
+      best_i = ms;
+      best_z = z;
+    }
+  }
+  h->bd.interp_izb[pi] = best_z;
+  ++h->epoch;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if (rc == FCM_OK && chosen) rc = fcm_get_tuning(h, chosen);
+  if (rc) return rc;
+  return sync_stream(h);
+}
+
 int fcm_mobility_solve(fcm_handle h, uint64_t seed, const gmres_opts* opts, gmres_report* rep) {
   if (int rc = fcm_mobility_prepare(h, seed)) return rc;
   return solve_single(h, opts, rep, nullptr);

@@ -588,6 +588,115 @@ __global__ void __launch_bounds__(256) spread_fast_kernel(SpreadListArgs A, Fiel
   }
 }
 
+// Branch-free variant of the same tiling: the staging thread of a blob shifts
+// its weights into tile coordinates -- F_c * g_x at each of the 8 tile x
+// nodes (24 values), g_y at the 8 tile rows, g_z at the TZ tile columns, zero
+// outside the stencil -- so that every lane of a wavefront whose patch the
+// blob touches does the same 24 fused multiply-adds, reading the x/force
+// table as LDS broadcasts and only its own y and z weights.
+static constexpr int SD_BATCH = 64;
+
+template <typename T>
+__global__ void __launch_bounds__(256) spread_dense_kernel(SpreadListArgs A, FieldPtrs<T> fp) {
+  if (fp.gate && *fp.gate) return;
+  const GridGeom& g = A.g;
+  const int ng = A.ng, omin = 1 - ng / 2, NW = A.NW;
+  const int TZ = 8 * NW;
+  const int nty = g.ny / 8, ntz = g.nz / TZ;
+  const int ntiles = (g.nx / 8) * nty * ntz;
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  if (tile >= ntiles) return;
+  int tt = tile;
+  const int tz = tt % ntz; tt /= ntz;
+  const int ty = tt % nty;
+  const int tx = tt / nty;
+  const int x0 = tx * 8, y0 = ty * 8, z0 = tz * TZ;
+
+  constexpr int TAB = SD_BATCH * (24 + 8 + 32);
+  constexpr int STAGE = 3 * 8 * 8 * 32;
+  __shared__ T buf[TAB > STAGE ? TAB : STAGE];
+  __shared__ int s_mask[SD_BATCH];
+  T (*s_fg)[24] = reinterpret_cast<T(*)[24]>(buf);
+  T (*s_gy)[8] = reinterpret_cast<T(*)[8]>(buf + SD_BATCH * 24);
+  T (*s_gz)[32] = reinterpret_cast<T(*)[32]>(buf + SD_BATCH * 32);
+
+  const int t = threadIdx.x;
+  const int lane = t & 63, wv = t >> 6;
+  const int lz = lane & 7, ly = lane >> 3;
+  const int zl = 8 * wv + lz;
+  const int L0 = A.tile_start[tile], L1 = A.tile_start[tile + 1];
+  T acc[8][3];
+#pragma unroll
+  for (int xi = 0; xi < 8; ++xi) acc[xi][0] = acc[xi][1] = acc[xi][2] = T(0);
+  const T* W = wsel<T>(A.w64, A.w32);
+
+  for (int base = L0; base < L1; base += SD_BATCH) {
+    const int cnt = min(SD_BATCH, L1 - base);
+    if (t < cnt) {
+      const int s = A.tile_list[base + t];
+      const int4 r = A.rec[s];
+      const T* ws = W + (size_t)s * 3 * ng;
+      const T f[3] = {T(A.F[3 * (size_t)r.w]), T(A.F[3 * (size_t)r.w + 1]), T(A.F[3 * (size_t)r.w + 2])};
+      const int sx = wrap_half(r.x + omin - x0, g.nx);
+      const int sy = wrap_half(r.y + omin - y0, g.ny);
+      const int sz = wrap_half(r.z + omin - z0, g.nz);
+#pragma unroll
+      for (int xi = 0; xi < 8; ++xi) {
+        int o = xi - sx;
+        if (o < 0) o += g.nx; else if (o >= g.nx) o -= g.nx;
+        const T gx = o < ng ? ws[o] : T(0);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s_fg[t][3 * xi + c] = f[c] * gx;
+      }
+#pragma unroll
+      for (int yi = 0; yi < 8; ++yi) {
+        int o = yi - sy;
+        if (o < 0) o += g.ny; else if (o >= g.ny) o -= g.ny;
+        s_gy[t][yi] = o < ng ? ws[ng + o] : T(0);
+      }
+      int mask = 0;
+      for (int zi = 0; zi < TZ; ++zi) {
+        int o = zi - sz;
+        if (o < 0) o += g.nz; else if (o >= g.nz) o -= g.nz;
+        const bool in = o < ng;
+        s_gz[t][zi] = in ? ws[2 * ng + o] : T(0);
+        if (in) mask |= 1 << (zi >> 3);
+      }
+      s_mask[t] = mask;
+    }
+    __syncthreads();
+    if (wv < NW) {
+      for (int p = 0; p < cnt; ++p) {
+        if (!((s_mask[p] >> wv) & 1)) continue;
+        const T w = s_gy[p][ly] * s_gz[p][zl];
+        const T* fg = s_fg[p];
+#pragma unroll
+        for (int xi = 0; xi < 8; ++xi)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) acc[xi][c] += w * fg[3 * xi + c];
+      }
+    }
+    __syncthreads();
+  }
+  T* stage = buf;
+  if (wv < NW) {
+#pragma unroll
+    for (int xi = 0; xi < 8; ++xi)
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) stage[((cc * 8 + xi) * 8 + ly) * TZ + zl] = acc[xi][cc];
+  }
+  __syncthreads();
+  const size_t rstride = 2 * (size_t)g.cpitch;
+  const int total = 3 * 8 * 8 * TZ;
+  for (int e = t; e < total; e += 256) {
+    const int zi = e % TZ;
+    const int row = e / TZ;
+    const int yi = row & 7, xi = (row >> 3) & 7, cc = row >> 6;
+    const size_t idx = ((size_t)(x0 + xi) * g.ny + (y0 + yi)) * rstride + z0 + zi;
+    reinterpret_cast<T*>(fp.p[cc])[idx] = stage[e];
+  }
+}
+
 bool spread_fast_ok(const GridGeom& g, const BlobBins& bb, int ng) {
   return ng <= 8 && g.nx >= 16 && g.nx % 8 == 0 && g.ny % 8 == 0 && g.nz % 8 == 0 && bb.bx == 4 && bb.by == 8 &&
          bb.bz == 8;
@@ -617,7 +726,10 @@ hipError_t spread(const GridGeom& g, const BlobBins& bb, const BlobData& bd, con
     const int NW = g.nz % 32 == 0 ? 4 : (g.nz % 16 == 0 ? 2 : 1);
     const int nt = spread_tiles(g);
     SpreadListArgs A{g, bd.ng, NW, bd.tile_start, bd.tile_list, bd.rec, bd.w64, bd.w32, F};
-    hipLaunchKernelGGL((spread_fast_kernel<T>), dim3(8 * ((nt + 7) / 8)), dim3(256), 0, st, A, fp);
+    if (bd.spread_variant[sizeof(T) == 4 ? 1 : 0] == 1)
+      hipLaunchKernelGGL((spread_dense_kernel<T>), dim3(8 * ((nt + 7) / 8)), dim3(256), 0, st, A, fp);
+    else
+      hipLaunchKernelGGL((spread_fast_kernel<T>), dim3(8 * ((nt + 7) / 8)), dim3(256), 0, st, A, fp);
     return hipGetLastError();
   }
   SpreadArgs A{g, bb, bd.ng, bd.bin_start, bd.rec, bd.w64, bd.w32, F};
@@ -905,7 +1017,7 @@ hipError_t interp(const GridGeom& g, const BlobBins& bb, const BlobData& bd, con
   InterpArgs A{g, bb, bd.ng, bd.bin_start, bd.rec, bd.w64, bd.w32, V, kt, g.dx * g.dx * g.dx};
   if (bd.nb == 0) return hipSuccess;
   if (interp_fast_ok(g, bb, bd.ng) && (bd.ng == 7 || bd.ng == 8)) {
-    const int IZB = bb.nbz % 2 == 0 ? 2 : 1;
+    const int IZB = (bd.interp_izb[sizeof(T) == 4 ? 1 : 0] == 1 || bb.nbz % 2 != 0) ? 1 : 2;
     const int TZ = 8 * IZB;
     const size_t lds = IfNc<T>::value * sizeof(T) * (size_t)(8 + bd.ng - 1) * (8 + bd.ng - 1) * (TZ + bd.ng - 1);
     const int ntiles = (g.nx / 8) * (g.ny / 8) * (g.nz / TZ);

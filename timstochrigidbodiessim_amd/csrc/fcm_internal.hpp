@@ -79,6 +79,11 @@ struct BlobData {
   int* tile_start = nullptr;  // ntiles + 1
   int* tile_cursor = nullptr; // ntiles (scan scratch)
   int* tile_list = nullptr;   // sorted slots of the blobs relevant to each tile (<= 8 per blob)
+  // launch-shape choices (tunable per grid): spreading kernel 0 = stencil-start
+  // specialised, 1 = branch-free shifted tables
+  // indexed by precision (0 FP64, 1 FP32)
+  int spread_variant[2] = {0, 0};
+  int interp_izb[2] = {0, 0};  // interpolation tile depth in z bins: 0 = 2 when it divides, 1, 2
 };
 
 // spreading tiles of the main path (0 if the grid does not qualify)

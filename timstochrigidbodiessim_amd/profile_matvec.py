@@ -1,0 +1,53 @@
+"""Run the saddle-point mat-vec of the 256^3 workload a few times per
+precision, plus a plain device copy of known size, as a target for
+rocprofv3 (kernel statistics and FETCH_SIZE / WRITE_SIZE counters):
+
+    python -m timstochrigidbodiessim_amd.profile_matvec [--reps 3] [--grid 256]
+
+The copy moves `--copy-mb` MB (read + write) with torch's copy kernel; its
+counter values calibrate the byte counts of the FCM kernels."""
+import argparse
+import sys
+import time
+
+import numpy as np
+
+from . import _lib as L
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--rods", type=int, default=20000)
+    ap.add_argument("--spheres", type=int, default=12)
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--copy-mb", type=int, default=512)
+    a = ap.parse_args()
+    n = a.grid
+    t0 = time.perf_counter()
+    ref, x, q, F6, _ = L.synth_rods(2, a.rods, a.spheres, 1.86, [float(n)] * 3, max_attempts=20)
+    nb = a.rods * a.spheres
+    cfg = L.make_config(n, ng=7, dx=1.0, a=1.86, mu=1.0, kT=1.0, dt=0.20216, max_blobs=nb, max_bodies=a.rods,
+                        max_blobs_per_body=a.spheres, max_krylov=4, fp32=True, random_field=True)
+    with L.FcmSolver(cfg) as s:
+        s.set_geometry(ref)
+        s.set_bodies(x, q)
+        print(f"[profile] set-up {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+        for name, prec in (("fp32", L.FCM_PREC_FP32), ("fp64", L.FCM_PREC_FP64)):
+            ms = s.profile_matvec(prec, a.reps)
+            print(f"[profile] {name}: " + ", ".join(f"{k} {v:.3f} ms" for k, v in ms.items()), flush=True)
+    try:
+        import torch
+        m = a.copy_mb * (1 << 20) // 4
+        src = torch.ones(m, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(a.reps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[profile] copy: {a.copy_mb} MiB read + {a.copy_mb} MiB written per launch", flush=True)
+    except Exception as e:  # calibration is optional
+        print(f"[profile] copy kernel skipped: {e!r}", flush=True)
+
+
+if __name__ == "__main__":
+    main()
