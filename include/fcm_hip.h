@@ -225,8 +225,9 @@ int fcm_batch_run(fcm_batch b);
 /* ---- launch shapes (no counterpart in the reference) ---------------------------- */
 /* Tunable launch shapes of a handle's mat-vec, per precision ([FCM_PREC_FP64],
  * [FCM_PREC_FP32]): the spreading kernel (0: stencil-start specialised x loop,
- * 1: branch-free tile-shifted weight tables) and the interpolation tile depth
- * (1 or 2 bins of 8 z nodes; 0 picks 2 when it divides).  Results do not depend
+ * 1: branch-free tile-shifted weight tables) and the interpolation tile (1 or 2:
+ * 8x8x8 or 8x8x16 nodes with all three components staged; 3: 8x16x16 (FP32) or
+ * 8x16x8 (FP64) staged one component at a time; 0 picks 2 when it divides).  Results do not depend
  * on the choice beyond rounding; FP64 parity holds for all. */
 typedef struct {
   int spread_variant[2];
@@ -245,6 +246,10 @@ int fcm_autotune(fcm_handle h, int precision, int reps, fcm_tuning* chosen);
  * mean device time of: spread, z forward, y forward, x forward+projection+x
  * inverse, y inverse, z inverse, interpolation(+K), K^T. */
 int fcm_profile_matvec(fcm_handle h, int precision, int reps, double* ms_per_pass);
+/* A plain streaming copy of `bytes` (16-byte words) between two scratch buffers,
+ * `reps` times, on the handle's stream; the reference point for HBM byte counters
+ * and for the achievable copy bandwidth.  *ms_per_copy = mean device time. */
+int fcm_profile_copy(fcm_handle h, size_t bytes, int reps, double* ms_per_copy);
 
 /* ---- synthetic inputs (host only; no GPU needed) ---------------------------- */
 /* Rods of nbpb spheres spaced 2a on the body x axis, centres uniform in the box

@@ -31,7 +31,7 @@ EXPORTS = [
     "fcm_batch_create", "fcm_batch_destroy", "fcm_batch_submit", "fcm_batch_step", "fcm_batch_take",
     "fcm_batch_run",
     "fcm_get_tuning", "fcm_set_tuning", "fcm_autotune",
-    "fcm_profile_matvec", "fcm_synth_rods",
+    "fcm_profile_matvec", "fcm_profile_copy", "fcm_synth_rods",
 ]
 
 PASS_NAMES = ["spread", "z_forward", "y_forward", "x_project", "y_inverse", "z_inverse", "interp", "KT"]
@@ -131,6 +131,7 @@ def lib():
         L.fcm_set_tuning.argtypes = [H, C.POINTER(FcmTuning)]
         L.fcm_autotune.argtypes = [H, C.c_int, C.c_int, C.POINTER(FcmTuning)]
         L.fcm_profile_matvec.argtypes = [H, C.c_int, C.c_int, _dp]
+        L.fcm_profile_copy.argtypes = [H, C.c_size_t, C.c_int, _dp]
         L.fcm_synth_rods.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_double, _dp, C.c_int, _dp, _dp,
                                      _dp, _dp, C.POINTER(C.c_int)]
         _lib = L
@@ -397,6 +398,12 @@ class FcmSolver:
         ms = np.zeros(8)
         _chk(self.L.fcm_profile_matvec(self.h, precision, reps, _ptr(ms)))
         return dict(zip(PASS_NAMES, ms.tolist()))
+
+    def profile_copy(self, nbytes, reps=5):
+        """Mean device ms of a plain copy of nbytes (the byte-counter reference)."""
+        ms = np.zeros(1)
+        _chk(self.L.fcm_profile_copy(self.h, nbytes, reps, _ptr(ms)))
+        return float(ms[0])
 
     def solution(self):
         s = np.zeros(self.n)

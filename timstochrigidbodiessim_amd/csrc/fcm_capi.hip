@@ -78,7 +78,7 @@ GridGeom make_geom(const fcm_config* c) {
   GridGeom g;
   g.nx = c->nx; g.ny = c->ny; g.nz = c->nz;
   g.nzc = c->nz / 2 + 1;
-  g.cpitch = round_up(g.nzc, 8);  // 128-byte rows for FP64
+  g.cpitch = round_up(g.nzc, 16);  // rows start on 128-byte lines in FP32 and FP64
   g.dx = c->dx;
   return g;
 }
@@ -916,7 +916,7 @@ int fcm_set_tuning(fcm_handle h, const fcm_tuning* t) {
   if (int rc = check(h)) return rc;
   if (!t) return fail(FCM_ERR_ARG, "null tuning");
   for (int p = 0; p < 2; ++p)
-    if (t->spread_variant[p] < 0 || t->spread_variant[p] > 1 || t->interp_izb[p] < 0 || t->interp_izb[p] > 2)
+    if (t->spread_variant[p] < 0 || t->spread_variant[p] > 1 || t->interp_izb[p] < 0 || t->interp_izb[p] > 3)
       return fail(FCM_ERR_ARG, "bad tuning");
   HIPCHK(hipStreamSynchronize(h->st));
   for (int p = 0; p < 2; ++p) {
@@ -974,7 +974,7 @@ int fcm_autotune(fcm_handle h, int precision, int reps, fcm_tuning* chosen) {
   h->bd.spread_variant[pi] = best_v;
   int best_z = h->bd.interp_izb[pi];
   float best_i = 0;
-  for (int z = 1; z <= 2 && !rc; ++z) {
+  for (int z = 1; z <= 3 && !rc; ++z) {
     h->bd.interp_izb[pi] = z;
     float ms = 0;
     rc = time_it(interp_once, ms);
@@ -991,6 +991,37 @@ int fcm_autotune(fcm_handle h, int precision, int reps, fcm_tuning* chosen) {
   if (rc == FCM_OK && chosen) rc = fcm_get_tuning(h, chosen);
   if (rc) return rc;
   return sync_stream(h);
+}
+
+int fcm_profile_copy(fcm_handle h, size_t bytes, int reps, double* ms_per_copy) {
+  if (int rc = check(h)) return rc;
+  if (!ms_per_copy || bytes < 16 || reps < 1) return fail(FCM_ERR_ARG, "bad copy arguments");
+  bytes &= ~(size_t)15;
+  void *a = nullptr, *b = nullptr;
+  HIPCHK(hipStreamSynchronize(h->st));
+  HIPCHK(hipMalloc(&a, bytes));
+  if (hipMalloc(&b, bytes) != hipSuccess) {
+    (void)hipFree(a);
+    return fail(FCM_ERR_NOMEM, "copy buffers");
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMemsetAsync(a, 0, bytes, h->st);
+  if (e == hipSuccess) e = copy16(a, b, bytes, h->st);  // warm-up
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipEventRecord(e0, h->st);
+  for (int r = 0; r < reps && e == hipSuccess; ++r) e = copy16(a, b, bytes, h->st);
+  if (e == hipSuccess) e = hipEventRecord(e1, h->st);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  HIPCHK(e);
+  *ms_per_copy = ms / reps;
+  return FCM_OK;
 }
 
 int fcm_mobility_solve(fcm_handle h, uint64_t seed, const gmres_opts* opts, gmres_report* rep) {

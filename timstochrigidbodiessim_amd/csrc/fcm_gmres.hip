@@ -42,15 +42,31 @@ __device__ double block_sum(double v, double* sh) {
   return r;
 }
 
-// Apply the previous rotations and a new one to the unrotated column
-// col[0..k+1] (gmres_solver.cpp:214-252).  commit: store the rotated column,
-// the rotation and beta.  Returns |beta_{k+1}| / ||b||.  One thread.
-__device__ double givens_column(const GmDev& G, int k, double* col, bool commit) {
-  for (int i = 1; i <= k; ++i) {
-    const double temp = G.CS[i - 1] * col[i - 1] + G.SN[i - 1] * col[i];
-    col[i] = G.CS[i - 1] * col[i] - G.SN[i - 1] * col[i - 1];
-    col[i - 1] = temp;
+// The Givens algebra of gmres_solver.cpp:214-252 on a column held in LDS.
+// All threads call givens_stage (copies rotations 0..k-1 to LDS); thread 0
+// alone calls givens_sweep, which applies them to col[0..k+1] in order and
+// forms the new rotation; commit stores the rotation and beta, and the rotated
+// column is then written to R by all threads (givens_store).  Returns
+// |beta_{k+1}| / ||b||.
+static constexpr int GCAP = 2048 + 4;
+
+__device__ __forceinline__ void givens_stage(const GmDev& G, int k, double* sCS, double* sSN) {
+  for (int i = threadIdx.x; i < k; i += blockDim.x) {
+    sCS[i] = G.CS[i];
+    sSN[i] = G.SN[i];
   }
+}
+
+__device__ double givens_sweep(const GmDev& G, int k, double* col, const double* sCS, const double* sSN,
+                               bool commit) {
+  double a = col[0];
+#pragma unroll 8
+  for (int i = 1; i <= k; ++i) {
+    const double b = col[i], c = sCS[i - 1], sn = sSN[i - 1];
+    col[i - 1] = c * a + sn * b;  // temp = CS h(i-1) + SN h(i)
+    a = c * b - sn * a;           // h(i) = CS h(i) - SN h(i-1)
+  }
+  col[k] = a;
   double cs, sn;
   if (col[k + 1] == 0.0) {
     cs = 1.0;
@@ -67,13 +83,16 @@ __device__ double givens_column(const GmDev& G, int k, double* col, bool commit)
   const double b1 = -sn * G.beta[k];
   if (commit) {
     col[k] = cs * col[k] + sn * col[k + 1];
-    for (int i = 0; i <= k; ++i) *Rp(G, i, k) = col[i];
     G.CS[k] = cs;
     G.SN[k] = sn;
     G.beta[k + 1] = b1;
     G.beta[k] = cs * G.beta[k];
   }
   return fabs(b1) / G.dctl[1];
+}
+
+__device__ __forceinline__ void givens_store(const GmDev& G, int k, const double* col) {
+  for (int i = threadIdx.x; i <= k; i += blockDim.x) *Rp(G, i, k) = col[i];
 }
 
 __device__ __forceinline__ void set_done(const GmDev& G, int status, int ncols, double rel) {
@@ -115,6 +134,10 @@ __global__ void gm_q0_kernel(GmDev G, const double* __restrict__ b, double* __re
 // ---- basis passes ------------------------------------------------------------
 // mode 0 (main): columns 0..k against v1 = Q col k (if provisional) and v2 = w
 // mode 1 (finalise): columns 0..k+1 against v1 = Q col k+1 (only if fin)
+// Grid (row blocks, DOT_CG): each block takes every DOT_CG-th group of 8
+// columns, so that enough waves stream the basis at once.
+static constexpr int DOT_CG = 4;
+
 __global__ void __launch_bounds__(KRT) gm_dots_kernel(GmDev G, const double* __restrict__ Q, size_t ldq,
                                                       const double* __restrict__ wbuf, size_t n, int mode,
                                                       double* __restrict__ partial, int nblk) {
@@ -123,9 +146,10 @@ __global__ void __launch_bounds__(KRT) gm_dots_kernel(GmDev G, const double* __r
   const int k = G.ictl[GM_K];
   const bool prov = G.ictl[GM_PROV] != 0;
   const int ncols = mode == 0 ? k + 1 : k + 2;
+  if ((int)blockIdx.y * 8 >= ncols) return;
   const double* v1 = mode == 0 ? (prov ? Q + (size_t)k * ldq : nullptr) : Q + (size_t)(k + 1) * ldq;
   const double* v2 = mode == 0 ? wbuf : nullptr;
-  __shared__ double red[4 * 8];
+  __shared__ double red[4 * 16];
   const size_t r0 = (size_t)blockIdx.x * KRB + threadIdx.x;
   double a[KRP], b[KRP];
 #pragma unroll
@@ -135,35 +159,45 @@ __global__ void __launch_bounds__(KRT) gm_dots_kernel(GmDev G, const double* __r
     b[q] = (v2 && r < n) ? v2[r] : 0.0;
   }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int c0 = 0; c0 < ncols; c0 += 4) {
-    double acc[8];
+  for (int c0 = blockIdx.y * 8; c0 < ncols; c0 += DOT_CG * 8) {
+    double qv[8][KRP];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      acc[c] = 0.0;
-      acc[4 + c] = 0.0;
-      if (c0 + c < ncols) {
-        const double* q = Q + (size_t)(c0 + c) * ldq;
+    for (int c = 0; c < 8; ++c) {
+      const double* q = Q + (size_t)(c0 + c) * ldq;
 #pragma unroll
-        for (int qq = 0; qq < KRP; ++qq) {
-          const size_t r = r0 + (size_t)qq * KRT;
-          if (r < n) {
-            const double qv = q[r];
-            acc[c] += qv * a[qq];
-            acc[4 + c] += qv * b[qq];
-          }
-        }
+      for (int qq = 0; qq < KRP; ++qq) {
+        const size_t r = r0 + (size_t)qq * KRT;
+        qv[c][qq] = (c0 + c < ncols && r < n) ? q[r] : 0.0;
       }
     }
+    double acc[16];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) acc[c] = wave_sum(acc[c]);
+    for (int c = 0; c < 8; ++c) {
+      acc[c] = 0.0;
+      acc[8 + c] = 0.0;
+#pragma unroll
+      for (int qq = 0; qq < KRP; ++qq) {
+        acc[c] += qv[c][qq] * a[qq];
+        acc[8 + c] += qv[c][qq] * b[qq];
+      }
+    }
+    if (v1) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) acc[c] = wave_sum(acc[c]);
+    }
+    if (v2) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) acc[8 + c] = wave_sum(acc[8 + c]);
+    }
     if (lane == 0)
 #pragma unroll
-      for (int c = 0; c < 8; ++c) red[wv * 8 + c] = acc[c];
+      for (int c = 0; c < 16; ++c) red[wv * 16 + c] = acc[c];
     __syncthreads();
-    if (threadIdx.x < 8) {
-      const int c = threadIdx.x & 3, which = threadIdx.x >> 2;
-      const double v = (red[threadIdx.x] + red[8 + threadIdx.x]) + (red[16 + threadIdx.x] + red[24 + threadIdx.x]);
-      if (c0 + c < ncols) partial[((size_t)which * ncols + c0 + c) * nblk + blockIdx.x] = v;
+    if (threadIdx.x < 16) {
+      const int c = threadIdx.x & 7, which = threadIdx.x >> 3;
+      const bool have = which == 0 ? v1 != nullptr : v2 != nullptr;
+      const double v = (red[threadIdx.x] + red[16 + threadIdx.x]) + (red[32 + threadIdx.x] + red[48 + threadIdx.x]);
+      if (have && c0 + c < ncols) partial[((size_t)which * ncols + c0 + c) * nblk + blockIdx.x] = v;
     }
     __syncthreads();
   }
@@ -218,17 +252,27 @@ __global__ void __launch_bounds__(KRT) gm_update_kernel(GmDev G, double* __restr
 #pragma unroll
   for (int q = 0; q < KRP; ++q) as[q] = at[q] = 0.0;
   const int mm = m > mt ? m : mt;
-  for (int c = 0; c < mm; ++c) {
-    const double sv = (qt && c < m) ? S(c) : 0.0;
-    const double tv = (w && c < mt) ? S(m + c) : 0.0;
-    const double* q = Q + (size_t)c * ldq;
+  constexpr int UC = 4;  // columns whose loads are in flight together
+  for (int c0 = 0; c0 < mm; c0 += UC) {
+    double qv[UC][KRP];
 #pragma unroll
-    for (int qq = 0; qq < KRP; ++qq) {
-      const size_t r = r0 + (size_t)qq * KRT;
-      if (r < n) {
-        const double qv = q[r];
-        as[qq] += sv * qv;
-        at[qq] += tv * qv;
+    for (int u = 0; u < UC; ++u) {
+      const double* q = Q + (size_t)(c0 + u) * ldq;
+#pragma unroll
+      for (int qq = 0; qq < KRP; ++qq) {
+        const size_t r = r0 + (size_t)qq * KRT;
+        qv[u][qq] = (c0 + u < mm && r < n) ? q[r] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UC; ++u) {
+      const int c = c0 + u;
+      const double sv = (qt && c < m) ? S(c) : 0.0;
+      const double tv = (w && c < mt) ? S(m + c) : 0.0;
+#pragma unroll
+      for (int qq = 0; qq < KRP; ++qq) {
+        as[qq] += sv * qv[u][qq];
+        at[qq] += tv * qv[u][qq];
       }
     }
   }
@@ -259,11 +303,12 @@ __global__ void __launch_bounds__(KRT) gm_update_kernel(GmDev G, double* __restr
 }
 
 // ---- control kernels (one workgroup) --------------------------------------------
-// after the main dots pass
+// after the main dots pass: finalise column k-1 (when the previous vector was
+// provisional), commit it to the least squares, and set up the update
 __global__ void __launch_bounds__(GT) gm_ctl1_kernel(GmDev G) {
   if (G.ictl[GM_STATUS] != 0) return;
   __shared__ double sh[8];
-  __shared__ double col[2048 + 4];
+  __shared__ double col[GCAP], sCS[GCAP], sSN[GCAP];
   __shared__ int s_done;
   const int k = G.ictl[GM_K];
   const bool prov = G.ictl[GM_PROV] != 0;
@@ -282,12 +327,19 @@ __global__ void __launch_bounds__(GT) gm_ctl1_kernel(GmDev G) {
     const double nu = sqrt(fmax(alpha - ss, 0.0));
     const double eta = G.dctl[2];
     // finalise column k-1 of H
-    for (int i = threadIdx.x; i < k; i += GT) *Hp(G, i, k - 1) = G.hprov[i] + eta * s[i];
-    if (threadIdx.x == 0) *Hp(G, k, k - 1) = eta * nu;
+    for (int i = threadIdx.x; i < k; i += GT) {
+      const double h = G.hprov[i] + eta * s[i];
+      *Hp(G, i, k - 1) = h;
+      col[i] = h;
+    }
+    if (threadIdx.x == 0) {
+      *Hp(G, k, k - 1) = eta * nu;
+      col[k] = eta * nu;
+    }
+    givens_stage(G, k - 1, sCS, sSN);
     __syncthreads();
     if (threadIdx.x == 0) {
-      for (int i = 0; i <= k; ++i) col[i] = *Hp(G, i, k - 1);
-      const double rel = givens_column(G, k - 1, col, true);
+      const double rel = givens_sweep(G, k - 1, col, sCS, sSN, true);
       G.hist[k - 1] = rel;
       s_done = 0;
       if (rel < G.dctl[0] || k == G.ictl[GM_MAXIT]) {
@@ -298,26 +350,19 @@ __global__ void __launch_bounds__(GT) gm_ctl1_kernel(GmDev G) {
         s_done = 1;
       }
       G.dctl[3] = nu;
-    }
-    __syncthreads();
-    if (s_done) return;
-    // projections of A P^-1 q_k on q_0..q_k, corrected for the provisional input
-    for (int l = threadIdx.x; l <= k; l += GT) {
-      const double proj = l < k ? t[l] : (tau - st) / nu;
-      double c = 0.0;
-      for (int i = (l > 0 ? l - 1 : 0); i < k; ++i) c += *Hp(G, l, i) * s[i];
-      G.hnew[l] = (proj - c) / nu;
-      if (l < k) {
-        G.coef[l] = s[l];
-        G.coef[k + l] = t[l];
-      } else {
-        G.coef[2 * k] = 1.0 / nu;
-        G.coef[2 * k + 1] = proj;
-      }
-    }
-    if (threadIdx.x == 0) {
+      // the projections of A P^-1 q_k on q_0..q_k are finished by gm_hs_kernel
+      G.ictl[GM_HS] = 1;
       G.ictl[GM_M] = k;
       G.ictl[GM_MT] = k;
+      G.coef[2 * k] = 1.0 / nu;
+      G.coef[2 * k + 1] = (tau - st) / nu;
+    }
+    __syncthreads();
+    givens_store(G, k - 1, col);
+    if (s_done) return;
+    for (int l = threadIdx.x; l < k; l += GT) {
+      G.coef[l] = s[l];
+      G.coef[k + l] = t[l];
     }
   } else {
     for (int l = threadIdx.x; l <= k; l += GT) {
@@ -329,37 +374,67 @@ __global__ void __launch_bounds__(GT) gm_ctl1_kernel(GmDev G) {
       G.coef[k + 2] = 0.0;
       G.ictl[GM_M] = 0;
       G.ictl[GM_MT] = k + 1;
+      G.ictl[GM_HS] = 0;
       G.dctl[3] = 1.0;
     }
+  }
+}
+
+// hnew_l = (proj_l - sum_i H(l,i) s_i) / nu for l = 0..k: the projections of
+// A P^-1 q_k on the final basis, corrected for the provisional input.  One
+// wavefront per row of H (rows are contiguous); grid = max_krylov + 1 blocks.
+__global__ void __launch_bounds__(64) gm_hs_kernel(GmDev G) {
+  if (G.ictl[GM_STATUS] != 0 || G.ictl[GM_HS] == 0) return;
+  const int k = G.ictl[GM_K];
+  const int l = blockIdx.x;
+  if (l > k) return;
+  const double* s = G.red;
+  const double* t = G.red + (k + 1);
+  const double* row = Hp(G, l, 0);
+  double c = 0.0;
+  for (int i = (l > 0 ? l - 1 : 0) + (int)threadIdx.x; i < k; i += 64) c += row[i] * s[i];
+  c = wave_sum(c);
+  if (threadIdx.x == 0) {
+    const double proj = l < k ? t[l] : G.coef[2 * k + 1];
+    G.hnew[l] = (proj - c) / G.dctl[3];
   }
 }
 
 // after the main update: normalisation, breakdown, predicted convergence
 __global__ void __launch_bounds__(GT) gm_ctl2_kernel(GmDev G) {
   if (G.ictl[GM_STATUS] != 0) return;
-  __shared__ double col[2048 + 4];
+  __shared__ double col[GCAP], sCS[GCAP], sSN[GCAP];
+  __shared__ int s_brk;
   const int k = G.ictl[GM_K];
   const double etar = sqrt(G.red[0]);
   const double eta = etar / G.dctl[3];
-  for (int l = threadIdx.x; l <= k; l += GT) G.hprov[l] = G.hnew[l];
+  for (int l = threadIdx.x; l <= k; l += GT) {
+    const double h = G.hnew[l];
+    G.hprov[l] = h;
+    col[l] = h;
+  }
+  givens_stage(G, k, sCS, sSN);
   __syncthreads();
   if (threadIdx.x == 0) {
     G.dctl[2] = eta;
-    if (!(etar > 0.0)) {
-      for (int i = 0; i <= k; ++i) *Hp(G, i, k) = G.hprov[i];
+    s_brk = !(etar > 0.0);
+    if (s_brk) {
+      // exact breakdown: the Krylov space is invariant, column k is exact
+      for (int i = 0; i <= k; ++i) *Hp(G, i, k) = col[i];
       *Hp(G, k + 1, k) = 0.0;
-      for (int i = 0; i <= k + 1; ++i) col[i] = *Hp(G, i, k);
-      const double rel = givens_column(G, k, col, true);
+      col[k + 1] = 0.0;
+      const double rel = givens_sweep(G, k, col, sCS, sSN, true);
       G.hist[k] = rel;
       set_done(G, 3, k + 1, rel);
-      return;
+    } else {
+      G.dctl[5] = 1.0 / etar;
+      col[k + 1] = eta;
+      const double est = givens_sweep(G, k, col, sCS, sSN, false);
+      G.ictl[GM_FIN] = (est < G.dctl[0] || k + 1 == G.ictl[GM_MAXIT]) ? 1 : 0;
     }
-    G.dctl[5] = 1.0 / etar;
-    for (int i = 0; i <= k; ++i) col[i] = G.hprov[i];
-    col[k + 1] = eta;
-    const double est = givens_column(G, k, col, false);
-    G.ictl[GM_FIN] = (est < G.dctl[0] || k + 1 == G.ictl[GM_MAXIT]) ? 1 : 0;
   }
+  __syncthreads();
+  if (s_brk) givens_store(G, k, col);
 }
 
 // Q col k+1 = w / ||w||
@@ -377,7 +452,7 @@ __global__ void gm_scale_kernel(GmDev G, const double* __restrict__ wbuf, double
 __global__ void __launch_bounds__(GT) gm_ctl3_kernel(GmDev G) {
   if (G.ictl[GM_STATUS] != 0 || G.ictl[GM_FIN] == 0) return;
   __shared__ double sh[8];
-  __shared__ double col[2048 + 4];
+  __shared__ double col[GCAP], sCS[GCAP], sSN[GCAP];
   __shared__ int s_done;
   const int k = G.ictl[GM_K];
   const double* s = G.red;  // k+2 values: s_0..s_k, alpha
@@ -386,12 +461,19 @@ __global__ void __launch_bounds__(GT) gm_ctl3_kernel(GmDev G) {
   ss = block_sum(ss, sh);
   const double nu2 = sqrt(fmax(s[k + 1] - ss, 0.0));
   const double eta = G.dctl[2];
-  for (int i = threadIdx.x; i <= k; i += GT) *Hp(G, i, k) = G.hprov[i] + eta * s[i];
-  if (threadIdx.x == 0) *Hp(G, k + 1, k) = eta * nu2;
+  for (int i = threadIdx.x; i <= k; i += GT) {
+    const double h = G.hprov[i] + eta * s[i];
+    *Hp(G, i, k) = h;
+    col[i] = h;
+  }
+  if (threadIdx.x == 0) {
+    *Hp(G, k + 1, k) = eta * nu2;
+    col[k + 1] = eta * nu2;
+  }
+  givens_stage(G, k, sCS, sSN);
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int i = 0; i <= k + 1; ++i) col[i] = *Hp(G, i, k);
-    const double rel = givens_column(G, k, col, true);
+    const double rel = givens_sweep(G, k, col, sCS, sSN, true);
     G.hist[k] = rel;
     s_done = 0;
     if (rel < G.dctl[0] || k + 1 == G.ictl[GM_MAXIT]) {
@@ -403,6 +485,7 @@ __global__ void __launch_bounds__(GT) gm_ctl3_kernel(GmDev G) {
     }
   }
   __syncthreads();
+  givens_store(G, k, col);
   if (s_done) return;
   for (int i = threadIdx.x; i <= k; i += GT) G.coef[i] = s[i];
   if (threadIdx.x == 0) {
@@ -489,16 +572,17 @@ hipError_t gm_launch_start(const GmDev& G, double tol, int max_iter, const doubl
 
 hipError_t gm_launch_basis(const GmDev& G, double* Q, size_t ldq, double* wbuf, size_t n, double* partial,
                            int nblk, hipStream_t st) {
-  // main dots, reduce, control 1, update, norm reduce, control 2, scale
-  hipLaunchKernelGGL(gm_dots_kernel, dim3(nblk), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 0, partial, nblk);
+  // main dots, reduce, control 1, H s, update, norm reduce, control 2, scale
+  hipLaunchKernelGGL(gm_dots_kernel, dim3(nblk, DOT_CG), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 0, partial, nblk);
   hipLaunchKernelGGL(gm_reduce_kernel, dim3(256), dim3(256), 0, st, G, partial, nblk, 0);
   hipLaunchKernelGGL(gm_ctl1_kernel, dim3(1), dim3(GT), 0, st, G);
+  hipLaunchKernelGGL(gm_hs_kernel, dim3(G.M + 1), dim3(64), 0, st, G);
   hipLaunchKernelGGL(gm_update_kernel, dim3(nblk), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 0, partial);
   hipLaunchKernelGGL(gm_reduce_kernel, dim3(1), dim3(256), 0, st, G, partial, nblk, 2);
   hipLaunchKernelGGL(gm_ctl2_kernel, dim3(1), dim3(GT), 0, st, G);
   hipLaunchKernelGGL(gm_scale_kernel, dim3(1024), dim3(256), 0, st, G, wbuf, Q, ldq, n);
   // finalisation of the new vector when convergence is predicted
-  hipLaunchKernelGGL(gm_dots_kernel, dim3(nblk), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 1, partial, nblk);
+  hipLaunchKernelGGL(gm_dots_kernel, dim3(nblk, DOT_CG), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 1, partial, nblk);
   hipLaunchKernelGGL(gm_reduce_kernel, dim3(256), dim3(256), 0, st, G, partial, nblk, 1);
   hipLaunchKernelGGL(gm_ctl3_kernel, dim3(1), dim3(GT), 0, st, G);
   hipLaunchKernelGGL(gm_update_kernel, dim3(nblk), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 1, partial);

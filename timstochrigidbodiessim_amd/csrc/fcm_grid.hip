@@ -1007,6 +1007,124 @@ __global__ void __launch_bounds__(256) interp_fast_kernel(InterpArgs A, FieldPtr
   }
 }
 
+// Interpolation on 8 x TY x TZ tiles (TY, TZ = 16 or 8), one velocity
+// component per pass: the node box of the tile plus the stencil halo, one
+// component at a time, fits in 27-35 KB of LDS, so four workgroups share a CU
+// and the halo is re-read 3.3x instead of 4.2x.  Blobs are taken by anchor
+// bin (the tile owns 2 x 2^(TY/16) x 2^(TZ/16)... bins, listed as x,y bin pairs
+// with contiguous z bins), one wavefront per blob, lanes over (jy, jz), the x
+// stencil in a register loop, DPP reduction.
+template <typename T, int NG, int TY, int TZ>
+__global__ void __launch_bounds__(256) interp_slab_kernel(InterpArgs A, FieldPtrs<T> fp) {
+  if (fp.gate && *fp.gate) return;
+  constexpr int OMIN = 1 - NG / 2;
+  constexpr int HX = 8 + NG - 1, HY = TY + NG - 1, HZ = TZ + NG - 1;
+  constexpr int NBOX = HX * HY * HZ;
+  constexpr int NYB = TY / 8, NZB = TZ / 8;  // y and z bins per tile
+  constexpr int NR = 2 * NYB;                // bin ranges: (x bin, y bin) pairs
+  constexpr int ROWS = HX * HY;
+  static_assert(HZ <= 32, "z rows are loaded by 32 lanes");
+  __shared__ T box[NBOX];
+  __shared__ double vres[256][3];
+  __shared__ int s_lo[NR], s_cnt[NR + 1];
+  const GridGeom& g = A.g;
+  const BlobBins& bb = A.bb;
+  const int nty = g.ny / TY, ntz = g.nz / TZ;
+  const int ntiles = (g.nx / 8) * nty * ntz;
+  int tile = xcd_tile(blockIdx.x, ntiles);
+  if (tile >= ntiles) return;
+  const int tz = tile % ntz; tile /= ntz;
+  const int ty = tile % nty;
+  const int tx = tile / nty;
+  const int x0 = tx * 8, y0 = ty * TY, z0 = tz * TZ;
+  const int t = threadIdx.x;
+  if (t == 0) {
+    int acc = 0;
+    for (int r = 0; r < NR; ++r) {
+      const int xb = 2 * tx + (r / NYB), yb = ty * NYB + (r % NYB);
+      const int b0 = (xb * bb.nby + yb) * bb.nbz + tz * NZB;
+      s_lo[r] = A.bin_start[b0];
+      s_cnt[r] = acc;
+      acc += A.bin_start[b0 + NZB] - A.bin_start[b0];
+    }
+    s_cnt[NR] = acc;
+  }
+  __syncthreads();
+  const int nblob = s_cnt[NR];
+  if (nblob == 0) return;
+  const size_t rstride = 2 * (size_t)g.cpitch;
+  const T* W = wsel<T>(A.w64, A.w32);
+  const int lane = t & 63, wv = t >> 6;
+  const int zl = t & 31, rg = t >> 5;  // lanes over z, 8 row groups
+  int zg = z0 + OMIN + zl;
+  if (zg < 0) zg += g.nz; else if (zg >= g.nz) zg -= g.nz;
+  const bool zok = zl < HZ;
+  const int jy = lane / NG, jz = lane - (lane / NG) * NG;
+  const bool pair_ok = lane < NG * NG;
+
+  for (int chunk = 0; chunk < nblob; chunk += 256) {
+    const int nch = min(256, nblob - chunk);
+    for (int c = 0; c < 3; ++c) {
+      __syncthreads();
+      const T* __restrict__ src = reinterpret_cast<const T*>(fp.p[c]);
+      // stage the component's node box, 8 rows per pass, loads issued in groups
+      constexpr int PASSES = (ROWS + 7) / 8;
+      constexpr int GRP = 8;
+#pragma unroll
+      for (int p0 = 0; p0 < PASSES; p0 += GRP) {
+        T v[GRP];
+#pragma unroll
+        for (int i = 0; i < GRP; ++i) {
+          const int row = rg + 8 * (p0 + i);
+          v[i] = T(0);
+          if (p0 + i < PASSES && row < ROWS && zok) {
+            const int hx = row / HY, hy = row - (row / HY) * HY;
+            int xg = x0 + OMIN + hx, yg = y0 + OMIN + hy;
+            if (xg < 0) xg += g.nx; else if (xg >= g.nx) xg -= g.nx;
+            if (yg < 0) yg += g.ny; else if (yg >= g.ny) yg -= g.ny;
+            v[i] = src[((size_t)xg * g.ny + yg) * rstride + zg];
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < GRP; ++i) {
+          const int row = rg + 8 * (p0 + i);
+          if (p0 + i < PASSES && row < ROWS && zok) box[row * HZ + zl] = v[i];
+        }
+      }
+      __syncthreads();
+      for (int k = wv; k < nch; k += 4) {
+        const int kk = chunk + k;
+        int r = 0;
+#pragma unroll
+        for (int q = 1; q < NR; ++q) r += (kk >= s_cnt[q]) ? 1 : 0;
+        const int sidx = __builtin_amdgcn_readfirstlane(s_lo[r] + (kk - s_cnt[r]));
+        const int4 rc = A.rec[sidx];
+        const T* ws = W + (size_t)sidx * 3 * NG;
+        const int lx = rc.x - x0, ly = rc.y - y0, lz = rc.z - z0;
+        T a = T(0);
+        if (pair_ok) {
+          const int hyz = (ly + jy) * HZ + (lz + jz);
+          T sv = T(0);
+#pragma unroll
+          for (int jx = 0; jx < NG; ++jx) sv += ws[jx] * box[(lx + jx) * HY * HZ + hyz];
+          a = ws[NG + jy] * ws[2 * NG + jz] * sv;
+        }
+        const T v = wave_sum(a);
+        if (lane == 0) vres[k][c] = (double)v;
+      }
+    }
+    __syncthreads();
+    for (int k = t; k < nch; k += 256) {
+      const int kk = chunk + k;
+      int r = 0;
+#pragma unroll
+      for (int q = 1; q < NR; ++q) r += (kk >= s_cnt[q]) ? 1 : 0;
+      const int sidx = s_lo[r] + (kk - s_cnt[r]);
+      write_velocity(A, A.rec[sidx].w, vres[k][0], vres[k][1], vres[k][2]);
+    }
+  }
+}
+
 static bool interp_fast_ok(const GridGeom& g, const BlobBins& bb, int ng) {
   return spread_fast_ok(g, bb, ng) && bb.nbx % 2 == 0;
 }
@@ -1016,6 +1134,17 @@ hipError_t interp(const GridGeom& g, const BlobBins& bb, const BlobData& bd, con
                   const KTerm& kt, hipStream_t st) {
   InterpArgs A{g, bb, bd.ng, bd.bin_start, bd.rec, bd.w64, bd.w32, V, kt, g.dx * g.dx * g.dx};
   if (bd.nb == 0) return hipSuccess;
+  const int izb_choice = bd.interp_izb[sizeof(T) == 4 ? 1 : 0];
+  if (izb_choice == 3 && interp_fast_ok(g, bb, bd.ng) && (bd.ng == 7 || bd.ng == 8) && g.nz % 16 == 0 &&
+      g.ny % (sizeof(T) == 4 ? 16 : 16) == 0) {
+    constexpr int TZ = sizeof(T) == 4 ? 16 : 8;
+    const int ntiles = (g.nx / 8) * (g.ny / 16) * (g.nz / TZ);
+    if (bd.ng == 7)
+      hipLaunchKernelGGL((interp_slab_kernel<T, 7, 16, TZ>), dim3(8 * ((ntiles + 7) / 8)), dim3(256), 0, st, A, fp);
+    else
+      hipLaunchKernelGGL((interp_slab_kernel<T, 8, 16, TZ>), dim3(8 * ((ntiles + 7) / 8)), dim3(256), 0, st, A, fp);
+    return hipGetLastError();
+  }
   if (interp_fast_ok(g, bb, bd.ng) && (bd.ng == 7 || bd.ng == 8)) {
     const int IZB = (bd.interp_izb[sizeof(T) == 4 ? 1 : 0] == 1 || bb.nbz % 2 != 0) ? 1 : 2;
     const int TZ = 8 * IZB;

@@ -181,13 +181,14 @@ hipError_t update2(const double* Q, size_t ldq, int m, int mt, const double* coe
 hipError_t combine(const double* Q, size_t ldq, int ncols, const double* c, double* y, size_t n, hipStream_t st);
 // y = a*x (a from device pointer, taken as 1/sqrt(*a) if invsqrt)
 hipError_t scale_vec(const double* x, double* y, size_t n, const double* a, bool invsqrt, hipStream_t st);
+hipError_t copy16(const void* src, void* dst, size_t bytes, hipStream_t st);
 hipError_t axpby(double a, const double* x, double b, double* y, size_t n, hipStream_t st);  // y = a x + b y
 hipError_t rhs_assemble(const double* Vrand, const double* F6, double* b, size_t n3, size_t n6, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // device-resident GMRES control (fcm_gmres.hip)
 // ---------------------------------------------------------------------------
-enum { GM_K = 0, GM_PROV = 1, GM_STATUS = 2, GM_NCOLS = 3, GM_FIN = 4, GM_MAXIT = 5, GM_M = 6, GM_MT = 7, GM_NINT = 8 };
+enum { GM_K = 0, GM_PROV = 1, GM_STATUS = 2, GM_NCOLS = 3, GM_FIN = 4, GM_MAXIT = 5, GM_M = 6, GM_MT = 7, GM_HS = 8, GM_NINT = 9 };
 
 struct GmDev {
   int* ictl = nullptr;     // GM_* words; status 0 = running, 1 + FCM_GMRES_* when done

@@ -164,9 +164,124 @@ __global__ void __launch_bounds__(128) pinv_kernel(BodyData b, const double* __r
   }
 }
 
+// The same products with one wavefront per body and four bodies per
+// workgroup: ref_Minv (transposed, so that lanes read consecutive words) and
+// ref_N are staged in LDS once per workgroup.  Lane i < 3 n_b owns row i;
+// every sum runs over k in the order of pinv_kernel, so the results are the
+// same bit for bit.  Used when 3 n_b <= 64.
+static constexpr int PW_BODIES = 4;
+static constexpr int PW_SCR = 4 * 64 + 6 + 6 + 9;  // bp, tt, gg, oo, sv, vv, R per wave
+
+__global__ void __launch_bounds__(256) pinv_wave_kernel(BodyData b, const double* __restrict__ in,
+                                                        double* __restrict__ out, double* __restrict__ wKT) {
+  extern __shared__ double sm[];
+  if (b.gate && *b.gate) return;
+  if (b.kcol) in += (size_t)(*b.kcol) * b.ldq;
+  const int n3 = 3 * b.nbpb;
+  double* MT = sm;              // n3 x n3, MT[k n3 + i] = Minv[i n3 + k]
+  double* NR = MT + n3 * n3;    // 36
+  double* ref = NR + 36;        // n3
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  double* bp = ref + n3 + wv * PW_SCR;
+  double* tt = bp + 64;
+  double* gg = tt + 64;
+  double* oo = gg + 64;
+  double* sv = oo + 64;
+  double* vv = sv + 6;
+  double* Rs = vv + 6;
+  for (int e = t; e < n3 * n3; e += 256) {
+    const int i = e / n3, k = e - (e / n3) * n3;
+    MT[k * n3 + i] = b.Minv[e];
+  }
+  if (t < 36) NR[t] = b.Nref[t];
+  for (int i = t; i < n3; i += 256) ref[i] = b.ref[i];
+  const int n = blockIdx.x * PW_BODIES + wv;
+  const bool live = n < b.NB;
+  const size_t nlam = 3 * (size_t)b.NB * b.nbpb;
+  if (live && lane < 9) Rs[lane] = b.R[9 * (size_t)n + lane];
+  __syncthreads();
+  const int i = lane, m = i / 3, c = i - (i / 3) * 3;
+  const bool row = live && i < n3;
+  const double* inl = in + (size_t)n3 * n;
+  if (row) {
+    const double* v = inl + 3 * m;
+    bp[i] = Rs[c] * v[0] + Rs[3 + c] * v[1] + Rs[6 + c] * v[2];  // (R^T v)_c
+  }
+  __syncthreads();
+  if (row) {
+    double s = 0.0;
+    for (int k = 0; k < n3; ++k) s += MT[k * n3 + i] * bp[k];
+    tt[i] = s;
+  }
+  __syncthreads();
+  if (live && lane < 6) {
+    const int cc = lane % 3;
+    const double* v = in + nlam + 6 * (size_t)n + (lane < 3 ? 0 : 3);
+    const double rv = Rs[cc] * v[0] + Rs[3 + cc] * v[1] + Rs[6 + cc] * v[2];
+    double k = 0.0;
+    if (lane < 3) {
+      for (int mm = 0; mm < b.nbpb; ++mm) k += tt[3 * mm + cc];
+    } else {
+      const int c1 = (cc + 1) % 3, c2i = (cc + 2) % 3;
+      for (int mm = 0; mm < b.nbpb; ++mm) k += ref[3 * mm + c1] * tt[3 * mm + c2i] - ref[3 * mm + c2i] * tt[3 * mm + c1];
+    }
+    sv[lane] = -rv - k;
+  }
+  __syncthreads();
+  if (live && lane < 6) {
+    double s = 0.0;
+    for (int k = 0; k < 6; ++k) s += NR[lane * 6 + k] * sv[k];
+    vv[lane] = s;
+  }
+  __syncthreads();
+  if (row) {
+    const int c1 = (c + 1) % 3, c2i = (c + 2) % 3;
+    gg[i] = vv[c] + (vv[3 + c1] * ref[3 * m + c2i] - vv[3 + c2i] * ref[3 * m + c1]) + bp[i];
+  }
+  __syncthreads();
+  if (row) {
+    double s = 0.0;
+    for (int k = 0; k < n3; ++k) s += MT[k * n3 + i] * gg[k];
+    oo[i] = s;
+  }
+  __syncthreads();
+  if (row) {
+    const double* o = oo + 3 * m;
+    const double v = Rs[3 * c] * o[0] + Rs[3 * c + 1] * o[1] + Rs[3 * c + 2] * o[2];
+    out[(size_t)n3 * n + i] = v;
+    gg[i] = v;
+  }
+  if (live && lane < 6) {
+    const int cc = lane % 3;
+    const double* v = vv + (lane < 3 ? 0 : 3);
+    out[nlam + 6 * (size_t)n + lane] = Rs[3 * cc] * v[0] + Rs[3 * cc + 1] * v[1] + Rs[3 * cc + 2] * v[2];
+  }
+  if (wKT) {
+    __syncthreads();
+    if (live && lane < 6) {
+      const int cc = lane % 3;
+      double s = 0.0;
+      const double* d = b.dpos + 3 * (size_t)n * b.nbpb;
+      if (lane < 3) {
+        for (int mm = 0; mm < b.nbpb; ++mm) s += gg[3 * mm + cc];
+      } else {
+        const int c1 = (cc + 1) % 3, c2i = (cc + 2) % 3;
+        for (int mm = 0; mm < b.nbpb; ++mm) s += d[3 * mm + c1] * gg[3 * mm + c2i] - d[3 * mm + c2i] * gg[3 * mm + c1];
+      }
+      wKT[6 * (size_t)n + lane] = -s;
+    }
+  }
+}
+
 hipError_t apply_pinv(const BodyData& b, const double* in, double* out, double* wKT, hipStream_t st) {
   if (b.NB == 0) return hipSuccess;
   const int n3 = 3 * b.nbpb;
+  if (n3 <= 64) {
+    const size_t lds = sizeof(double) * ((size_t)n3 * n3 + 36 + n3 + PW_BODIES * (size_t)PW_SCR);
+    hipLaunchKernelGGL(pinv_wave_kernel, dim3((b.NB + PW_BODIES - 1) / PW_BODIES), dim3(256), lds, st, b, in, out,
+                       wKT);
+    return hipGetLastError();
+  }
   const size_t lds = sizeof(double) * (5 * (size_t)n3 + 21);
   hipLaunchKernelGGL(pinv_kernel, dim3(b.NB), dim3(128), lds, st, b, in, out, wKT);
   return hipGetLastError();
@@ -510,6 +625,21 @@ hipError_t scale_vec(const double* x, double* y, size_t n, const double* a, bool
 
 hipError_t axpby(double a, const double* x, double b, double* y, size_t n, hipStream_t st) {
   hipLaunchKernelGGL(axpby_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, x, b, y, n);
+  return hipGetLastError();
+}
+
+// plain streaming copy (16-byte words), the calibration kernel of the byte counters
+__global__ void __launch_bounds__(256) copy16_kernel(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                     size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+hipError_t copy16(const void* src, void* dst, size_t bytes, hipStream_t st) {
+  const size_t n = bytes / 16;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(copy16_kernel, dim3(8192), dim3(256), 0, st, static_cast<const float4*>(src),
+                     static_cast<float4*>(dst), n);
   return hipGetLastError();
 }
 

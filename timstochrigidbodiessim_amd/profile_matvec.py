@@ -4,8 +4,9 @@ rocprofv3 (kernel statistics and FETCH_SIZE / WRITE_SIZE counters):
 
     python -m timstochrigidbodiessim_amd.profile_matvec [--reps 3] [--grid 256]
 
-The copy moves `--copy-mb` MB (read + write) with torch's copy kernel; its
-counter values calibrate the byte counts of the FCM kernels."""
+The copy (fcm_profile_copy) reads and writes `--copy-mb` MiB with a plain
+streaming kernel; its counter values calibrate the byte counts of the FCM
+kernels."""
 import argparse
 import sys
 import time
@@ -36,17 +37,10 @@ def main():
         for name, prec in (("fp32", L.FCM_PREC_FP32), ("fp64", L.FCM_PREC_FP64)):
             ms = s.profile_matvec(prec, a.reps)
             print(f"[profile] {name}: " + ", ".join(f"{k} {v:.3f} ms" for k, v in ms.items()), flush=True)
-    try:
-        import torch
-        m = a.copy_mb * (1 << 20) // 4
-        src = torch.ones(m, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(a.reps):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(f"[profile] copy: {a.copy_mb} MiB read + {a.copy_mb} MiB written per launch", flush=True)
-    except Exception as e:  # calibration is optional
-        print(f"[profile] copy kernel skipped: {e!r}", flush=True)
+        nbytes = a.copy_mb << 20
+        ms = s.profile_copy(nbytes, a.reps)
+        print(f"[profile] copy: {a.copy_mb} MiB read + {a.copy_mb} MiB written per launch, {ms:.3f} ms, "
+              f"{2 * nbytes / (ms * 1e-3) / 1e9:.0f} GB/s", flush=True)
 
 
 if __name__ == "__main__":
