@@ -225,7 +225,8 @@ int fcm_batch_run(fcm_batch b);
 /* ---- launch shapes (no counterpart in the reference) ---------------------------- */
 /* Tunable launch shapes of a handle's mat-vec, per precision ([FCM_PREC_FP64],
  * [FCM_PREC_FP32]): the spreading kernel (0: stencil-start specialised x loop,
- * 1: branch-free tile-shifted weight tables) and the interpolation tile (1 or 2:
+ * 1: branch-free tile-shifted weight tables, 2: as 1 with a ballot over the
+ * blobs touching each wavefront's patch) and the interpolation tile (1 or 2:
  * 8x8x8 or 8x8x16 nodes with all three components staged; 3: 8x16x16 (FP32) or
  * 8x16x8 (FP64) staged one component at a time; 0 picks 2 when it divides).  Results do not depend
  * on the choice beyond rounding; FP64 parity holds for all. */

@@ -916,7 +916,7 @@ int fcm_set_tuning(fcm_handle h, const fcm_tuning* t) {
   if (int rc = check(h)) return rc;
   if (!t) return fail(FCM_ERR_ARG, "null tuning");
   for (int p = 0; p < 2; ++p)
-    if (t->spread_variant[p] < 0 || t->spread_variant[p] > 1 || t->interp_izb[p] < 0 || t->interp_izb[p] > 3)
+    if (t->spread_variant[p] < 0 || t->spread_variant[p] > 2 || t->interp_izb[p] < 0 || t->interp_izb[p] > 3)
       return fail(FCM_ERR_ARG, "bad tuning");
   HIPCHK(hipStreamSynchronize(h->st));
   for (int p = 0; p < 2; ++p) {
@@ -962,7 +962,7 @@ int fcm_autotune(fcm_handle h, int precision, int reps, fcm_tuning* chosen) {
   int rc = FCM_OK;
   float best = 0;
   int best_v = h->bd.spread_variant[pi];
-  for (int v = 0; v <= 1 && !rc; ++v) {
+  for (int v = 0; v <= 2 && !rc; ++v) {
     h->bd.spread_variant[pi] = v;
     float ms = 0;
     rc = time_it(spread_once, ms);

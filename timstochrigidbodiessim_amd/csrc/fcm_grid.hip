@@ -697,6 +697,135 @@ __global__ void __launch_bounds__(256) spread_dense_kernel(SpreadListArgs A, Fie
   }
 }
 
+// Variant 2: as variant 1, with the staging spread over the whole workgroup
+// (one blob per thread, SB blobs per batch: 256 in FP32, 128 in FP64), the z
+// weights kept raw (a lane picks its own from the blob's stencil start), and a
+// wavefront finding the blobs that touch its patch with one ballot per 64
+// blobs and walking the set bits in a scalar loop.  Per blob a lane reads its
+// y and z weight and three broadcast 16-byte words (8 shifted x weights and
+// the force), forms w F_c and does 24 fused multiply-adds.
+template <typename T> struct SpreadBatch { static constexpr int value = sizeof(T) == 4 ? 256 : 128; };
+
+template <typename T>
+__global__ void __launch_bounds__(256) spread_bits_kernel(SpreadListArgs A, FieldPtrs<T> fp) {
+  if (fp.gate && *fp.gate) return;
+  const GridGeom& g = A.g;
+  const int ng = A.ng, omin = 1 - ng / 2, NW = A.NW;
+  const int TZ = 8 * NW;
+  const int nty = g.ny / 8, ntz = g.nz / TZ;
+  const int ntiles = (g.nx / 8) * nty * ntz;
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  if (tile >= ntiles) return;
+  int tt = tile;
+  const int tz = tt % ntz; tt /= ntz;
+  const int ty = tt % nty;
+  const int tx = tt / nty;
+  const int x0 = tx * 8, y0 = ty * 8, z0 = tz * TZ;
+
+  constexpr int B = SpreadBatch<T>::value;
+  constexpr int TAB = B * (8 + 4 + 8 + 8);
+  constexpr int STAGE = 3 * 8 * 8 * 32;
+  __shared__ __align__(16) T buf[TAB > STAGE ? TAB : STAGE];
+  __shared__ int s_mask[B], s_sz[B];
+  T (*s_gx)[8] = reinterpret_cast<T(*)[8]>(buf);
+  T (*s_f)[4] = reinterpret_cast<T(*)[4]>(buf + B * 8);
+  T (*s_gy)[8] = reinterpret_cast<T(*)[8]>(buf + B * 12);
+  T (*s_gz)[8] = reinterpret_cast<T(*)[8]>(buf + B * 20);
+
+  const int t = threadIdx.x;
+  const int lane = t & 63, wv = t >> 6;
+  const int lz = lane & 7, ly = lane >> 3;
+  const int zl = 8 * wv + lz;
+  const int L0 = A.tile_start[tile], L1 = A.tile_start[tile + 1];
+  T acc[8][3];
+#pragma unroll
+  for (int xi = 0; xi < 8; ++xi) acc[xi][0] = acc[xi][1] = acc[xi][2] = T(0);
+  const T* W = wsel<T>(A.w64, A.w32);
+
+  for (int base = L0; base < L1; base += B) {
+    const int cnt = min(B, L1 - base);
+    if (t < cnt) {
+      const int s = A.tile_list[base + t];
+      const int4 r = A.rec[s];
+      const T* ws = W + (size_t)s * 3 * ng;
+      s_f[t][0] = T(A.F[3 * (size_t)r.w]);
+      s_f[t][1] = T(A.F[3 * (size_t)r.w + 1]);
+      s_f[t][2] = T(A.F[3 * (size_t)r.w + 2]);
+      s_f[t][3] = T(0);
+      const int sx = wrap_half(r.x + omin - x0, g.nx);
+      const int sy = wrap_half(r.y + omin - y0, g.ny);
+      const int sz = wrap_half(r.z + omin - z0, g.nz);
+#pragma unroll
+      for (int xi = 0; xi < 8; ++xi) {
+        int o = xi - sx;
+        if (o < 0) o += g.nx; else if (o >= g.nx) o -= g.nx;
+        s_gx[t][xi] = o < ng ? ws[o] : T(0);
+      }
+#pragma unroll
+      for (int yi = 0; yi < 8; ++yi) {
+        int o = yi - sy;
+        if (o < 0) o += g.ny; else if (o >= g.ny) o -= g.ny;
+        s_gy[t][yi] = o < ng ? ws[ng + o] : T(0);
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i) s_gz[t][i] = i < ng ? ws[2 * ng + i] : T(0);
+      // wavefront w covers tile z in [8w, 8w + 8): the stencil covers z = sz + o, o < ng (mod nz)
+      int mask = 0;
+      for (int w = 0; w < NW; ++w) {
+        int o0 = 8 * w - sz;  // first offset of the slab, wrapped into [0, nz)
+        if (o0 < 0) o0 += g.nz; else if (o0 >= g.nz) o0 -= g.nz;
+        const int o7 = o0 + 7 < g.nz ? o0 + 7 : o0 + 7 - g.nz;  // last offset
+        if (o0 < ng || o7 < ng || (o0 > o7)) mask |= 1 << w;  // o0 > o7: the slab wraps through offset 0
+      }
+      s_mask[t] = mask;
+      s_sz[t] = sz;
+    }
+    __syncthreads();
+    if (wv < NW) {
+#pragma unroll
+      for (int q = 0; q < B / 64; ++q) {
+        const int pl = 64 * q + lane;
+        const bool mine = pl < cnt && ((s_mask[pl] >> wv) & 1);
+        unsigned long long bits = __ballot(mine);
+        while (bits) {
+          const int p = 64 * q + __builtin_ctzll(bits);
+          bits &= bits - 1;
+          int oz = zl - s_sz[p];
+          if (oz < 0) oz += g.nz; else if (oz >= g.nz) oz -= g.nz;
+          const T gz = s_gz[p][oz < 8 ? oz : 7];
+          const T w = s_gy[p][ly] * (oz < ng ? gz : T(0));
+          const T wf0 = w * s_f[p][0], wf1 = w * s_f[p][1], wf2 = w * s_f[p][2];
+#pragma unroll
+          for (int xi = 0; xi < 8; ++xi) {
+            const T gx = s_gx[p][xi];
+            acc[xi][0] += wf0 * gx;
+            acc[xi][1] += wf1 * gx;
+            acc[xi][2] += wf2 * gx;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  T* stage = buf;
+  if (wv < NW) {
+#pragma unroll
+    for (int xi = 0; xi < 8; ++xi)
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) stage[((cc * 8 + xi) * 8 + ly) * TZ + zl] = acc[xi][cc];
+  }
+  __syncthreads();
+  const size_t rstride = 2 * (size_t)g.cpitch;
+  const int total = 3 * 8 * 8 * TZ;
+  for (int e = t; e < total; e += 256) {
+    const int zi = e % TZ;
+    const int row = e / TZ;
+    const int yi = row & 7, xi = (row >> 3) & 7, cc = row >> 6;
+    const size_t idx = ((size_t)(x0 + xi) * g.ny + (y0 + yi)) * rstride + z0 + zi;
+    reinterpret_cast<T*>(fp.p[cc])[idx] = stage[e];
+  }
+}
+
 bool spread_fast_ok(const GridGeom& g, const BlobBins& bb, int ng) {
   return ng <= 8 && g.nx >= 16 && g.nx % 8 == 0 && g.ny % 8 == 0 && g.nz % 8 == 0 && bb.bx == 4 && bb.by == 8 &&
          bb.bz == 8;
@@ -726,7 +855,10 @@ hipError_t spread(const GridGeom& g, const BlobBins& bb, const BlobData& bd, con
     const int NW = g.nz % 32 == 0 ? 4 : (g.nz % 16 == 0 ? 2 : 1);
     const int nt = spread_tiles(g);
     SpreadListArgs A{g, bd.ng, NW, bd.tile_start, bd.tile_list, bd.rec, bd.w64, bd.w32, F};
-    if (bd.spread_variant[sizeof(T) == 4 ? 1 : 0] == 1)
+    const int variant = bd.spread_variant[sizeof(T) == 4 ? 1 : 0];
+    if (variant == 2)
+      hipLaunchKernelGGL((spread_bits_kernel<T>), dim3(8 * ((nt + 7) / 8)), dim3(256), 0, st, A, fp);
+    else if (variant == 1)
       hipLaunchKernelGGL((spread_dense_kernel<T>), dim3(8 * ((nt + 7) / 8)), dim3(256), 0, st, A, fp);
     else
       hipLaunchKernelGGL((spread_fast_kernel<T>), dim3(8 * ((nt + 7) / 8)), dim3(256), 0, st, A, fp);
@@ -1055,10 +1187,22 @@ __global__ void __launch_bounds__(256) interp_slab_kernel(InterpArgs A, FieldPtr
   const size_t rstride = 2 * (size_t)g.cpitch;
   const T* W = wsel<T>(A.w64, A.w32);
   const int lane = t & 63, wv = t >> 6;
-  const int zl = t & 31, rg = t >> 5;  // lanes over z, 8 row groups
-  int zg = z0 + OMIN + zl;
-  if (zg < 0) zg += g.nz; else if (zg >= g.nz) zg -= g.nz;
-  const bool zok = zl < HZ;
+  // the box is loaded plane by plane in x; thread t owns the (y, z) entries
+  // e = t + 256 j of every plane, whose offsets are fixed for the tile
+  constexpr int HYZ = HY * HZ;
+  constexpr int JN = (HYZ + 255) / 256;
+  size_t yzoff[JN];
+  bool yzok[JN];
+#pragma unroll
+  for (int j = 0; j < JN; ++j) {
+    const int e = t + 256 * j;
+    yzok[j] = e < HYZ;
+    const int hy = e / HZ, hz = e - (e / HZ) * HZ;
+    int yg = y0 + OMIN + hy, zg = z0 + OMIN + hz;
+    if (yg < 0) yg += g.ny; else if (yg >= g.ny) yg -= g.ny;
+    if (zg < 0) zg += g.nz; else if (zg >= g.nz) zg -= g.nz;
+    yzoff[j] = yzok[j] ? (size_t)yg * rstride + zg : 0;
+  }
   const int jy = lane / NG, jz = lane - (lane / NG) * NG;
   const bool pair_ok = lane < NG * NG;
 
@@ -1067,30 +1211,20 @@ __global__ void __launch_bounds__(256) interp_slab_kernel(InterpArgs A, FieldPtr
     for (int c = 0; c < 3; ++c) {
       __syncthreads();
       const T* __restrict__ src = reinterpret_cast<const T*>(fp.p[c]);
-      // stage the component's node box, 8 rows per pass, loads issued in groups
-      constexpr int PASSES = (ROWS + 7) / 8;
-      constexpr int GRP = 8;
+      T pv[HX][JN];
 #pragma unroll
-      for (int p0 = 0; p0 < PASSES; p0 += GRP) {
-        T v[GRP];
+      for (int hx = 0; hx < HX; ++hx) {
+        int xg = x0 + OMIN + hx;
+        if (xg < 0) xg += g.nx; else if (xg >= g.nx) xg -= g.nx;
+        const T* plane = src + (size_t)xg * g.ny * rstride;
 #pragma unroll
-        for (int i = 0; i < GRP; ++i) {
-          const int row = rg + 8 * (p0 + i);
-          v[i] = T(0);
-          if (p0 + i < PASSES && row < ROWS && zok) {
-            const int hx = row / HY, hy = row - (row / HY) * HY;
-            int xg = x0 + OMIN + hx, yg = y0 + OMIN + hy;
-            if (xg < 0) xg += g.nx; else if (xg >= g.nx) xg -= g.nx;
-            if (yg < 0) yg += g.ny; else if (yg >= g.ny) yg -= g.ny;
-            v[i] = src[((size_t)xg * g.ny + yg) * rstride + zg];
-          }
-        }
-#pragma unroll
-        for (int i = 0; i < GRP; ++i) {
-          const int row = rg + 8 * (p0 + i);
-          if (p0 + i < PASSES && row < ROWS && zok) box[row * HZ + zl] = v[i];
-        }
+        for (int j = 0; j < JN; ++j) pv[hx][j] = yzok[j] ? plane[yzoff[j]] : T(0);
       }
+#pragma unroll
+      for (int hx = 0; hx < HX; ++hx)
+#pragma unroll
+        for (int j = 0; j < JN; ++j)
+          if (yzok[j]) box[hx * HYZ + t + 256 * j] = pv[hx][j];
       __syncthreads();
       for (int k = wv; k < nch; k += 4) {
         const int kk = chunk + k;

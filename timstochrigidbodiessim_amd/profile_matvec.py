@@ -23,6 +23,9 @@ def main():
     ap.add_argument("--spheres", type=int, default=12)
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--copy-mb", type=int, default=512)
+    ap.add_argument("--spread-variant", type=int, default=None, help="0 or 1 (both precisions)")
+    ap.add_argument("--interp-izb", type=int, default=None, help="1, 2 or 3 (both precisions)")
+    ap.add_argument("--autotune", action="store_true")
     a = ap.parse_args()
     n = a.grid
     t0 = time.perf_counter()
@@ -34,6 +37,15 @@ def main():
         s.set_geometry(ref)
         s.set_bodies(x, q)
         print(f"[profile] set-up {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+        if a.autotune:
+            s.autotune(L.FCM_PREC_FP32, 3)
+            s.autotune(L.FCM_PREC_FP64, 3)
+        if a.spread_variant is not None or a.interp_izb is not None:
+            t = s.tuning()
+            sv = [a.spread_variant] * 2 if a.spread_variant is not None else t["spread_variant"]
+            iz = [a.interp_izb] * 2 if a.interp_izb is not None else t["interp_izb"]
+            s.set_tuning(sv, iz)
+        print(f"[profile] launch shapes {s.tuning()}", flush=True)
         for name, prec in (("fp32", L.FCM_PREC_FP32), ("fp64", L.FCM_PREC_FP64)):
             ms = s.profile_matvec(prec, a.reps)
             print(f"[profile] {name}: " + ", ".join(f"{k} {v:.3f} ms" for k, v in ms.items()), flush=True)
