@@ -40,20 +40,22 @@ def percentile(xs, p):
     return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
 
 
-def pass_bytes(nx, ny, nz, nb, word):
+def pass_bytes(nx, ny, nz, nb, word, column=False):
     """Algorithmic HBM bytes per pass of one saddle mat-vec: S real words per
     field (S = nx*ny*2*(nz/2+1)); spread writes 3 fields, every FFT pass reads
     and writes 3 fields, interpolation reads 3 fields; per blob the spread
     reads position+force+permutation (52 B) and interpolation reads
-    position+permutation and writes the velocity (52 B)."""
+    position+permutation and writes the velocity (52 B).  On the column path
+    the z passes happen inside spreading (which then writes k-space rows) and
+    interpolation (which reads them), so they move no bytes of their own."""
     S = nx * ny * 2 * (nz // 2 + 1)
     return {
         "spread": 3 * S * word + 52 * nb,
-        "z_forward": 6 * S * word,
+        "z_forward": 0 if column else 6 * S * word,
         "y_forward": 6 * S * word,
         "x_project": 6 * S * word,
         "y_inverse": 6 * S * word,
-        "z_inverse": 6 * S * word,
+        "z_inverse": 0 if column else 6 * S * word,
         "interp": 3 * S * word + 52 * nb,
     }
 
@@ -357,17 +359,18 @@ def main():
 
     # roofline of the dominant kernel (per-pass device time, HIP events on the solver's stream)
     prof = {}
+    column = s.tuning()["column"]
     for name, prec, word in (("fp32", fa.FCM_PREC_FP32, 4), ("fp64", fa.FCM_PREC_FP64, 8)):
         ms = s.profile_matvec(prec, args.profile_reps)
-        by = pass_bytes(n, n, n, nb, word)
+        by = pass_bytes(n, n, n, nb, word, column[prec] == 1)
         prof[name] = {k: {"ms": ms[k], "GBps": (by[k] / (ms[k] * 1e-3) / 1e9 if k in by and ms[k] > 0 else None)}
-                      for k in ms}
+                      for k in ms if ms[k] > 0}
         log(f"[bench] passes {name}: " + ", ".join(f"{k} {v['ms']:.3f} ms" + (f" {v['GBps']:.0f} GB/s" if v['GBps'] else "")
                                                  for k, v in prof[name].items()))
     head = prof["fp32"]
     dom = max((k for k in head if k != "KT"), key=lambda k: head[k]["ms"])
     achieved = head[dom]["GBps"]
-    mv_bytes = sum(pass_bytes(n, n, n, nb, 4).values())
+    mv_bytes = sum(pass_bytes(n, n, n, nb, 4, column[fa.FCM_PREC_FP32] == 1).values())
     mv_ms = sum(v["ms"] for v in head.values())
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

@@ -229,10 +229,18 @@ int fcm_batch_run(fcm_batch b);
  * blobs touching each wavefront's patch) and the interpolation tile (1 or 2:
  * 8x8x8 or 8x8x16 nodes with all three components staged; 3: 8x16x16 (FP32) or
  * 8x16x8 (FP64) staged one component at a time; 0 picks 2 when it divides).  Results do not depend
- * on the choice beyond rounding; FP64 parity holds for all. */
+ * on the choice beyond rounding; FP64 parity holds for all.  `column` selects
+ * the mobility product's path: 1 (the default where the grid qualifies: x and y
+ * multiples of 4 and >= 16, nz in {32,48,64,96,128,192,256,512}, ng <= 8)
+ * spreads onto x-y patches of whole z rows and does the forward z pass in the
+ * same kernel, and does the inverse z pass inside interpolation; 0 runs the
+ * spreading, five FFT passes and interpolation as separate kernels.  The
+ * stand-alone entry points (fcm_spread, fcm_stokes, fcm_interp) always use the
+ * separate kernels. */
 typedef struct {
   int spread_variant[2];
   int interp_izb[2];
+  int column[2];
 } fcm_tuning;
 int fcm_get_tuning(fcm_handle h, fcm_tuning* t);
 int fcm_set_tuning(fcm_handle h, const fcm_tuning* t);
@@ -245,7 +253,10 @@ int fcm_autotune(fcm_handle h, int precision, int reps, fcm_tuning* chosen);
 /* Runs one saddle-point mat-vec `reps` times on a fixed input with HIP events
  * between its passes, on the handle's stream; ms_per_pass[0..7] receives the
  * mean device time of: spread, z forward, y forward, x forward+projection+x
- * inverse, y inverse, z inverse, interpolation(+K), K^T. */
+ * inverse, y inverse, z inverse, interpolation(+K), K^T.  On the column path
+ * the spread slot holds the fused spread + z forward kernel, the interpolation
+ * slot the fused z inverse + interpolation kernel and its reduction, and the
+ * two z slots are 0. */
 int fcm_profile_matvec(fcm_handle h, int precision, int reps, double* ms_per_pass);
 /* A plain streaming copy of `bytes` (16-byte words) between two scratch buffers,
  * `reps` times, on the handle's stream; the reference point for HBM byte counters

@@ -245,3 +245,48 @@ def test_spread_interp_launch_variants_match_oracle(dims, ng, nb):
         t = s.autotune(fa.FCM_PREC_FP32, reps=2)
         say("autotuned", t)
         assert t["spread_variant"][1] in (0, 1, 2) and t["interp_izb"][1] in (1, 2, 3)
+
+
+# the column path of the mobility product on every z length it is built for,
+# dense lists (several staging rounds per patch), ng from 2 to 8, wrap-around
+# stencils on small boxes and non-cubic grids
+COLUMN_CASES = [((32, 32, 32), 7, 200), ((16, 16, 32), 7, 2000), ((32, 16, 48), 8, 300), ((16, 24, 96), 5, 150),
+                ((24, 16, 64), 2, 50), ((32, 32, 128), 6, 400), ((16, 16, 192), 7, 200), ((16, 16, 256), 7, 300),
+                ((16, 16, 512), 8, 300)]
+
+
+@pytest.mark.parametrize("dims,ng,nb", COLUMN_CASES)
+def test_column_mobility_matches_oracle(dims, ng, nb):
+    say("column mobility", dims, ng, nb)
+    dx = 0.9
+    gc, oc = pair_cfg(*dims, ng=ng, dx=dx, a=1.5, max_blobs=nb)
+    rng = np.random.default_rng(15)
+    Y = _blobs(rng, nb, dims, dx)
+    F = rng.standard_normal(3 * nb)
+    o = po.Fcm(oc, nb)
+    o.set_up_gaussians(Y)
+    ref = o.mobility(F)
+    with fa.FcmSolver(gc) as s:
+        s.set_up_gaussians(Y)
+        t = s.tuning()
+        assert t["column"] == [1, 1], t
+        col = {}
+        for prec, tol in ((fa.FCM_PREC_FP64, FP64_TOL), (fa.FCM_PREC_FP32, FP32_TOL)):
+            V = s.mobility(F, prec)
+            assert rel(V, ref) < tol, ("column", prec, rel(V, ref))
+            assert np.array_equal(V, s.mobility(F, prec)), "column path is not bitwise repeatable"
+            col[prec] = V
+        s.set_tuning(t["spread_variant"], t["interp_izb"], column=(0, 0))
+        for prec, tol in ((fa.FCM_PREC_FP64, FP64_TOL), (fa.FCM_PREC_FP32, FP32_TOL)):
+            V = s.mobility(F, prec)
+            assert rel(V, ref) < tol, ("separate", prec, rel(V, ref))
+            assert rel(V, col[prec]) < tol
+
+
+def test_column_path_needs_a_qualifying_grid():
+    gc, _ = pair_cfg(12, 12, 12, max_blobs=20)
+    with fa.FcmSolver(gc) as s:
+        t = s.tuning()
+        assert t["column"] == [0, 0]
+        with pytest.raises(fa.FcmError):
+            s.set_tuning(t["spread_variant"], t["interp_izb"], column=(1, 1))

@@ -38,10 +38,11 @@ PASS_NAMES = ["spread", "z_forward", "y_forward", "x_project", "y_inverse", "z_i
 
 
 class FcmTuning(C.Structure):
-    _fields_ = [("spread_variant", C.c_int * 2), ("interp_izb", C.c_int * 2)]
+    _fields_ = [("spread_variant", C.c_int * 2), ("interp_izb", C.c_int * 2), ("column", C.c_int * 2)]
 
     def as_dict(self):
-        return {"spread_variant": list(self.spread_variant), "interp_izb": list(self.interp_izb)}
+        return {"spread_variant": list(self.spread_variant), "interp_izb": list(self.interp_izb),
+                "column": list(self.column)}
 
 
 class FcmConfig(C.Structure):
@@ -381,8 +382,10 @@ class FcmSolver:
         _chk(self.L.fcm_get_tuning(self.h, C.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, spread_variant=(0, 0), interp_izb=(0, 0)):
-        t = FcmTuning((C.c_int * 2)(*spread_variant), (C.c_int * 2)(*interp_izb))
+    def set_tuning(self, spread_variant=(0, 0), interp_izb=(0, 0), column=None):
+        if column is None:
+            column = self.tuning()["column"]
+        t = FcmTuning((C.c_int * 2)(*spread_variant), (C.c_int * 2)(*interp_izb), (C.c_int * 2)(*column))
         _chk(self.L.fcm_set_tuning(self.h, C.byref(t)))
 
     def autotune(self, precision=FCM_PREC_FP32, reps=5):

@@ -107,6 +107,14 @@ Sizes sizes_of(const fcm_config* c) {
   {
     GridGeom g2 = make_geom(c);
     t += 3 * (size_t)(spread_tiles(g2) + 1) * 4;
+    BlobBins bb2;
+    bb2.bx = largest_divisor_le(c->nx, 4);
+    bb2.by = largest_divisor_le(c->ny, 8);
+    bb2.bz = largest_divisor_le(c->nz, 8);
+    const int cx = col_patch_cx(g2, bb2, c->ng);
+    if (cx > 0)
+      t += (3 * (size_t)col_groups(g2, cx) + 1) * 4 + col_list_capacity(nb) * 20 + col_zst_words(g2, cx) * 4 +
+           (size_t)nb * COL_SLOTS_MAX * 3 * 8;
   }
   const int n3b = 3 * c->max_blobs_per_body;
   t += (size_t)(3 * n3b + n3b * n3b + 36) * 8 + (size_t)c->max_bodies * (3 + 4 + 9 + 6) * 8 + (size_t)nb * 3 * 8;
@@ -184,6 +192,38 @@ int sync_stream(fcm_handle h) {
 
 namespace fcm_host {
 
+// spread -> Stokes solve -> interpolation on one precision's fields; on the
+// column path the z passes are done inside spreading and interpolation
+template <typename T>
+int mobility_passes(fcm_handle h, const FieldPtrs<T>& fp, const TwiddleSet<T>& tw, bool column, const double* Fin,
+                    double* Vout, const KTerm& kt, hipStream_t st) {
+  if (column && h->bd.col_cx > 0) {
+    HIPCHK(spread_zfwd<T>(h->g, h->bd, Fin, fp, tw, st));
+    for (int p = 1; p <= 3; ++p) HIPCHK(stokes_pass<T>(h->g, fp, p, h->pp, tw, st));
+    HIPCHK(zinv_interp<T>(h->g, h->bd, fp, tw, Vout, kt, st));
+    return FCM_OK;
+  }
+  HIPCHK(spread<T>(h->g, h->bb, h->bd, Fin, fp, st));
+  HIPCHK(stokes_solve<T>(h->g, fp, h->pp, tw, st));
+  HIPCHK(interp<T>(h->g, h->bb, h->bd, fp, Vout, kt, st));
+  return FCM_OK;
+}
+
+// the passes of the column path with events between them (the z slots stay empty)
+template <typename T>
+int profile_column(fcm_handle h, const FieldPtrs<T>& fp, const TwiddleSet<T>& tw, const KTerm& kt, hipEvent_t* ev) {
+  HIPCHK(spread_zfwd<T>(h->g, h->bd, h->xacc, fp, tw, h->st));
+  HIPCHK(hipEventRecord(ev[1], h->st));
+  HIPCHK(hipEventRecord(ev[2], h->st));
+  for (int p = 1; p <= 3; ++p) {
+    HIPCHK(stokes_pass<T>(h->g, fp, p, h->pp, tw, h->st));
+    HIPCHK(hipEventRecord(ev[2 + p], h->st));
+  }
+  HIPCHK(hipEventRecord(ev[6], h->st));
+  HIPCHK(zinv_interp<T>(h->g, h->bd, fp, tw, h->tmp, kt, h->st));
+  return FCM_OK;
+}
+
 int mobility_dev(fcm_handle h, const double* Fin, double* Vout, int precision, const KTerm& kt, hipStream_t st,
                  const int* gate) {
   if (!h->have_blobs) return fail(FCM_ERR_STATE, "no blob positions set");
@@ -191,17 +231,11 @@ int mobility_dev(fcm_handle h, const double* Fin, double* Vout, int precision, c
     if (!h->f32[0]) return fail(FCM_ERR_STATE, "FP32 path not allocated (fcm_config.fp32 = 0)");
     FieldPtrs<float> fp = h->fp32();
     fp.gate = gate;
-    HIPCHK(spread<float>(h->g, h->bb, h->bd, Fin, fp, st));
-    HIPCHK(stokes_solve<float>(h->g, fp, h->pp, h->tw32, st));
-    HIPCHK(interp<float>(h->g, h->bb, h->bd, fp, Vout, kt, st));
-  } else {
-    FieldPtrs<double> fp = h->fp64();
-    fp.gate = gate;
-    HIPCHK(spread<double>(h->g, h->bb, h->bd, Fin, fp, st));
-    HIPCHK(stokes_solve<double>(h->g, fp, h->pp, h->tw64, st));
-    HIPCHK(interp<double>(h->g, h->bb, h->bd, fp, Vout, kt, st));
+    return mobility_passes<float>(h, fp, h->tw32, h->bd.col_fused[1] != 0, Fin, Vout, kt, st);
   }
-  return FCM_OK;
+  FieldPtrs<double> fp = h->fp64();
+  fp.gate = gate;
+  return mobility_passes<double>(h, fp, h->tw64, h->bd.col_fused[0] != 0, Fin, Vout, kt, st);
 }
 
 int saddle_dev(fcm_handle h, const double* in, double* out, int precision, hipStream_t st) {
@@ -409,6 +443,20 @@ int fcm_create(const fcm_config* cfg, fcm_handle* out) {
     if (e == hipSuccess) h->bd.tile_start = dalloc<int>(h, nt + 1, e);
     if (e == hipSuccess) h->bd.tile_cursor = dalloc<int>(h, nt, e);
     if (e == hipSuccess) h->bd.tile_list = dalloc<int>(h, 8 * (size_t)nb, e);
+  }
+  h->bd.ng = cfg->ng;
+  h->bd.col_cx = col_patch_cx(h->g, bb, cfg->ng);
+  if (h->bd.col_cx > 0) {
+    const int ng_ = col_groups(h->g, h->bd.col_cx);
+    if (e == hipSuccess) h->bd.col_count = dalloc<int>(h, ng_, e);
+    if (e == hipSuccess) h->bd.col_start = dalloc<int>(h, (size_t)ng_ + 1, e);
+    if (e == hipSuccess) h->bd.col_cursor = dalloc<int>(h, ng_, e);
+    if (e == hipSuccess) h->bd.col_list = dalloc<int>(h, col_list_capacity(nb), e);
+    if (e == hipSuccess) h->bd.col_ent = dalloc<int4>(h, col_list_capacity(nb), e);
+    if (e == hipSuccess) h->bd.col_zst = dalloc<int>(h, col_zst_words(h->g, h->bd.col_cx), e);
+    if (e == hipSuccess) h->bd.col_part = dalloc<double>(h, (size_t)nb * COL_SLOTS_MAX * 3, e);
+  } else {
+    h->bd.col_fused[0] = h->bd.col_fused[1] = 0;
   }
   h->bd.ng = cfg->ng;
   const int n3b = 3 * cfg->max_blobs_per_body;
@@ -868,9 +916,14 @@ int fcm_profile_matvec(fcm_handle h, int precision, int reps, double* ms_per_pas
   kt.nbpb = h->nbpb;
   BodyData b = h->body();
   int rc = FCM_OK;
+  const bool column = h->bd.col_cx > 0 && h->bd.col_fused[precision == FCM_PREC_FP32 ? 1 : 0] != 0;
   for (int r = 0; r < reps && rc == FCM_OK; ++r) {
     HIPCHK(hipEventRecord(ev[0], h->st));
-    if (precision == FCM_PREC_FP32) {
+    if (column) {
+      rc = precision == FCM_PREC_FP32 ? profile_column<float>(h, h->fp32(), h->tw32, kt, ev)
+                                      : profile_column<double>(h, h->fp64(), h->tw64, kt, ev);
+      if (rc) break;
+    } else if (precision == FCM_PREC_FP32) {
       HIPCHK(spread<float>(h->g, h->bb, h->bd, h->xacc, h->fp32(), h->st));
       HIPCHK(hipEventRecord(ev[1], h->st));
       for (int p = 0; p < 5; ++p) {
@@ -908,6 +961,7 @@ int fcm_get_tuning(fcm_handle h, fcm_tuning* t) {
   for (int p = 0; p < 2; ++p) {
     t->spread_variant[p] = h->bd.spread_variant[p];
     t->interp_izb[p] = h->bd.interp_izb[p];
+    t->column[p] = h->bd.col_fused[p];
   }
   return FCM_OK;
 }
@@ -916,12 +970,14 @@ int fcm_set_tuning(fcm_handle h, const fcm_tuning* t) {
   if (int rc = check(h)) return rc;
   if (!t) return fail(FCM_ERR_ARG, "null tuning");
   for (int p = 0; p < 2; ++p)
-    if (t->spread_variant[p] < 0 || t->spread_variant[p] > 2 || t->interp_izb[p] < 0 || t->interp_izb[p] > 3)
+    if (t->spread_variant[p] < 0 || t->spread_variant[p] > 2 || t->interp_izb[p] < 0 || t->interp_izb[p] > 3 ||
+        t->column[p] < 0 || t->column[p] > 1 || (t->column[p] == 1 && h->bd.col_cx <= 0))
       return fail(FCM_ERR_ARG, "bad tuning");
   HIPCHK(hipStreamSynchronize(h->st));
   for (int p = 0; p < 2; ++p) {
     h->bd.spread_variant[p] = t->spread_variant[p];
     h->bd.interp_izb[p] = t->interp_izb[p];
+    h->bd.col_fused[p] = t->column[p];
   }
   ++h->epoch;
   return FCM_OK;
@@ -984,6 +1040,22 @@ int fcm_autotune(fcm_handle h, int precision, int reps, fcm_tuning* chosen) {
     }
   }
   h->bd.interp_izb[pi] = best_z;
+  if (!rc && h->bd.col_cx > 0) {
+    // the column path against the separate passes, whole mobility products
+    auto product_once = [&]() -> int {
+      if (precision == FCM_PREC_FP32)
+        return mobility_passes<float>(h, h->fp32(), h->tw32, h->bd.col_fused[1] != 0, h->tmp, h->Vtmp, kt, h->st);
+      return mobility_passes<double>(h, h->fp64(), h->tw64, h->bd.col_fused[0] != 0, h->tmp, h->Vtmp, kt, h->st);
+    };
+    float ms_col = 0, ms_sep = 0;
+    h->bd.col_fused[pi] = 1;
+    rc = time_it(product_once, ms_col);
+    if (!rc) {
+      h->bd.col_fused[pi] = 0;
+      rc = time_it(product_once, ms_sep);
+    }
+    h->bd.col_fused[pi] = (!rc && ms_col <= ms_sep) ? 1 : 0;
+  }
   ++h->epoch;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

@@ -1,0 +1,720 @@
+// fcm_column.hip -- spreading and interpolation of the mobility product, each
+// fused with the z pass of the FFT that touches the same grid rows.
+//
+// A patch is cx x 4 grid rows (x, y) with all of z.  For every patch the blobs
+// whose x-y stencil touches it are listed once per set of positions, ordered by
+// the z of their first stencil node ("stencil start"), with a prefix table
+// over z.  One workgroup owns a patch and one thread owns one z:
+//
+//   spread + z R2C (col_spread_kernel): the thread sums F gx gy gz over exactly
+//   the blobs whose z stencil covers its z -- a contiguous range of the ordered
+//   list -- for every node of the patch rows, in registers.  The rows go
+//   through LDS into the nz/2-point complex FFT and the R2C untangling, and
+//   only k-space rows are written (fcm_fluid_solver.cpp:552-600 followed by the
+//   z stage of fft_solver.cpp:32-36).
+//
+//   z C2R + interpolation (col_interp_kernel): the k-space rows are brought
+//   back to real space in LDS (imaginary parts of kz = 0 and nz/2 ignored, as
+//   FFTW's halfcomplex format stores none), each thread keeps its z column of
+//   the patch in registers and, for every blob covering its z, forms
+//   gz sum_xy gx gy u.  The ng values of a blob are summed in a fixed order and
+//   the patch's partial sum goes to the blob's slot for this patch;
+//   col_reduce_kernel adds the (at most 5 x 3) slots of a blob in a fixed
+//   order and writes dx^3 J u - K V (fcm_fluid_solver.cpp:363-424 preceded by
+//   the z stage of fft_solver.cpp:38-42).
+//
+// Nothing is accumulated with atomics, so both are bitwise repeatable.  Per
+// product the grid is written once and read once in k space, instead of real
+// space being written, read back, transformed and written again on each side.
+#include "fcm_internal.hpp"
+
+namespace fcm {
+
+__device__ __forceinline__ int col_wrapi(int v, int n) {
+  const int r = v % n;
+  return r < 0 ? r + n : r;
+}
+__device__ __forceinline__ int col_fdiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+__device__ __forceinline__ int col_wrap_half(int v, int n) {
+  if (v < -(n / 2)) v += n; else if (v >= n - n / 2) v -= n;
+  return v;
+}
+// blocks are spread over the 8 XCDs so that each works on a contiguous range
+// of patches (neighbouring patches share blobs in that XCD's L2)
+__device__ __forceinline__ int col_xcd(int b, int n) {
+  const int per = (n + 7) / 8;
+  return (b % 8) * per + b / 8;
+}
+
+template <typename T> __device__ __forceinline__ const T* col_w(const double* w64, const float* w32);
+template <> __device__ __forceinline__ const double* col_w<double>(const double* w64, const float*) { return w64; }
+template <> __device__ __forceinline__ const float* col_w<float>(const double*, const float* w32) { return w32; }
+
+struct ColArgs {
+  GridGeom g;
+  int ng, npx, npy;
+  const int4* ent;  // entry records of the patch lists (col_entry)
+  const int* zst;
+  const double* w64;
+  const float* w32;
+};
+
+// One entry of a patch list: (sorted blob slot, blob id, packed offsets, index
+// of the blob's partial-sum slot for this patch).  The packed word holds the
+// stencil start relative to the patch in x and y (+8, 5 bits each) and the
+// wrapped stencil start in z (bits 10 and up).
+__device__ __forceinline__ int col_rx(int pk) { return (pk & 31) - 8; }
+__device__ __forceinline__ int col_ry(int pk) { return ((pk >> 5) & 31) - 8; }
+__device__ __forceinline__ int col_sz(int pk) { return pk >> 10; }
+
+// ---------------------------------------------------------------------------
+// patch lists
+// ---------------------------------------------------------------------------
+struct ColListArgs {
+  GridGeom g;
+  BlobBins bb;
+  int ng, cx, npx, npy;
+  const int* bin_start;
+  const int4* rec;
+  int* count;
+  const int* start;
+  int* list;
+  int4* ent;
+  int* zst;
+};
+
+__device__ __forceinline__ long long col_key(const int4* rec, int s, int omin, int nz) {
+  return ((long long)col_wrapi(rec[s].z + omin, nz) << 32) | (unsigned)s;
+}
+
+// One 64-thread block per patch; thread b takes the group of blobs with
+// stencil start z in [8b, 8b + 8).  Pass 0 counts, pass 1 writes the group in
+// (stencil start z, slot) order and its part of the z prefix table.
+__global__ void __launch_bounds__(64) col_list_kernel(ColListArgs A, int pass) {
+  const GridGeom& g = A.g;
+  const BlobBins& bb = A.bb;
+  const int ng = A.ng, omin = 1 - ng / 2, cx = A.cx;
+  const int p = blockIdx.x;
+  const int px = p / A.npy, py = p - (p / A.npy) * A.npy;
+  const int x0 = cx * px, y0 = COL_CY * py;
+  // anchor bins whose stencils can reach the patch in x and y
+  int xlo = col_fdiv(x0 - omin - ng + 1, bb.bx);
+  int xn = col_fdiv(x0 + cx - 1 - omin, bb.bx) - xlo + 1;
+  if (xn >= bb.nbx) { xlo = 0; xn = bb.nbx; }
+  int ylo = col_fdiv(y0 - omin - ng + 1, bb.by);
+  int yn = col_fdiv(y0 + COL_CY - 1 - omin, bb.by) - ylo + 1;
+  if (yn >= bb.nby) { ylo = 0; yn = bb.nby; }
+  const int nbz = bb.nbz;
+  for (int b = threadIdx.x; b < nbz; b += 64) {
+    // stencil starts [8b, 8b + 8) <=> anchors [8b - omin, 8b + 7 - omin]
+    const int zlo = col_fdiv(8 * b - omin, bb.bz), zhi = col_fdiv(8 * b + 7 - omin, bb.bz);
+    const int gi = p * nbz + b;
+    const int pos0 = pass == 1 ? A.start[gi] : 0;
+    int pos = pos0;
+    for (int i = 0; i < xn; ++i)
+      for (int j = 0; j < yn; ++j) {
+        const int bxy = col_wrapi(xlo + i, bb.nbx) * bb.nby + col_wrapi(ylo + j, bb.nby);
+        for (int zb = zlo; zb <= zhi; ++zb) {
+          const int bin = bxy * nbz + col_wrapi(zb, nbz);
+          const int s1 = A.bin_start[bin + 1];
+          for (int s = A.bin_start[bin]; s < s1; ++s) {
+            const int4 r = A.rec[s];
+            if ((col_wrapi(r.z + omin, g.nz) >> 3) != b) continue;
+            const int rx = col_wrap_half(col_wrapi(r.x + omin, g.nx) - x0, g.nx);
+            const int ry = col_wrap_half(col_wrapi(r.y + omin, g.ny) - y0, g.ny);
+            if (rx > cx - 1 || rx + ng - 1 < 0 || ry > COL_CY - 1 || ry + ng - 1 < 0) continue;
+            if (pass == 1) A.list[pos] = s;
+            ++pos;
+          }
+        }
+      }
+    if (pass == 0) {
+      A.count[gi] = pos;
+      continue;
+    }
+    for (int i = pos0 + 1; i < pos; ++i) {  // insertion sort of the group
+      const int v = A.list[i];
+      const long long kv = col_key(A.rec, v, omin, g.nz);
+      int j = i - 1;
+      while (j >= pos0 && col_key(A.rec, A.list[j], omin, g.nz) > kv) {
+        A.list[j + 1] = A.list[j];
+        --j;
+      }
+      A.list[j + 1] = v;
+    }
+    int* zst = A.zst + (size_t)p * (g.nz + 1);
+    int i = pos0;
+    for (int k = 0; k < 8; ++k) {
+      const int z = 8 * b + k;
+      while (i < pos && col_wrapi(A.rec[A.list[i]].z + omin, g.nz) < z) ++i;
+      zst[z] = i;
+    }
+    if (b == nbz - 1) zst[g.nz] = pos;
+    for (int e = pos0; e < pos; ++e) {
+      const int sl = A.list[e];
+      const int4 r = A.rec[sl];
+      const int sxw = col_wrapi(r.x + omin, g.nx), syw = col_wrapi(r.y + omin, g.ny);
+      const int rx = col_wrap_half(sxw - x0, g.nx), ry = col_wrap_half(syw - y0, g.ny);
+      const int dst = col_wrapi(px - sxw / cx, A.npx) * 3 + col_wrapi(py - syw / COL_CY, A.npy);
+      A.ent[e] = make_int4(sl, r.w, (rx + 8) | ((ry + 8) << 5) | (col_wrapi(r.z + omin, g.nz) << 10),
+                           sl * COL_SLOTS_MAX + dst);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launch shapes
+// ---------------------------------------------------------------------------
+constexpr int col_threads(int nz) { return (nz + 63) / 64 * 64; }  // one thread per z
+constexpr int col_max(int a, int b) { return a > b ? a : b; }
+// largest odd number <= v (LDS planes of an odd number of words are not
+// aligned with one another in the 64 banks)
+constexpr int col_odd_le(int v) { return (v % 2 == 1) ? v : v - 1; }
+
+// points per thread of the z pass: the plan whose rounds pad the 3 cx 4 rows
+// least, then the one with the smaller LDS scratch
+constexpr int col_pm(int NZ, int CX) {
+  const int nt = col_threads(NZ), rows = 3 * CX * COL_CY;
+  int best = 16, best_pad = 1 << 30, best_lds = 1 << 30;
+  for (int pm : {16, 8}) {
+    const FftPlan pl = make_plan(NZ / 2, pm);
+    if (!pl.ok || nt % pl.T != 0) continue;
+    const int rr = nt / pl.T;
+    const int pad = (rows + rr - 1) / rr * rr;
+    const int lds = (NZ / 2 + 1) * (rr + 1);
+    if (pad < best_pad || (pad == best_pad && lds < best_lds)) {
+      best_pad = pad;
+      best_lds = lds;
+      best = pm;
+    }
+  }
+  return best;
+}
+
+template <typename T, int NZ, int CX, int PM>
+struct ColumnShape {
+  static constexpr int NT = col_threads(NZ);
+  static constexpr int M = NZ / 2;                // complex FFT length of the z pass
+  static constexpr FftPlan PL = make_plan(M, PM);
+  static constexpr int P = PL.P, TC = PL.T;
+  static constexpr int RR = NT / TC;              // rows transformed at once (one round)
+  static constexpr int NROW = 3 * CX * COL_CY;    // rows of a patch (3 components)
+  static constexpr int NRND = (NROW + RR - 1) / RR;
+  static_assert(PL.ok && NT % TC == 0, "z pass plan");
+  static constexpr int LS = RR + 1;               // FFT scratch: element e of row r at e LS + r
+  static constexpr int RS = M + 1;                // row-major rows: complex words per row
+  static constexpr int FFT_BYTES = (M + 1) * LS * 2 * (int)sizeof(T);
+  static_assert(RR * RS <= (M + 1) * LS, "row-major rows fit the scratch");
+  // staging of the blob lists: budget per workgroup, then planes of CAP entries
+  // (about 38 KB per 256 threads in FP32, 52 KB in FP64: four or three workgroups per CU)
+  static constexpr int BUDGET = col_max((sizeof(T) == 4 ? 38912 : 53248) / 256 * NT, FFT_BYTES);
+  static constexpr int S_ENT = 19 * (int)sizeof(T) + 4;   // gx 4, gy 4, F 3, gz 8 | stencil-start z
+  static constexpr int S_CAP = col_odd_le(BUDGET / S_ENT);
+  static constexpr int S_LDS = col_max(FFT_BYTES, S_CAP * S_ENT);
+  static constexpr int I_ENT = 40 * (int)sizeof(T) + 8;   // gx 4, gy 4, gz 8, slots 3 x 8 | z, partial slot
+  static constexpr int I_CAP = col_odd_le(BUDGET / I_ENT);
+  static constexpr int I_LDS = col_max(FFT_BYTES, I_CAP * I_ENT);
+  static_assert(S_CAP >= 15 && I_CAP >= 15, "staging");
+  static_assert(S_LDS + 4 * (NZ + 1) <= 163840 && I_LDS + 4 * (NZ + 1) <= 163840, "LDS");
+};
+
+// the range [a1, b1) u [a2, b2) of the patch list holds exactly the blobs
+// whose stencil covers z (stencil start in [z - ng + 1, z] modulo nz)
+template <int NZ>
+__device__ __forceinline__ void col_ranges(const int* s_zst, int z, int ng, int& a1, int& b1, int& a2, int& b2) {
+  a1 = b1 = a2 = b2 = 0;
+  if (z >= NZ) return;
+  const int lo = z - ng + 1;
+  if (lo >= 0) {
+    a1 = s_zst[lo];
+    b1 = s_zst[z + 1];
+  } else {
+    a1 = s_zst[NZ + lo];
+    b1 = s_zst[NZ];
+    a2 = s_zst[0];
+    b2 = s_zst[z + 1];
+  }
+}
+
+// x and y weights of a blob over the patch (zero outside the stencil)
+template <typename T, int CX>
+__device__ __forceinline__ void col_xy_weights(const T* ws, int ng, int rx, int ry, T* gx, T* gy) {
+#pragma unroll
+  for (int ii = 0; ii < 4; ++ii) {
+    const int o = ii - rx;
+    gx[ii] = (ii < CX && o >= 0 && o < ng) ? ws[o] : T(0);
+  }
+#pragma unroll
+  for (int jj = 0; jj < 4; ++jj) {
+    const int o = jj - ry;
+    gy[jj] = (o >= 0 && o < ng) ? ws[ng + o] : T(0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// spreading + forward z R2C
+//
+// LDS while accumulating: planes of S_CAP entries (entry i of field f at
+// f S_CAP + i), so that the lanes of a wavefront, which read neighbouring
+// entries of the list, read neighbouring banks.  Then, per round of RR rows:
+// the rows go row-major through LDS into registers, the FFT runs with a
+// padded scratch stride, and the untangled k-space rows are stored.
+// ---------------------------------------------------------------------------
+template <typename T, int NZ, int CX, int PM>
+__global__ void __launch_bounds__(col_threads(NZ))
+    col_spread_kernel(ColArgs A, const double* __restrict__ F, FieldPtrs<T> fp, const c2<T>* __restrict__ tw,
+                      const c2<T>* __restrict__ twz) {
+  using SH = ColumnShape<T, NZ, CX, PM>;
+  using V = c2<T>;
+  constexpr int NT = SH::NT, M = SH::M, P = SH::P, TC = SH::TC, RR = SH::RR, NROW = SH::NROW, NRND = SH::NRND;
+  constexpr int CY = COL_CY, CAP = SH::S_CAP, LS = SH::LS, RS = SH::RS;
+  __shared__ __align__(16) unsigned char smem[SH::S_LDS];
+  __shared__ int s_zst[NZ + 1];
+  if (fp.gate && *fp.gate) return;
+  const GridGeom& g = A.g;
+  const int npatch = A.npx * A.npy;
+  const int p = col_xcd(blockIdx.x, npatch);
+  if (p >= npatch) return;
+  const int px = p / A.npy, py = p - (p / A.npy) * A.npy;
+  const int x0 = CX * px, y0 = CY * py;
+  const int t = threadIdx.x;
+  const int ng = A.ng, omin = 1 - ng / 2;
+  for (int i = t; i <= NZ; i += NT) s_zst[i] = A.zst[(size_t)p * (NZ + 1) + i];
+  __syncthreads();
+  const int L0 = s_zst[0], L1 = s_zst[NZ];
+  int a1, b1, a2, b2;
+  col_ranges<NZ>(s_zst, t, ng, a1, b1, a2, b2);
+
+  T* sw = reinterpret_cast<T*>(smem);  // planes: gx 0-3, gy 4-7, F 8-10, gz 11-18
+  int* ssz = reinterpret_cast<int*>(sw + 19 * CAP);
+  const T* W = col_w<T>(A.w64, A.w32);
+  T acc[NROW];  // row (c CX + ii) CY + jj of this thread's z
+#pragma unroll
+  for (int r = 0; r < NROW; ++r) acc[r] = T(0);
+
+  for (int c0 = L0; c0 < L1; c0 += CAP) {
+    const int cnt = min(CAP, L1 - c0);
+    __syncthreads();
+    // two entries per thread per step, all of their loads in flight together
+    for (int i0 = 0; i0 < cnt; i0 += 2 * NT) {
+      int4 e[2];
+      T gx[2][4], gy[2][4], gz[2][8], f[2][3];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int i = i0 + t + u * NT;
+        e[u] = i < cnt ? A.ent[c0 + i] : make_int4(0, 0, 8 | (8 << 5), 0);
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const T* ws = W + (size_t)e[u].x * 3 * ng;
+        col_xy_weights<T, CX>(ws, ng, col_rx(e[u].z), col_ry(e[u].z), gx[u], gy[u]);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) gz[u][k] = k < ng ? ws[2 * ng + k] : T(0);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) f[u][c] = T(F[3 * (size_t)e[u].y + c]);
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int i = i0 + t + u * NT;
+        if (i < cnt) {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            sw[k * CAP + i] = gx[u][k];
+            sw[(4 + k) * CAP + i] = gy[u][k];
+          }
+#pragma unroll
+          for (int c = 0; c < 3; ++c) sw[(8 + c) * CAP + i] = f[u][c];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) sw[(11 + k) * CAP + i] = gz[u][k];
+          ssz[i] = col_sz(e[u].z);
+        }
+      }
+    }
+    __syncthreads();
+    const int e1a = max(a1, c0), e1b = min(b1, c0 + cnt);
+    const int e2a = max(a2, c0), e2b = min(b2, c0 + cnt);
+    const int n1 = max(0, e1b - e1a), n2 = max(0, e2b - e2a);
+    for (int k = 0; k < n1 + n2; ++k) {
+      const int i = (k < n1 ? e1a + k : e2a + (k - n1)) - c0;
+      int o = t - ssz[i];
+      if (o < 0) o += NZ;
+      const T w = sw[(11 + o) * CAP + i];
+      T gx[CX], gy[CY];
+#pragma unroll
+      for (int q = 0; q < CX; ++q) gx[q] = sw[q * CAP + i];
+#pragma unroll
+      for (int q = 0; q < CY; ++q) gy[q] = sw[(4 + q) * CAP + i];
+      const T wf0 = w * sw[8 * CAP + i], wf1 = w * sw[9 * CAP + i], wf2 = w * sw[10 * CAP + i];
+#pragma unroll
+      for (int jj = 0; jj < CY; ++jj) {
+        const T wy0 = wf0 * gy[jj], wy1 = wf1 * gy[jj], wy2 = wf2 * gy[jj];
+#pragma unroll
+        for (int ii = 0; ii < CX; ++ii) {
+          acc[(0 * CX + ii) * CY + jj] += wy0 * gx[ii];
+          acc[(1 * CX + ii) * CY + jj] += wy1 * gx[ii];
+          acc[(2 * CX + ii) * CY + jj] += wy2 * gx[ii];
+        }
+      }
+    }
+  }
+  V* area = reinterpret_cast<V*>(smem);
+  T* rowsT = reinterpret_cast<T*>(smem);
+  const int rloc = t / TC, tcol = t - (t / TC) * TC;
+#pragma unroll
+  for (int q = 0; q < NRND; ++q) {
+    __syncthreads();  // staging (q = 0) or the previous round's scratch is free
+    if (t < NZ) {
+#pragma unroll
+      for (int r = 0; r < RR; ++r)
+        if (q * RR + r < NROW) rowsT[r * 2 * RS + t] = acc[q * RR + r];
+    }
+    __syncthreads();
+    V v[P];
+#pragma unroll
+    for (int s = 0; s < P; ++s) v[s] = area[rloc * RS + fft_in_index<M, false, PM>(tcol, s)];
+    __syncthreads();
+    fft_column<T, M, -1, false, PM>(v, area, LS, rloc, tcol, tw);
+#pragma unroll
+    for (int s = 0; s < P; ++s) area[fft_out_index<M, false, PM>(tcol, s) * LS + rloc] = v[s];
+    __syncthreads();
+    const int row = q * RR + rloc;
+    if (row < NROW) {
+      const int c = row / (CX * CY), rem = row - (row / (CX * CY)) * (CX * CY);
+      const int ii = rem / CY, jj = rem - (rem / CY) * CY;
+      V* out = fp.p[c] + ((size_t)(x0 + ii) * g.ny + (y0 + jj)) * g.cpitch;
+      // untangle: X[k] = E[k] + W^k O[k], E = (Z[k] + conj Z[M-k])/2, O = (Z[k] - conj Z[M-k])/(2i)
+      for (int k = tcol; k <= M; k += TC) {
+        const V zk = area[(k == M ? 0 : k) * LS + rloc];
+        const V zm = cconj(area[(k == 0 ? 0 : M - k) * LS + rloc]);
+        const V e = cscale(cadd(zk, zm), T(0.5));
+        const V d = csub(zk, zm);
+        const V o = mk<T>(T(0.5) * d.y, T(-0.5) * d.x);
+        out[k] = cadd(e, cmul(twz[k], o));
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// inverse z C2R + interpolation partial sums
+//
+// Per round of RR rows: the k-space rows are loaded into the padded FFT
+// scratch, transformed, and read back row-major into this thread's z column
+// (registers).  Then the blob lists are staged in planes as for spreading,
+// and each (blob, stencil z) value lands in its own LDS slot, so the ng
+// values of a blob are added in a fixed order.
+// ---------------------------------------------------------------------------
+template <typename T, int NZ, int CX, int PM>
+__global__ void __launch_bounds__(col_threads(NZ))
+    col_interp_kernel(ColArgs A, FieldPtrs<T> fp, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ twz,
+                      T* __restrict__ part) {
+  using SH = ColumnShape<T, NZ, CX, PM>;
+  using V = c2<T>;
+  constexpr int NT = SH::NT, M = SH::M, P = SH::P, TC = SH::TC, RR = SH::RR, NROW = SH::NROW, NRND = SH::NRND;
+  constexpr int CY = COL_CY, CAP = SH::I_CAP, LS = SH::LS, RS = SH::RS;
+  __shared__ __align__(16) unsigned char smem[SH::I_LDS];
+  __shared__ int s_zst[NZ + 1];
+  if (fp.gate && *fp.gate) return;
+  const GridGeom& g = A.g;
+  const int npatch = A.npx * A.npy;
+  const int p = col_xcd(blockIdx.x, npatch);
+  if (p >= npatch) return;
+  const int px = p / A.npy, py = p - (p / A.npy) * A.npy;
+  const int x0 = CX * px, y0 = CY * py;
+  const int t = threadIdx.x;
+  const int ng = A.ng, omin = 1 - ng / 2;
+  for (int i = t; i <= NZ; i += NT) s_zst[i] = A.zst[(size_t)p * (NZ + 1) + i];
+
+  V* area = reinterpret_cast<V*>(smem);
+  const T* rowsT = reinterpret_cast<const T*>(smem);
+  const int rloc = t / TC, tcol = t - (t / TC) * TC;
+  // every k-space word of the patch this thread will transform, loaded at once
+  constexpr int KS = M / TC + 1;  // words k = tcol + j TC <= M per row
+  V kin[NRND][KS];
+#pragma unroll
+  for (int q = 0; q < NRND; ++q) {
+    const int row = q * RR + rloc;
+    const bool valid = row < NROW;
+    const int c = row / (CX * CY), rem = row - (row / (CX * CY)) * (CX * CY);
+    const int ii = rem / CY, jj = rem - (rem / CY) * CY;
+    const V* in = fp.p[valid ? c : 0] + ((size_t)(x0 + (valid ? ii : 0)) * g.ny + (y0 + (valid ? jj : 0))) * g.cpitch;
+#pragma unroll
+    for (int j = 0; j < KS; ++j) {
+      const int k = tcol + j * TC;
+      kin[q][j] = (valid && k <= M) ? in[k] : mk<T>(T(0), T(0));
+    }
+  }
+  T u[NROW];  // row (c CX + ii) CY + jj at this thread's z
+#pragma unroll
+  for (int q = 0; q < NRND; ++q) {
+    if (q > 0) __syncthreads();  // the previous round's rows have been read
+#pragma unroll
+    for (int j = 0; j < KS; ++j) {
+      const int k = tcol + j * TC;
+      if (k <= M) {
+        V x = kin[q][j];
+        if (k == 0 || k == M) x.y = T(0);
+        area[k * LS + rloc] = x;
+      }
+    }
+    __syncthreads();
+    V v[P];
+#pragma unroll
+    for (int s = 0; s < P; ++s) {
+      const int k = fft_in_index<M, true, PM>(tcol, s);
+      const V xk = area[k * LS + rloc];
+      const V xm = cconj(area[(M - k) * LS + rloc]);
+      const V sum = cadd(xk, xm);
+      const V dif = csub(xk, xm);
+      const V rot = cmulc(dif, twz[k]);        // W^{-k} (X[k] - conj X[M-k])
+      v[s] = cadd(sum, mk<T>(-rot.y, rot.x));  // + i * rot
+    }
+    __syncthreads();
+    fft_column<T, M, +1, true, PM>(v, area, LS, rloc, tcol, tw);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < P; ++s) area[rloc * RS + fft_out_index<M, true, PM>(tcol, s)] = v[s];
+    __syncthreads();
+    if (t < NZ) {
+#pragma unroll
+      for (int r = 0; r < RR; ++r)
+        if (q * RR + r < NROW) u[q * RR + r] = rowsT[r * 2 * RS + t];
+    }
+  }
+  const int L0 = s_zst[0], L1 = s_zst[NZ];
+  int a1, b1, a2, b2;
+  col_ranges<NZ>(s_zst, t, ng, a1, b1, a2, b2);
+  T* sw = reinterpret_cast<T*>(smem);  // planes: gx 0-3, gy 4-7, gz 8-15, slot (c, o) 16 + 8 c + o
+  int* ssz = reinterpret_cast<int*>(sw + 40 * CAP);
+  int* sout = ssz + CAP;  // partial-sum slot of the blob for this patch
+  const T* W = col_w<T>(A.w64, A.w32);
+  for (int c0 = L0; c0 < L1; c0 += CAP) {
+    const int cnt = min(CAP, L1 - c0);
+    __syncthreads();
+    for (int i0 = 0; i0 < cnt; i0 += 2 * NT) {
+      int4 e[2];
+      T gx[2][4], gy[2][4], gz[2][8];
+#pragma unroll
+      for (int uu = 0; uu < 2; ++uu) {
+        const int i = i0 + t + uu * NT;
+        e[uu] = i < cnt ? A.ent[c0 + i] : make_int4(0, 0, 8 | (8 << 5), 0);
+      }
+#pragma unroll
+      for (int uu = 0; uu < 2; ++uu) {
+        const T* ws = W + (size_t)e[uu].x * 3 * ng;
+        col_xy_weights<T, CX>(ws, ng, col_rx(e[uu].z), col_ry(e[uu].z), gx[uu], gy[uu]);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) gz[uu][k] = k < ng ? ws[2 * ng + k] : T(0);
+      }
+#pragma unroll
+      for (int uu = 0; uu < 2; ++uu) {
+        const int i = i0 + t + uu * NT;
+        if (i < cnt) {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            sw[k * CAP + i] = gx[uu][k];
+            sw[(4 + k) * CAP + i] = gy[uu][k];
+          }
+#pragma unroll
+          for (int k = 0; k < 8; ++k) sw[(8 + k) * CAP + i] = gz[uu][k];
+          ssz[i] = col_sz(e[uu].z);
+          sout[i] = e[uu].w;
+        }
+      }
+    }
+    __syncthreads();
+    const int e1a = max(a1, c0), e1b = min(b1, c0 + cnt);
+    const int e2a = max(a2, c0), e2b = min(b2, c0 + cnt);
+    const int n1 = max(0, e1b - e1a), n2 = max(0, e2b - e2a);
+    for (int k = 0; k < n1 + n2; ++k) {
+      const int i = (k < n1 ? e1a + k : e2a + (k - n1)) - c0;
+      int o = t - ssz[i];
+      if (o < 0) o += NZ;
+      const T w = sw[(8 + o) * CAP + i];
+      T gx[CX], gy[CY];
+#pragma unroll
+      for (int q = 0; q < CX; ++q) gx[q] = sw[q * CAP + i];
+#pragma unroll
+      for (int q = 0; q < CY; ++q) gy[q] = sw[(4 + q) * CAP + i];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        T a = T(0);
+#pragma unroll
+        for (int ii = 0; ii < CX; ++ii) {
+          T b = T(0);
+#pragma unroll
+          for (int jj = 0; jj < CY; ++jj) b += gy[jj] * u[(c * CX + ii) * CY + jj];
+          a += gx[ii] * b;
+        }
+        sw[(16 + 8 * c + o) * CAP + i] = w * a;
+      }
+    }
+    __syncthreads();
+    for (int i = t; i < cnt; i += NT) {
+      T s0 = T(0), s1 = T(0), s2 = T(0);
+      for (int o = 0; o < ng; ++o) {
+        s0 += sw[(16 + o) * CAP + i];
+        s1 += sw[(24 + o) * CAP + i];
+        s2 += sw[(32 + o) * CAP + i];
+      }
+      T* out = part + (size_t)sout[i] * 3;
+      out[0] = s0;
+      out[1] = s1;
+      out[2] = s2;
+    }
+  }
+}
+
+// sum of a blob's patch partials in a fixed order, then dx^3 J u - K V
+template <typename T>
+__global__ void __launch_bounds__(256) col_reduce_kernel(GridGeom g, int ng, int cx, int nb, const int4* __restrict__ rec,
+                                                         const T* __restrict__ part, double* V, KTerm kt, double dx3,
+                                                         const int* gate) {
+  if (gate && *gate) return;
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= nb) return;
+  const int omin = 1 - ng / 2;
+  const int4 r = rec[s];
+  const int sxw = col_wrapi(r.x + omin, g.nx), syw = col_wrapi(r.y + omin, g.ny);
+  const int nxp = (sxw % cx + ng - 1) / cx + 1, nyp = (syw % COL_CY + ng - 1) / COL_CY + 1;
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+  for (int dx = 0; dx < nxp; ++dx)
+    for (int dy = 0; dy < nyp; ++dy) {
+      const T* pp = part + ((size_t)s * COL_SLOTS_MAX + dx * 3 + dy) * 3;
+      v0 += (double)pp[0];
+      v1 += (double)pp[1];
+      v2 += (double)pp[2];
+    }
+  store_blob_velocity(V, kt, dx3, r.w, v0, v1, v2);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+#define FCM_COL_NZ(X) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(512)
+
+static bool col_nz_supported(int nz) {
+  switch (nz) {
+#define FCM_COL_CASE(v) case v:
+    FCM_COL_NZ(FCM_COL_CASE)
+#undef FCM_COL_CASE
+    return true;
+    default:
+      return false;
+  }
+}
+
+// patch x extent per nz: FP64 rows of a 4 x 4 patch exceed the LDS at nz = 512
+constexpr int col_cx_for(int nz) { return nz <= 256 ? 4 : 2; }
+
+int col_patch_cx(const GridGeom& g, const BlobBins& bb, int ng) {
+  if (ng < 2 || ng > 8) return 0;
+  if (bb.bx != 4 || bb.by != 8 || bb.bz != 8) return 0;
+  if (g.nx < 16 || g.ny < 16 || g.nz < 16 || g.ny % COL_CY != 0 || !col_nz_supported(g.nz)) return 0;
+  const int cx = col_cx_for(g.nz);
+  return g.nx % cx == 0 ? cx : 0;
+}
+
+size_t col_list_capacity(int nb) { return (size_t)COL_SLOTS_MAX * (size_t)(nb > 0 ? nb : 1); }
+
+size_t col_zst_words(const GridGeom& g, int cx) {
+  if (cx <= 0) return 1;
+  return (size_t)(g.nx / cx) * (g.ny / COL_CY) * (g.nz + 1);
+}
+
+int col_groups(const GridGeom& g, int cx) {
+  if (cx <= 0) return 1;
+  return (g.nx / cx) * (g.ny / COL_CY) * (g.nz / 8);
+}
+
+hipError_t build_col_lists(const GridGeom& g, const BlobBins& bb, BlobData& bd, hipStream_t st) {
+  const int cx = bd.col_cx;
+  if (cx <= 0 || !bd.col_list || !bd.col_ent) return hipSuccess;
+  ColListArgs A{g, bb, bd.ng, cx, g.nx / cx, g.ny / COL_CY, bd.bin_start, bd.rec, bd.col_count, bd.col_start,
+                bd.col_list, bd.col_ent, bd.col_zst};
+  const int npatch = A.npx * A.npy;
+  hipLaunchKernelGGL(col_list_kernel, dim3(npatch), dim3(64), 0, st, A, 0);
+  hipError_t e = exclusive_scan(bd.col_count, bd.col_start, bd.col_cursor, col_groups(g, cx), st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(col_list_kernel, dim3(npatch), dim3(64), 0, st, A, 1);
+  return hipGetLastError();
+}
+
+namespace {
+
+template <typename T, int NZ, int CX>
+hipError_t launch_spread(const ColArgs& A, const double* F, const FieldPtrs<T>& fp, const TwiddleSet<T>& tw,
+                         hipStream_t st) {
+  constexpr int PM = col_pm(NZ, CX);
+  using SH = ColumnShape<T, NZ, CX, PM>;
+  const int npatch = A.npx * A.npy;
+  hipLaunchKernelGGL((col_spread_kernel<T, NZ, CX, PM>), dim3(8 * ((npatch + 7) / 8)), dim3(SH::NT), 0, st, A, F, fp,
+                     tw.z, tw.zu);
+  return hipGetLastError();
+}
+
+template <typename T, int NZ, int CX>
+hipError_t launch_interp(const ColArgs& A, const FieldPtrs<T>& fp, const TwiddleSet<T>& tw, T* part, hipStream_t st) {
+  constexpr int PM = col_pm(NZ, CX);
+  using SH = ColumnShape<T, NZ, CX, PM>;
+  const int npatch = A.npx * A.npy;
+  hipLaunchKernelGGL((col_interp_kernel<T, NZ, CX, PM>), dim3(8 * ((npatch + 7) / 8)), dim3(SH::NT), 0, st, A, fp,
+                     tw.z, tw.zu, part);
+  return hipGetLastError();
+}
+
+ColArgs col_args(const GridGeom& g, const BlobData& bd) {
+  return ColArgs{g, bd.ng, g.nx / bd.col_cx, g.ny / COL_CY, bd.col_ent, bd.col_zst, bd.w64, bd.w32};
+}
+
+}  // namespace
+
+template <typename T>
+hipError_t spread_zfwd(const GridGeom& g, const BlobData& bd, const double* F, const FieldPtrs<T>& fp,
+                       const TwiddleSet<T>& tw, hipStream_t st) {
+  if (bd.col_cx <= 0) return hipErrorInvalidValue;
+  const ColArgs A = col_args(g, bd);
+  switch (g.nz) {
+#define FCM_COL_SPREAD(v) \
+  case v: return bd.col_cx == col_cx_for(v) ? launch_spread<T, v, col_cx_for(v)>(A, F, fp, tw, st) : hipErrorInvalidValue;
+    FCM_COL_NZ(FCM_COL_SPREAD)
+#undef FCM_COL_SPREAD
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+template <typename T>
+hipError_t zinv_interp(const GridGeom& g, const BlobData& bd, const FieldPtrs<T>& fp, const TwiddleSet<T>& tw,
+                       double* V, const KTerm& kt, hipStream_t st) {
+  if (bd.col_cx <= 0) return hipErrorInvalidValue;
+  if (bd.nb == 0) return hipSuccess;
+  const ColArgs A = col_args(g, bd);
+  T* part = static_cast<T*>(bd.col_part);
+  hipError_t e = hipErrorInvalidValue;
+  switch (g.nz) {
+#define FCM_COL_INTERP(v)                                                                                      \
+  case v:                                                                                                      \
+    e = bd.col_cx == col_cx_for(v) ? launch_interp<T, v, col_cx_for(v)>(A, fp, tw, part, st) : hipErrorInvalidValue; \
+    break;
+    FCM_COL_NZ(FCM_COL_INTERP)
+#undef FCM_COL_INTERP
+    default:
+      break;
+  }
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((col_reduce_kernel<T>), dim3((bd.nb + 255) / 256), dim3(256), 0, st, g, bd.ng, bd.col_cx, bd.nb,
+                     bd.rec, (const T*)part, V, kt, g.dx * g.dx * g.dx, fp.gate);
+  return hipGetLastError();
+}
+
+template hipError_t spread_zfwd<double>(const GridGeom&, const BlobData&, const double*, const FieldPtrs<double>&,
+                                        const TwiddleSet<double>&, hipStream_t);
+template hipError_t spread_zfwd<float>(const GridGeom&, const BlobData&, const double*, const FieldPtrs<float>&,
+                                       const TwiddleSet<float>&, hipStream_t);
+template hipError_t zinv_interp<double>(const GridGeom&, const BlobData&, const FieldPtrs<double>&,
+                                        const TwiddleSet<double>&, double*, const KTerm&, hipStream_t);
+template hipError_t zinv_interp<float>(const GridGeom&, const BlobData&, const FieldPtrs<float>&,
+                                       const TwiddleSet<float>&, double*, const KTerm&, hipStream_t);
+
+}  // namespace fcm

@@ -83,7 +83,8 @@ __global__ void __launch_bounds__(256) zfwd_kernel(FieldPtrs<T> fp, int rows, in
   using SH = RowShape<T, M>;
   constexpr FftPlan PL = make_plan(M);
   constexpr int P = PL.P, TC = SH::TC, RR = SH::RR;
-  __shared__ V lds[(M + 1) * RR];
+  constexpr int LS = RR | 1;  // odd element stride: rows of a column fall in different LDS banks
+  __shared__ V lds[(M + 1) * LS];
   const int t = threadIdx.x;
   const int tcol = t % TC, rloc = t / TC;
   const int row = blockIdx.x * RR + rloc;
@@ -111,14 +112,14 @@ __global__ void __launch_bounds__(256) zfwd_kernel(FieldPtrs<T> fp, int rows, in
       v[s] = valid ? in[e] : mk<T>(T(0), T(0));
     }
   }
-  fft_column<T, M, -1, false>(v, lds, RR, rloc, tcol, tw);
+  fft_column<T, M, -1, false>(v, lds, LS, rloc, tcol, tw);
 #pragma unroll
-  for (int s = 0; s < P; ++s) lds[fft_out_index<M, false>(tcol, s) * RR + rloc] = v[s];
+  for (int s = 0; s < P; ++s) lds[fft_out_index<M, false>(tcol, s) * LS + rloc] = v[s];
   __syncthreads();
   // untangle: X[k] = E[k] + W^k O[k], E = (Z[k] + conj Z[M-k])/2, O = (Z[k] - conj Z[M-k])/(2i)
   for (int k = tcol; k <= M; k += TC) {
-    const V zk = lds[(k == M ? 0 : k) * RR + rloc];
-    const V zm = cconj(lds[(k == 0 ? 0 : M - k) * RR + rloc]);
+    const V zk = lds[(k == M ? 0 : k) * LS + rloc];
+    const V zm = cconj(lds[(k == 0 ? 0 : M - k) * LS + rloc]);
     const V e = cscale(cadd(zk, zm), T(0.5));
     const V d = csub(zk, zm);
     const V o = mk<T>(T(0.5) * d.y, T(-0.5) * d.x);  // d / (2i)
@@ -136,7 +137,8 @@ __global__ void __launch_bounds__(256) zinv_kernel(FieldPtrs<T> fp, int rows, in
   using SH = RowShape<T, M>;
   constexpr FftPlan PL = make_plan(M);
   constexpr int P = PL.P, TC = SH::TC, RR = SH::RR;
-  __shared__ V lds[(M + 1) * RR];
+  constexpr int LS = RR | 1;  // odd element stride: rows of a column fall in different LDS banks
+  __shared__ V lds[(M + 1) * LS];
   const int t = threadIdx.x;
   const int tcol = t % TC, rloc = t / TC;
   const int row = blockIdx.x * RR + rloc;
@@ -145,22 +147,22 @@ __global__ void __launch_bounds__(256) zinv_kernel(FieldPtrs<T> fp, int rows, in
   for (int k = tcol; k <= M; k += TC) {
     V x = valid ? base[k] : mk<T>(T(0), T(0));
     if (k == 0 || k == M) x.y = T(0);
-    lds[k * RR + rloc] = x;
+    lds[k * LS + rloc] = x;
   }
   __syncthreads();
   V v[P];
 #pragma unroll
   for (int s = 0; s < P; ++s) {
     const int k = fft_in_index<M, true>(tcol, s);
-    const V xk = lds[k * RR + rloc];
-    const V xm = cconj(lds[(M - k) * RR + rloc]);
+    const V xk = lds[k * LS + rloc];
+    const V xm = cconj(lds[(M - k) * LS + rloc]);
     const V sum = cadd(xk, xm);
     const V dif = csub(xk, xm);
     const V rot = cmulc(dif, twz[k]);          // W^{-k} (X[k] - conj X[M-k])
     v[s] = cadd(sum, mk<T>(-rot.y, rot.x));    // + i * rot
   }
   __syncthreads();
-  fft_column<T, M, +1, true>(v, lds, RR, rloc, tcol, tw);
+  fft_column<T, M, +1, true>(v, lds, LS, rloc, tcol, tw);
   if (valid) {
 #pragma unroll
     for (int s = 0; s < P; ++s) base[fft_out_index<M, true>(tcol, s)] = v[s];

@@ -127,6 +127,11 @@ hipError_t sort_by_key(const int* key_of, int n, int nkeys, const int* count, in
   return hipGetLastError();
 }
 
+hipError_t exclusive_scan(const int* count, int* start, int* cursor, int n, hipStream_t st) {
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, count, start, cursor, n);
+  return hipGetLastError();
+}
+
 hipError_t fill_ints(int* p, int n, int v, hipStream_t st) {
   if (n > 0) hipLaunchKernelGGL(fill_int_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
   return hipGetLastError();
@@ -147,7 +152,9 @@ hipError_t setup_gaussians(const GridGeom& g, const BlobBins& bb, BlobData& bd, 
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  return build_tile_lists(g, bb, bd, st);
+  e = build_tile_lists(g, bb, bd, st);
+  if (e != hipSuccess) return e;
+  return build_col_lists(g, bb, bd, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -888,19 +895,7 @@ struct InterpArgs {
 };
 
 __device__ __forceinline__ void write_velocity(const InterpArgs& A, int id, double v0, double v1, double v2) {
-  v0 *= A.dx3; v1 *= A.dx3; v2 *= A.dx3;
-  if (A.kt.zV) {
-    const int body = id / A.kt.nbpb;
-    const double* u = A.kt.zV + 6 * (size_t)body;
-    const double* d = A.kt.dpos + 3 * (size_t)id;
-    // U + Omega x d   (collection_of_rigid_bodies.cpp:723)
-    v0 -= u[0] + (u[4] * d[2] - u[5] * d[1]);
-    v1 -= u[1] + (u[5] * d[0] - u[3] * d[2]);
-    v2 -= u[2] + (u[3] * d[1] - u[4] * d[0]);
-  }
-  A.V[3 * (size_t)id] = v0;
-  A.V[3 * (size_t)id + 1] = v1;
-  A.V[3 * (size_t)id + 2] = v2;
+  store_blob_velocity(A.V, A.kt, A.dx3, id, v0, v1, v2);
 }
 
 // one bin per workgroup; node box of (bin + ng - 1) per axis staged in LDS

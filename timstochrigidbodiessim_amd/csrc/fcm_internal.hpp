@@ -84,7 +84,20 @@ struct BlobData {
   // indexed by precision (0 FP64, 1 FP32)
   int spread_variant[2] = {0, 0};
   int interp_izb[2] = {0, 0};  // interpolation tile depth in z bins: 0 = 2 when it divides, 1, 2
+  // column path of the mobility product (fcm_column.hip): x-y patches of
+  // col_cx x COL_CY nodes with all of z; per patch the blobs whose x-y stencil
+  // touches it, ordered by stencil-start z, and a z prefix table
+  int col_cx = 0;              // patch x extent (0: the grid does not qualify)
+  int* col_count = nullptr;    // npatch x nbz: entries per (patch, z group)
+  int* col_start = nullptr;    // npatch x nbz + 1 (exclusive scan)
+  int* col_cursor = nullptr;   // scan scratch
+  int* col_list = nullptr;     // sorted blob slots, COL_SLOTS_MAX per blob at most
+  int4* col_ent = nullptr;     // entry records in list order (slot, id, packed offsets, partial slot)
+  int* col_zst = nullptr;      // npatch x (nz + 1): first entry with stencil-start z >= k
+  void* col_part = nullptr;    // interpolation partial sums: nb x COL_SLOTS_MAX x 3 (T)
+  int col_fused[2] = {1, 1};   // per precision (0 FP64, 1 FP32): use the column path in the mobility product
 };
+
 
 // spreading tiles of the main path (0 if the grid does not qualify)
 bool spread_fast_ok(const GridGeom& g, const BlobBins& bb, int ng);
@@ -99,12 +112,52 @@ struct KTerm {
   int nbpb = 1;
 };
 
+// V[id] = dx^3 (v0, v1, v2) - (U + Omega x d_id) when the K term is present
+// (the first row of the saddle product, gmres_solver.cpp:72-83 with
+// collection_of_rigid_bodies.cpp:723)
+__device__ __forceinline__ void store_blob_velocity(double* V, const KTerm& kt, double dx3, int id, double v0,
+                                                    double v1, double v2) {
+  v0 *= dx3; v1 *= dx3; v2 *= dx3;
+  if (kt.zV) {
+    const int body = id / kt.nbpb;
+    const double* u = kt.zV + 6 * (size_t)body;
+    const double* d = kt.dpos + 3 * (size_t)id;
+    v0 -= u[0] + (u[4] * d[2] - u[5] * d[1]);
+    v1 -= u[1] + (u[5] * d[0] - u[3] * d[2]);
+    v2 -= u[2] + (u[3] * d[1] - u[4] * d[0]);
+  }
+  V[3 * (size_t)id] = v0;
+  V[3 * (size_t)id + 1] = v1;
+  V[3 * (size_t)id + 2] = v2;
+}
+
 template <typename T>
 hipError_t spread(const GridGeom& g, const BlobBins& bb, const BlobData& bd, const double* F, const FieldPtrs<T>& fp,
                   hipStream_t st);
 template <typename T>
 hipError_t interp(const GridGeom& g, const BlobBins& bb, const BlobData& bd, const FieldPtrs<T>& fp, double* V,
                   const KTerm& kt, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// column path (fcm_column.hip): spreading fused with the forward z R2C pass,
+// and the inverse z C2R pass fused with interpolation
+// ---------------------------------------------------------------------------
+constexpr int COL_CY = 4;          // patch y extent
+constexpr int COL_SLOTS_MAX = 15;  // partial sums per blob: up to 5 x 3 patches
+// patch x extent usable on this grid for both precisions (0: none)
+int col_patch_cx(const GridGeom& g, const BlobBins& bb, int ng);
+size_t col_list_capacity(int nb);
+size_t col_zst_words(const GridGeom& g, int cx);
+int col_groups(const GridGeom& g, int cx);  // npatch x nbz
+hipError_t build_col_lists(const GridGeom& g, const BlobBins& bb, BlobData& bd, hipStream_t st);
+template <typename T>
+hipError_t spread_zfwd(const GridGeom& g, const BlobData& bd, const double* F, const FieldPtrs<T>& fp,
+                       const TwiddleSet<T>& tw, hipStream_t st);
+// inverse z pass + interpolation partial sums, then the fixed-order reduction
+// into V (with the K term)
+template <typename T>
+hipError_t zinv_interp(const GridGeom& g, const BlobData& bd, const FieldPtrs<T>& fp, const TwiddleSet<T>& tw,
+                       double* V, const KTerm& kt, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // rigid bodies, preconditioner, Krylov kernels
@@ -134,6 +187,8 @@ hipError_t bodies_step(const GridGeom& g, double* x, double* q, int NB, const do
 hipError_t sort_by_key(const int* key_of, int n, int nkeys, const int* count, int* start, int* cursor, int* sorted,
                        hipStream_t st);
 hipError_t fill_ints(int* p, int n, int v, hipStream_t st);
+// start[0..n] = exclusive scan of count[0..n), cursor[0..n) = 0 (one workgroup)
+hipError_t exclusive_scan(const int* count, int* start, int* cursor, int n, hipStream_t st);
 
 // soft-sphere barrier forces between blobs of different bodies within a cutoff
 // (linked_list.cpp:80-102,192-281,362-382) on a cell list of cells >= cutoff
