@@ -60,6 +60,33 @@ def pass_bytes(nx, ny, nz, nb, word, column=False):
     }
 
 
+# kernel behind each profiled pass of the FP32 mobility product (column path, separate passes)
+KERNEL_OF_PASS = {
+    True: {"spread": "col_spread_kernel<float, 256, 4, 8>", "interp": "col_interp_kernel<float, 256, 4, 8>",
+           "y_forward": "cols_kernel<float, 256, -1>", "x_project": "xproj_kernel<float, 256>",
+           "y_inverse": "cols_kernel<float, 256, 1>", "interp_reduce": "col_reduce_kernel<float>"},
+    False: {"z_forward": "zfwd_kernel<float, 128, false>", "z_inverse": "zinv_kernel<float, 128>",
+            "y_forward": "cols_kernel<float, 256, -1>", "x_project": "xproj_kernel<float, 256>",
+            "y_inverse": "cols_kernel<float, 256, 1>"},
+}
+
+
+def measured_traffic(kernel, grid, nb):
+    """HBM bytes per launch of `kernel` on this workload from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{grid}cubed_{nb}blobs.md")
+    if not os.path.exists(path):
+        return None
+    section = None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("## "):
+                section = line[3:].strip()
+            elif line.startswith("traffic_bytes_per_launch:") and section and section in kernel:
+                return int(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(args, ref, x, q, F6, fp64_iters):
     """Times the CPU oracle (OpenMP over x slabs) on a bounded sample of the
     same workload: one fluctuating-stress field and one saddle-point mat-vec
@@ -370,10 +397,13 @@ def main():
     head = prof["fp32"]
     dom = max((k for k in head if k != "KT"), key=lambda k: head[k]["ms"])
     achieved = head[dom]["GBps"]
+    kernel = KERNEL_OF_PASS[column[fa.FCM_PREC_FP32] == 1].get(dom, dom)
     mv_bytes = sum(pass_bytes(n, n, n, nb, 4, column[fa.FCM_PREC_FP32] == 1).values())
     mv_ms = sum(v["ms"] for v in head.values())
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+    roofline = {"bound": "hbm", "kernel": kernel, "pass": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(kernel, n, args.rods * args.spheres),
+                "algorithmic_bytes": pass_bytes(n, n, n, nb, 4, column[fa.FCM_PREC_FP32] == 1)[dom],
                 "matvec_fp32": {"ms": round(mv_ms, 4), "GBps": round(mv_bytes / (mv_ms * 1e-3) / 1e9, 1)}}
 
     cpu = None

@@ -155,8 +155,9 @@ typedef struct {
   int max_iter;        /* GMRES_MAX_ITER (config.hpp:63)                             */
   int precision;       /* FCM_PREC_FP64: plain solve; FCM_PREC_FP32: FP32 bulk inner
                           solves with FP64 iterative refinement on the true residual */
-  double inner_tol;    /* FP32 form: relative tolerance of each inner solve (0: auto) */
-  int max_refine;      /* FP32 form: maximum outer refinements (0: 8)               */
+  double inner_tol;    /* FP32 form: relative tolerance of each inner solve
+                          (0: max(1e-2, tol ||b|| / (2 ||r||)))                  */
+  int max_refine;      /* FP32 form: maximum outer refinements (0: 12)              */
 } gmres_opts;
 
 typedef struct {
@@ -251,12 +252,13 @@ int fcm_autotune(fcm_handle h, int precision, int reps, fcm_tuning* chosen);
 
 /* ---- measurement hook (no counterpart in the reference) ----------------------- */
 /* Runs one saddle-point mat-vec `reps` times on a fixed input with HIP events
- * between its passes, on the handle's stream; ms_per_pass[0..7] receives the
+ * between its passes, on the handle's stream; ms_per_pass[0..8] receives the
  * mean device time of: spread, z forward, y forward, x forward+projection+x
- * inverse, y inverse, z inverse, interpolation(+K), K^T.  On the column path
- * the spread slot holds the fused spread + z forward kernel, the interpolation
- * slot the fused z inverse + interpolation kernel and its reduction, and the
- * two z slots are 0. */
+ * inverse, y inverse, z inverse, interpolation(+K), K^T, and the reduction of
+ * the interpolation partial sums.  On the column path the spread slot holds
+ * the fused spread + z forward kernel, the interpolation slot the fused z
+ * inverse + interpolation kernel, and the two z slots are 0; on the separate
+ * path the last slot is 0. */
 int fcm_profile_matvec(fcm_handle h, int precision, int reps, double* ms_per_pass);
 /* A plain streaming copy of `bytes` (16-byte words) between two scratch buffers,
  * `reps` times, on the handle's stream; the reference point for HBM byte counters

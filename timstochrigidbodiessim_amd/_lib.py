@@ -34,7 +34,8 @@ EXPORTS = [
     "fcm_profile_matvec", "fcm_profile_copy", "fcm_synth_rods",
 ]
 
-PASS_NAMES = ["spread", "z_forward", "y_forward", "x_project", "y_inverse", "z_inverse", "interp", "KT"]
+PASS_NAMES = ["spread", "z_forward", "y_forward", "x_project", "y_inverse", "z_inverse", "interp", "KT",
+              "interp_reduce"]
 
 
 class FcmTuning(C.Structure):
@@ -398,7 +399,7 @@ class FcmSolver:
         _chk(self.L.fcm_mobility_prepare(self.h, seed))
 
     def profile_matvec(self, precision=FCM_PREC_FP64, reps=10):
-        ms = np.zeros(8)
+        ms = np.zeros(len(PASS_NAMES))
         _chk(self.L.fcm_profile_matvec(self.h, precision, reps, _ptr(ms)))
         return dict(zip(PASS_NAMES, ms.tolist()))
 

@@ -33,7 +33,7 @@ struct Job {
   fcm_handle h = nullptr;
   int prec = FCM_PREC_FP64;
   double tol = 1e-8, inner_tol = 0.0;
-  int max_iter = 1, max_ref = 8;
+  int max_iter = 1, max_ref = 12;
   Phase phase = Phase::INNER;
   gmres_report rep{};
   long long valid_after = 0;  // first snapshot that reflects the last enqueued transition
@@ -174,8 +174,12 @@ int inner_max_iter(const Job* j) {
   return (int)std::max<long long>(1, std::min<long long>({left, (long long)h->cfg.max_krylov, (long long)h->nsys()}));
 }
 
+// Each inner FP32 solve reduces the current residual by 1e-2 (or just enough to
+// reach the tolerance): restarting the Krylov space every ~75 iterations keeps
+// the FP64 orthogonalisation, whose cost grows with the square of the basis
+// size, well below that of one long solve, at about the same iteration count.
 double inner_tol(const Job* j, double bn, double rn) {
-  double itol = j->inner_tol > 0 ? j->inner_tol : std::max(0.5 * j->tol * bn / rn, 1e-5);
+  double itol = j->inner_tol > 0 ? j->inner_tol : std::max(0.5 * j->tol * bn / rn, 1e-2);
   return std::min(itol, 0.5);
 }
 
@@ -413,7 +417,7 @@ int batch_submit(fcm_batch_s* B, fcm_handle h, const gmres_opts* o) {
   j->tol = o ? o->tol : 1e-8;
   j->inner_tol = o ? o->inner_tol : 0.0;
   j->max_iter = max_iter;
-  j->max_ref = (o && o->max_refine > 0) ? o->max_refine : 8;
+  j->max_ref = (o && o->max_refine > 0) ? o->max_refine : 12;
   j->rep.status = FCM_GMRES_MAX_ITER;
   j->rep.true_rel_residual = -1.0;
   j->slot = slot;

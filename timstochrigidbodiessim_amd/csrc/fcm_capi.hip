@@ -220,7 +220,7 @@ int profile_column(fcm_handle h, const FieldPtrs<T>& fp, const TwiddleSet<T>& tw
     HIPCHK(hipEventRecord(ev[2 + p], h->st));
   }
   HIPCHK(hipEventRecord(ev[6], h->st));
-  HIPCHK(zinv_interp<T>(h->g, h->bd, fp, tw, h->tmp, kt, h->st));
+  HIPCHK(zinv_interp<T>(h->g, h->bd, fp, tw, h->tmp, kt, h->st, ev[7]));
   return FCM_OK;
 }
 
@@ -907,9 +907,10 @@ int fcm_profile_matvec(fcm_handle h, int precision, int reps, double* ms_per_pas
     v[i] = (double)(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
   }
   HIPCHK(hipMemcpyAsync(h->xacc, v.data(), n * sizeof(double), hipMemcpyHostToDevice, h->st));
-  hipEvent_t ev[9];
-  for (int i = 0; i < 9; ++i) HIPCHK(hipEventCreate(&ev[i]));
-  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // events: 0 start, 1 spread, 2..6 the five FFT slots, 7 interpolation, 8 reduction, 9 K^T
+  hipEvent_t ev[10];
+  for (int i = 0; i < 10; ++i) HIPCHK(hipEventCreate(&ev[i]));
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   KTerm kt;
   kt.zV = h->xacc + n3;
   kt.dpos = h->d_dpos;
@@ -940,18 +941,24 @@ int fcm_profile_matvec(fcm_handle h, int precision, int reps, double* ms_per_pas
       }
       HIPCHK(interp<double>(h->g, h->bb, h->bd, h->fp64(), h->tmp, kt, h->st));
     }
-    HIPCHK(hipEventRecord(ev[7], h->st));
-    HIPCHK(mul_KT(b, h->xacc, h->tmp + n3, h->st));
+    if (!column) HIPCHK(hipEventRecord(ev[7], h->st));
     HIPCHK(hipEventRecord(ev[8], h->st));
-    HIPCHK(hipEventSynchronize(ev[8]));
-    for (int i = 0; i < 8; ++i) {
+    HIPCHK(mul_KT(b, h->xacc, h->tmp + n3, h->st));
+    HIPCHK(hipEventRecord(ev[9], h->st));
+    HIPCHK(hipEventSynchronize(ev[9]));
+    for (int i = 0; i < 9; ++i) {
       float ms = 0;
       HIPCHK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
       acc[i] += ms;
     }
   }
-  for (int i = 0; i < 9; ++i) (void)hipEventDestroy(ev[i]);
-  for (int i = 0; i < 8; ++i) ms_per_pass[i] = acc[i] / reps;
+  for (int i = 0; i < 10; ++i) (void)hipEventDestroy(ev[i]);
+  // slots: spread, z fwd, y fwd, x, y inv, z inv, interpolation, K^T, reduction
+  const double mean[9] = {acc[0] / reps, acc[1] / reps, acc[2] / reps, acc[3] / reps, acc[4] / reps,
+                          acc[5] / reps, acc[6] / reps, acc[8] / reps, acc[7] / reps};
+  for (int i = 0; i < 9; ++i) ms_per_pass[i] = mean[i];
+  if (column) ms_per_pass[1] = ms_per_pass[5] = 0.0;  // back-to-back events only: the z passes ran inside
+  else ms_per_pass[8] = 0.0;
   return rc;
 }
 

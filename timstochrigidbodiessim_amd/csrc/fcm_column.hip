@@ -167,6 +167,7 @@ __global__ void __launch_bounds__(64) col_list_kernel(ColListArgs A, int pass) {
 // ---------------------------------------------------------------------------
 constexpr int col_threads(int nz) { return (nz + 63) / 64 * 64; }  // one thread per z
 constexpr int col_max(int a, int b) { return a > b ? a : b; }
+constexpr int col_min(int a, int b) { return a < b ? a : b; }
 // largest odd number <= v (LDS planes of an odd number of words are not
 // aligned with one another in the 64 banks)
 constexpr int col_odd_le(int v) { return (v % 2 == 1) ? v : v - 1; }
@@ -201,13 +202,16 @@ struct ColumnShape {
   static constexpr int NROW = 3 * CX * COL_CY;    // rows of a patch (3 components)
   static constexpr int NRND = (NROW + RR - 1) / RR;
   static_assert(PL.ok && NT % TC == 0, "z pass plan");
-  static constexpr int LS = RR + 1;               // FFT scratch: element e of row r at e LS + r
+  // rounds transformed together (one set of barriers): all of them in FP32,
+  // one at a time in FP64, whose scratch would otherwise crowd the LDS
+  static constexpr int NCB = (sizeof(T) == 4 && (M + 1) * (NRND * RR + 1) * 8 <= 81920) ? NRND : 1;
+  static constexpr int LS = NCB * RR + 1;         // FFT scratch: element e of row r at e LS + r
   static constexpr int RS = M + 1;                // row-major rows: complex words per row
   static constexpr int FFT_BYTES = (M + 1) * LS * 2 * (int)sizeof(T);
-  static_assert(RR * RS <= (M + 1) * LS, "row-major rows fit the scratch");
+  static_assert(NCB * RR * RS <= (M + 1) * LS, "row-major rows fit the scratch");
   // staging of the blob lists: budget per workgroup, then planes of CAP entries
   // (about 38 KB per 256 threads in FP32, 52 KB in FP64: four or three workgroups per CU)
-  static constexpr int BUDGET = col_max((sizeof(T) == 4 ? 38912 : 53248) / 256 * NT, FFT_BYTES);
+  static constexpr int BUDGET = col_max(col_min((sizeof(T) == 4 ? 38912 : 53248) / 256 * NT, 65536), FFT_BYTES);
   static constexpr int S_ENT = 19 * (int)sizeof(T) + 4;   // gx 4, gy 4, F 3, gz 8 | stencil-start z
   static constexpr int S_CAP = col_odd_le(BUDGET / S_ENT);
   static constexpr int S_LDS = col_max(FFT_BYTES, S_CAP * S_ENT);
@@ -267,9 +271,10 @@ __global__ void __launch_bounds__(col_threads(NZ))
   using SH = ColumnShape<T, NZ, CX, PM>;
   using V = c2<T>;
   constexpr int NT = SH::NT, M = SH::M, P = SH::P, TC = SH::TC, RR = SH::RR, NROW = SH::NROW, NRND = SH::NRND;
-  constexpr int CY = COL_CY, CAP = SH::S_CAP, LS = SH::LS, RS = SH::RS;
+  constexpr int CY = COL_CY, CAP = SH::S_CAP, LS = SH::LS, RS = SH::RS, NCB = SH::NCB;
   __shared__ __align__(16) unsigned char smem[SH::S_LDS];
   __shared__ int s_zst[NZ + 1];
+  __shared__ V s_tw[M], s_twz[M + 1];
   if (fp.gate && *fp.gate) return;
   const GridGeom& g = A.g;
   const int npatch = A.npx * A.npy;
@@ -280,6 +285,10 @@ __global__ void __launch_bounds__(col_threads(NZ))
   const int t = threadIdx.x;
   const int ng = A.ng, omin = 1 - ng / 2;
   for (int i = t; i <= NZ; i += NT) s_zst[i] = A.zst[(size_t)p * (NZ + 1) + i];
+  for (int i = t; i <= M; i += NT) {
+    if (i < M) s_tw[i] = tw[i];
+    s_twz[i] = twz[i];
+  }
   __syncthreads();
   const int L0 = s_zst[0], L1 = s_zst[NZ];
   int a1, b1, a2, b2;
@@ -361,35 +370,42 @@ __global__ void __launch_bounds__(col_threads(NZ))
   T* rowsT = reinterpret_cast<T*>(smem);
   const int rloc = t / TC, tcol = t - (t / TC) * TC;
 #pragma unroll
-  for (int q = 0; q < NRND; ++q) {
-    __syncthreads();  // staging (q = 0) or the previous round's scratch is free
+  for (int qg = 0; qg < NRND; qg += NCB) {
+    __syncthreads();  // staging (first group) or the previous group's scratch is free
     if (t < NZ) {
 #pragma unroll
-      for (int r = 0; r < RR; ++r)
-        if (q * RR + r < NROW) rowsT[r * 2 * RS + t] = acc[q * RR + r];
+      for (int r = 0; r < NCB * RR; ++r)
+        if (qg * RR + r < NROW) rowsT[r * 2 * RS + t] = acc[qg * RR + r];
     }
     __syncthreads();
-    V v[P];
+    V v[NCB][P];
 #pragma unroll
-    for (int s = 0; s < P; ++s) v[s] = area[rloc * RS + fft_in_index<M, false, PM>(tcol, s)];
-    __syncthreads();
-    fft_column<T, M, -1, false, PM>(v, area, LS, rloc, tcol, tw);
+    for (int c = 0; c < NCB; ++c)
 #pragma unroll
-    for (int s = 0; s < P; ++s) area[fft_out_index<M, false, PM>(tcol, s) * LS + rloc] = v[s];
+      for (int s = 0; s < P; ++s) v[c][s] = area[(rloc + c * RR) * RS + fft_in_index<M, false, PM>(tcol, s)];
     __syncthreads();
-    const int row = q * RR + rloc;
-    if (row < NROW) {
-      const int c = row / (CX * CY), rem = row - (row / (CX * CY)) * (CX * CY);
-      const int ii = rem / CY, jj = rem - (rem / CY) * CY;
-      V* out = fp.p[c] + ((size_t)(x0 + ii) * g.ny + (y0 + jj)) * g.cpitch;
-      // untangle: X[k] = E[k] + W^k O[k], E = (Z[k] + conj Z[M-k])/2, O = (Z[k] - conj Z[M-k])/(2i)
-      for (int k = tcol; k <= M; k += TC) {
-        const V zk = area[(k == M ? 0 : k) * LS + rloc];
-        const V zm = cconj(area[(k == 0 ? 0 : M - k) * LS + rloc]);
-        const V e = cscale(cadd(zk, zm), T(0.5));
-        const V d = csub(zk, zm);
-        const V o = mk<T>(T(0.5) * d.y, T(-0.5) * d.x);
-        out[k] = cadd(e, cmul(twz[k], o));
+    fft_columns<T, M, -1, false, PM, NCB>(v, area, LS, rloc, RR, tcol, s_tw);
+#pragma unroll
+    for (int c = 0; c < NCB; ++c)
+#pragma unroll
+      for (int s = 0; s < P; ++s) area[fft_out_index<M, false, PM>(tcol, s) * LS + rloc + c * RR] = v[c][s];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NCB; ++c) {
+      const int row = (qg + c) * RR + rloc;
+      if (row < NROW) {
+        const int cc = row / (CX * CY), rem = row - (row / (CX * CY)) * (CX * CY);
+        const int ii = rem / CY, jj = rem - (rem / CY) * CY;
+        V* out = fp.p[cc] + ((size_t)(x0 + ii) * g.ny + (y0 + jj)) * g.cpitch;
+        // untangle: X[k] = E[k] + W^k O[k], E = (Z[k] + conj Z[M-k])/2, O = (Z[k] - conj Z[M-k])/(2i)
+        for (int k = tcol; k <= M; k += TC) {
+          const V zk = area[(k == M ? 0 : k) * LS + rloc + c * RR];
+          const V zm = cconj(area[(k == 0 ? 0 : M - k) * LS + rloc + c * RR]);
+          const V e = cscale(cadd(zk, zm), T(0.5));
+          const V d = csub(zk, zm);
+          const V o = mk<T>(T(0.5) * d.y, T(-0.5) * d.x);
+          out[k] = cadd(e, cmul(s_twz[k], o));
+        }
       }
     }
   }
@@ -411,9 +427,10 @@ __global__ void __launch_bounds__(col_threads(NZ))
   using SH = ColumnShape<T, NZ, CX, PM>;
   using V = c2<T>;
   constexpr int NT = SH::NT, M = SH::M, P = SH::P, TC = SH::TC, RR = SH::RR, NROW = SH::NROW, NRND = SH::NRND;
-  constexpr int CY = COL_CY, CAP = SH::I_CAP, LS = SH::LS, RS = SH::RS;
+  constexpr int CY = COL_CY, CAP = SH::I_CAP, LS = SH::LS, RS = SH::RS, NCB = SH::NCB;
   __shared__ __align__(16) unsigned char smem[SH::I_LDS];
   __shared__ int s_zst[NZ + 1];
+  __shared__ V s_tw[M], s_twz[M + 1];
   if (fp.gate && *fp.gate) return;
   const GridGeom& g = A.g;
   const int npatch = A.npx * A.npy;
@@ -424,6 +441,10 @@ __global__ void __launch_bounds__(col_threads(NZ))
   const int t = threadIdx.x;
   const int ng = A.ng, omin = 1 - ng / 2;
   for (int i = t; i <= NZ; i += NT) s_zst[i] = A.zst[(size_t)p * (NZ + 1) + i];
+  for (int i = t; i <= M; i += NT) {
+    if (i < M) s_tw[i] = tw[i];
+    s_twz[i] = twz[i];
+  }
 
   V* area = reinterpret_cast<V*>(smem);
   const T* rowsT = reinterpret_cast<const T*>(smem);
@@ -446,39 +467,45 @@ __global__ void __launch_bounds__(col_threads(NZ))
   }
   T u[NROW];  // row (c CX + ii) CY + jj at this thread's z
 #pragma unroll
-  for (int q = 0; q < NRND; ++q) {
-    if (q > 0) __syncthreads();  // the previous round's rows have been read
+  for (int qg = 0; qg < NRND; qg += NCB) {
+    if (qg > 0) __syncthreads();  // the previous group's rows have been read
 #pragma unroll
-    for (int j = 0; j < KS; ++j) {
-      const int k = tcol + j * TC;
-      if (k <= M) {
-        V x = kin[q][j];
-        if (k == 0 || k == M) x.y = T(0);
-        area[k * LS + rloc] = x;
+    for (int c = 0; c < NCB; ++c)
+#pragma unroll
+      for (int j = 0; j < KS; ++j) {
+        const int k = tcol + j * TC;
+        if (k <= M) {
+          V x = kin[qg + c][j];
+          if (k == 0 || k == M) x.y = T(0);
+          area[k * LS + rloc + c * RR] = x;
+        }
       }
-    }
     __syncthreads();
-    V v[P];
+    V v[NCB][P];
 #pragma unroll
-    for (int s = 0; s < P; ++s) {
-      const int k = fft_in_index<M, true, PM>(tcol, s);
-      const V xk = area[k * LS + rloc];
-      const V xm = cconj(area[(M - k) * LS + rloc]);
-      const V sum = cadd(xk, xm);
-      const V dif = csub(xk, xm);
-      const V rot = cmulc(dif, twz[k]);        // W^{-k} (X[k] - conj X[M-k])
-      v[s] = cadd(sum, mk<T>(-rot.y, rot.x));  // + i * rot
-    }
+    for (int c = 0; c < NCB; ++c)
+#pragma unroll
+      for (int s = 0; s < P; ++s) {
+        const int k = fft_in_index<M, true, PM>(tcol, s);
+        const V xk = area[k * LS + rloc + c * RR];
+        const V xm = cconj(area[(M - k) * LS + rloc + c * RR]);
+        const V sum = cadd(xk, xm);
+        const V dif = csub(xk, xm);
+        const V rot = cmulc(dif, s_twz[k]);         // W^{-k} (X[k] - conj X[M-k])
+        v[c][s] = cadd(sum, mk<T>(-rot.y, rot.x));  // + i * rot
+      }
     __syncthreads();
-    fft_column<T, M, +1, true, PM>(v, area, LS, rloc, tcol, tw);
+    fft_columns<T, M, +1, true, PM, NCB>(v, area, LS, rloc, RR, tcol, s_tw);
     __syncthreads();
 #pragma unroll
-    for (int s = 0; s < P; ++s) area[rloc * RS + fft_out_index<M, true, PM>(tcol, s)] = v[s];
+    for (int c = 0; c < NCB; ++c)
+#pragma unroll
+      for (int s = 0; s < P; ++s) area[(rloc + c * RR) * RS + fft_out_index<M, true, PM>(tcol, s)] = v[c][s];
     __syncthreads();
     if (t < NZ) {
 #pragma unroll
-      for (int r = 0; r < RR; ++r)
-        if (q * RR + r < NROW) u[q * RR + r] = rowsT[r * 2 * RS + t];
+      for (int r = 0; r < NCB * RR; ++r)
+        if (qg * RR + r < NROW) u[qg * RR + r] = rowsT[r * 2 * RS + t];
     }
   }
   const int L0 = s_zst[0], L1 = s_zst[NZ];
@@ -686,7 +713,7 @@ hipError_t spread_zfwd(const GridGeom& g, const BlobData& bd, const double* F, c
 
 template <typename T>
 hipError_t zinv_interp(const GridGeom& g, const BlobData& bd, const FieldPtrs<T>& fp, const TwiddleSet<T>& tw,
-                       double* V, const KTerm& kt, hipStream_t st) {
+                       double* V, const KTerm& kt, hipStream_t st, hipEvent_t between) {
   if (bd.col_cx <= 0) return hipErrorInvalidValue;
   if (bd.nb == 0) return hipSuccess;
   const ColArgs A = col_args(g, bd);
@@ -703,6 +730,7 @@ hipError_t zinv_interp(const GridGeom& g, const BlobData& bd, const FieldPtrs<T>
       break;
   }
   if (e != hipSuccess) return e;
+  if (between && (e = hipEventRecord(between, st)) != hipSuccess) return e;
   hipLaunchKernelGGL((col_reduce_kernel<T>), dim3((bd.nb + 255) / 256), dim3(256), 0, st, g, bd.ng, bd.col_cx, bd.nb,
                      bd.rec, (const T*)part, V, kt, g.dx * g.dx * g.dx, fp.gate);
   return hipGetLastError();
@@ -713,8 +741,8 @@ template hipError_t spread_zfwd<double>(const GridGeom&, const BlobData&, const 
 template hipError_t spread_zfwd<float>(const GridGeom&, const BlobData&, const double*, const FieldPtrs<float>&,
                                        const TwiddleSet<float>&, hipStream_t);
 template hipError_t zinv_interp<double>(const GridGeom&, const BlobData&, const FieldPtrs<double>&,
-                                        const TwiddleSet<double>&, double*, const KTerm&, hipStream_t);
+                                        const TwiddleSet<double>&, double*, const KTerm&, hipStream_t, hipEvent_t);
 template hipError_t zinv_interp<float>(const GridGeom&, const BlobData&, const FieldPtrs<float>&,
-                                       const TwiddleSet<float>&, double*, const KTerm&, hipStream_t);
+                                       const TwiddleSet<float>&, double*, const KTerm&, hipStream_t, hipEvent_t);
 
 }  // namespace fcm

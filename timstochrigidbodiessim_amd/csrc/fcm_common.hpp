@@ -280,6 +280,71 @@ __device__ __forceinline__ void fft_column(c2<T>* v, c2<T>* lds, int ls, int col
   FftStage<T, N, S, REV, 0, PM>::run(v, lds, ls, col, tcol, tw);
 }
 
+// The same transform for NC columns per thread (column c of this thread at
+// lds column col + c * cs), with one set of barriers for all of them and each
+// twiddle factor loaded once for the NC columns.
+template <typename T, int N, int S, bool REV, int STG, int PM, int NC>
+struct FftStageB {
+  static constexpr FftPlan PL = REV ? reverse_plan(make_plan(N, PM)) : make_plan(N, PM);
+  static constexpr int P = PL.P;
+  static constexpr int TT = PL.T;
+  static constexpr int R = PL.r[STG];
+  static constexpr int NS = plan_ns_before(PL, STG);
+  static constexpr bool LAST = (STG == PL.ns - 1);
+
+  static __device__ __forceinline__ void run(c2<T> (&v)[NC][P], c2<T>* lds, int ls, int col, int cs, int tcol,
+                                             const c2<T>* __restrict__ tw) {
+#pragma unroll
+    for (int i = 0; i < P / R; ++i) {
+      const int j = tcol + i * TT;
+      if (NS > 1) {
+        const int k = j % NS;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const int m = r * k * (N / (NS * R));
+          c2<T> w = tw[m];
+          if (S > 0) w.y = -w.y;
+#pragma unroll
+          for (int c = 0; c < NC; ++c) v[c][i * R + r] = cmul(v[c][i * R + r], w);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < NC; ++c) Dft<T, R, S>::run(v[c] + i * R);
+    }
+    if constexpr (!LAST) {
+      constexpr int R2 = PL.r[STG + 1];
+#pragma unroll
+      for (int i = 0; i < P / R; ++i) {
+        const int j = tcol + i * TT;
+        const int k = j % NS;
+        const int base = (j / NS) * NS * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+          for (int c = 0; c < NC; ++c) lds[(base + r * NS) * ls + col + c * cs] = v[c][i * R + r];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < P / R2; ++i) {
+        const int j = tcol + i * TT;
+#pragma unroll
+        for (int r = 0; r < R2; ++r)
+#pragma unroll
+          for (int c = 0; c < NC; ++c) v[c][i * R2 + r] = lds[(j + r * (N / R2)) * ls + col + c * cs];
+      }
+      __syncthreads();
+      FftStageB<T, N, S, REV, STG + 1, PM, NC>::run(v, lds, ls, col, cs, tcol, tw);
+    }
+  }
+};
+
+template <typename T, int N, int S, bool REV, int PM, int NC>
+__device__ __forceinline__ void fft_columns(c2<T> (&v)[NC][REV ? reverse_plan(make_plan(N, PM)).P : make_plan(N, PM).P],
+                                            c2<T>* lds, int ls, int col, int cs, int tcol,
+                                            const c2<T>* __restrict__ tw) {
+  FftStageB<T, N, S, REV, 0, PM, NC>::run(v, lds, ls, col, cs, tcol, tw);
+}
+
 // element index held in register slot (i, r) for the first radix of a plan
 template <int N, bool REV, int PM = 16>
 __device__ __forceinline__ int fft_in_index(int tcol, int slot) {

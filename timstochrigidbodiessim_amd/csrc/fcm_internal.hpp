@@ -157,7 +157,7 @@ hipError_t spread_zfwd(const GridGeom& g, const BlobData& bd, const double* F, c
 // into V (with the K term)
 template <typename T>
 hipError_t zinv_interp(const GridGeom& g, const BlobData& bd, const FieldPtrs<T>& fp, const TwiddleSet<T>& tw,
-                       double* V, const KTerm& kt, hipStream_t st);
+                       double* V, const KTerm& kt, hipStream_t st, hipEvent_t between = nullptr);
 
 // ---------------------------------------------------------------------------
 // rigid bodies, preconditioner, Krylov kernels

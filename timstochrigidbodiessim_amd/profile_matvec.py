@@ -3,17 +3,21 @@ precision, plus a plain device copy of known size, as a target for
 rocprofv3 (kernel statistics and FETCH_SIZE / WRITE_SIZE counters):
 
     python -m timstochrigidbodiessim_amd.profile_matvec [--reps 3] [--grid 256]
+    python3 timstochrigidbodiessim_amd/profile_matvec.py ...   (the same, as a file)
 
 The copy (fcm_profile_copy) reads and writes `--copy-mb` MiB with a plain
 streaming kernel; its counter values calibrate the byte counts of the FCM
 kernels."""
 import argparse
+import os
 import sys
 import time
 
-import numpy as np
-
-from . import _lib as L
+if __package__ in (None, ""):  # run as a file (rocprofv3 -- python3 .../profile_matvec.py)
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from timstochrigidbodiessim_amd import _lib as L
+else:
+    from . import _lib as L
 
 
 def main():
