@@ -60,7 +60,8 @@ struct ColArgs {
 };
 
 // One entry of a patch list: (sorted blob slot, blob id, packed offsets, index
-// of the blob's partial-sum slot for this patch).  The packed word holds the
+// of the blob's partial-sum slot for this patch: slot-major, dst nb + blob, so
+// that the reduction reads consecutive blobs).  The packed word holds the
 // stencil start relative to the patch in x and y (+8, 5 bits each) and the
 // wrapped stencil start in z (bits 10 and up).
 __device__ __forceinline__ int col_rx(int pk) { return (pk & 31) - 8; }
@@ -73,7 +74,7 @@ __device__ __forceinline__ int col_sz(int pk) { return pk >> 10; }
 struct ColListArgs {
   GridGeom g;
   BlobBins bb;
-  int ng, cx, npx, npy;
+  int ng, cx, npx, npy, nb;
   const int* bin_start;
   const int4* rec;
   int* count;
@@ -157,7 +158,7 @@ __global__ void __launch_bounds__(64) col_list_kernel(ColListArgs A, int pass) {
       const int rx = col_wrap_half(sxw - x0, g.nx), ry = col_wrap_half(syw - y0, g.ny);
       const int dst = col_wrapi(px - sxw / cx, A.npx) * 3 + col_wrapi(py - syw / COL_CY, A.npy);
       A.ent[e] = make_int4(sl, r.w, (rx + 8) | ((ry + 8) << 5) | (col_wrapi(r.z + omin, g.nz) << 10),
-                           sl * COL_SLOTS_MAX + dst);
+                           dst * A.nb + sl);
     }
   }
 }
@@ -607,7 +608,7 @@ __global__ void __launch_bounds__(256) col_reduce_kernel(GridGeom g, int ng, int
   double v0 = 0.0, v1 = 0.0, v2 = 0.0;
   for (int dx = 0; dx < nxp; ++dx)
     for (int dy = 0; dy < nyp; ++dy) {
-      const T* pp = part + ((size_t)s * COL_SLOTS_MAX + dx * 3 + dy) * 3;
+      const T* pp = part + ((size_t)(dx * 3 + dy) * nb + s) * 3;
       v0 += (double)pp[0];
       v1 += (double)pp[1];
       v2 += (double)pp[2];
@@ -657,7 +658,7 @@ int col_groups(const GridGeom& g, int cx) {
 hipError_t build_col_lists(const GridGeom& g, const BlobBins& bb, BlobData& bd, hipStream_t st) {
   const int cx = bd.col_cx;
   if (cx <= 0 || !bd.col_list || !bd.col_ent) return hipSuccess;
-  ColListArgs A{g, bb, bd.ng, cx, g.nx / cx, g.ny / COL_CY, bd.bin_start, bd.rec, bd.col_count, bd.col_start,
+  ColListArgs A{g, bb, bd.ng, cx, g.nx / cx, g.ny / COL_CY, bd.nb, bd.bin_start, bd.rec, bd.col_count, bd.col_start,
                 bd.col_list, bd.col_ent, bd.col_zst};
   const int npatch = A.npx * A.npy;
   hipLaunchKernelGGL(col_list_kernel, dim3(npatch), dim3(64), 0, st, A, 0);

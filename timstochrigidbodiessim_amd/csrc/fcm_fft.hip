@@ -20,11 +20,11 @@ namespace fcm {
 // ---------------------------------------------------------------------------
 // block shapes
 // ---------------------------------------------------------------------------
-template <typename T, int N, int PM = 16> struct ColShape {
+template <typename T, int N, int PM = 16, int TMAX = 256> struct ColShape {
   static constexpr FftPlan PL = make_plan(N, PM);
   static constexpr int TC = PL.T;
-  // columns per block: at most 16, at most 256 threads, at most 64 KiB LDS
-  static constexpr int C0 = 256 / TC < 16 ? 256 / TC : 16;
+  // columns per block: at most 16, at most TMAX threads, at most 64 KiB LDS
+  static constexpr int C0 = TMAX / TC < 16 ? TMAX / TC : 16;
   static constexpr int CL = (65536 / (N * (int)sizeof(c2<T>))) < C0 ? (65536 / (N * (int)sizeof(c2<T>))) : C0;
   static constexpr int C = CL < 1 ? 1 : CL;
   static constexpr int THREADS = C * TC;
@@ -231,15 +231,19 @@ __device__ __forceinline__ int signed_k(int k, int n) { return k <= n / 2 ? k : 
 // forward x FFT of 3 fields -> projection -> inverse x FFT, in place.
 // points per thread of the fused x passes: three (or more) fields stay in
 // registers between the forward and inverse transforms
+// (8 points per thread in FP32 as well, with 512-thread workgroups, halves the
+// registers but doubles the LDS exchanges per transform: 0.146 ms against
+// 0.102 ms at 256^3, so FP32 keeps 16)
 template <typename T> struct XPM { static constexpr int value = sizeof(T) == 8 ? 8 : 16; };
+template <typename T> struct XTMAX { static constexpr int value = 256; };
 
 template <typename T, int N>
-__global__ void __launch_bounds__(256) xproj_kernel(FieldPtrs<T> fp, int ny, int cpitch, int nzc,
+__global__ void __launch_bounds__(XTMAX<T>::value) xproj_kernel(FieldPtrs<T> fp, int ny, int cpitch, int nzc,
                                                     ProjParams pp, const c2<T>* __restrict__ tw) {
   if (fp.gate && *fp.gate) return;
   using V = c2<T>;
   constexpr int PM = XPM<T>::value;
-  using SH = ColShape<T, N, PM>;
+  using SH = ColShape<T, N, PM, XTMAX<T>::value>;
   constexpr FftPlan PL = make_plan(N, PM);
   constexpr int P = PL.P, TC = SH::TC, C = SH::C;
   __shared__ V lds[N * C];
@@ -286,12 +290,12 @@ __global__ void __launch_bounds__(256) xproj_kernel(FieldPtrs<T> fp, int ny, int
 // x FFT, written to fields 0..2 of fp (in place over sxx, syy, szz).
 // Stream/field order: sxx syy szz sxy sxz syz.
 template <typename T, int N>
-__global__ void __launch_bounds__(256) xstress_kernel(FieldPtrs<T> fp, int ny, int cpitch, int nzc,
+__global__ void __launch_bounds__(XTMAX<T>::value) xstress_kernel(FieldPtrs<T> fp, int ny, int cpitch, int nzc,
                                                       ProjParams pp, const c2<T>* __restrict__ tw) {
   if (fp.gate && *fp.gate) return;
   using V = c2<T>;
   constexpr int PM = XPM<T>::value;
-  using SH = ColShape<T, N, PM>;
+  using SH = ColShape<T, N, PM, XTMAX<T>::value>;
   constexpr FftPlan PL = make_plan(N, PM);
   constexpr int P = PL.P, TC = SH::TC, C = SH::C;
   __shared__ V lds[N * C];
@@ -419,7 +423,7 @@ static hipError_t launch_xproj(const GridGeom& g, const FieldPtrs<T>& fp, bool s
   switch (g.nx) {
 #define FCM_CASE(v)                                                                                             \
   case v: {                                                                                                     \
-    using SH = ColShape<T, v, XPM<T>::value>;                                                                   \
+    using SH = ColShape<T, v, XPM<T>::value, XTMAX<T>::value>;                                                  \
     dim3 grid((g.nzc + SH::C - 1) / SH::C, g.ny);                                                               \
     if (stress)                                                                                                 \
       hipLaunchKernelGGL((xstress_kernel<T, v>), grid, dim3(SH::THREADS), 0, st, fp, g.ny, g.cpitch, g.nzc, pp, tws.x); \

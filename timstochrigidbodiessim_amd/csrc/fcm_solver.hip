@@ -164,122 +164,131 @@ __global__ void __launch_bounds__(128) pinv_kernel(BodyData b, const double* __r
   }
 }
 
-// The same products with one wavefront per body and four bodies per
-// workgroup: ref_Minv (transposed, so that lanes read consecutive words) and
-// ref_N are staged in LDS once per workgroup.  Lane i < 3 n_b owns row i;
-// every sum runs over k in the order of pinv_kernel, so the results are the
-// same bit for bit.  Used when 3 n_b <= 64.
-static constexpr int PW_BODIES = 4;
-static constexpr int PW_SCR = 4 * 64 + 6 + 6 + 9;  // bp, tt, gg, oo, sv, vv, R per wave
-
-__global__ void __launch_bounds__(256) pinv_wave_kernel(BodyData b, const double* __restrict__ in,
+// The same products with one thread per (body, row): a pass of a workgroup
+// takes BP = 256 / 3n_b bodies (7 at n_b = 12, 98% of the lanes), and the
+// workgroup loops over passes so that ref_Minv (transposed: the threads of a
+// body read consecutive words) and ref_N are staged in LDS once.  Every sum
+// runs over k in the order of pinv_kernel, so the results are the same bit for
+// bit.  Used when 3 n_b <= 72 (the LDS stays under 64 KiB).
+__global__ void __launch_bounds__(256) pinv_pack_kernel(BodyData b, const double* __restrict__ in,
                                                         double* __restrict__ out, double* __restrict__ wKT) {
   extern __shared__ double sm[];
   if (b.gate && *b.gate) return;
   if (b.kcol) in += (size_t)(*b.kcol) * b.ldq;
   const int n3 = 3 * b.nbpb;
-  double* MT = sm;              // n3 x n3, MT[k n3 + i] = Minv[i n3 + k]
-  double* NR = MT + n3 * n3;    // 36
-  double* ref = NR + 36;        // n3
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  double* bp = ref + n3 + wv * PW_SCR;
-  double* tt = bp + 64;
-  double* gg = tt + 64;
-  double* oo = gg + 64;
-  double* sv = oo + 64;
-  double* vv = sv + 6;
-  double* Rs = vv + 6;
+  const int BP = 256 / n3;
+  double* MT = sm;               // n3 x n3, MT[k n3 + i] = Minv[i n3 + k]
+  double* NR = MT + n3 * n3;     // 36
+  double* ref = NR + 36;         // n3
+  double* bp = ref + n3;         // BP x n3 each: R^T v, Minv bp, g, Minv g
+  double* tt = bp + BP * n3;
+  double* gg = tt + BP * n3;
+  double* oo = gg + BP * n3;
+  double* sv = oo + BP * n3;     // BP x 6
+  double* vv = sv + BP * 6;      // BP x 6
+  double* Rs = vv + BP * 6;      // BP x 9
+  const int t = threadIdx.x;
   for (int e = t; e < n3 * n3; e += 256) {
-    const int i = e / n3, k = e - (e / n3) * n3;
-    MT[k * n3 + i] = b.Minv[e];
+    const int k = e / n3, i = e - (e / n3) * n3;
+    MT[e] = b.Minv[(size_t)i * n3 + k];
   }
   if (t < 36) NR[t] = b.Nref[t];
   for (int i = t; i < n3; i += 256) ref[i] = b.ref[i];
-  const int n = blockIdx.x * PW_BODIES + wv;
-  const bool live = n < b.NB;
+  const int q = t / n3, i = t - (t / n3) * n3, m = i / 3, c = i - (i / 3) * 3;
+  const int c1 = (c + 1) % 3, c2i = (c + 2) % 3;
   const size_t nlam = 3 * (size_t)b.NB * b.nbpb;
-  if (live && lane < 9) Rs[lane] = b.R[9 * (size_t)n + lane];
-  __syncthreads();
-  const int i = lane, m = i / 3, c = i - (i / 3) * 3;
-  const bool row = live && i < n3;
-  const double* inl = in + (size_t)n3 * n;
-  if (row) {
-    const double* v = inl + 3 * m;
-    bp[i] = Rs[c] * v[0] + Rs[3 + c] * v[1] + Rs[6 + c] * v[2];  // (R^T v)_c
-  }
-  __syncthreads();
-  if (row) {
-    double s = 0.0;
-    for (int k = 0; k < n3; ++k) s += MT[k * n3 + i] * bp[k];
-    tt[i] = s;
-  }
-  __syncthreads();
-  if (live && lane < 6) {
-    const int cc = lane % 3;
-    const double* v = in + nlam + 6 * (size_t)n + (lane < 3 ? 0 : 3);
-    const double rv = Rs[cc] * v[0] + Rs[3 + cc] * v[1] + Rs[6 + cc] * v[2];
-    double k = 0.0;
-    if (lane < 3) {
-      for (int mm = 0; mm < b.nbpb; ++mm) k += tt[3 * mm + cc];
-    } else {
-      const int c1 = (cc + 1) % 3, c2i = (cc + 2) % 3;
-      for (int mm = 0; mm < b.nbpb; ++mm) k += ref[3 * mm + c1] * tt[3 * mm + c2i] - ref[3 * mm + c2i] * tt[3 * mm + c1];
+  double* bpq = bp + q * n3;
+  double* ttq = tt + q * n3;
+  double* ggq = gg + q * n3;
+  double* ooq = oo + q * n3;
+  double* svq = sv + q * 6;
+  double* vvq = vv + q * 6;
+  const double* Rq = Rs + q * 9;
+  for (int base = blockIdx.x * BP; base < b.NB; base += gridDim.x * BP) {
+    for (int e = t; e < BP * 9; e += 256) {
+      const int nn = base + e / 9;
+      Rs[e] = nn < b.NB ? b.R[9 * (size_t)nn + (e - (e / 9) * 9)] : 0.0;
     }
-    sv[lane] = -rv - k;
-  }
-  __syncthreads();
-  if (live && lane < 6) {
-    double s = 0.0;
-    for (int k = 0; k < 6; ++k) s += NR[lane * 6 + k] * sv[k];
-    vv[lane] = s;
-  }
-  __syncthreads();
-  if (row) {
-    const int c1 = (c + 1) % 3, c2i = (c + 2) % 3;
-    gg[i] = vv[c] + (vv[3 + c1] * ref[3 * m + c2i] - vv[3 + c2i] * ref[3 * m + c1]) + bp[i];
-  }
-  __syncthreads();
-  if (row) {
-    double s = 0.0;
-    for (int k = 0; k < n3; ++k) s += MT[k * n3 + i] * gg[k];
-    oo[i] = s;
-  }
-  __syncthreads();
-  if (row) {
-    const double* o = oo + 3 * m;
-    const double v = Rs[3 * c] * o[0] + Rs[3 * c + 1] * o[1] + Rs[3 * c + 2] * o[2];
-    out[(size_t)n3 * n + i] = v;
-    gg[i] = v;
-  }
-  if (live && lane < 6) {
-    const int cc = lane % 3;
-    const double* v = vv + (lane < 3 ? 0 : 3);
-    out[nlam + 6 * (size_t)n + lane] = Rs[3 * cc] * v[0] + Rs[3 * cc + 1] * v[1] + Rs[3 * cc + 2] * v[2];
-  }
-  if (wKT) {
     __syncthreads();
-    if (live && lane < 6) {
-      const int cc = lane % 3;
-      double s = 0.0;
-      const double* d = b.dpos + 3 * (size_t)n * b.nbpb;
-      if (lane < 3) {
-        for (int mm = 0; mm < b.nbpb; ++mm) s += gg[3 * mm + cc];
-      } else {
-        const int c1 = (cc + 1) % 3, c2i = (cc + 2) % 3;
-        for (int mm = 0; mm < b.nbpb; ++mm) s += d[3 * mm + c1] * gg[3 * mm + c2i] - d[3 * mm + c2i] * gg[3 * mm + c1];
-      }
-      wKT[6 * (size_t)n + lane] = -s;
+    const int n = base + q;
+    const bool live = q < BP && n < b.NB;
+    if (live) {
+      const double* v = in + (size_t)n3 * n + 3 * m;
+      bpq[i] = Rq[c] * v[0] + Rq[3 + c] * v[1] + Rq[6 + c] * v[2];  // (R^T v)_c
     }
+    __syncthreads();
+    if (live) {
+      double s = 0.0;
+      for (int k = 0; k < n3; ++k) s += MT[k * n3 + i] * bpq[k];
+      ttq[i] = s;
+    }
+    __syncthreads();
+    if (live && i < 6) {
+      const int cc = i % 3;
+      const double* v = in + nlam + 6 * (size_t)n + (i < 3 ? 0 : 3);
+      const double rv = Rq[cc] * v[0] + Rq[3 + cc] * v[1] + Rq[6 + cc] * v[2];
+      double k = 0.0;
+      if (i < 3) {
+        for (int mm = 0; mm < b.nbpb; ++mm) k += ttq[3 * mm + cc];
+      } else {
+        const int d1 = (cc + 1) % 3, d2 = (cc + 2) % 3;
+        for (int mm = 0; mm < b.nbpb; ++mm) k += ref[3 * mm + d1] * ttq[3 * mm + d2] - ref[3 * mm + d2] * ttq[3 * mm + d1];
+      }
+      svq[i] = -rv - k;
+    }
+    __syncthreads();
+    if (live && i < 6) {
+      double s = 0.0;
+      for (int k = 0; k < 6; ++k) s += NR[i * 6 + k] * svq[k];
+      vvq[i] = s;
+    }
+    __syncthreads();
+    if (live) ggq[i] = vvq[c] + (vvq[3 + c1] * ref[3 * m + c2i] - vvq[3 + c2i] * ref[3 * m + c1]) + bpq[i];
+    __syncthreads();
+    if (live) {
+      double s = 0.0;
+      for (int k = 0; k < n3; ++k) s += MT[k * n3 + i] * ggq[k];
+      ooq[i] = s;
+    }
+    __syncthreads();
+    if (live) {
+      const double* o = ooq + 3 * m;
+      const double v = Rq[3 * c] * o[0] + Rq[3 * c + 1] * o[1] + Rq[3 * c + 2] * o[2];
+      out[(size_t)n3 * n + i] = v;
+      ggq[i] = v;
+    }
+    if (live && i < 6) {
+      const int cc = i % 3;
+      const double* v = vvq + (i < 3 ? 0 : 3);
+      out[nlam + 6 * (size_t)n + i] = Rq[3 * cc] * v[0] + Rq[3 * cc + 1] * v[1] + Rq[3 * cc + 2] * v[2];
+    }
+    if (wKT) {
+      __syncthreads();
+      if (live && i < 6) {
+        const int cc = i % 3;
+        double s = 0.0;
+        const double* d = b.dpos + 3 * (size_t)n * b.nbpb;
+        if (i < 3) {
+          for (int mm = 0; mm < b.nbpb; ++mm) s += ggq[3 * mm + cc];
+        } else {
+          const int d1 = (cc + 1) % 3, d2 = (cc + 2) % 3;
+          for (int mm = 0; mm < b.nbpb; ++mm) s += d[3 * mm + d1] * ggq[3 * mm + d2] - d[3 * mm + d2] * ggq[3 * mm + d1];
+        }
+        wKT[6 * (size_t)n + i] = -s;
+      }
+    }
+    __syncthreads();  // the next pass overwrites R and the per-body rows
   }
 }
 
 hipError_t apply_pinv(const BodyData& b, const double* in, double* out, double* wKT, hipStream_t st) {
   if (b.NB == 0) return hipSuccess;
   const int n3 = 3 * b.nbpb;
-  if (n3 <= 64) {
-    const size_t lds = sizeof(double) * ((size_t)n3 * n3 + 36 + n3 + PW_BODIES * (size_t)PW_SCR);
-    hipLaunchKernelGGL(pinv_wave_kernel, dim3((b.NB + PW_BODIES - 1) / PW_BODIES), dim3(256), lds, st, b, in, out,
-                       wKT);
+  if (n3 <= 72) {
+    const int BP = 256 / n3;
+    const size_t lds = sizeof(double) * ((size_t)n3 * n3 + 36 + n3 + (size_t)BP * (4 * n3 + 21));
+    const int groups = (b.NB + BP - 1) / BP;
+    hipLaunchKernelGGL(pinv_pack_kernel, dim3(groups < 2048 ? groups : 2048), dim3(256), lds, st, b, in, out, wKT);
     return hipGetLastError();
   }
   const size_t lds = sizeof(double) * (5 * (size_t)n3 + 21);
