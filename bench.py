@@ -181,7 +181,7 @@ def bench_batch(args):
                    "steps_per_round": args.steps_per_round},
         "one_at_a_time_solves_per_s": round(len(insts) / t_alone, 3),
         "batch_speedup_vs_one_at_a_time": round(t_alone / t_batch, 3),
-        "converged": conv, "iterations_total": sum(r["iterations"] for r in reps),
+        "converged": conv, "iteration_cap": args.max_iter, "iterations_total": sum(r["iterations"] for r in reps),
     }
     for s, *_ in insts:
         s.close()
@@ -219,7 +219,8 @@ def bench_serve(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32 grid passes + f64 Krylov/refinement",
         "data": "synthetic (seeded Poisson stream, rods, forces and stress)",
         "config": {"workload": f"{args.requests} requests at {args.rate}/s, grids 64..256^3, 100-20000 rods x 10, "
-                               f"kT=1, tol {args.tol:g}", "parallelism": "single GPU"},
+                               f"kT=1, tol {args.tol:g}, iteration cap {args.max_iter}", "parallelism": "single GPU"},
+        "converged": h["converged"],
         "p50_ms": round(1e3 * h["p50_latency_s"], 2), "p99_ms": round(1e3 * h["p99_latency_s"], 2),
         "fp32_refine": out["fp32_refine"], "fp64": out["fp64"],
     }
@@ -292,7 +293,9 @@ def main():
     ap.add_argument("--rods", type=int, default=20000)
     ap.add_argument("--spheres", type=int, default=12)
     ap.add_argument("--tol", type=float, default=1e-8)
-    ap.add_argument("--max-iter", type=int, default=300)
+    ap.add_argument("--max-iter", type=int, default=None,
+                    help="GMRES iteration cap (total over refinements) and Krylov capacity; default 600, "
+                         "800 for batch and serve (instances near volume fraction 1 need 500-700 iterations)")
     ap.add_argument("--dt", type=float, default=np.pi * 1.86 ** 3 / 100.0)
     ap.add_argument("--place-attempts", type=int, default=20)
     ap.add_argument("--profile-reps", type=int, default=10)
@@ -301,6 +304,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
 
+    if args.max_iter is None:
+        args.max_iter = 800 if args.workload in ("batch", "serve") else 600
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: the whole 256^3 grid lives on one MI355X and no collective or "

@@ -415,8 +415,9 @@ __global__ void __launch_bounds__(col_threads(NZ))
 // ---------------------------------------------------------------------------
 // inverse z C2R + interpolation partial sums
 //
-// Per round of RR rows: the k-space rows are loaded into the padded FFT
-// scratch, transformed, and read back row-major into this thread's z column
+// Per group of rounds: each thread loads the k-space words its FFT points
+// need (X[k] and X[M-k]) from HBM, the rows are transformed through the padded
+// LDS scratch and read back row-major into this thread's z column
 // (registers).  Then the blob lists are staged in planes as for spreading,
 // and each (blob, stencil z) value lands in its own LDS slot, so the ng
 // values of a blob are added in a fixed order.
@@ -450,52 +451,33 @@ __global__ void __launch_bounds__(col_threads(NZ))
   V* area = reinterpret_cast<V*>(smem);
   const T* rowsT = reinterpret_cast<const T*>(smem);
   const int rloc = t / TC, tcol = t - (t / TC) * TC;
-  // every k-space word of the patch this thread will transform, loaded at once
-  constexpr int KS = M / TC + 1;  // words k = tcol + j TC <= M per row
-  V kin[NRND][KS];
-#pragma unroll
-  for (int q = 0; q < NRND; ++q) {
-    const int row = q * RR + rloc;
-    const bool valid = row < NROW;
-    const int c = row / (CX * CY), rem = row - (row / (CX * CY)) * (CX * CY);
-    const int ii = rem / CY, jj = rem - (rem / CY) * CY;
-    const V* in = fp.p[valid ? c : 0] + ((size_t)(x0 + (valid ? ii : 0)) * g.ny + (y0 + (valid ? jj : 0))) * g.cpitch;
-#pragma unroll
-    for (int j = 0; j < KS; ++j) {
-      const int k = tcol + j * TC;
-      kin[q][j] = (valid && k <= M) ? in[k] : mk<T>(T(0), T(0));
-    }
-  }
   T u[NROW];  // row (c CX + ii) CY + jj at this thread's z
 #pragma unroll
   for (int qg = 0; qg < NRND; qg += NCB) {
-    if (qg > 0) __syncthreads();  // the previous group's rows have been read
-#pragma unroll
-    for (int c = 0; c < NCB; ++c)
-#pragma unroll
-      for (int j = 0; j < KS; ++j) {
-        const int k = tcol + j * TC;
-        if (k <= M) {
-          V x = kin[qg + c][j];
-          if (k == 0 || k == M) x.y = T(0);
-          area[k * LS + rloc + c * RR] = x;
-        }
-      }
-    __syncthreads();
+    // C2R input of this thread's FFT points, straight from HBM: each point k
+    // needs X[k] and X[M - k] (imaginary parts of k = 0 and M ignored)
     V v[NCB][P];
 #pragma unroll
-    for (int c = 0; c < NCB; ++c)
+    for (int c = 0; c < NCB; ++c) {
+      const int row = (qg + c) * RR + rloc;
+      const bool valid = row < NROW;
+      const int cc = row / (CX * CY), rem = row - (row / (CX * CY)) * (CX * CY);
+      const int ii = rem / CY, jj = rem - (rem / CY) * CY;
+      const V* in = fp.p[valid ? cc : 0] +
+                    ((size_t)(x0 + (valid ? ii : 0)) * g.ny + (y0 + (valid ? jj : 0))) * g.cpitch;
 #pragma unroll
       for (int s = 0; s < P; ++s) {
         const int k = fft_in_index<M, true, PM>(tcol, s);
-        const V xk = area[k * LS + rloc + c * RR];
-        const V xm = cconj(area[(M - k) * LS + rloc + c * RR]);
-        const V sum = cadd(xk, xm);
-        const V dif = csub(xk, xm);
-        const V rot = cmulc(dif, s_twz[k]);         // W^{-k} (X[k] - conj X[M-k])
+        V xk = valid ? in[k] : mk<T>(T(0), T(0));
+        V xm = valid ? in[M - k] : mk<T>(T(0), T(0));
+        if (k == 0) { xk.y = T(0); xm.y = T(0); }
+        const V sum = cadd(xk, cconj(xm));
+        const V dif = csub(xk, cconj(xm));
+        const V rot = cmulc(dif, twz[k]);        // W^{-k} (X[k] - conj X[M-k])
         v[c][s] = cadd(sum, mk<T>(-rot.y, rot.x));  // + i * rot
       }
-    __syncthreads();
+    }
+    __syncthreads();  // the previous group's rows have been read (and the twiddles are in LDS)
     fft_columns<T, M, +1, true, PM, NCB>(v, area, LS, rloc, RR, tcol, s_tw);
     __syncthreads();
 #pragma unroll
