@@ -185,21 +185,28 @@ __global__ void __launch_bounds__(256) cols_kernel(FieldPtrs<T> fp, size_t plane
   constexpr int P = PL.P, TC = SH::TC, C = SH::C;
   constexpr bool REV = (S > 0);
   __shared__ V lds[N * C];
+  __shared__ V s_tw[N];
   const int t = threadIdx.x;
+  for (int i = t; i < N; i += SH::THREADS) s_tw[i] = tw[i];
+  __syncthreads();
   const int col = t % C, tcol = t / C;
   const int kz = blockIdx.x * C + col;
   const bool valid = kz < nzc;
-  V* data = fp.p[blockIdx.z] + (size_t)blockIdx.y * plane_stride + (valid ? kz : 0);
+  // a uniform plane base plus 32-bit byte offsets (scalar-base addressing)
+  char* base = reinterpret_cast<char*>(fp.p[blockIdx.z] + (size_t)blockIdx.y * plane_stride);
+  const unsigned colb = (unsigned)((valid ? kz : 0) * sizeof(V));
+  const unsigned eb = (unsigned)(estride * sizeof(V));
   V v[P];
 #pragma unroll
   for (int s = 0; s < P; ++s) {
     const int e = fft_in_index<N, REV>(tcol, s);
-    v[s] = valid ? data[(size_t)e * estride] : mk<T>(T(0), T(0));
+    v[s] = valid ? *reinterpret_cast<const V*>(base + (colb + (unsigned)e * eb)) : mk<T>(T(0), T(0));
   }
-  fft_column<T, N, S, REV>(v, lds, C, col, tcol, tw);
+  fft_column<T, N, S, REV>(v, lds, C, col, tcol, s_tw);
   if (valid) {
 #pragma unroll
-    for (int s = 0; s < P; ++s) data[(size_t)fft_out_index<N, REV>(tcol, s) * estride] = v[s];
+    for (int s = 0; s < P; ++s)
+      *reinterpret_cast<V*>(base + (colb + (unsigned)fft_out_index<N, REV>(tcol, s) * eb)) = v[s];
   }
 }
 
@@ -247,23 +254,27 @@ __global__ void __launch_bounds__(XTMAX<T>::value) xproj_kernel(FieldPtrs<T> fp,
   constexpr FftPlan PL = make_plan(N, PM);
   constexpr int P = PL.P, TC = SH::TC, C = SH::C;
   __shared__ V lds[N * C];
+  __shared__ V s_tw[N];
   const int t = threadIdx.x;
+  for (int i = t; i < N; i += SH::THREADS) s_tw[i] = tw[i];
+  __syncthreads();
   const int col = t % C, tcol = t / C;
   const int kz = blockIdx.x * C + col;
   const int ky = blockIdx.y;
   const bool valid = kz < nzc;
-  const size_t estride = (size_t)ny * cpitch;
-  const size_t off = (size_t)ky * cpitch + (valid ? kz : 0);
+  // uniform bases (row ky of each field) plus 32-bit byte offsets
+  const unsigned colb = (unsigned)((valid ? kz : 0) * sizeof(V));
+  const unsigned eb = (unsigned)((size_t)ny * cpitch * sizeof(V));
   V v[3][P];
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
-    const V* d = fp.p[f] + off;
+    const char* d = reinterpret_cast<const char*>(fp.p[f] + (size_t)ky * cpitch);
 #pragma unroll
     for (int s = 0; s < P; ++s) {
       const int e = fft_in_index<N, false, PM>(tcol, s);
-      v[f][s] = valid ? d[(size_t)e * estride] : mk<T>(T(0), T(0));
+      v[f][s] = valid ? *reinterpret_cast<const V*>(d + (colb + (unsigned)e * eb)) : mk<T>(T(0), T(0));
     }
-    fft_column<T, N, -1, false, PM>(v[f], lds, C, col, tcol, tw);
+    fft_column<T, N, -1, false, PM>(v[f], lds, C, col, tcol, s_tw);
   }
   const T q2 = T(signed_k(ky, pp.ny) * pp.facy);
   const T q3 = T(kz * pp.facz);
@@ -276,11 +287,12 @@ __global__ void __launch_bounds__(XTMAX<T>::value) xproj_kernel(FieldPtrs<T> fp,
   }
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
-    fft_column<T, N, +1, true, PM>(v[f], lds, C, col, tcol, tw);
+    fft_column<T, N, +1, true, PM>(v[f], lds, C, col, tcol, s_tw);
     if (valid) {
-      V* d = fp.p[f] + off;
+      char* d = reinterpret_cast<char*>(fp.p[f] + (size_t)ky * cpitch);
 #pragma unroll
-      for (int s = 0; s < P; ++s) d[(size_t)fft_out_index<N, true, PM>(tcol, s) * estride] = v[f][s];
+      for (int s = 0; s < P; ++s)
+        *reinterpret_cast<V*>(d + (colb + (unsigned)fft_out_index<N, true, PM>(tcol, s) * eb)) = v[f][s];
     }
   }
 }
