@@ -117,7 +117,7 @@ Sizes sizes_of(const fcm_config* c) {
            (size_t)nb * COL_SLOTS_MAX * 3 * 8;
   }
   const int n3b = 3 * c->max_blobs_per_body;
-  t += (size_t)(3 * n3b + n3b * n3b + 36) * 8 + (size_t)c->max_bodies * (3 + 4 + 9 + 6) * 8 + (size_t)nb * 3 * 8;
+  t += (size_t)(3 * n3b + 2 * n3b * n3b + 36) * 8 + (size_t)c->max_bodies * (3 + 4 + 9 + 6) * 8 + (size_t)nb * 3 * 8;
   t += (size_t)(c->max_krylov + 1) * s.ldq * 8 + 6 * s.ldq * 8 + 2 * (size_t)nb * 3 * 8;
   const size_t nblk = (s.nmax + 1023) / 1024;
   t += nblk * (2 * (size_t)c->max_krylov + 6) * 8 + 2 * (2 * (size_t)c->max_krylov + 16) * 8;
@@ -462,6 +462,7 @@ int fcm_create(const fcm_config* cfg, fcm_handle* out) {
   const int n3b = 3 * cfg->max_blobs_per_body;
   if (e == hipSuccess) h->d_ref = dalloc<double>(h, n3b, e);
   if (e == hipSuccess) h->d_Minv = dalloc<double>(h, (size_t)n3b * n3b, e);
+  if (e == hipSuccess) h->d_MinvT = dalloc<double>(h, (size_t)n3b * n3b, e);
   if (e == hipSuccess) h->d_N = dalloc<double>(h, 36, e);
   if (e == hipSuccess) h->d_x = dalloc<double>(h, 3 * (size_t)cfg->max_bodies, e);
   if (e == hipSuccess) h->d_q = dalloc<double>(h, 4 * (size_t)cfg->max_bodies, e);
@@ -783,7 +784,12 @@ int fcm_set_ref_matrices(fcm_handle h, const double* Minv, const double* N) {
   if (!h->have_geometry) return fail(FCM_ERR_STATE, "rb_set_geometry first");
   if (!Minv || !N) return fail(FCM_ERR_ARG, "null matrices");
   const size_t n3 = 3 * (size_t)h->nbpb;
+  // the transposed copy lets the preconditioner stage ref_Minv with contiguous loads
+  std::vector<double> MT(n3 * n3);
+  for (size_t i = 0; i < n3; ++i)
+    for (size_t k = 0; k < n3; ++k) MT[k * n3 + i] = Minv[i * n3 + k];
   HIPCHK(hipMemcpyAsync(h->d_Minv, Minv, n3 * n3 * sizeof(double), hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(h->d_MinvT, MT.data(), n3 * n3 * sizeof(double), hipMemcpyHostToDevice, h->st));
   HIPCHK(hipMemcpyAsync(h->d_N, N, 36 * sizeof(double), hipMemcpyHostToDevice, h->st));
   h->have_ref = true;
   return sync_stream(h);

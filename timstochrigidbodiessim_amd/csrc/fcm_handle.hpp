@@ -50,7 +50,7 @@ struct fcm_handle_s {
   // bodies
   int nbpb = 0, NB = 0;
   double *d_ref = nullptr, *d_x = nullptr, *d_q = nullptr, *d_R = nullptr, *d_dpos = nullptr;
-  double *d_Minv = nullptr, *d_N = nullptr, *d_F6 = nullptr;
+  double *d_Minv = nullptr, *d_MinvT = nullptr, *d_N = nullptr, *d_F6 = nullptr;
   double* d_fblob = nullptr;    // 3 Nb blob forces (barrier forces), blob order
   double* d_F6tot = nullptr;    // 6 NB body forces + K^T blob forces, for the RHS
   double* d_V6 = nullptr;       // 6 NB body velocities for the end-of-step update
@@ -96,6 +96,7 @@ struct fcm_handle_s {
     b.dpos = d_dpos;
     b.Y = dY;
     b.Minv = d_Minv;
+    b.MinvT = d_MinvT;
     b.Nref = d_N;
     return b;
   }

@@ -171,6 +171,7 @@ struct BodyData {
   double* dpos = nullptr;        // 3 nb (world-frame offsets R r_m)
   double* Y = nullptr;           // 3 nb wrapped positions
   const double* Minv = nullptr;  // (3 nbpb)^2
+  const double* MinvT = nullptr; // (3 nbpb)^2, transposed
   const double* Nref = nullptr;  // 36
   const int* gate = nullptr;     // P^-1 kernel returns when *gate != 0
   const int* kcol = nullptr;     // if set, the P^-1 input is column *kcol of a basis with stride ldq

@@ -167,7 +167,8 @@ __global__ void __launch_bounds__(128) pinv_kernel(BodyData b, const double* __r
 // The same products with one thread per (body, row): a pass of a workgroup
 // takes BP = 256 / 3n_b bodies (7 at n_b = 12, 98% of the lanes), and the
 // workgroup loops over passes so that ref_Minv (transposed: the threads of a
-// body read consecutive words) and ref_N are staged in LDS once.  Every sum
+// body read consecutive words; copied from the transposed matrix kept in HBM,
+// a contiguous read) and ref_N are staged in LDS once.  Every sum
 // runs over k in the order of pinv_kernel, so the results are the same bit for
 // bit.  Used when 3 n_b <= 72 (the LDS stays under 64 KiB).
 __global__ void __launch_bounds__(256) pinv_pack_kernel(BodyData b, const double* __restrict__ in,
@@ -186,12 +187,8 @@ __global__ void __launch_bounds__(256) pinv_pack_kernel(BodyData b, const double
   double* oo = gg + BP * n3;
   double* sv = oo + BP * n3;     // BP x 6
   double* vv = sv + BP * 6;      // BP x 6
-  double* Rs = vv + BP * 6;      // BP x 9
   const int t = threadIdx.x;
-  for (int e = t; e < n3 * n3; e += 256) {
-    const int k = e / n3, i = e - (e / n3) * n3;
-    MT[e] = b.Minv[(size_t)i * n3 + k];
-  }
+  for (int e = t; e < n3 * n3; e += 256) MT[e] = b.MinvT[e];
   if (t < 36) NR[t] = b.Nref[t];
   for (int i = t; i < n3; i += 256) ref[i] = b.ref[i];
   const int q = t / n3, i = t - (t / n3) * n3, m = i / 3, c = i - (i / 3) * 3;
@@ -203,35 +200,46 @@ __global__ void __launch_bounds__(256) pinv_pack_kernel(BodyData b, const double
   double* ooq = oo + q * n3;
   double* svq = sv + q * 6;
   double* vvq = vv + q * 6;
-  const double* Rq = Rs + q * 9;
   for (int base = blockIdx.x * BP; base < b.NB; base += gridDim.x * BP) {
-    for (int e = t; e < BP * 9; e += 256) {
-      const int nn = base + e / 9;
-      Rs[e] = nn < b.NB ? b.R[9 * (size_t)nn + (e - (e / 9) * 9)] : 0.0;
-    }
-    __syncthreads();
     const int n = base + q;
     const bool live = q < BP && n < b.NB;
+    // every global load of the pass first: the body's rotation (the same 9
+    // words for the threads of a body), this row's blob vector, the body
+    // vector (rows 0..5) and the blob offset R r_m (for -K^T of the output)
+    double R[9], v3[3] = {0.0, 0.0, 0.0}, w3[3] = {0.0, 0.0, 0.0}, d3[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int r = 0; r < 9; ++r) R[r] = live ? b.R[9 * (size_t)n + r] : 0.0;
     if (live) {
       const double* v = in + (size_t)n3 * n + 3 * m;
-      bpq[i] = Rq[c] * v[0] + Rq[3 + c] * v[1] + Rq[6 + c] * v[2];  // (R^T v)_c
+      v3[0] = v[0]; v3[1] = v[1]; v3[2] = v[2];
+      if (i < 6) {
+        const double* vb = in + nlam + 6 * (size_t)n + (i < 3 ? 0 : 3);
+        w3[0] = vb[0]; w3[1] = vb[1]; w3[2] = vb[2];
+      }
+      if (wKT) {
+        const double* d = b.dpos + 3 * ((size_t)n * b.nbpb + m);
+        d3[0] = d[0]; d3[1] = d[1]; d3[2] = d[2];
+      }
     }
+    if (live) bpq[i] = R[c] * v3[0] + R[3 + c] * v3[1] + R[6 + c] * v3[2];  // (R^T v)_c
     __syncthreads();
     if (live) {
       double s = 0.0;
+#pragma unroll 12
       for (int k = 0; k < n3; ++k) s += MT[k * n3 + i] * bpq[k];
       ttq[i] = s;
     }
     __syncthreads();
     if (live && i < 6) {
       const int cc = i % 3;
-      const double* v = in + nlam + 6 * (size_t)n + (i < 3 ? 0 : 3);
-      const double rv = Rq[cc] * v[0] + Rq[3 + cc] * v[1] + Rq[6 + cc] * v[2];
+      const double rv = R[cc] * w3[0] + R[3 + cc] * w3[1] + R[6 + cc] * w3[2];
       double k = 0.0;
       if (i < 3) {
+#pragma unroll 4
         for (int mm = 0; mm < b.nbpb; ++mm) k += ttq[3 * mm + cc];
       } else {
         const int d1 = (cc + 1) % 3, d2 = (cc + 2) % 3;
+#pragma unroll 4
         for (int mm = 0; mm < b.nbpb; ++mm) k += ref[3 * mm + d1] * ttq[3 * mm + d2] - ref[3 * mm + d2] * ttq[3 * mm + d1];
       }
       svq[i] = -rv - k;
@@ -247,37 +255,44 @@ __global__ void __launch_bounds__(256) pinv_pack_kernel(BodyData b, const double
     __syncthreads();
     if (live) {
       double s = 0.0;
+#pragma unroll 12
       for (int k = 0; k < n3; ++k) s += MT[k * n3 + i] * ggq[k];
       ooq[i] = s;
     }
     __syncthreads();
     if (live) {
       const double* o = ooq + 3 * m;
-      const double v = Rq[3 * c] * o[0] + Rq[3 * c + 1] * o[1] + Rq[3 * c + 2] * o[2];
+      const double v = R[3 * c] * o[0] + R[3 * c + 1] * o[1] + R[3 * c + 2] * o[2];
       out[(size_t)n3 * n + i] = v;
       ggq[i] = v;
     }
     if (live && i < 6) {
       const int cc = i % 3;
       const double* v = vvq + (i < 3 ? 0 : 3);
-      out[nlam + 6 * (size_t)n + i] = Rq[3 * cc] * v[0] + Rq[3 * cc + 1] * v[1] + Rq[3 * cc + 2] * v[2];
+      out[nlam + 6 * (size_t)n + i] = R[3 * cc] * v[0] + R[3 * cc + 1] * v[1] + R[3 * cc + 2] * v[2];
     }
     if (wKT) {
+      __syncthreads();
+      // torque term of this row: (d_m x lambda_m)_c, from the blob's three outputs
+      if (live) {
+        const double* l = ggq + 3 * m;
+        ttq[i] = d3[c1] * l[c2i] - d3[c2i] * l[c1];
+      }
       __syncthreads();
       if (live && i < 6) {
         const int cc = i % 3;
         double s = 0.0;
-        const double* d = b.dpos + 3 * (size_t)n * b.nbpb;
         if (i < 3) {
-          for (int mm = 0; mm < b.nbpb; ++mm) s += ggq[3 * mm + cc];
+  #pragma unroll 4
+        for (int mm = 0; mm < b.nbpb; ++mm) s += ggq[3 * mm + cc];
         } else {
-          const int d1 = (cc + 1) % 3, d2 = (cc + 2) % 3;
-          for (int mm = 0; mm < b.nbpb; ++mm) s += d[3 * mm + d1] * ggq[3 * mm + d2] - d[3 * mm + d2] * ggq[3 * mm + d1];
+  #pragma unroll 4
+        for (int mm = 0; mm < b.nbpb; ++mm) s += ttq[3 * mm + cc];
         }
         wKT[6 * (size_t)n + i] = -s;
       }
     }
-    __syncthreads();  // the next pass overwrites R and the per-body rows
+    __syncthreads();  // the next pass overwrites the per-body rows
   }
 }
 
@@ -286,7 +301,7 @@ hipError_t apply_pinv(const BodyData& b, const double* in, double* out, double* 
   const int n3 = 3 * b.nbpb;
   if (n3 <= 72) {
     const int BP = 256 / n3;
-    const size_t lds = sizeof(double) * ((size_t)n3 * n3 + 36 + n3 + (size_t)BP * (4 * n3 + 21));
+    const size_t lds = sizeof(double) * ((size_t)n3 * n3 + 36 + n3 + (size_t)BP * (4 * n3 + 12));
     const int groups = (b.NB + BP - 1) / BP;
     hipLaunchKernelGGL(pinv_pack_kernel, dim3(groups < 2048 ? groups : 2048), dim3(256), lds, st, b, in, out, wKT);
     return hipGetLastError();
