@@ -251,8 +251,9 @@ int fcm_set_tuning(fcm_handle h, const fcm_tuning* t);
 int fcm_autotune(fcm_handle h, int precision, int reps, fcm_tuning* chosen);
 
 /* ---- measurement hook (no counterpart in the reference) ----------------------- */
-/* Runs one saddle-point mat-vec `reps` times on a fixed input with HIP events
- * between its passes, on the handle's stream; ms_per_pass[0..8] receives the
+/* Runs one saddle-point mat-vec `reps` times (after one untimed warm-up) on a
+ * fixed input with HIP events between its passes, on the handle's stream
+ * (`reps` + 1 launches of each kernel in all); ms_per_pass[0..8] receives the
  * mean device time of: spread, z forward, y forward, x forward+projection+x
  * inverse, y inverse, z inverse, interpolation(+K), K^T, and the reduction of
  * the interpolation partial sums.  On the column path the spread slot holds

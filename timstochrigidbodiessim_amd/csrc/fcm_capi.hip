@@ -924,7 +924,8 @@ int fcm_profile_matvec(fcm_handle h, int precision, int reps, double* ms_per_pas
   BodyData b = h->body();
   int rc = FCM_OK;
   const bool column = h->bd.col_cx > 0 && h->bd.col_fused[precision == FCM_PREC_FP32 ? 1 : 0] != 0;
-  for (int r = 0; r < reps && rc == FCM_OK; ++r) {
+  // rep -1 is an untimed warm-up (first launches load code objects)
+  for (int r = -1; r < reps && rc == FCM_OK; ++r) {
     HIPCHK(hipEventRecord(ev[0], h->st));
     if (column) {
       rc = precision == FCM_PREC_FP32 ? profile_column<float>(h, h->fp32(), h->tw32, kt, ev)
@@ -952,7 +953,7 @@ int fcm_profile_matvec(fcm_handle h, int precision, int reps, double* ms_per_pas
     HIPCHK(mul_KT(b, h->xacc, h->tmp + n3, h->st));
     HIPCHK(hipEventRecord(ev[9], h->st));
     HIPCHK(hipEventSynchronize(ev[9]));
-    for (int i = 0; i < 9; ++i) {
+    for (int i = 0; i < 9 && r >= 0; ++i) {
       float ms = 0;
       HIPCHK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
       acc[i] += ms;

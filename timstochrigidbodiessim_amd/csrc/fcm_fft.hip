@@ -192,7 +192,9 @@ __global__ void __launch_bounds__(256) cols_kernel(FieldPtrs<T> fp, size_t plane
   const int col = t % C, tcol = t / C;
   const int kz = blockIdx.x * C + col;
   const bool valid = kz < nzc;
-  // a uniform plane base plus 32-bit byte offsets (scalar-base addressing)
+  // a uniform plane base plus 32-bit byte offsets (scalar-base addressing);
+  // the Nyquist-column remap of xproj_kernel does not pay here, as the Nyquist
+  // columns of different planes are a plane apart (about 2% slower at 256^3)
   char* base = reinterpret_cast<char*>(fp.p[blockIdx.z] + (size_t)blockIdx.y * plane_stride);
   const unsigned colb = (unsigned)((valid ? kz : 0) * sizeof(V));
   const unsigned eb = (unsigned)(estride * sizeof(V));
@@ -259,16 +261,24 @@ __global__ void __launch_bounds__(XTMAX<T>::value) xproj_kernel(FieldPtrs<T> fp,
   for (int i = t; i < N; i += SH::THREADS) s_tw[i] = tw[i];
   __syncthreads();
   const int col = t % C, tcol = t / C;
-  const int kz = blockIdx.x * C + col;
-  const int ky = blockIdx.y;
-  const bool valid = kz < nzc;
-  // uniform bases (row ky of each field) plus 32-bit byte offsets
-  const unsigned colb = (unsigned)((valid ? kz : 0) * sizeof(V));
+  // When nz/2 is a multiple of C, the last column block of a y row would hold
+  // the Nyquist column kz = nz/2 alone; instead, the last blocks each take the
+  // Nyquist columns of C consecutive y rows (a row apart), and the surplus ones
+  // return (0.095 -> 0.092 ms FP32 at 256^3).
+  const int M = nzc - 1;
+  const bool tail = (M % C == 0) && (int)blockIdx.x == M / C;
+  if (tail && (int)blockIdx.y * C >= ny) return;
+  const int ky = tail ? (int)blockIdx.y * C + col : (int)blockIdx.y;
+  const int kz = tail ? M : (int)blockIdx.x * C + col;
+  const bool valid = tail ? ky < ny : kz < nzc;
+  // uniform bases plus 32-bit byte offsets
+  const size_t rowb = tail ? (size_t)0 : (size_t)blockIdx.y * cpitch;
+  const unsigned colb = valid ? (unsigned)(((tail ? (size_t)ky * cpitch : (size_t)0) + kz) * sizeof(V)) : 0u;
   const unsigned eb = (unsigned)((size_t)ny * cpitch * sizeof(V));
   V v[3][P];
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
-    const char* d = reinterpret_cast<const char*>(fp.p[f] + (size_t)ky * cpitch);
+    const char* d = reinterpret_cast<const char*>(fp.p[f] + rowb);
 #pragma unroll
     for (int s = 0; s < P; ++s) {
       const int e = fft_in_index<N, false, PM>(tcol, s);
@@ -289,7 +299,7 @@ __global__ void __launch_bounds__(XTMAX<T>::value) xproj_kernel(FieldPtrs<T> fp,
   for (int f = 0; f < 3; ++f) {
     fft_column<T, N, +1, true, PM>(v[f], lds, C, col, tcol, s_tw);
     if (valid) {
-      char* d = reinterpret_cast<char*>(fp.p[f] + (size_t)ky * cpitch);
+      char* d = reinterpret_cast<char*>(fp.p[f] + rowb);
 #pragma unroll
       for (int s = 0; s < P; ++s)
         *reinterpret_cast<V*>(d + (colb + (unsigned)fft_out_index<N, true, PM>(tcol, s) * eb)) = v[f][s];
