@@ -193,6 +193,42 @@ constexpr int col_pm(int NZ, int CX) {
   return best;
 }
 
+// LDS layout of the z pass scratch: element e of FFT column col at
+// e ls + (e >> sh) sb + col.
+// Chosen per shape with a model of the 64 LDS banks (a b64 access is served
+// 32 lanes at a time, a b128 access 16 at a time) over every exchange of the
+// transform, within the LDS the kernel has anyway at its occupancy; ls = 0
+// keeps the plain layout (ls = rows + 1, no skew).  At nz = 256 in FP32 this
+// takes the bank cycles of the exchanges of one round from 228 to 124 per
+// wavefront (forward) and from 208 to 80 (inverse).
+struct ColLayout {
+  int ls, sh, sb;
+};
+constexpr ColLayout col_layout(int tbytes, int nz) {
+  if (tbytes == 4) {
+    switch (nz) {
+      case 32: return {72, 2, 8};
+      case 48: return {48, 2, 8};
+      case 64: return {24, 3, 8};
+      case 96: return {48, 1, 4};
+      case 128: return {20, 3, 4};
+      case 192: return {14, 1, 0};
+      case 256: return {18, 4, 2};
+      case 512: return {17, 5, 3};
+      default: return {0, 0, 0};
+    }
+  }
+  switch (nz) {
+    case 32: return {40, 3, 8};
+    case 48: return {20, 1, 0};
+    case 64: return {20, 3, 4};
+    case 96: return {18, 1, 0};
+    case 128: return {18, 3, 2};
+    case 256: return {17, 4, 1};
+    default: return {0, 0, 0};  // nz = 512: the skew costs the inverse kernel a wave per SIMD
+  }
+}
+
 template <typename T, int NZ, int CX, int PM>
 struct ColumnShape {
   static constexpr int NT = col_threads(NZ);
@@ -203,16 +239,28 @@ struct ColumnShape {
   static constexpr int NROW = 3 * CX * COL_CY;    // rows of a patch (3 components)
   static constexpr int NRND = (NROW + RR - 1) / RR;
   static_assert(PL.ok && NT % TC == 0, "z pass plan");
-  // rounds transformed together (one set of barriers): all of them in FP32,
-  // one at a time in FP64, whose scratch would otherwise crowd the LDS
-  static constexpr int NCB = (sizeof(T) == 4 && (M + 1) * (NRND * RR + 1) * 8 <= 81920) ? NRND : 1;
-  static constexpr int LS = NCB * RR + 1;         // FFT scratch: element e of row r at e LS + r
+  // LDS for staging the blob lists (about 38 KB per 256 threads in FP32, 52 KB
+  // in FP64: four or two workgroups per CU at nz = 256)
+  static constexpr int STAGE = col_min((sizeof(T) == 4 ? 38912 : 53248) / 256 * NT, 65536);
+  static constexpr int plain_words(int n) { return col_max((M - 1) * (n * RR + 1) + n * RR, n * RR * (M + 1)); }
+  // all rounds transformed together (one set of barriers) when their scratch
+  // fits in the staging LDS (FP32 only), else one round at a time: a larger
+  // scratch would cost workgroups per CU (at nz = 256 in FP32, three rounds at
+  // once held three workgroups per CU instead of four and took 15% longer)
+  static constexpr int NCB = (sizeof(T) == 4 && plain_words(NRND) * 8 <= STAGE) ? NRND : 1;
+  static constexpr ColLayout LAY = col_layout((int)sizeof(T), NZ);
+  // FFT scratch: element e of row r at e LS + (e >> SSH) SB + r
+  static constexpr int LS = LAY.ls > 0 ? LAY.ls : NCB * RR + 1;
+  static constexpr int SSH = LAY.ls > 0 ? LAY.sh : 0, SB = LAY.ls > 0 ? LAY.sb : 0;
+  static_assert(LS >= NCB * RR, "FFT columns do not overlap");
   static constexpr int RS = M + 1;                // row-major rows: complex words per row
-  static constexpr int FFT_BYTES = (M + 1) * LS * 2 * (int)sizeof(T);
-  static_assert(NCB * RR * RS <= (M + 1) * LS, "row-major rows fit the scratch");
+  static constexpr int FFT_WORDS =
+      col_max((M - 1) * LS + ((M - 1) >> SSH) * SB + NCB * RR, NCB * RR * RS);
+  static constexpr int FFT_BYTES = FFT_WORDS * 2 * (int)sizeof(T);
+  static __device__ __forceinline__ int slot(int e, int col) { return fft_slot<SSH, SB>(e, LS, col); }
+  static __device__ __forceinline__ int row0(int r) { return r * RS; }
   // staging of the blob lists: budget per workgroup, then planes of CAP entries
-  // (about 38 KB per 256 threads in FP32, 52 KB in FP64: four or three workgroups per CU)
-  static constexpr int BUDGET = col_max(col_min((sizeof(T) == 4 ? 38912 : 53248) / 256 * NT, 65536), FFT_BYTES);
+  static constexpr int BUDGET = col_max(STAGE, FFT_BYTES);
   static constexpr int S_ENT = 19 * (int)sizeof(T) + 4;   // gx 4, gy 4, F 3, gz 8 | stencil-start z
   static constexpr int S_CAP = col_odd_le(BUDGET / S_ENT);
   static constexpr int S_LDS = col_max(FFT_BYTES, S_CAP * S_ENT);
@@ -220,24 +268,26 @@ struct ColumnShape {
   static constexpr int I_CAP = col_odd_le(BUDGET / I_ENT);
   static constexpr int I_LDS = col_max(FFT_BYTES, I_CAP * I_ENT);
   static_assert(S_CAP >= 15 && I_CAP >= 15, "staging");
-  static_assert(S_LDS + 4 * (NZ + 1) <= 163840 && I_LDS + 4 * (NZ + 1) <= 163840, "LDS");
+  static_assert(S_LDS + (M + M / 2 + 1) * 2 * (int)sizeof(T) <= 163840 &&
+                    I_LDS + M * 2 * (int)sizeof(T) <= 163840,
+                "LDS");
 };
 
 // the range [a1, b1) u [a2, b2) of the patch list holds exactly the blobs
 // whose stencil covers z (stencil start in [z - ng + 1, z] modulo nz)
 template <int NZ>
-__device__ __forceinline__ void col_ranges(const int* s_zst, int z, int ng, int& a1, int& b1, int& a2, int& b2) {
+__device__ __forceinline__ void col_ranges(const int* __restrict__ zs, int z, int ng, int& a1, int& b1, int& a2, int& b2) {
   a1 = b1 = a2 = b2 = 0;
   if (z >= NZ) return;
   const int lo = z - ng + 1;
   if (lo >= 0) {
-    a1 = s_zst[lo];
-    b1 = s_zst[z + 1];
+    a1 = zs[lo];
+    b1 = zs[z + 1];
   } else {
-    a1 = s_zst[NZ + lo];
-    b1 = s_zst[NZ];
-    a2 = s_zst[0];
-    b2 = s_zst[z + 1];
+    a1 = zs[NZ + lo];
+    b1 = zs[NZ];
+    a2 = zs[0];
+    b2 = zs[z + 1];
   }
 }
 
@@ -272,10 +322,9 @@ __global__ void __launch_bounds__(col_threads(NZ))
   using SH = ColumnShape<T, NZ, CX, PM>;
   using V = c2<T>;
   constexpr int NT = SH::NT, M = SH::M, P = SH::P, TC = SH::TC, RR = SH::RR, NROW = SH::NROW, NRND = SH::NRND;
-  constexpr int CY = COL_CY, CAP = SH::S_CAP, LS = SH::LS, RS = SH::RS, NCB = SH::NCB;
+  constexpr int CY = COL_CY, CAP = SH::S_CAP, LS = SH::LS, NCB = SH::NCB;
   __shared__ __align__(16) unsigned char smem[SH::S_LDS];
-  __shared__ int s_zst[NZ + 1];
-  __shared__ V s_tw[M], s_twz[M + 1];
+  __shared__ V s_tw[M], s_twz[M / 2 + 1];  // untangle twiddles W^k, k <= M/2
   if (fp.gate && *fp.gate) return;
   const GridGeom& g = A.g;
   const int npatch = A.npx * A.npy;
@@ -285,15 +334,16 @@ __global__ void __launch_bounds__(col_threads(NZ))
   const int x0 = CX * px, y0 = CY * py;
   const int t = threadIdx.x;
   const int ng = A.ng, omin = 1 - ng / 2;
-  for (int i = t; i <= NZ; i += NT) s_zst[i] = A.zst[(size_t)p * (NZ + 1) + i];
-  for (int i = t; i <= M; i += NT) {
-    if (i < M) s_tw[i] = tw[i];
-    s_twz[i] = twz[i];
+  for (int i = t; i < M; i += NT) {
+    s_tw[i] = tw[i];
+    if (i <= M / 2) s_twz[i] = twz[i];
   }
   __syncthreads();
-  const int L0 = s_zst[0], L1 = s_zst[NZ];
+  // this patch's z prefix table, read from HBM (a few words per thread)
+  const int* zs = A.zst + (size_t)p * (NZ + 1);
+  const int L0 = zs[0], L1 = zs[NZ];
   int a1, b1, a2, b2;
-  col_ranges<NZ>(s_zst, t, ng, a1, b1, a2, b2);
+  col_ranges<NZ>(zs, t, ng, a1, b1, a2, b2);
 
   T* sw = reinterpret_cast<T*>(smem);  // planes: gx 0-3, gy 4-7, F 8-10, gz 11-18
   int* ssz = reinterpret_cast<int*>(sw + 19 * CAP);
@@ -376,20 +426,20 @@ __global__ void __launch_bounds__(col_threads(NZ))
     if (t < NZ) {
 #pragma unroll
       for (int r = 0; r < NCB * RR; ++r)
-        if (qg * RR + r < NROW) rowsT[r * 2 * RS + t] = acc[qg * RR + r];
+        if (qg * RR + r < NROW) rowsT[2 * SH::row0(r) + t] = acc[qg * RR + r];
     }
     __syncthreads();
     V v[NCB][P];
 #pragma unroll
     for (int c = 0; c < NCB; ++c)
 #pragma unroll
-      for (int s = 0; s < P; ++s) v[c][s] = area[(rloc + c * RR) * RS + fft_in_index<M, false, PM>(tcol, s)];
+      for (int s = 0; s < P; ++s) v[c][s] = area[SH::row0(rloc + c * RR) + fft_in_index<M, false, PM>(tcol, s)];
     __syncthreads();
-    fft_columns<T, M, -1, false, PM, NCB>(v, area, LS, rloc, RR, tcol, s_tw);
+    fft_columns<T, M, -1, false, PM, NCB, SH::SSH, SH::SB>(v, area, LS, rloc, RR, tcol, s_tw);
 #pragma unroll
     for (int c = 0; c < NCB; ++c)
 #pragma unroll
-      for (int s = 0; s < P; ++s) area[fft_out_index<M, false, PM>(tcol, s) * LS + rloc + c * RR] = v[c][s];
+      for (int s = 0; s < P; ++s) area[SH::slot(fft_out_index<M, false, PM>(tcol, s), rloc + c * RR)] = v[c][s];
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < NCB; ++c) {
@@ -398,14 +448,13 @@ __global__ void __launch_bounds__(col_threads(NZ))
         const int cc = row / (CX * CY), rem = row - (row / (CX * CY)) * (CX * CY);
         const int ii = rem / CY, jj = rem - (rem / CY) * CY;
         V* out = fp.p[cc] + ((size_t)(x0 + ii) * g.ny + (y0 + jj)) * g.cpitch;
-        // untangle: X[k] = E[k] + W^k O[k], E = (Z[k] + conj Z[M-k])/2, O = (Z[k] - conj Z[M-k])/(2i)
-        for (int k = tcol; k <= M; k += TC) {
-          const V zk = area[(k == M ? 0 : k) * LS + rloc + c * RR];
-          const V zm = cconj(area[(k == 0 ? 0 : M - k) * LS + rloc + c * RR]);
-          const V e = cscale(cadd(zk, zm), T(0.5));
-          const V d = csub(zk, zm);
-          const V o = mk<T>(T(0.5) * d.y, T(-0.5) * d.x);
-          out[k] = cadd(e, cmul(s_twz[k], o));
+        // untangle, one bin pair (k, M - k) per step
+        const int zc = rloc + c * RR;
+        for (int k = tcol; k <= M / 2; k += TC) {
+          V xk, xmk;
+          r2c_untangle_pair<T>(area[SH::slot(k, zc)], area[SH::slot(k == 0 ? 0 : M - k, zc)], s_twz[k], xk, xmk);
+          out[k] = xk;
+          if (M - k != k) out[M - k] = xmk;
         }
       }
     }
@@ -429,10 +478,9 @@ __global__ void __launch_bounds__(col_threads(NZ))
   using SH = ColumnShape<T, NZ, CX, PM>;
   using V = c2<T>;
   constexpr int NT = SH::NT, M = SH::M, P = SH::P, TC = SH::TC, RR = SH::RR, NROW = SH::NROW, NRND = SH::NRND;
-  constexpr int CY = COL_CY, CAP = SH::I_CAP, LS = SH::LS, RS = SH::RS, NCB = SH::NCB;
+  constexpr int CY = COL_CY, CAP = SH::I_CAP, LS = SH::LS, NCB = SH::NCB;
   __shared__ __align__(16) unsigned char smem[SH::I_LDS];
-  __shared__ int s_zst[NZ + 1];
-  __shared__ V s_tw[M], s_twz[M + 1];
+  __shared__ V s_tw[M];
   if (fp.gate && *fp.gate) return;
   const GridGeom& g = A.g;
   const int npatch = A.npx * A.npy;
@@ -442,11 +490,7 @@ __global__ void __launch_bounds__(col_threads(NZ))
   const int x0 = CX * px, y0 = CY * py;
   const int t = threadIdx.x;
   const int ng = A.ng, omin = 1 - ng / 2;
-  for (int i = t; i <= NZ; i += NT) s_zst[i] = A.zst[(size_t)p * (NZ + 1) + i];
-  for (int i = t; i <= M; i += NT) {
-    if (i < M) s_tw[i] = tw[i];
-    s_twz[i] = twz[i];
-  }
+  for (int i = t; i < M; i += NT) s_tw[i] = tw[i];
 
   V* area = reinterpret_cast<V*>(smem);
   const T* rowsT = reinterpret_cast<const T*>(smem);
@@ -478,22 +522,24 @@ __global__ void __launch_bounds__(col_threads(NZ))
       }
     }
     __syncthreads();  // the previous group's rows have been read (and the twiddles are in LDS)
-    fft_columns<T, M, +1, true, PM, NCB>(v, area, LS, rloc, RR, tcol, s_tw);
+    fft_columns<T, M, +1, true, PM, NCB, SH::SSH, SH::SB>(v, area, LS, rloc, RR, tcol, s_tw);
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < NCB; ++c)
 #pragma unroll
-      for (int s = 0; s < P; ++s) area[(rloc + c * RR) * RS + fft_out_index<M, true, PM>(tcol, s)] = v[c][s];
+      for (int s = 0; s < P; ++s) area[SH::row0(rloc + c * RR) + fft_out_index<M, true, PM>(tcol, s)] = v[c][s];
     __syncthreads();
     if (t < NZ) {
 #pragma unroll
       for (int r = 0; r < NCB * RR; ++r)
-        if (qg * RR + r < NROW) u[qg * RR + r] = rowsT[r * 2 * RS + t];
+        if (qg * RR + r < NROW) u[qg * RR + r] = rowsT[2 * SH::row0(r) + t];
     }
   }
-  const int L0 = s_zst[0], L1 = s_zst[NZ];
+  // this patch's z prefix table, read from HBM (a few words per thread)
+  const int* zs = A.zst + (size_t)p * (NZ + 1);
+  const int L0 = zs[0], L1 = zs[NZ];
   int a1, b1, a2, b2;
-  col_ranges<NZ>(s_zst, t, ng, a1, b1, a2, b2);
+  col_ranges<NZ>(zs, t, ng, a1, b1, a2, b2);
   T* sw = reinterpret_cast<T*>(smem);  // planes: gx 0-3, gy 4-7, gz 8-15, slot (c, o) 16 + 8 c + o
   int* ssz = reinterpret_cast<int*>(sw + 40 * CAP);
   int* sout = ssz + CAP;  // partial-sum slot of the blob for this patch

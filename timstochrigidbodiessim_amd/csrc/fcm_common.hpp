@@ -35,6 +35,22 @@ template <int S, typename V> __host__ __device__ __forceinline__ V cmul_si(V a) 
   return r;
 }
 
+// Real-to-complex untangle of Z, the M-point complex FFT of 2M reals packed as
+// (even, odd) pairs, for the bin pair (k, M - k), 0 <= k <= M/2:
+//   E = (Z[k] + conj Z[M-k]) / 2,  O = (Z[k] - conj Z[M-k]) / (2i)  (Z indices mod M),
+//   X[k] = E + W^k O,  X[M-k] = conj(E - W^k O),  W = exp(-i pi / M),
+// the second from E[M-k] = conj E[k], O[M-k] = conj O[k] and W^(M-k) = -conj W^k.
+template <typename T>
+__device__ __forceinline__ void r2c_untangle_pair(c2<T> zk, c2<T> zmk, c2<T> wk, c2<T>& xk, c2<T>& xmk) {
+  const c2<T> zm = cconj(zmk);
+  const c2<T> e = cscale(cadd(zk, zm), T(0.5));
+  const c2<T> d = csub(zk, zm);
+  const c2<T> o = mk<T>(T(0.5) * d.y, T(-0.5) * d.x);  // d / (2i)
+  const c2<T> t = cmul(wk, o);
+  xk = cadd(e, t);
+  xmk = cconj(csub(e, t));
+}
+
 // ---------------------------------------------------------------------------
 // FFT plans: a length N is split into stages of radix 16, 8, 4, 2, 3 or 5.
 // Each thread of a column holds P points; every radix divides P, so each
@@ -282,8 +298,15 @@ __device__ __forceinline__ void fft_column(c2<T>* v, c2<T>* lds, int ls, int col
 
 // The same transform for NC columns per thread (column c of this thread at
 // lds column col + c * cs), with one set of barriers for all of them and each
-// twiddle factor loaded once for the NC columns.
-template <typename T, int N, int S, bool REV, int STG, int PM, int NC>
+// twiddle factor loaded once for the NC columns.  Element e of a column lies at
+// lds[e ls + (e >> SSH) SB + column]: the skew (SB > 0) spreads the exchanges
+// of a wavefront over more LDS banks for some shapes.
+template <int SSH, int SB>
+__device__ __forceinline__ int fft_slot(int e, int ls, int col) {
+  return e * ls + (SB != 0 ? (e >> SSH) * SB : 0) + col;
+}
+
+template <typename T, int N, int S, bool REV, int STG, int PM, int NC, int SSH = 0, int SB = 0>
 struct FftStageB {
   static constexpr FftPlan PL = REV ? reverse_plan(make_plan(N, PM)) : make_plan(N, PM);
   static constexpr int P = PL.P;
@@ -321,7 +344,7 @@ struct FftStageB {
 #pragma unroll
         for (int r = 0; r < R; ++r)
 #pragma unroll
-          for (int c = 0; c < NC; ++c) lds[(base + r * NS) * ls + col + c * cs] = v[c][i * R + r];
+          for (int c = 0; c < NC; ++c) lds[fft_slot<SSH, SB>(base + r * NS, ls, col + c * cs)] = v[c][i * R + r];
       }
       __syncthreads();
 #pragma unroll
@@ -330,19 +353,19 @@ struct FftStageB {
 #pragma unroll
         for (int r = 0; r < R2; ++r)
 #pragma unroll
-          for (int c = 0; c < NC; ++c) v[c][i * R2 + r] = lds[(j + r * (N / R2)) * ls + col + c * cs];
+          for (int c = 0; c < NC; ++c) v[c][i * R2 + r] = lds[fft_slot<SSH, SB>(j + r * (N / R2), ls, col + c * cs)];
       }
       __syncthreads();
-      FftStageB<T, N, S, REV, STG + 1, PM, NC>::run(v, lds, ls, col, cs, tcol, tw);
+      FftStageB<T, N, S, REV, STG + 1, PM, NC, SSH, SB>::run(v, lds, ls, col, cs, tcol, tw);
     }
   }
 };
 
-template <typename T, int N, int S, bool REV, int PM, int NC>
+template <typename T, int N, int S, bool REV, int PM, int NC, int SSH = 0, int SB = 0>
 __device__ __forceinline__ void fft_columns(c2<T> (&v)[NC][REV ? reverse_plan(make_plan(N, PM)).P : make_plan(N, PM).P],
                                             c2<T>* lds, int ls, int col, int cs, int tcol,
                                             const c2<T>* __restrict__ tw) {
-  FftStageB<T, N, S, REV, 0, PM, NC>::run(v, lds, ls, col, cs, tcol, tw);
+  FftStageB<T, N, S, REV, 0, PM, NC, SSH, SB>::run(v, lds, ls, col, cs, tcol, tw);
 }
 
 // element index held in register slot (i, r) for the first radix of a plan

@@ -116,15 +116,14 @@ __global__ void __launch_bounds__(256) zfwd_kernel(FieldPtrs<T> fp, int rows, in
 #pragma unroll
   for (int s = 0; s < P; ++s) lds[fft_out_index<M, false>(tcol, s) * LS + rloc] = v[s];
   __syncthreads();
-  // untangle: X[k] = E[k] + W^k O[k], E = (Z[k] + conj Z[M-k])/2, O = (Z[k] - conj Z[M-k])/(2i)
-  for (int k = tcol; k <= M; k += TC) {
-    const V zk = lds[(k == M ? 0 : k) * LS + rloc];
-    const V zm = cconj(lds[(k == 0 ? 0 : M - k) * LS + rloc]);
-    const V e = cscale(cadd(zk, zm), T(0.5));
-    const V d = csub(zk, zm);
-    const V o = mk<T>(T(0.5) * d.y, T(-0.5) * d.x);  // d / (2i)
-    const V x = cadd(e, cmul(twz[k], o));
-    if (valid) base[k] = x;
+  // untangle, one bin pair (k, M - k) per step
+  for (int k = tcol; k <= M / 2; k += TC) {
+    V xk, xmk;
+    r2c_untangle_pair<T>(lds[k * LS + rloc], lds[(k == 0 ? 0 : M - k) * LS + rloc], twz[k], xk, xmk);
+    if (valid) {
+      base[k] = xk;
+      if (M - k != k) base[M - k] = xmk;
+    }
   }
 }
 
