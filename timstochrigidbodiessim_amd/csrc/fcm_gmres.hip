@@ -118,6 +118,7 @@ __global__ void __launch_bounds__(GT) gm_start_kernel(GmDev G, double tol, int m
     G.ictl[GM_FIN] = 0;
     G.ictl[GM_MAXIT] = max_iter;
     G.ictl[GM_NCOLS] = 0;
+    G.ictl[GM_CNT] = 0;
     G.ictl[GM_STATUS] = nrm == 0.0 ? 1 + 2 /* zero rhs */ : 0;
     G.beta[0] = nrm;
   }
@@ -203,29 +204,53 @@ __global__ void __launch_bounds__(KRT) gm_dots_kernel(GmDev G, const double* __r
   }
 }
 
-// red[c] = sum_blk partial[c][blk] for the rows the dots pass wrote; fixed order
-__global__ void __launch_bounds__(256) gm_reduce_kernel(GmDev G, const double* __restrict__ partial, int nblk,
-                                                        int mode) {
-  const int status = G.ictl[GM_STATUS];
-  if (status != 0) return;
-  if (mode == 1 && G.ictl[GM_FIN] == 0) return;
-  const int k = G.ictl[GM_K];
-  // mode 0: 2 (k+1) rows; mode 1: k+2 rows; mode 2: the single norm row
-  const int rows = mode == 0 ? 2 * (k + 1) : (mode == 1 ? k + 2 : 1);
-  __shared__ double s[256];
+// G.red[c] = sum_blk partial[c][blk] for rows c = blockIdx.x, blockIdx.x +
+// gridDim.x, ... < rows; fixed order (thread-strided sums, then a tree in LDS)
+__device__ void reduce_rows(const GmDev& G, const double* __restrict__ partial, int nblk, int rows, double* s) {
   for (int c = blockIdx.x; c < rows; c += gridDim.x) {
     const double* p = partial + (size_t)c * nblk;
     double a = 0.0;
-    for (int i = threadIdx.x; i < nblk; i += 256) a += p[i];
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x) a += p[i];
     s[threadIdx.x] = a;
     __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
       if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
       __syncthreads();
     }
     if (threadIdx.x == 0) G.red[c] = s[0];
     __syncthreads();
   }
+}
+
+// rows of a dots pass: mode 0 the main pass (2 (k+1) rows), mode 1 the
+// finalising pass (k+2 rows, only when it ran)
+__global__ void __launch_bounds__(256) gm_reduce_kernel(GmDev G, const double* __restrict__ partial, int nblk,
+                                                        int mode) {
+  if (G.ictl[GM_STATUS] != 0 || (mode == 1 && G.ictl[GM_FIN] == 0)) return;
+  __shared__ double s[256];
+  const int k = G.ictl[GM_K];
+  reduce_rows(G, partial, nblk, mode == 0 ? 2 * (k + 1) : k + 2, s);
+}
+
+// Called by every block of a grid after its writes: true in the block that
+// arrives last, which then sees the writes of all the others (device-scope
+// fences around the arrival count) and resets the count for the next kernel.
+__device__ bool last_block(const GmDev& G, int* s_flag) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) *s_flag = atomicAdd(G.ictl + GM_CNT, 1) == (int)gridDim.x - 1;
+  __syncthreads();
+  if (!*s_flag) return false;
+  __threadfence();
+  if (threadIdx.x == 0) G.ictl[GM_CNT] = 0;
+  return true;
+}
+
+// end of an iteration: the next one works on column k + 1
+__device__ __forceinline__ void end_iteration(const GmDev& G) {
+  G.ictl[GM_PROV] = G.ictl[GM_FIN] ? 0 : 1;
+  G.ictl[GM_FIN] = 0;
+  G.ictl[GM_K] += 1;
 }
 
 // mode 0: q = (Q col k - sum_{i<m} s_i Q_i) / nu written back (if m >= 0 and
@@ -235,13 +260,20 @@ __global__ void __launch_bounds__(KRT) gm_update_kernel(GmDev G, double* __restr
                                                         double* __restrict__ wbuf, size_t n, int mode,
                                                         double* __restrict__ partial) {
   const int status = G.ictl[GM_STATUS];
-  if (status != 0 || (mode == 1 && G.ictl[GM_FIN] == 0)) return;
+  if (status != 0) return;
+  if (mode == 1 && G.ictl[GM_FIN] == 0) {
+    // no finalisation this step: block 0 ends the iteration (the other blocks
+    // read nothing it changes)
+    if (blockIdx.x == 0 && threadIdx.x == 0) end_iteration(G);
+    return;
+  }
   const int k = G.ictl[GM_K];
   const int m = G.ictl[GM_M], mt = G.ictl[GM_MT];
   double* qt = mode == 0 ? (m > 0 ? Q + (size_t)k * ldq : nullptr) : Q + (size_t)(k + 1) * ldq;
   double* w = mode == 0 ? wbuf : nullptr;
   __shared__ double cs[2048 + 4];
   __shared__ double red[4];
+  __shared__ int s_last;
   const int ncoef = m + mt + 2;
   for (int i = threadIdx.x; i < ncoef && i < 2048 + 4; i += KRT) cs[i] = G.coef[i];
   __syncthreads();
@@ -299,6 +331,8 @@ __global__ void __launch_bounds__(KRT) gm_update_kernel(GmDev G, double* __restr
     if (lane == 0) red[wv] = acc;
     __syncthreads();
     if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+  } else if (last_block(G, &s_last)) {
+    if (threadIdx.x == 0) end_iteration(G);  // after every block has read k
   }
 }
 
@@ -401,11 +435,13 @@ __global__ void __launch_bounds__(64) gm_hs_kernel(GmDev G) {
 }
 
 // after the main update: normalisation, breakdown, predicted convergence
-__global__ void __launch_bounds__(GT) gm_ctl2_kernel(GmDev G) {
+// (one block; ||w||^2 from the update pass's block partials, fixed order)
+__global__ void __launch_bounds__(GT) gm_ctl2_kernel(GmDev G, const double* __restrict__ partial, int nblk) {
   if (G.ictl[GM_STATUS] != 0) return;
   __shared__ double col[GCAP], sCS[GCAP], sSN[GCAP];
   __shared__ int s_brk;
   const int k = G.ictl[GM_K];
+  reduce_rows(G, partial, nblk, 1, col);
   const double etar = sqrt(G.red[0]);
   const double eta = etar / G.dctl[3];
   for (int l = threadIdx.x; l <= k; l += GT) {
@@ -496,16 +532,6 @@ __global__ void __launch_bounds__(GT) gm_ctl3_kernel(GmDev G) {
   }
 }
 
-// end of an iteration
-__global__ void gm_end_kernel(GmDev G) {
-  if (G.ictl[GM_STATUS] != 0) return;
-  if (threadIdx.x == 0) {
-    G.ictl[GM_PROV] = G.ictl[GM_FIN] ? 0 : 1;
-    G.ictl[GM_FIN] = 0;
-    G.ictl[GM_K] += 1;
-  }
-}
-
 // back substitution (gmres_solver.cpp:260-276) into G.y; one wavefront
 __global__ void __launch_bounds__(64) gm_backsub_kernel(GmDev G) {
   const int st = G.ictl[GM_STATUS] - 1;
@@ -572,21 +598,20 @@ hipError_t gm_launch_start(const GmDev& G, double tol, int max_iter, const doubl
 
 hipError_t gm_launch_basis(const GmDev& G, double* Q, size_t ldq, double* wbuf, size_t n, double* partial,
                            int nblk, hipStream_t st) {
-  // main dots, reduce, control 1, H s, update, norm reduce, control 2, scale
+  // main dots, reduce, control 1, H s, update, norm reduce + control 2, scale
   hipLaunchKernelGGL(gm_dots_kernel, dim3(nblk, DOT_CG), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 0, partial, nblk);
   hipLaunchKernelGGL(gm_reduce_kernel, dim3(256), dim3(256), 0, st, G, partial, nblk, 0);
   hipLaunchKernelGGL(gm_ctl1_kernel, dim3(1), dim3(GT), 0, st, G);
   hipLaunchKernelGGL(gm_hs_kernel, dim3(G.M + 1), dim3(64), 0, st, G);
   hipLaunchKernelGGL(gm_update_kernel, dim3(nblk), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 0, partial);
-  hipLaunchKernelGGL(gm_reduce_kernel, dim3(1), dim3(256), 0, st, G, partial, nblk, 2);
-  hipLaunchKernelGGL(gm_ctl2_kernel, dim3(1), dim3(GT), 0, st, G);
+  hipLaunchKernelGGL(gm_ctl2_kernel, dim3(1), dim3(GT), 0, st, G, partial, nblk);
   hipLaunchKernelGGL(gm_scale_kernel, dim3(1024), dim3(256), 0, st, G, wbuf, Q, ldq, n);
-  // finalisation of the new vector when convergence is predicted
+  // finalisation of the new vector when convergence is predicted; the last
+  // update pass also ends the iteration
   hipLaunchKernelGGL(gm_dots_kernel, dim3(nblk, DOT_CG), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 1, partial, nblk);
   hipLaunchKernelGGL(gm_reduce_kernel, dim3(256), dim3(256), 0, st, G, partial, nblk, 1);
   hipLaunchKernelGGL(gm_ctl3_kernel, dim3(1), dim3(GT), 0, st, G);
   hipLaunchKernelGGL(gm_update_kernel, dim3(nblk), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 1, partial);
-  hipLaunchKernelGGL(gm_end_kernel, dim3(1), dim3(64), 0, st, G);
   return hipGetLastError();
 }
 
