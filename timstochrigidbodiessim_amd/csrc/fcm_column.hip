@@ -621,26 +621,43 @@ __global__ void __launch_bounds__(col_threads(NZ))
   }
 }
 
-// sum of a blob's patch partials in a fixed order, then dx^3 J u - K V
-template <typename T>
-__global__ void __launch_bounds__(256) col_reduce_kernel(GridGeom g, int ng, int cx, int nb, const int4* __restrict__ rec,
+// sum of a blob's patch partials in a fixed order, then dx^3 J u - K V.  The
+// loads of every slot a blob can have (XM x 3 for patches CX wide, ng <= 8)
+// are issued before the record that says which of them are its own arrives;
+// the others are read but not added.
+template <typename T, int CX>
+__global__ void __launch_bounds__(256) col_reduce_kernel(GridGeom g, int ng, int nb, const int4* __restrict__ rec,
                                                          const T* __restrict__ part, double* V, KTerm kt, double dx3,
                                                          const int* gate) {
+  constexpr int XM = (CX - 1 + 7) / CX + 1, YM = (COL_CY - 1 + 7) / COL_CY + 1;
+  static_assert(XM * YM <= COL_SLOTS_MAX, "partial-sum slots");
   if (gate && *gate) return;
   const int s = blockIdx.x * 256 + threadIdx.x;
   if (s >= nb) return;
+  T pv[XM][YM][3];
+#pragma unroll
+  for (int dx = 0; dx < XM; ++dx)
+#pragma unroll
+    for (int dy = 0; dy < YM; ++dy) {
+      const T* pp = part + ((size_t)(dx * 3 + dy) * nb + s) * 3;
+      pv[dx][dy][0] = pp[0];
+      pv[dx][dy][1] = pp[1];
+      pv[dx][dy][2] = pp[2];
+    }
   const int omin = 1 - ng / 2;
   const int4 r = rec[s];
   const int sxw = col_wrapi(r.x + omin, g.nx), syw = col_wrapi(r.y + omin, g.ny);
-  const int nxp = (sxw % cx + ng - 1) / cx + 1, nyp = (syw % COL_CY + ng - 1) / COL_CY + 1;
+  const int nxp = (sxw % CX + ng - 1) / CX + 1, nyp = (syw % COL_CY + ng - 1) / COL_CY + 1;
   double v0 = 0.0, v1 = 0.0, v2 = 0.0;
-  for (int dx = 0; dx < nxp; ++dx)
-    for (int dy = 0; dy < nyp; ++dy) {
-      const T* pp = part + ((size_t)(dx * 3 + dy) * nb + s) * 3;
-      v0 += (double)pp[0];
-      v1 += (double)pp[1];
-      v2 += (double)pp[2];
-    }
+#pragma unroll
+  for (int dx = 0; dx < XM; ++dx)
+#pragma unroll
+    for (int dy = 0; dy < YM; ++dy)
+      if (dx < nxp && dy < nyp) {
+        v0 += (double)pv[dx][dy][0];
+        v1 += (double)pv[dx][dy][1];
+        v2 += (double)pv[dx][dy][2];
+      }
   store_blob_velocity(V, kt, dx3, r.w, v0, v1, v2);
 }
 
@@ -760,8 +777,13 @@ hipError_t zinv_interp(const GridGeom& g, const BlobData& bd, const FieldPtrs<T>
   }
   if (e != hipSuccess) return e;
   if (between && (e = hipEventRecord(between, st)) != hipSuccess) return e;
-  hipLaunchKernelGGL((col_reduce_kernel<T>), dim3((bd.nb + 255) / 256), dim3(256), 0, st, g, bd.ng, bd.col_cx, bd.nb,
-                     bd.rec, (const T*)part, V, kt, g.dx * g.dx * g.dx, fp.gate);
+  const dim3 grid((bd.nb + 255) / 256);
+  if (bd.col_cx == 4)
+    hipLaunchKernelGGL((col_reduce_kernel<T, 4>), grid, dim3(256), 0, st, g, bd.ng, bd.nb, bd.rec, (const T*)part, V,
+                       kt, g.dx * g.dx * g.dx, fp.gate);
+  else
+    hipLaunchKernelGGL((col_reduce_kernel<T, 2>), grid, dim3(256), 0, st, g, bd.ng, bd.nb, bd.rec, (const T*)part, V,
+                       kt, g.dx * g.dx * g.dx, fp.gate);
   return hipGetLastError();
 }
 
