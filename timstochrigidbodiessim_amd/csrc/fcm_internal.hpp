@@ -244,10 +244,7 @@ hipError_t rhs_assemble(const double* Vrand, const double* F6, double* b, size_t
 // ---------------------------------------------------------------------------
 // device-resident GMRES control (fcm_gmres.hip)
 // ---------------------------------------------------------------------------
-// GM_CNT counts the finished blocks of a reduction whose last block goes on
-// with the control step (zero between kernels)
-enum { GM_K = 0, GM_PROV = 1, GM_STATUS = 2, GM_NCOLS = 3, GM_FIN = 4, GM_MAXIT = 5, GM_M = 6, GM_MT = 7, GM_HS = 8,
-       GM_CNT = 9, GM_NINT = 10 };
+enum { GM_K = 0, GM_PROV = 1, GM_STATUS = 2, GM_NCOLS = 3, GM_FIN = 4, GM_MAXIT = 5, GM_M = 6, GM_MT = 7, GM_HS = 8, GM_NINT = 9 };
 
 struct GmDev {
   int* ictl = nullptr;     // GM_* words; status 0 = running, 1 + FCM_GMRES_* when done
