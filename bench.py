@@ -64,7 +64,7 @@ def pass_bytes(nx, ny, nz, nb, word, column=False):
 KERNEL_OF_PASS = {
     True: {"spread": "col_spread_kernel<float, 256, 4, 8>", "interp": "col_interp_kernel<float, 256, 4, 8>",
            "y_forward": "cols_kernel<float, 256, -1>", "x_project": "xproj_kernel<float, 256>",
-           "y_inverse": "cols_kernel<float, 256, 1>", "interp_reduce": "col_reduce_kernel<float>"},
+           "y_inverse": "cols_kernel<float, 256, 1>", "interp_reduce": "col_reduce_kernel<float, 4>"},
     False: {"z_forward": "zfwd_kernel<float, 128, false>", "z_inverse": "zinv_kernel<float, 128>",
             "y_forward": "cols_kernel<float, 256, -1>", "x_project": "xproj_kernel<float, 256>",
             "y_inverse": "cols_kernel<float, 256, 1>"},
