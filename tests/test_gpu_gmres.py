@@ -115,3 +115,38 @@ def test_one_body_reference_pose_one_iteration():
         assert rep["iterations"] == 1 and rep["status"] in ("converged", "breakdown")
         res = s.saddle_matvec(sol) - rhs
         assert np.linalg.norm(res) < 1e-10 * np.linalg.norm(rhs)
+
+
+def test_long_vector_solve_residual():
+    """A system of more than 2^18 unknowns, where the basis passes move two
+    rows per access: FP64 and FP32+refine solves reach the tolerance, checked
+    with the oracle's own saddle product, and agree with each other."""
+    say("long vectors")
+    NB, nbpb, a = 7000, 12, 0.8
+    gc, oc = pair_cfg(96, a=a, max_blobs=NB * nbpb, max_bodies=NB, max_blobs_per_body=nbpb, max_krylov=600)
+    ref, x, q, F6, nov = fa.synth_rods(5, NB, nbpb, a, [96.0] * 3, max_attempts=200)
+    n = 3 * NB * nbpb + 6 * NB
+    assert n >= 1 << 18
+    rhs = np.concatenate([np.zeros(3 * NB * nbpb), -F6.ravel()])
+    with fa.FcmSolver(gc) as s:
+        s.set_geometry(ref)
+        s.set_bodies(x, q)
+        Mg, Minv, N = s.ref_matrices()
+        sol64, rep64 = s.solve(rhs, tol=1e-8, max_iter=600)
+        sol32, rep32 = s.solve(rhs, tol=1e-8, max_iter=600, precision=fa.FCM_PREC_FP32)
+        say("fp64", rep64, "fp32+refine", rep32)
+    assert rep64["status"] == "converged" and rep32["status"] == "converged"
+    assert rep32["true_rel_residual"] < 1e-8
+    o = po.Fcm(oc, NB * nbpb)
+    o.set_up_gaussians(po.fill_blob_pos(oc, ref, x, q))
+    b = po.Bodies(ref, x, q, Minv, N)
+    for sol in (sol64, sol32):
+        true = np.linalg.norm(rhs - po.saddle_matvec(o, b, sol)) / np.linalg.norm(rhs)
+        assert true < 2e-8, true
+    # two solutions within the same residual bound differ by what the conditioning
+    # allows: 1.1e-5 over the whole vector (the blob forces of these small spheres),
+    # 4e-7 in the body velocities, as measured
+    nl = 3 * NB * nbpb
+    say("fp32 vs fp64: all", rel(sol32, sol64), "body velocities", rel(sol32[nl:], sol64[nl:]))
+    assert rel(sol32, sol64) < 1e-4
+    assert rel(sol32[nl:], sol64[nl:]) < 1e-5
