@@ -229,6 +229,32 @@ constexpr ColLayout col_layout(int tbytes, int nz) {
   }
 }
 
+// Lane pairing of the C2R input.  The point k of the inverse z transform needs
+// X[k] and X[M - k]; with the first radix of the inverse plan equal to 2, the
+// word X[M - k] of thread tcol's slot s is thread (TC - tcol)'s slot P - 1 - s,
+// and for tcol = 0 it is the thread's own slot c2r_self_slot(s) (none for k = 0,
+// whose partner is the Nyquist bin X[M]).  Checked at compile time over every
+// thread; where it does not hold, both words are loaded.
+constexpr int c2r_in_index(const FftPlan& pl, int tcol, int s) {
+  return tcol + (s / pl.r[0]) * pl.T + (s % pl.r[0]) * (pl.N / pl.r[0]);
+}
+constexpr int c2r_self_slot(const FftPlan& pl, int s) {
+  const int want = (pl.N - c2r_in_index(pl, 0, s)) % pl.N;
+  if (want == 0) return -1;
+  for (int q = 0; q < pl.P; ++q)
+    if (c2r_in_index(pl, 0, q) == want) return q;
+  return -2;
+}
+constexpr bool c2r_lane_pairs(const FftPlan& pl) {
+  if (pl.T > 64 || 64 % pl.T != 0) return false;
+  for (int tcol = 1; tcol < pl.T; ++tcol)
+    for (int s = 0; s < pl.P; ++s)
+      if (c2r_in_index(pl, pl.T - tcol, pl.P - 1 - s) != pl.N - c2r_in_index(pl, tcol, s)) return false;
+  for (int s = 0; s < pl.P; ++s)
+    if (c2r_self_slot(pl, s) == -2 || (c2r_self_slot(pl, s) == -1) != (c2r_in_index(pl, 0, s) == 0)) return false;
+  return true;
+}
+
 template <typename T, int NZ, int CX, int PM>
 struct ColumnShape {
   static constexpr int NT = col_threads(NZ);
@@ -480,7 +506,7 @@ __global__ void __launch_bounds__(col_threads(NZ))
   constexpr int NT = SH::NT, M = SH::M, P = SH::P, TC = SH::TC, RR = SH::RR, NROW = SH::NROW, NRND = SH::NRND;
   constexpr int CY = COL_CY, CAP = SH::I_CAP, LS = SH::LS, NCB = SH::NCB;
   __shared__ __align__(16) unsigned char smem[SH::I_LDS];
-  __shared__ V s_tw[M];
+  __shared__ V s_tw[M], s_twz[M];  // FFT twiddles, C2R twiddles W^k
   if (fp.gate && *fp.gate) return;
   const GridGeom& g = A.g;
   const int npatch = A.npx * A.npy;
@@ -490,7 +516,11 @@ __global__ void __launch_bounds__(col_threads(NZ))
   const int x0 = CX * px, y0 = CY * py;
   const int t = threadIdx.x;
   const int ng = A.ng, omin = 1 - ng / 2;
-  for (int i = t; i < M; i += NT) s_tw[i] = tw[i];
+  for (int i = t; i < M; i += NT) {
+    s_tw[i] = tw[i];
+    s_twz[i] = twz[i];
+  }
+  __syncthreads();
 
   V* area = reinterpret_cast<V*>(smem);
   const T* rowsT = reinterpret_cast<const T*>(smem);
@@ -499,7 +529,11 @@ __global__ void __launch_bounds__(col_threads(NZ))
 #pragma unroll
   for (int qg = 0; qg < NRND; qg += NCB) {
     // C2R input of this thread's FFT points, straight from HBM: each point k
-    // needs X[k] and X[M - k] (imaginary parts of k = 0 and M ignored)
+    // needs X[k] and X[M - k] (imaginary parts of k = 0 and M ignored); the
+    // second word comes from the partner lane where the plan allows
+    constexpr FftPlan IPL = reverse_plan(make_plan(M, PM));
+    constexpr bool PAIRS = c2r_lane_pairs(IPL);
+    const int partner = (t & 63) - tcol + ((TC - tcol) & (TC - 1));  // lane of thread TC - tcol of this row
     V v[NCB][P];
 #pragma unroll
     for (int c = 0; c < NCB; ++c) {
@@ -509,15 +543,28 @@ __global__ void __launch_bounds__(col_threads(NZ))
       const int ii = rem / CY, jj = rem - (rem / CY) * CY;
       const V* in = fp.p[valid ? cc : 0] +
                     ((size_t)(x0 + (valid ? ii : 0)) * g.ny + (y0 + (valid ? jj : 0))) * g.cpitch;
+      V xk[P];
+#pragma unroll
+      for (int s = 0; s < P; ++s) xk[s] = valid ? in[fft_in_index<M, true, PM>(tcol, s)] : mk<T>(T(0), T(0));
+      V xn = mk<T>(T(0), T(0));  // X[M], the partner of k = 0 (thread 0 of the row)
+      if (PAIRS && valid && tcol == 0) xn = in[M];
 #pragma unroll
       for (int s = 0; s < P; ++s) {
         const int k = fft_in_index<M, true, PM>(tcol, s);
-        V xk = valid ? in[k] : mk<T>(T(0), T(0));
-        V xm = valid ? in[M - k] : mk<T>(T(0), T(0));
-        if (k == 0) { xk.y = T(0); xm.y = T(0); }
-        const V sum = cadd(xk, cconj(xm));
-        const V dif = csub(xk, cconj(xm));
-        const V rot = cmulc(dif, twz[k]);        // W^{-k} (X[k] - conj X[M-k])
+        V xm;
+        if constexpr (PAIRS) {
+          const V sh = mk<T>(__shfl(xk[P - 1 - s].x, partner), __shfl(xk[P - 1 - s].y, partner));
+          const int self = c2r_self_slot(IPL, s);  // a constant once unrolled
+          const V own = self >= 0 ? xk[self >= 0 ? self : 0] : xn;
+          xm = tcol == 0 ? own : sh;
+        } else {
+          xm = valid ? in[M - k] : mk<T>(T(0), T(0));
+        }
+        V xs = xk[s];
+        if (k == 0) { xs.y = T(0); xm.y = T(0); }
+        const V sum = cadd(xs, cconj(xm));
+        const V dif = csub(xs, cconj(xm));
+        const V rot = cmulc(dif, s_twz[k]);      // W^{-k} (X[k] - conj X[M-k])
         v[c][s] = cadd(sum, mk<T>(-rot.y, rot.x));  // + i * rot
       }
     }
