@@ -291,7 +291,10 @@ struct ColumnShape {
   static constexpr int S_CAP = col_odd_le(BUDGET / S_ENT);
   static constexpr int S_LDS = col_max(FFT_BYTES, S_CAP * S_ENT);
   static constexpr int I_ENT = 40 * (int)sizeof(T) + 8;   // gx 4, gy 4, gz 8, slots 3 x 8 | z, partial slot
-  static constexpr int I_CAP = col_odd_le(BUDGET / I_ENT);
+  // FP64 interpolation at 256 threads fits 168 VGPRs (three waves per SIMD), so
+  // its staging is trimmed to let three workgroups share a CU's LDS
+  static constexpr int I_BUDGET = (sizeof(T) == 8 && NT == 256) ? col_max(col_min(BUDGET, 49664), FFT_BYTES) : BUDGET;
+  static constexpr int I_CAP = col_odd_le(I_BUDGET / I_ENT);
   static constexpr int I_LDS = col_max(FFT_BYTES, I_CAP * I_ENT);
   static_assert(S_CAP >= 15 && I_CAP >= 15, "staging");
   static_assert(S_LDS + (M + M / 2 + 1) * 2 * (int)sizeof(T) <= 163840 &&
