@@ -60,6 +60,8 @@ typedef struct {
   int max_krylov;      /* capacity: GMRES iterations (basis columns - 1), GMRES_MAX_ITER */
   int fp32;            /* also allocate the FP32 bulk path (FCM_PREC_FP32 solves)       */
   int random_field;    /* allocate the six stress fields for the fluctuating velocity   */
+  int slip_channel;    /* SLIP_CHANNEL (config.hpp:33), 0 or 1: image system in y, slip
+                          walls at y = 0 and y = LY/2, blobs in between                */
 } fcm_config;
 
 /* ---- lifetime ------------------------------------------------------------ */

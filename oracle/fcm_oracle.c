@@ -435,6 +435,31 @@ void or_fcm_get_gaussians(const or_fcm* f, double* gx, double* gy, double* gz, i
 /* fcm_fluid_solver.cpp:552-600 (fully periodic branch).  Each thread owns an
  * x-slab and walks every blob in order, exactly as one MPI rank does with its
  * slab filter at :570, so every node's sum is taken in blob order. */
+/* Slip-channel image system on the force fields (fcm_fluid_solver.cpp:602-630):
+ * fy vanishes on the wall rows j = 0 and j = ny/2, and every row j > ny/2 is
+ * replaced by the mirror image of row ny - j, fx and fz even, fy odd.  Whatever
+ * spreading deposited in those rows (blobs whose stencil wraps through y = 0 or
+ * reaches past ny/2) is discarded. */
+static void channel_image(or_fcm* f) {
+  const or_cfg* c = &f->c;
+  const int ny = c->ny, half = ny / 2, pad = f->pad;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < c->nx; ++i) {
+    const size_t x_index = (size_t)i * ny * pad;
+    for (int k = 0; k < pad; ++k) {
+      FY[x_index + k] = 0.0;
+      FY[x_index + (size_t)half * pad + k] = 0.0;
+      for (int j = half + 1; j < ny; ++j) {
+        const size_t index = x_index + (size_t)j * pad + k;
+        const size_t image = x_index + (size_t)(ny - j) * pad + k;
+        FX[index] = FX[image];
+        FY[index] = -FY[image];
+        FZ[index] = FZ[image];
+      }
+    }
+  }
+}
+
 void or_fcm_spread(or_fcm* f, const double* F) {
   const or_cfg* c = &f->c;
   const int ng = c->ng, ny = c->ny, pad = f->pad, nb = f->nb;
@@ -468,6 +493,7 @@ void or_fcm_spread(or_fcm* f, const double* F) {
       }
     }
   }
+  if (c->slip_channel) channel_image(f);
 }
 
 /* Fourier-space Stokes projection over every stored real slot, Nyquist modes
@@ -523,7 +549,9 @@ static void interp_from(or_fcm* f, const double* u1, const double* u2, const dou
     for (int i = 0; i < ng; ++i) {
       const size_t x_index = (size_t)f->indx[(size_t)np * ng + i] * ny * pad;
       for (int j = 0; j < ng; ++j) {
-        const size_t y_index = (size_t)f->indy[(size_t)np * ng + j] * pad;
+        const int jj = f->indy[(size_t)np * ng + j];
+        if (c->slip_channel && jj >= ny / 2) continue;  /* truncated Gaussians, :387-413 */
+        const size_t y_index = (size_t)jj * pad;
         for (int k = 0; k < ng; ++k) {
           const size_t index = x_index + y_index + f->indz[(size_t)np * ng + k];
           const double temp = f->gaussx[(size_t)np * ng + i] * f->gaussy[(size_t)np * ng + j] * f->gaussz[(size_t)np * ng + k] * norm;
@@ -560,13 +588,29 @@ void or_fcm_fill_stress(or_fcm* f, uint64_t seed) {
   double* dst[6] = {SXX, SYY, SZZ, SXY, SXZ, SYZ};
   const double scale[6] = {var2, var2, var2, var1, var1, var1};
   const uint64_t nnodes = (uint64_t)c->nx * c->ny * c->nz;
+  const int ny = c->ny, half = ny / 2;
   for (int s = 0; s < 6; ++s) {
     double* d = dst[s];
+    const int odd = (s == 3 || s == 5);  /* sxy, syz change sign under y -> -y */
 #pragma omp parallel for schedule(static)
-    for (int ij = 0; ij < c->nx * c->ny; ++ij) {
+    for (int ij = 0; ij < c->nx * ny; ++ij) {
+      /* slip channel (fcm_fluid_solver.cpp:186-217): the wall rows j = 0 and
+       * j = ny/2 carry sqrt(2) times the variance-scaled draw and no sxy, syz;
+       * a row j > ny/2 is the image of row ny - j, sxy and syz negated */
+      const int j = ij % ny;
+      int src = ij;
+      double sc = scale[s];
+      if (c->slip_channel) {
+        if (j == 0 || j == half) {
+          sc = odd ? 0.0 : sqrt(2.0) * scale[s];
+        } else if (j > half) {
+          src = ij - j + (ny - j);
+          if (odd) sc = -sc;
+        }
+      }
       for (int k = 0; k < f->pad; ++k) {
         d[(size_t)ij * f->pad + k] =
-            (k < c->nz) ? scale[s] * or_stress_normal(seed, s, (uint64_t)ij * c->nz + k, nnodes) : 0.0;
+            (k < c->nz && sc != 0.0) ? sc * or_stress_normal(seed, s, (uint64_t)src * c->nz + k, nnodes) : 0.0;
       }
     }
   }
@@ -728,6 +772,12 @@ void or_barrier_forces(const or_cfg* c, int Nb, int nbpb, const double* Y, doubl
       const double r = sqrt(r2);
       const double mag = r < two_a ? U0 / b : U0 * exp((two_a - r) / b) / b;  /* :377 */
       for (int k = 0; k < 3; ++k) f[k] += mag * (d[k] / r);
+    }
+    if (c->slip_channel) {
+      /* linear wall springs of range 2.2a (collection_of_rigid_bodies.cpp:640-662) */
+      const double Lw = 2.2 * c->a, Kw = (40.0 * c->kT / c->a) / Lw, y = Y[3 * i + 1];
+      if (y <= Lw) f[1] += Kw * (Lw - y);
+      else if ((0.5 * L[1] - y <= Lw) && (y <= 0.5 * L[1])) f[1] -= Kw * (Lw - 0.5 * L[1] + y);
     }
     for (int k = 0; k < 3; ++k) F[3 * i + k] = f[k];
   }
@@ -926,7 +976,8 @@ static void build_Kref(int nbpb, const double* r, double* K /* 3nbpb x 6 */) {
 int or_ref_matrices(or_fcm* f, int nbpb, const double* blob_ref, double* Msingle, double* Minv, double* N) {
   const or_cfg* c = &f->c;
   const int n3 = 3 * nbpb;
-  const double x[3] = {0.5 * c->dx * c->nx, 0.5 * c->dx * c->ny, 0.5 * c->dx * c->nz};
+  /* collection_of_rigid_bodies.cpp:139-147: box centre, or y = LY/4 (mid-channel) */
+  const double x[3] = {0.5 * c->dx * c->nx, (c->slip_channel ? 0.25 : 0.5) * c->dx * c->ny, 0.5 * c->dx * c->nz};
   const double q[4] = {1.0, 0.0, 0.0, 0.0};
   double* Y = (double*)malloc(sizeof(double) * n3);
   or_fill_blob_pos(c, 1, nbpb, blob_ref, x, q, Y);

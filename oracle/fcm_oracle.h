@@ -33,6 +33,8 @@ typedef struct {
   double mu;        /* FLUID_VISCOSITY             config.hpp:49    */
   double kT;        /* ENERGY_SCALE                config.hpp:50    */
   double dt;        /* DT                          config.hpp:84    */
+  int slip_channel; /* SLIP_CHANNEL                config.hpp:33: image system in y,
+                       fluid in 0 < y < LY/2 between slip walls at y = 0, LY/2 */
 } or_cfg;
 
 typedef struct or_fcm or_fcm;

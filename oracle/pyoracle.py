@@ -20,7 +20,7 @@ _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 class OrCfg(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("ng", C.c_int),
                 ("dx", C.c_double), ("a", C.c_double), ("mu", C.c_double),
-                ("kT", C.c_double), ("dt", C.c_double)]
+                ("kT", C.c_double), ("dt", C.c_double), ("slip_channel", C.c_int)]
 
 
 class OrBodies(C.Structure):
@@ -95,10 +95,10 @@ def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
-def cfg(nx, ny=None, nz=None, ng=7, dx=1.0, a=1.86, mu=1.0, kT=0.0, dt=1.0):
+def cfg(nx, ny=None, nz=None, ng=7, dx=1.0, a=1.86, mu=1.0, kT=0.0, dt=1.0, slip_channel=False):
     ny = nx if ny is None else ny
     nz = nx if nz is None else nz
-    return OrCfg(nx, ny, nz, ng, dx, a, mu, kT, dt)
+    return OrCfg(nx, ny, nz, ng, dx, a, mu, kT, dt, int(bool(slip_channel)))
 
 
 def pad_of(c):

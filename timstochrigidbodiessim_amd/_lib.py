@@ -51,7 +51,7 @@ class FcmConfig(C.Structure):
                 ("dx", C.c_double), ("a", C.c_double), ("mu", C.c_double), ("kT", C.c_double),
                 ("dt", C.c_double), ("max_blobs", C.c_int), ("max_bodies", C.c_int),
                 ("max_blobs_per_body", C.c_int), ("max_krylov", C.c_int), ("fp32", C.c_int),
-                ("random_field", C.c_int)]
+                ("random_field", C.c_int), ("slip_channel", C.c_int)]
 
 
 class GmresOpts(C.Structure):
@@ -179,11 +179,12 @@ def device_memory():
 
 
 def make_config(nx, ny=None, nz=None, ng=7, dx=1.0, a=1.86, mu=1.0, kT=0.0, dt=1.0, max_blobs=1,
-                max_bodies=1, max_blobs_per_body=1, max_krylov=200, fp32=False, random_field=False):
+                max_bodies=1, max_blobs_per_body=1, max_krylov=200, fp32=False, random_field=False,
+                slip_channel=False):
     ny = nx if ny is None else ny
     nz = nx if nz is None else nz
     return FcmConfig(nx, ny, nz, ng, dx, a, mu, kT, dt, max_blobs, max_bodies, max_blobs_per_body,
-                     max_krylov, int(fp32), int(random_field))
+                     max_krylov, int(fp32), int(random_field), int(bool(slip_channel)))
 
 
 def footprint(cfg):

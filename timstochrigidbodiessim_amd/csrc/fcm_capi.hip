@@ -61,6 +61,7 @@ bool valid_cfg(const fcm_config* c, std::string& why) {
   }
   if (!fft_len_supported(c->nz / 2)) { why = "unsupported z length " + std::to_string(c->nz); return false; }
   if (c->ng < 1 || c->ng > 16) { why = "ng must be in 1..16"; return false; }
+  if (c->slip_channel != 0 && c->slip_channel != 1) { why = "slip_channel must be 0 or 1"; return false; }
   if (c->ng > std::min(c->nx, std::min(c->ny, c->nz))) { why = "ng exceeds the grid"; return false; }
   if (!(c->dx > 0) || !(c->a > 0) || !(c->mu > 0)) { why = "dx, a and mu must be positive"; return false; }
   if (c->max_blobs < 1 || c->max_bodies < 1 || c->max_blobs_per_body < 1 || c->max_krylov < 1) {
@@ -80,6 +81,7 @@ GridGeom make_geom(const fcm_config* c) {
   g.nzc = c->nz / 2 + 1;
   g.cpitch = round_up(g.nzc, 16);  // rows start on 128-byte lines in FP32 and FP64
   g.dx = c->dx;
+  g.channel = c->slip_channel ? 1 : 0;
   return g;
 }
 
@@ -104,6 +106,7 @@ Sizes sizes_of(const fcm_config* c) {
   if (c->fp32) t += 3 * s.field32 + s.twbytes32;
   if (c->random_field) t += 6 * s.field64;
   t += (size_t)nb * (3 * 8 + 4 + 4 + 16 + 3 * c->ng * 12 + 8 * 4) + (3 * nbins + 2) * 4;
+  if (c->slip_channel) t += (size_t)nb * 3 * c->ng * 12;  // truncated interpolation weights
   {
     GridGeom g2 = make_geom(c);
     t += 3 * (size_t)(spread_tiles(g2) + 1) * 4;
@@ -437,6 +440,13 @@ int fcm_create(const fcm_config* cfg, fcm_handle* out) {
   if (e == hipSuccess) h->bd.rec = dalloc<int4>(h, nb, e);
   if (e == hipSuccess) h->bd.w64 = dalloc<double>(h, 3 * (size_t)nb * cfg->ng, e);
   if (e == hipSuccess) h->bd.w32 = dalloc<float>(h, 3 * (size_t)nb * cfg->ng, e);
+  if (cfg->slip_channel) {
+    if (e == hipSuccess) h->bd.iw64 = dalloc<double>(h, 3 * (size_t)nb * cfg->ng, e);
+    if (e == hipSuccess) h->bd.iw32 = dalloc<float>(h, 3 * (size_t)nb * cfg->ng, e);
+  } else {
+    h->bd.iw64 = h->bd.w64;
+    h->bd.iw32 = h->bd.w32;
+  }
   {
     const int nt = spread_tiles(h->g);
     if (e == hipSuccess) h->bd.tile_count = dalloc<int>(h, nt, e);
@@ -800,12 +810,14 @@ int fcm_ref_matrices(fcm_handle h, double* Msingle, double* Minv, double* N) {
   if (!h->have_geometry) return fail(FCM_ERR_STATE, "rb_set_geometry first");
   const fcm_config& c = h->cfg;
   const int nbpb = h->nbpb, n3 = 3 * nbpb;
-  // body 0 at the box centre, identity orientation (collection_of_rigid_bodies.cpp:141-149)
+  // body 0 at the box centre, or at y = LY/4 in a slip channel, identity
+  // orientation (collection_of_rigid_bodies.cpp:139-149)
   const double L[3] = {c.dx * c.nx, c.dx * c.ny, c.dx * c.nz};
+  const double at[3] = {0.5, c.slip_channel ? 0.25 : 0.5, 0.5};
   std::vector<double> Y(n3);
   for (int m = 0; m < nbpb; ++m)
     for (int d = 0; d < 3; ++d) {
-      const double p = 0.5 * L[d] + h->h_ref[3 * m + d];
+      const double p = at[d] * L[d] + h->h_ref[3 * m + d];
       Y[3 * m + d] = p - L[d] * std::floor(p / L[d]);
     }
   HIPCHK(hipMemcpyAsync(h->dY, Y.data(), n3 * sizeof(double), hipMemcpyHostToDevice, h->st));
@@ -1167,7 +1179,7 @@ int ll_barrier_forces(fcm_handle h, double cutoff, double* F3) {
     h->sw.cap_cells = (int)ncells;
   }
   const int nb = h->NB * h->nbpb;
-  HIPCHK(barrier_forces(h->dY, nb, h->nbpb, cg, cutoff, c.kT, c.a, h->sw, h->d_fblob, h->st));
+  HIPCHK(barrier_forces(h->dY, nb, h->nbpb, cg, cutoff, c.kT, c.a, c.slip_channel != 0, h->sw, h->d_fblob, h->st));
   h->have_blob_forces = true;
   h->have_rhs = false;
   if (F3) HIPCHK(hipMemcpyAsync(F3, h->d_fblob, 3 * (size_t)nb * sizeof(double), hipMemcpyDeviceToHost, h->st));

@@ -786,8 +786,9 @@ hipError_t launch_interp(const ColArgs& A, const FieldPtrs<T>& fp, const Twiddle
   return hipGetLastError();
 }
 
-ColArgs col_args(const GridGeom& g, const BlobData& bd) {
-  return ColArgs{g, bd.ng, g.nx / bd.col_cx, g.ny / COL_CY, bd.col_ent, bd.col_zst, bd.w64, bd.w32};
+ColArgs col_args(const GridGeom& g, const BlobData& bd, bool interp) {
+  return ColArgs{g,      bd.ng, g.nx / bd.col_cx, g.ny / COL_CY, bd.col_ent, bd.col_zst, interp ? bd.iw64 : bd.w64,
+                 interp ? bd.iw32 : bd.w32};
 }
 
 }  // namespace
@@ -796,15 +797,17 @@ template <typename T>
 hipError_t spread_zfwd(const GridGeom& g, const BlobData& bd, const double* F, const FieldPtrs<T>& fp,
                        const TwiddleSet<T>& tw, hipStream_t st) {
   if (bd.col_cx <= 0) return hipErrorInvalidValue;
-  const ColArgs A = col_args(g, bd);
+  const ColArgs A = col_args(g, bd, false);
+  hipError_t e = hipErrorInvalidValue;
   switch (g.nz) {
 #define FCM_COL_SPREAD(v) \
-  case v: return bd.col_cx == col_cx_for(v) ? launch_spread<T, v, col_cx_for(v)>(A, F, fp, tw, st) : hipErrorInvalidValue;
+  case v: e = bd.col_cx == col_cx_for(v) ? launch_spread<T, v, col_cx_for(v)>(A, F, fp, tw, st) : hipErrorInvalidValue; break;
     FCM_COL_NZ(FCM_COL_SPREAD)
 #undef FCM_COL_SPREAD
     default:
-      return hipErrorInvalidValue;
+      break;
   }
+  return e != hipSuccess ? e : channel_images<T>(g, fp, st);
 }
 
 template <typename T>
@@ -812,7 +815,7 @@ hipError_t zinv_interp(const GridGeom& g, const BlobData& bd, const FieldPtrs<T>
                        double* V, const KTerm& kt, hipStream_t st, hipEvent_t between) {
   if (bd.col_cx <= 0) return hipErrorInvalidValue;
   if (bd.nb == 0) return hipSuccess;
-  const ColArgs A = col_args(g, bd);
+  const ColArgs A = col_args(g, bd, true);
   T* part = static_cast<T*>(bd.col_part);
   hipError_t e = hipErrorInvalidValue;
   switch (g.nz) {

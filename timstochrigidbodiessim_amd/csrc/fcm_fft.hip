@@ -47,7 +47,9 @@ template <typename T, int M> struct RowShape {
 struct StressGen {
   unsigned long long seed;
   double scale[6];
-  int comp0;  // stream component of field 0 of this launch
+  int comp0;    // stream component of field 0 of this launch
+  int ny;       // rows per x plane
+  int channel;  // slip-channel image system in y
 };
 
 __device__ __forceinline__ unsigned long long sm_mix(unsigned long long z) {
@@ -94,14 +96,29 @@ __global__ void __launch_bounds__(256) zfwd_kernel(FieldPtrs<T> fp, int rows, in
   if (GEN) {
     const unsigned long long nnodes = ((unsigned long long)nz_total_nodes_hi << 32) | (unsigned)nz_total_nodes_lo;
     const int comp = sg.comp0 + blockIdx.y;
-    const double sc = sg.scale[comp];
+    double sc = sg.scale[comp];
+    int srow = row;
+    if (sg.channel) {
+      // slip channel (fcm_fluid_solver.cpp:186-217): the wall rows y = 0 and
+      // y = ny/2 carry sqrt(2) times the scaled draw and no sxy, syz (stream
+      // components 3 and 5); a row y > ny/2 repeats row ny - y, sxy and syz negated
+      const int y = row % sg.ny, half = sg.ny / 2;
+      const bool odd = comp == 3 || comp == 5;
+      if (y == 0 || y == half) {
+        sc = odd ? 0.0 : sqrt(2.0) * sc;
+      } else if (y > half) {
+        srow = row - y + (sg.ny - y);
+        if (odd) sc = -sc;
+      }
+    }
+    const bool draw = valid && sc != 0.0;
     const int nz = 2 * M;
 #pragma unroll
     for (int s = 0; s < P; ++s) {
       const int e = fft_in_index<M, false>(tcol, s);
-      const unsigned long long node = (unsigned long long)row * nz + 2 * e;
-      const double a = valid ? sc * stress_normal(sg.seed, comp, node, nnodes) : 0.0;
-      const double b = valid ? sc * stress_normal(sg.seed, comp, node + 1, nnodes) : 0.0;
+      const unsigned long long node = (unsigned long long)srow * nz + 2 * e;
+      const double a = draw ? sc * stress_normal(sg.seed, comp, node, nnodes) : 0.0;
+      const double b = draw ? sc * stress_normal(sg.seed, comp, node + 1, nnodes) : 0.0;
       v[s] = mk<T>(T(a), T(b));
     }
   } else {
@@ -507,6 +524,8 @@ hipError_t random_velocity(const GridGeom& g, const FieldPtrs<T>& stress6, bool 
   sg.seed = seed;
   for (int i = 0; i < 6; ++i) sg.scale[i] = scale[i];
   sg.comp0 = 0;
+  sg.ny = g.ny;
+  sg.channel = g.channel;
   hipError_t e = launch_z<T>(g, stress6, 6, true, tws, st, generate ? &sg : nullptr);
   if (e == hipSuccess) e = launch_cols<T>(g, stress6, 6, 1, true, tws, st);
   if (e == hipSuccess) e = launch_xproj<T>(g, stress6, true, pp, tws, st);

@@ -91,7 +91,7 @@ __global__ void bin_sort_kernel(const int* start, int nbins, int* sorted) {
 }
 
 __global__ void record_kernel(const double* __restrict__ Y, const int* __restrict__ sorted, int nb, GridGeom g,
-                              int ng, double a, int4* rec, double* w64, float* w32) {
+                              int ng, double a, int4* rec, double* w64, float* w32, double* iw64, float* iw32) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nb) return;
   const int id = sorted[s];
@@ -110,6 +110,13 @@ __global__ void record_kernel(const double* __restrict__ Y, const int* __restric
       const double w = anorm * exp(-xx * xx / anorm2);
       w64[((size_t)s * 3 + d) * ng + i] = w;
       w32[((size_t)s * 3 + d) * ng + i] = (float)w;
+      if (g.channel) {
+        // interpolation drops stencil rows at and above the upper wall
+        // (fcm_fluid_solver.cpp:387-413): the wrapped row index decides
+        const double wi = (d == 1 && wrapi(xg, g.ny) >= g.ny / 2) ? 0.0 : w;
+        iw64[((size_t)s * 3 + d) * ng + i] = wi;
+        iw32[((size_t)s * 3 + d) * ng + i] = (float)wi;
+      }
     }
   }
   rec[s] = make_int4(wa[0], wa[1], wa[2], id);
@@ -148,7 +155,7 @@ hipError_t setup_gaussians(const GridGeom& g, const BlobBins& bb, BlobData& bd, 
                        bd.bin_cursor, bd.sorted);
     hipLaunchKernelGGL(bin_sort_kernel, dim3((nbins + 255) / 256), dim3(256), 0, st, bd.bin_start, nbins, bd.sorted);
     hipLaunchKernelGGL(record_kernel, dim3((nb + 255) / 256), dim3(256), 0, st, bd.Y, bd.sorted, nb, g, bd.ng, a,
-                       bd.rec, bd.w64, bd.w32);
+                       bd.rec, bd.w64, bd.w32, bd.iw64, bd.iw32);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -156,6 +163,47 @@ hipError_t setup_gaussians(const GridGeom& g, const BlobBins& bb, BlobData& bd, 
   if (e != hipSuccess) return e;
   return build_col_lists(g, bb, bd, st);
 }
+
+// ---------------------------------------------------------------------------
+// slip-channel images of the force fields (fcm_fluid_solver.cpp:602-630): fy is
+// zero on the wall rows y = 0 and y = ny/2, and each row y > ny/2 becomes the
+// mirror of row ny - y, fx and fz even, fy odd.  Whole rows are copied, so the
+// rule serves real rows and z-transformed rows alike (it acts on y only and
+// commutes with the z transform).  One thread per complex word, grid-stride.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) channel_image_kernel(FieldPtrs<T> fp, GridGeom g) {
+  if (fp.gate && *fp.gate) return;
+  const int half = g.ny / 2, cp = g.cpitch;
+  const size_t words = (size_t)g.nx * half * cp;
+  for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += (size_t)gridDim.x * blockDim.x) {
+    const int k = (int)(w % cp);
+    const size_t r = w / cp;
+    const int t = (int)(r % half);
+    const size_t plane = (r / half) * g.ny;  // first row of this x plane
+    if (t == 0) {
+      fp.p[1][plane * cp + k] = mk<T>(T(0), T(0));
+      fp.p[1][(plane + half) * cp + k] = mk<T>(T(0), T(0));
+    } else {
+      const size_t dst = (plane + half + t) * cp + k, src = (plane + half - t) * cp + k;
+      fp.p[0][dst] = fp.p[0][src];
+      const c2<T> v = fp.p[1][src];
+      fp.p[1][dst] = mk<T>(-v.x, -v.y);
+      fp.p[2][dst] = fp.p[2][src];
+    }
+  }
+}
+
+template <typename T>
+hipError_t channel_images(const GridGeom& g, const FieldPtrs<T>& fp, hipStream_t st) {
+  if (!g.channel) return hipSuccess;
+  const size_t words = (size_t)g.nx * (g.ny / 2) * g.cpitch;
+  const int blocks = (int)std::min<size_t>((words + 255) / 256, 8192);
+  hipLaunchKernelGGL((channel_image_kernel<T>), dim3(blocks), dim3(256), 0, st, fp, g);
+  return hipGetLastError();
+}
+template hipError_t channel_images<double>(const GridGeom&, const FieldPtrs<double>&, hipStream_t);
+template hipError_t channel_images<float>(const GridGeom&, const FieldPtrs<float>&, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // spreading
@@ -869,13 +917,15 @@ hipError_t spread(const GridGeom& g, const BlobBins& bb, const BlobData& bd, con
       hipLaunchKernelGGL((spread_dense_kernel<T>), dim3(8 * ((nt + 7) / 8)), dim3(256), 0, st, A, fp);
     else
       hipLaunchKernelGGL((spread_fast_kernel<T>), dim3(8 * ((nt + 7) / 8)), dim3(256), 0, st, A, fp);
-    return hipGetLastError();
+  } else {
+    SpreadArgs A{g, bb, bd.ng, bd.bin_start, bd.rec, bd.w64, bd.w32, F};
+    if (bd.ng > SP_NGMAX || bb.sx * bb.bx > SP_TXMAX || bb.sy * bb.by * bb.sz * bb.bz > 256)
+      return hipErrorInvalidValue;
+    const int ntiles = (g.nx / (bb.sx * bb.bx)) * (g.ny / (bb.sy * bb.by)) * (g.nz / (bb.sz * bb.bz));
+    hipLaunchKernelGGL((spread_kernel<T>), dim3(ntiles), dim3(256), 0, st, A, fp);
   }
-  SpreadArgs A{g, bb, bd.ng, bd.bin_start, bd.rec, bd.w64, bd.w32, F};
-  if (bd.ng > SP_NGMAX || bb.sx * bb.bx > SP_TXMAX || bb.sy * bb.by * bb.sz * bb.bz > 256) return hipErrorInvalidValue;
-  const int ntiles = (g.nx / (bb.sx * bb.bx)) * (g.ny / (bb.sy * bb.by)) * (g.nz / (bb.sz * bb.bz));
-  hipLaunchKernelGGL((spread_kernel<T>), dim3(ntiles), dim3(256), 0, st, A, fp);
-  return hipGetLastError();
+  const hipError_t e = hipGetLastError();
+  return e != hipSuccess ? e : channel_images<T>(g, fp, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -1261,7 +1311,7 @@ static bool interp_fast_ok(const GridGeom& g, const BlobBins& bb, int ng) {
 template <typename T>
 hipError_t interp(const GridGeom& g, const BlobBins& bb, const BlobData& bd, const FieldPtrs<T>& fp, double* V,
                   const KTerm& kt, hipStream_t st) {
-  InterpArgs A{g, bb, bd.ng, bd.bin_start, bd.rec, bd.w64, bd.w32, V, kt, g.dx * g.dx * g.dx};
+  InterpArgs A{g, bb, bd.ng, bd.bin_start, bd.rec, bd.iw64, bd.iw32, V, kt, g.dx * g.dx * g.dx};
   if (bd.nb == 0) return hipSuccess;
   const int izb_choice = bd.interp_izb[sizeof(T) == 4 ? 1 : 0];
   if (izb_choice == 3 && interp_fast_ok(g, bb, bd.ng) && (bd.ng == 7 || bd.ng == 8) && g.nz % 16 == 0 &&

@@ -10,6 +10,7 @@ namespace fcm {
 // hold nz reals in their first nz real slots.
 struct GridGeom {
   int nx = 0, ny = 0, nz = 0, nzc = 0, cpitch = 0;
+  int channel = 0;  // slip-channel image system in y (walls at rows 0 and ny/2)
   double dx = 1.0;
   size_t field_words() const { return (size_t)nx * ny * cpitch; }  // complex words per field
 };
@@ -75,6 +76,10 @@ struct BlobData {
   int4* anc = nullptr;        // nb, sorted order: unwrapped anchors
   double* w64 = nullptr;      // nb x 3 x ng, sorted order
   float* w32 = nullptr;       // nb x 3 x ng, sorted order
+  // interpolation weights: w64/w32 themselves, or in a slip channel a copy whose
+  // y weights are zero on rows >= ny/2 (truncated Gaussians)
+  double* iw64 = nullptr;
+  float* iw32 = nullptr;
   int* tile_count = nullptr;  // spreading tiles: relevant blobs per tile
   int* tile_start = nullptr;  // ntiles + 1
   int* tile_cursor = nullptr; // ntiles (scan scratch)
@@ -102,6 +107,10 @@ struct BlobData {
 // spreading tiles of the main path (0 if the grid does not qualify)
 bool spread_fast_ok(const GridGeom& g, const BlobBins& bb, int ng);
 int spread_tiles(const GridGeom& g);
+// slip-channel image rule on the three force fields after spreading (no-op
+// unless g.channel)
+template <typename T>
+hipError_t channel_images(const GridGeom& g, const FieldPtrs<T>& fp, hipStream_t st);
 
 hipError_t setup_gaussians(const GridGeom& g, const BlobBins& bb, BlobData& bd, double a, hipStream_t st);
 
@@ -206,7 +215,7 @@ struct StericWork {
   int cap_cells = 0;
 };
 hipError_t barrier_forces(const double* Y, int nb, int nbpb, const StericCells& cg, double cutoff, double kT, double a,
-                          StericWork& w, double* F, hipStream_t st);
+                          bool channel, StericWork& w, double* F, hipStream_t st);
 // out = P^{-1} in; if wKT != null also writes wKT = -K^T out_lambda (6 per body)
 hipError_t apply_pinv(const BodyData& b, const double* in, double* out, double* wKT, hipStream_t st);
 hipError_t mul_K(const BodyData& b, const double* in6, double* out3, hipStream_t st);

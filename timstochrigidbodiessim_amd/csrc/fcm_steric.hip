@@ -42,7 +42,7 @@ __device__ __forceinline__ int axis_cells(int c, int nc, int k) {
 __global__ void __launch_bounds__(256) ll_force_kernel(const double* __restrict__ Y, int nb, int nbpb, StericCells cg,
                                                        const int* __restrict__ start, const int* __restrict__ sorted,
                                                        double cut2, double U0, double bl, double two_a,
-                                                       double* __restrict__ F) {
+                                                       double wall_L, double wall_K, double* __restrict__ F) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nb) return;
   const double x1 = Y[3 * (size_t)i], y1 = Y[3 * (size_t)i + 1], z1 = Y[3 * (size_t)i + 2];
@@ -79,13 +79,19 @@ __global__ void __launch_bounds__(256) ll_force_kernel(const double* __restrict_
       }
     }
   }
+  if (wall_L > 0.0) {
+    // slip-channel wall springs (collection_of_rigid_bodies.cpp:640-662)
+    const double hy = 0.5 * cg.L[1];
+    if (y1 <= wall_L) fy += wall_K * (wall_L - y1);
+    else if ((hy - y1 <= wall_L) && (y1 <= hy)) fy -= wall_K * (wall_L - hy + y1);
+  }
   F[3 * (size_t)i] = fx;
   F[3 * (size_t)i + 1] = fy;
   F[3 * (size_t)i + 2] = fz;
 }
 
 hipError_t barrier_forces(const double* Y, int nb, int nbpb, const StericCells& cg, double cutoff, double kT, double a,
-                          StericWork& w, double* F, hipStream_t st) {
+                          bool channel, StericWork& w, double* F, hipStream_t st) {
   if (nb <= 0) return hipSuccess;
   const int ncells = cg.nc[0] * cg.nc[1] * cg.nc[2];
   if (ncells > w.cap_cells) return hipErrorInvalidValue;
@@ -95,8 +101,9 @@ hipError_t barrier_forces(const double* Y, int nb, int nbpb, const StericCells& 
   e = sort_by_key(w.cell_of, nb, ncells, w.count, w.start, w.cursor, w.sorted, st);
   if (e != hipSuccess) return e;
   const double U0 = 20.0 * kT, bl = 0.5 * a;  // linked_list.cpp:367-368
+  const double wall_L = channel ? 2.2 * a : 0.0, wall_K = channel ? (40.0 * kT / a) / wall_L : 0.0;
   hipLaunchKernelGGL(ll_force_kernel, dim3((nb + 255) / 256), dim3(256), 0, st, Y, nb, nbpb, cg, w.start, w.sorted,
-                     cutoff * cutoff, U0, bl, 2.0 * a, F);
+                     cutoff * cutoff, U0, bl, 2.0 * a, wall_L, wall_K, F);
   return hipGetLastError();
 }
 
