@@ -95,7 +95,7 @@ def cpu_baseline(args, ref, x, q, F6, fp64_iters):
     import pyoracle as po
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
     n = args.grid
-    oc = po.cfg(n, ng=7, dx=1.0, a=1.86, mu=1.0, kT=1.0, dt=args.dt)
+    oc = po.cfg(n, ng=7, dx=1.0, a=1.86, mu=1.0, kT=1.0, dt=args.dt, slip_channel=args.slip_channel)
     o = po.Fcm(oc, x.shape[0] * ref.shape[0])
     Y = po.fill_blob_pos(oc, ref, x, q)
     o.set_up_gaussians(Y)
@@ -290,7 +290,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=256)
-    ap.add_argument("--rods", type=int, default=20000)
+    ap.add_argument("--rods", type=int, default=None,
+                    help="rods of the solve256 workload: 20000, or 10000 with --slip-channel (the same volume "
+                         "fraction in half the box)")
     ap.add_argument("--spheres", type=int, default=12)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--max-iter", type=int, default=None,
@@ -302,10 +304,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--slip-channel", action="store_true",
+                    help="solve256 in the slip-channel geometry: walls at y = 0 and y = grid/2, rods inside")
     args = ap.parse_args()
 
     if args.max_iter is None:
         args.max_iter = 800 if args.workload in ("batch", "serve") else 600
+    if args.rods is None:
+        args.rods = 10000 if args.slip_channel else 20000
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: the whole 256^3 grid lives on one MI355X and no collective or "
@@ -323,14 +329,22 @@ def main():
 
     n = args.grid
     L = [float(n)] * 3
+    margin = 0.0
+    if args.slip_channel:
+        # rod centres at least one rod half-length (+0.5) from either wall, so
+        # every sphere lies between y = 0 and y = n/2
+        margin = 1.86 * (args.spheres - 1) + 0.5
+        L[1] = n / 2 - 2 * margin
     t0 = time.perf_counter()
     ref, x, q, F6, nov = fa.synth_rods(2, args.rods, args.spheres, 1.86, L, max_attempts=args.place_attempts)
+    x[:, 1] += margin
     nb = args.rods * args.spheres
-    phi = nb * 4.0 / 3.0 * np.pi * 1.86 ** 3 / n ** 3
+    phi = nb * 4.0 / 3.0 * np.pi * 1.86 ** 3 / (n ** 3 / (2 if args.slip_channel else 1))
     log(f"[bench] synthetic rods: {args.rods} x {args.spheres}, volume fraction {phi:.3f}, "
         f"{nov} rods placed with overlap ({time.perf_counter() - t0:.1f} s)")
     cfg = fa.make_config(n, ng=7, dx=1.0, a=1.86, mu=1.0, kT=1.0, dt=args.dt, max_blobs=nb, max_bodies=args.rods,
-                         max_blobs_per_body=args.spheres, max_krylov=args.max_iter, fp32=True, random_field=True)
+                         max_blobs_per_body=args.spheres, max_krylov=args.max_iter, fp32=True, random_field=True,
+                         slip_channel=args.slip_channel)
     s = fa.FcmSolver(cfg)
     s.set_geometry(ref)
     t0 = time.perf_counter()
@@ -423,7 +437,8 @@ def main():
 
     head_r = results["fp32_refine"]
     line = {
-        "metric": "FCM mobility solves/s at 256^3 (FP32+refine vs FP64) at 1e-8; p50/p99 latency",
+        "metric": "FCM mobility solves/s at 256^3 (FP32+refine vs FP64) at 1e-8; p50/p99 latency"
+                  + (" (slip channel)" if args.slip_channel else ""),
         "value": round(head_r["solves_per_s"], 4),
         "unit": "solves/s",
         "n_gpus": 1,
@@ -437,7 +452,9 @@ def main():
         "data": "synthetic (seeded rods, forces and stress stream)",
         "config": {"workload": f"{n}^3 grid, {args.rods} rods x {args.spheres} spheres ({nb} blobs), a=1.86, dx=1, "
                                f"mu=1, kT=1, ng=7, GMRES tol {args.tol:g}, volume fraction {phi:.3f}, "
-                               f"{nov} rods overlapping",
+                               f"{nov} rods overlapping"
+                               + (f", slip channel (walls at y = 0 and y = {n // 2}, rod centres in "
+                                  f"[{margin:.2f}, {n / 2 - margin:.2f}])" if args.slip_channel else ""),
                    "parallelism": "single GPU"},
         "p50_ms": round(head_r["p50_ms"], 3),
         "p99_ms": round(head_r["p99_ms"], 3),

@@ -200,14 +200,15 @@ namespace fcm_host {
 template <typename T>
 int mobility_passes(fcm_handle h, const FieldPtrs<T>& fp, const TwiddleSet<T>& tw, bool column, const double* Fin,
                     double* Vout, const KTerm& kt, hipStream_t st) {
+  // a slip channel's force images are applied by the forward y pass as it reads
   if (column && h->bd.col_cx > 0) {
-    HIPCHK(spread_zfwd<T>(h->g, h->bd, Fin, fp, tw, st));
-    for (int p = 1; p <= 3; ++p) HIPCHK(stokes_pass<T>(h->g, fp, p, h->pp, tw, st));
+    HIPCHK(spread_zfwd<T>(h->g, h->bd, Fin, fp, tw, st, false));
+    for (int p = 1; p <= 3; ++p) HIPCHK(stokes_pass<T>(h->g, fp, p, h->pp, tw, st, p == 1 && h->g.channel));
     HIPCHK(zinv_interp<T>(h->g, h->bd, fp, tw, Vout, kt, st));
     return FCM_OK;
   }
-  HIPCHK(spread<T>(h->g, h->bb, h->bd, Fin, fp, st));
-  HIPCHK(stokes_solve<T>(h->g, fp, h->pp, tw, st));
+  HIPCHK(spread<T>(h->g, h->bb, h->bd, Fin, fp, st, false));
+  HIPCHK(stokes_solve<T>(h->g, fp, h->pp, tw, st, h->g.channel != 0));
   HIPCHK(interp<T>(h->g, h->bb, h->bd, fp, Vout, kt, st));
   return FCM_OK;
 }
@@ -215,11 +216,11 @@ int mobility_passes(fcm_handle h, const FieldPtrs<T>& fp, const TwiddleSet<T>& t
 // the passes of the column path with events between them (the z slots stay empty)
 template <typename T>
 int profile_column(fcm_handle h, const FieldPtrs<T>& fp, const TwiddleSet<T>& tw, const KTerm& kt, hipEvent_t* ev) {
-  HIPCHK(spread_zfwd<T>(h->g, h->bd, h->xacc, fp, tw, h->st));
+  HIPCHK(spread_zfwd<T>(h->g, h->bd, h->xacc, fp, tw, h->st, false));
   HIPCHK(hipEventRecord(ev[1], h->st));
   HIPCHK(hipEventRecord(ev[2], h->st));
   for (int p = 1; p <= 3; ++p) {
-    HIPCHK(stokes_pass<T>(h->g, fp, p, h->pp, tw, h->st));
+    HIPCHK(stokes_pass<T>(h->g, fp, p, h->pp, tw, h->st, p == 1 && h->g.channel));
     HIPCHK(hipEventRecord(ev[2 + p], h->st));
   }
   HIPCHK(hipEventRecord(ev[6], h->st));
@@ -944,18 +945,18 @@ int fcm_profile_matvec(fcm_handle h, int precision, int reps, double* ms_per_pas
                                       : profile_column<double>(h, h->fp64(), h->tw64, kt, ev);
       if (rc) break;
     } else if (precision == FCM_PREC_FP32) {
-      HIPCHK(spread<float>(h->g, h->bb, h->bd, h->xacc, h->fp32(), h->st));
+      HIPCHK(spread<float>(h->g, h->bb, h->bd, h->xacc, h->fp32(), h->st, false));
       HIPCHK(hipEventRecord(ev[1], h->st));
       for (int p = 0; p < 5; ++p) {
-        HIPCHK(stokes_pass<float>(h->g, h->fp32(), p, h->pp, h->tw32, h->st));
+        HIPCHK(stokes_pass<float>(h->g, h->fp32(), p, h->pp, h->tw32, h->st, p == 1 && h->g.channel));
         HIPCHK(hipEventRecord(ev[2 + p], h->st));
       }
       HIPCHK(interp<float>(h->g, h->bb, h->bd, h->fp32(), h->tmp, kt, h->st));
     } else {
-      HIPCHK(spread<double>(h->g, h->bb, h->bd, h->xacc, h->fp64(), h->st));
+      HIPCHK(spread<double>(h->g, h->bb, h->bd, h->xacc, h->fp64(), h->st, false));
       HIPCHK(hipEventRecord(ev[1], h->st));
       for (int p = 0; p < 5; ++p) {
-        HIPCHK(stokes_pass<double>(h->g, h->fp64(), p, h->pp, h->tw64, h->st));
+        HIPCHK(stokes_pass<double>(h->g, h->fp64(), p, h->pp, h->tw64, h->st, p == 1 && h->g.channel));
         HIPCHK(hipEventRecord(ev[2 + p], h->st));
       }
       HIPCHK(interp<double>(h->g, h->bb, h->bd, h->fp64(), h->tmp, kt, h->st));
@@ -1031,8 +1032,8 @@ int fcm_autotune(fcm_handle h, int precision, int reps, fcm_tuning* chosen) {
   };
   const KTerm kt{};
   auto spread_once = [&]() -> int {
-    if (precision == FCM_PREC_FP32) HIPCHK(spread<float>(h->g, h->bb, h->bd, h->tmp, h->fp32(), h->st));
-    else HIPCHK(spread<double>(h->g, h->bb, h->bd, h->tmp, h->fp64(), h->st));
+    if (precision == FCM_PREC_FP32) HIPCHK(spread<float>(h->g, h->bb, h->bd, h->tmp, h->fp32(), h->st, false));
+    else HIPCHK(spread<double>(h->g, h->bb, h->bd, h->tmp, h->fp64(), h->st, false));
     return FCM_OK;
   };
   auto interp_once = [&]() -> int {

@@ -361,6 +361,9 @@ __global__ void __launch_bounds__(col_threads(NZ))
   if (p >= npatch) return;
   const int px = p / A.npy, py = p - (p / A.npy) * A.npy;
   const int x0 = CX * px, y0 = CY * py;
+  // slip channel: rows above ny/2 are images, made from their mirror rows when
+  // they are read, so a patch wholly above ny/2 has nothing to write
+  if (g.channel && y0 > g.ny / 2) return;
   const int t = threadIdx.x;
   const int ng = A.ng, omin = 1 - ng / 2;
   for (int i = t; i < M; i += NT) {
@@ -795,7 +798,7 @@ ColArgs col_args(const GridGeom& g, const BlobData& bd, bool interp) {
 
 template <typename T>
 hipError_t spread_zfwd(const GridGeom& g, const BlobData& bd, const double* F, const FieldPtrs<T>& fp,
-                       const TwiddleSet<T>& tw, hipStream_t st) {
+                       const TwiddleSet<T>& tw, hipStream_t st, bool images) {
   if (bd.col_cx <= 0) return hipErrorInvalidValue;
   const ColArgs A = col_args(g, bd, false);
   hipError_t e = hipErrorInvalidValue;
@@ -807,7 +810,7 @@ hipError_t spread_zfwd(const GridGeom& g, const BlobData& bd, const double* F, c
     default:
       break;
   }
-  return e != hipSuccess ? e : channel_images<T>(g, fp, st);
+  return e != hipSuccess || !images ? e : channel_images<T>(g, fp, st);
 }
 
 template <typename T>
@@ -841,9 +844,9 @@ hipError_t zinv_interp(const GridGeom& g, const BlobData& bd, const FieldPtrs<T>
 }
 
 template hipError_t spread_zfwd<double>(const GridGeom&, const BlobData&, const double*, const FieldPtrs<double>&,
-                                        const TwiddleSet<double>&, hipStream_t);
+                                        const TwiddleSet<double>&, hipStream_t, bool);
 template hipError_t spread_zfwd<float>(const GridGeom&, const BlobData&, const double*, const FieldPtrs<float>&,
-                                       const TwiddleSet<float>&, hipStream_t);
+                                       const TwiddleSet<float>&, hipStream_t, bool);
 template hipError_t zinv_interp<double>(const GridGeom&, const BlobData&, const FieldPtrs<double>&,
                                         const TwiddleSet<double>&, double*, const KTerm&, hipStream_t, hipEvent_t);
 template hipError_t zinv_interp<float>(const GridGeom&, const BlobData&, const FieldPtrs<float>&,

@@ -191,7 +191,12 @@ __global__ void __launch_bounds__(256) zinv_kernel(FieldPtrs<T> fp, int rows, in
 //   y pass: blockIdx.y = x index, base = x*ny*cpitch, estride = cpitch
 //   x pass: blockIdx.y = y index, base = y*cpitch,    estride = ny*cpitch
 // ---------------------------------------------------------------------------
-template <typename T, int N, int S>
+// MIRROR (forward y pass over the three force fields of a slip channel): the
+// slip-channel image rule is applied as the columns are read, so spreading
+// need not write it: element y > N/2 is read from N - y, field 1 (fy) negated
+// there and zero at y = 0 and y = N/2.  On a field that already holds the
+// images the read returns what is stored, so the rule may be applied twice.
+template <typename T, int N, int S, bool MIRROR = false>
 __global__ void __launch_bounds__(256) cols_kernel(FieldPtrs<T> fp, size_t plane_stride, size_t estride, int nzc,
                                                    const c2<T>* __restrict__ tw) {
   if (fp.gate && *fp.gate) return;
@@ -215,10 +220,26 @@ __global__ void __launch_bounds__(256) cols_kernel(FieldPtrs<T> fp, size_t plane
   const unsigned colb = (unsigned)((valid ? kz : 0) * sizeof(V));
   const unsigned eb = (unsigned)(estride * sizeof(V));
   V v[P];
+  if constexpr (MIRROR) {
+    static_assert(S < 0, "the image rule is read by the forward pass");
+    const bool fy = blockIdx.z == 1;
 #pragma unroll
-  for (int s = 0; s < P; ++s) {
-    const int e = fft_in_index<N, REV>(tcol, s);
-    v[s] = valid ? *reinterpret_cast<const V*>(base + (colb + (unsigned)e * eb)) : mk<T>(T(0), T(0));
+    for (int s = 0; s < P; ++s) {
+      const int e = fft_in_index<N, REV>(tcol, s);
+      const bool up = e > N / 2;
+      V x = valid ? *reinterpret_cast<const V*>(base + (colb + (unsigned)(up ? N - e : e) * eb)) : mk<T>(T(0), T(0));
+      if (fy) {
+        if (up) x = mk<T>(-x.x, -x.y);
+        else if (e == 0 || e == N / 2) x = mk<T>(T(0), T(0));
+      }
+      v[s] = x;
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < P; ++s) {
+      const int e = fft_in_index<N, REV>(tcol, s);
+      v[s] = valid ? *reinterpret_cast<const V*>(base + (colb + (unsigned)e * eb)) : mk<T>(T(0), T(0));
+    }
   }
   fft_column<T, N, S, REV>(v, lds, C, col, tcol, s_tw);
   if (valid) {
@@ -430,7 +451,8 @@ static hipError_t launch_z(const GridGeom& g, const FieldPtrs<T>& fp, int nf, bo
 
 template <typename T>
 static hipError_t launch_cols(const GridGeom& g, const FieldPtrs<T>& fp, int nf, int axis, bool fwd,
-                              const TwiddleSet<T>& tws, hipStream_t st) {
+                              const TwiddleSet<T>& tws, hipStream_t st, bool mirror = false) {
+  if (mirror && (axis != 1 || !fwd || nf != 3)) return hipErrorInvalidValue;
   const int N = axis == 1 ? g.ny : g.nx;
   const int nplanes = axis == 1 ? g.nx : g.ny;
   const size_t plane_stride = axis == 1 ? (size_t)g.ny * g.cpitch : (size_t)g.cpitch;
@@ -441,7 +463,10 @@ static hipError_t launch_cols(const GridGeom& g, const FieldPtrs<T>& fp, int nf,
   case v: {                                                                                                      \
     using SH = ColShape<T, v>;                                                                                   \
     dim3 grid((g.nzc + SH::C - 1) / SH::C, nplanes, nf);                                                         \
-    if (fwd)                                                                                                     \
+    if (mirror)                                                                                                  \
+      hipLaunchKernelGGL((cols_kernel<T, v, -1, true>), grid, dim3(SH::THREADS), 0, st, fp, plane_stride, estride,  \
+                         g.nzc, tw);                                                                             \
+    else if (fwd)                                                                                                \
       hipLaunchKernelGGL((cols_kernel<T, v, -1>), grid, dim3(SH::THREADS), 0, st, fp, plane_stride, estride, g.nzc, tw); \
     else                                                                                                         \
       hipLaunchKernelGGL((cols_kernel<T, v, 1>), grid, dim3(SH::THREADS), 0, st, fp, plane_stride, estride, g.nzc, tw); \
@@ -495,9 +520,10 @@ hipError_t fft3d_inverse(const GridGeom& g, const FieldPtrs<T>& fp, int nf, cons
 
 template <typename T>
 hipError_t stokes_solve(const GridGeom& g, const FieldPtrs<T>& fp, const ProjParams& pp, const TwiddleSet<T>& tws,
-                        hipStream_t st) {
+                        hipStream_t st, bool images) {
+  if (images && !g.channel) return hipErrorInvalidValue;
   hipError_t e = launch_z<T>(g, fp, 3, true, tws, st, nullptr);
-  if (e == hipSuccess) e = launch_cols<T>(g, fp, 3, 1, true, tws, st);
+  if (e == hipSuccess) e = launch_cols<T>(g, fp, 3, 1, true, tws, st, images);
   if (e == hipSuccess) e = launch_xproj<T>(g, fp, false, pp, tws, st);
   if (e == hipSuccess) e = launch_cols<T>(g, fp, 3, 1, false, tws, st);
   if (e == hipSuccess) e = launch_z<T>(g, fp, 3, false, tws, st, nullptr);
@@ -506,10 +532,11 @@ hipError_t stokes_solve(const GridGeom& g, const FieldPtrs<T>& fp, const ProjPar
 
 template <typename T>
 hipError_t stokes_pass(const GridGeom& g, const FieldPtrs<T>& fp, int pass, const ProjParams& pp,
-                       const TwiddleSet<T>& tws, hipStream_t st) {
+                       const TwiddleSet<T>& tws, hipStream_t st, bool images) {
+  if (images && (!g.channel || pass != 1)) return hipErrorInvalidValue;
   switch (pass) {
     case 0: return launch_z<T>(g, fp, 3, true, tws, st, nullptr);
-    case 1: return launch_cols<T>(g, fp, 3, 1, true, tws, st);
+    case 1: return launch_cols<T>(g, fp, 3, 1, true, tws, st, images);
     case 2: return launch_xproj<T>(g, fp, false, pp, tws, st);
     case 3: return launch_cols<T>(g, fp, 3, 1, false, tws, st);
     case 4: return launch_z<T>(g, fp, 3, false, tws, st, nullptr);
@@ -538,10 +565,10 @@ template hipError_t fft3d_forward<double>(const GridGeom&, const FieldPtrs<doubl
 template hipError_t fft3d_forward<float>(const GridGeom&, const FieldPtrs<float>&, int, const TwiddleSet<float>&, hipStream_t);
 template hipError_t fft3d_inverse<double>(const GridGeom&, const FieldPtrs<double>&, int, const TwiddleSet<double>&, hipStream_t);
 template hipError_t fft3d_inverse<float>(const GridGeom&, const FieldPtrs<float>&, int, const TwiddleSet<float>&, hipStream_t);
-template hipError_t stokes_solve<double>(const GridGeom&, const FieldPtrs<double>&, const ProjParams&, const TwiddleSet<double>&, hipStream_t);
-template hipError_t stokes_solve<float>(const GridGeom&, const FieldPtrs<float>&, const ProjParams&, const TwiddleSet<float>&, hipStream_t);
-template hipError_t stokes_pass<double>(const GridGeom&, const FieldPtrs<double>&, int, const ProjParams&, const TwiddleSet<double>&, hipStream_t);
-template hipError_t stokes_pass<float>(const GridGeom&, const FieldPtrs<float>&, int, const ProjParams&, const TwiddleSet<float>&, hipStream_t);
+template hipError_t stokes_solve<double>(const GridGeom&, const FieldPtrs<double>&, const ProjParams&, const TwiddleSet<double>&, hipStream_t, bool);
+template hipError_t stokes_solve<float>(const GridGeom&, const FieldPtrs<float>&, const ProjParams&, const TwiddleSet<float>&, hipStream_t, bool);
+template hipError_t stokes_pass<double>(const GridGeom&, const FieldPtrs<double>&, int, const ProjParams&, const TwiddleSet<double>&, hipStream_t, bool);
+template hipError_t stokes_pass<float>(const GridGeom&, const FieldPtrs<float>&, int, const ProjParams&, const TwiddleSet<float>&, hipStream_t, bool);
 template hipError_t random_velocity<double>(const GridGeom&, const FieldPtrs<double>&, bool, unsigned long long, const double*, const ProjParams&, const TwiddleSet<double>&, hipStream_t);
 template hipError_t random_velocity<float>(const GridGeom&, const FieldPtrs<float>&, bool, unsigned long long, const double*, const ProjParams&, const TwiddleSet<float>&, hipStream_t);
 

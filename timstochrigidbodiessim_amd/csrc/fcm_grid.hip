@@ -905,7 +905,7 @@ hipError_t build_tile_lists(const GridGeom& g, const BlobBins& bb, BlobData& bd,
 
 template <typename T>
 hipError_t spread(const GridGeom& g, const BlobBins& bb, const BlobData& bd, const double* F, const FieldPtrs<T>& fp,
-                  hipStream_t st) {
+                  hipStream_t st, bool images) {
   if (spread_fast_ok(g, bb, bd.ng)) {
     const int NW = g.nz % 32 == 0 ? 4 : (g.nz % 16 == 0 ? 2 : 1);
     const int nt = spread_tiles(g);
@@ -925,7 +925,7 @@ hipError_t spread(const GridGeom& g, const BlobBins& bb, const BlobData& bd, con
     hipLaunchKernelGGL((spread_kernel<T>), dim3(ntiles), dim3(256), 0, st, A, fp);
   }
   const hipError_t e = hipGetLastError();
-  return e != hipSuccess ? e : channel_images<T>(g, fp, st);
+  return e != hipSuccess || !images ? e : channel_images<T>(g, fp, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -1344,8 +1344,8 @@ hipError_t interp(const GridGeom& g, const BlobBins& bb, const BlobData& bd, con
   return hipGetLastError();
 }
 
-template hipError_t spread<double>(const GridGeom&, const BlobBins&, const BlobData&, const double*, const FieldPtrs<double>&, hipStream_t);
-template hipError_t spread<float>(const GridGeom&, const BlobBins&, const BlobData&, const double*, const FieldPtrs<float>&, hipStream_t);
+template hipError_t spread<double>(const GridGeom&, const BlobBins&, const BlobData&, const double*, const FieldPtrs<double>&, hipStream_t, bool);
+template hipError_t spread<float>(const GridGeom&, const BlobBins&, const BlobData&, const double*, const FieldPtrs<float>&, hipStream_t, bool);
 template hipError_t interp<double>(const GridGeom&, const BlobBins&, const BlobData&, const FieldPtrs<double>&, double*, const KTerm&, hipStream_t);
 template hipError_t interp<float>(const GridGeom&, const BlobBins&, const BlobData&, const FieldPtrs<float>&, double*, const KTerm&, hipStream_t);
 

@@ -41,14 +41,16 @@ template <typename T>
 hipError_t fft3d_forward(const GridGeom& g, const FieldPtrs<T>& fp, int nf, const TwiddleSet<T>& tws, hipStream_t st);
 template <typename T>
 hipError_t fft3d_inverse(const GridGeom& g, const FieldPtrs<T>& fp, int nf, const TwiddleSet<T>& tws, hipStream_t st);
+// images: the forward y pass applies the slip-channel image rule as it reads
+// (a slip channel's mobility product, whose spreading leaves the images out)
 template <typename T>
 hipError_t stokes_solve(const GridGeom& g, const FieldPtrs<T>& fp, const ProjParams& pp, const TwiddleSet<T>& tws,
-                        hipStream_t st);
+                        hipStream_t st, bool images = false);
 // one pass of stokes_solve: 0 z forward, 1 y forward, 2 x forward+projection+
 // x inverse, 3 y inverse, 4 z inverse (all three fields)
 template <typename T>
 hipError_t stokes_pass(const GridGeom& g, const FieldPtrs<T>& fp, int pass, const ProjParams& pp,
-                       const TwiddleSet<T>& tws, hipStream_t st);
+                       const TwiddleSet<T>& tws, hipStream_t st, bool images = false);
 template <typename T>
 hipError_t random_velocity(const GridGeom& g, const FieldPtrs<T>& stress6, bool generate, unsigned long long seed,
                            const double scale[6], const ProjParams& pp, const TwiddleSet<T>& tws, hipStream_t st);
@@ -140,9 +142,11 @@ __device__ __forceinline__ void store_blob_velocity(double* V, const KTerm& kt, 
   V[3 * (size_t)id + 2] = v2;
 }
 
+// images: apply the slip-channel image rule to the spread fields (the mobility
+// product passes false: its forward y pass applies the rule as it reads)
 template <typename T>
 hipError_t spread(const GridGeom& g, const BlobBins& bb, const BlobData& bd, const double* F, const FieldPtrs<T>& fp,
-                  hipStream_t st);
+                  hipStream_t st, bool images = true);
 template <typename T>
 hipError_t interp(const GridGeom& g, const BlobBins& bb, const BlobData& bd, const FieldPtrs<T>& fp, double* V,
                   const KTerm& kt, hipStream_t st);
@@ -161,7 +165,7 @@ int col_groups(const GridGeom& g, int cx);  // npatch x nbz
 hipError_t build_col_lists(const GridGeom& g, const BlobBins& bb, BlobData& bd, hipStream_t st);
 template <typename T>
 hipError_t spread_zfwd(const GridGeom& g, const BlobData& bd, const double* F, const FieldPtrs<T>& fp,
-                       const TwiddleSet<T>& tw, hipStream_t st);
+                       const TwiddleSet<T>& tw, hipStream_t st, bool images = true);
 // inverse z pass + interpolation partial sums, then the fixed-order reduction
 // into V (with the K term)
 template <typename T>
