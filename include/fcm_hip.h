@@ -150,6 +150,16 @@ int ll_barrier_forces(fcm_handle h, double cutoff, double* F3);
 int rb_end_of_step_update(fcm_handle h, const double* V6, double nu);
 /* current body positions (3 NB) and quaternions (4 NB); either may be NULL */
 int rb_get_bodies(fcm_handle h, double* x, double* q);
+/* check_for_channel_wall_overlap (collection_of_rigid_bodies.cpp:207-239): in a
+ * slip channel *overlap = 1 if any blob is at y == 0 or y >= LY/2 (wrapped
+ * positions), else 0; always 0 in a periodic box.  end_of_step_update returns
+ * this flag (:784-790) and main.cpp:304-326 then redoes the step. */
+int rb_channel_wall_overlap(fcm_handle h, int* overlap);
+/* reset_to_start_of_step (collection_of_rigid_bodies.cpp:792-796): the bodies,
+ * blob positions and Gaussians return to their state before the last
+ * rb_end_of_step_update (or fcm_em_step); the next step can then be redone
+ * with a fresh stress seed.  FCM_ERR_STATE if there is no step to undo. */
+int rb_reset_to_start_of_step(fcm_handle h);
 
 /* ---- gmres_solver ----------------------------------------------------------- */
 typedef struct {
@@ -199,7 +209,9 @@ int fcm_mobility_prepare(fcm_handle h, uint64_t seed);
 /* one Euler-Maruyama step (main.cpp:222-312 with EULER_MARUYAMA, nu = 1):
  * barrier forces (cutoff > 0), the seeded fluctuating field and its
  * interpolation, the RHS, the saddle-point solve and the body update, all on
- * the device. */
+ * the device.  In a slip channel the caller then asks rb_channel_wall_overlap
+ * and, on 1, calls rb_reset_to_start_of_step and steps again with a new seed
+ * (main.cpp:304-326). */
 int fcm_em_step(fcm_handle h, uint64_t seed, double cutoff, const gmres_opts* opts, gmres_report* rep);
 
 /* ---- batches of independent solves (no counterpart in the reference) ------ */

@@ -266,3 +266,48 @@ def test_channel_stokes_entry_takes_the_fields_as_given():
             s.stokes(prec)
             got = np.stack([po.from_padded(oc, s.get_field(c, prec)) for c in range(3)])
             assert rel(got, ref) < tol, (prec, rel(got, ref))
+
+
+def test_channel_wall_overlap_and_reset_to_start_of_step():
+    """check_for_channel_wall_overlap after a step that carries a blob past the
+    upper wall, then reset_to_start_of_step restores bodies and blobs exactly
+    (collection_of_rigid_bodies.cpp:207-239,759-796; main.cpp:304-326)."""
+    say("channel wall overlap / reset")
+    n, NB, nbpb, dt = 32, 4, 3, 0.2
+    gc, oc = pair_cfg(n, dt=dt, max_blobs=NB * nbpb, max_bodies=NB, max_blobs_per_body=nbpb, slip_channel=True)
+    ref = rod(nbpb, 1.86)
+    x = np.array([[8.0, 8.0, 8.0], [20.0, 12.0, 20.0], [10.0, 5.0, 5.0], [25.0, 9.0, 25.0]])
+    q = np.tile([1.0, 0.0, 0.0, 0.0], (NB, 1))
+    with fa.FcmSolver(gc) as s:
+        s.set_geometry(ref)
+        s.set_bodies(x, q)
+        with pytest.raises(fa.FcmError):
+            s.reset_to_start_of_step()  # no step to undo
+        assert not s.channel_wall_overlap()
+        Y0 = s.blob_positions()
+        V6 = np.zeros((NB, 6))
+        V6[1, 1] = 4.5 / dt  # body 1 (all blobs at y = 12) to y = 16.5 > LY/2
+        s.end_of_step_update(V6, nu=1.0)
+        xg, _ = s.bodies()
+        assert abs(xg[1, 1] - 16.5) < 1e-12
+        assert s.channel_wall_overlap()
+        s.reset_to_start_of_step()
+        xr, qr = s.bodies()
+        assert np.array_equal(xr, x) and np.array_equal(qr, q)
+        assert np.array_equal(s.blob_positions(), Y0)
+        assert not s.channel_wall_overlap()
+        V6[1, 1] = 3.9 / dt  # to y = 15.9: inside
+        s.end_of_step_update(V6, nu=1.0)
+        assert not s.channel_wall_overlap()
+        V6[:] = 0.0
+        V6[2, 1] = -5.0 / dt  # body 2 to y = 0 exactly: on the lower wall
+        s.end_of_step_update(V6, nu=1.0)
+        assert s.channel_wall_overlap()
+    gp, _ = pair_cfg(n, dt=dt, max_blobs=NB * nbpb, max_bodies=NB, max_blobs_per_body=nbpb)
+    with fa.FcmSolver(gp) as s:
+        s.set_geometry(ref)
+        s.set_bodies(x, q)
+        V6 = np.zeros((NB, 6))
+        V6[1, 1] = 4.5 / dt
+        s.end_of_step_update(V6, nu=1.0)
+        assert not s.channel_wall_overlap()  # no walls in a periodic box

@@ -1,0 +1,131 @@
+"""The reference's on-disk formats written and read by the C++ host classes
+(timstochrigidbodiessim_amd/host): <name>.dat and <name>_blob_forces.dat
+(write_data_to_file, collection_of_rigid_bodies.cpp:798-834),
+<name>_blob_references.dat (seed_the_bodies, :245-263), the initial-condition
+reader (:265-339), <name>.par (main.cpp:36-66) and <name>_velocities.dat
+(main.cpp:343-367).  No GPU is used.
+
+The reference writes with C++ streams (std::scientific, precision 6, each
+number followed by a space; .par in the stream's default format).  The files
+here are checked against Python's own printf-style formatting of the same
+values ("%.6e", "%g"), an independent implementation of the format those
+stream flags are defined by."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import timstochrigidbodiessim_amd as fa
+
+EXE = os.path.join(os.path.dirname(fa.__file__), "fcm_host_check")
+SCI = re.compile(r"^-?\d\.\d{6}e[+-]\d{2,3}$")
+
+# the fixed state the io mode writes (fcm_host_check.cpp, run_io)
+XS = [[1.5, 2.25, 3.125], [-0.0, 1e-300, 123456789.0], [63.999999951, 0.1, 7.0]]
+QS = [[0.5 + 0.1 * n, -0.5, 0.5, np.sqrt(0.75 - (0.5 + 0.1 * n) ** 2)] for n in range(3)]
+FORCES = [(i - 9) * 0.3125e-3 for i in range(18)]
+SOLN = [1.0 / (1.0 + i) for i in range(36)]
+
+
+def _run(tmp_path, states, sample):
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} is not built (python -m timstochrigidbodiessim_amd.build)")
+    ic = tmp_path / "ic_state.dat"
+    with open(ic, "w") as f:
+        for t, bodies in states:
+            f.write("%.6e " % t + "".join("%.6e " % v for b in bodies for v in b) + "\n")
+    name = str(tmp_path / "run")
+    r = subprocess.run([EXE, "io", name, str(ic), str(sample)], capture_output=True, text=True, timeout=60)
+    return r, name
+
+
+def _states(k):
+    rng = np.random.default_rng(k)
+    out = []
+    for s in range(k):
+        bodies = [list(rng.uniform(0, 64, 3)) + list(rng.standard_normal(4)) for _ in range(3)]
+        out.append((0.1 * s, bodies))
+    return out
+
+
+def _tokens(line):
+    assert line.endswith(" "), "every number is followed by a space"
+    toks = line[:-1].split(" ")
+    for t in toks:
+        assert SCI.match(t), t
+    return toks
+
+
+def _same(toks, values):
+    assert len(toks) == len(values)
+    for t, v in zip(toks, values):
+        assert t == "%.6e" % v, (t, v)
+
+
+def test_data_force_reference_and_velocity_files(tmp_path):
+    r, name = _run(tmp_path, _states(2), 0)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = open(name + ".dat").read().split("\n")
+    assert lines[-1] == "" and len(lines) == 3  # appended twice, newline-terminated
+    for line, t in zip(lines[:2], (0.0, 0.20216)):
+        toks = _tokens(line)
+        _same(toks, [t] + [v for n in range(3) for v in XS[n] + QS[n]])
+    assert _tokens(lines[0])[8] == "-0.000000e+00" and _tokens(lines[0])[9] == "1.000000e-300"
+    fl = open(name + "_blob_forces.dat").read().split("\n")
+    assert len(fl) == 3
+    for line, t in zip(fl[:2], (0.0, 0.20216)):
+        _same(_tokens(line), [t] + FORCES)
+    refs = open(name + "_blob_references.dat").read().split("\n")
+    assert len(refs) == 4 and refs[3] == ""
+    _same(_tokens(refs[0]), [-1.86, 1.86])
+    _same(_tokens(refs[1]), [0.0, 0.0])
+    _same(_tokens(refs[2]), [0.0, 0.0])
+    vel = open(name + "_velocities.dat").read().split("\n")
+    assert len(vel) == 2
+    _same(_tokens(vel[0]), [0.20216] + SOLN[18:])
+
+
+def test_par_file(tmp_path):
+    r, name = _run(tmp_path, _states(1), 0)
+    assert r.returncode == 0, r.stdout + r.stderr
+    want = [(3, "NUM_BODIES"), (1.86, "BLOB_RADIUS"), (1.0, "FLUID_VISCOSITY"), (1.0, "ENERGY_SCALE"),
+            (64.0, "LX"), (128.0, "LY"), (64.0, "LZ"), (1, "SLIP_CHANNEL"), (100, "STEPS_PER_DIFFUSIVE_TIME"),
+            (100, "NUM_DIFFUSIVE_TIMES"), (10, "DATA_SAVES_PER_DIFFUSIVE_TIME")]
+    assert open(name + ".par").read() == "".join("%g %%%% %s\n" % (v, k) for v, k in want)
+
+
+@pytest.mark.parametrize("k,sample", [(3, 0), (3, 2), (1, 0), (4, 1)])
+def test_initial_conditions_reader(tmp_path, k, sample):
+    states = _states(k)
+    r, _ = _run(tmp_path, states, sample)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = r.stdout.strip().split("\n")
+    assert out[0] == f"states {k}"
+    _, bodies = states[(k if sample == 0 else sample) - 1]
+    got = np.array([[float(v) for v in line.split()] for line in out[1:]])
+    want = np.array([[float("%.6e" % v) for v in b] for b in bodies])
+    assert np.array_equal(got, want)
+
+
+def test_initial_conditions_reader_round_trips_its_own_output(tmp_path):
+    """The .dat written by write_data_to_file is a valid state file: reading its
+    last state returns the second save's bodies to the printed precision."""
+    r, name = _run(tmp_path, _states(1), 0)
+    assert r.returncode == 0
+    r2 = subprocess.run([EXE, "io", str(tmp_path / "again"), name + ".dat", "0"], capture_output=True, text=True,
+                        timeout=60)
+    assert r2.returncode == 0, r2.stdout + r2.stderr
+    out = r2.stdout.strip().split("\n")
+    assert out[0] == "states 2"
+    got = np.array([[float(v) for v in line.split()] for line in out[1:]])
+    want = np.array([[float("%.6e" % v) for v in XS[n] + QS[n]] for n in range(3)])
+    assert np.array_equal(got, want)
+
+
+def test_initial_conditions_reader_rejects_missing_states(tmp_path):
+    r, _ = _run(tmp_path, _states(2), 3)
+    assert r.returncode == 3 and "state 3 of 2" in r.stderr
+    r, _ = _run(tmp_path, [], 0)
+    assert r.returncode == 3 and "no complete state" in r.stderr

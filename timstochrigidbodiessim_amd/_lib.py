@@ -25,7 +25,8 @@ EXPORTS = [
     "fcm_ref_matrices", "fcm_set_ref_matrices",
     "fft_transform", "fft_inverse_transform", "fcm_get_kspace", "fcm_set_kspace",
     "rb_set_geometry", "rb_set_bodies", "rb_set_body_forces", "rb_get_blob_positions",
-    "rb_mul_K", "rb_mul_KT", "ll_barrier_forces", "rb_end_of_step_update", "rb_get_bodies", "fcm_em_step",
+    "rb_mul_K", "rb_mul_KT", "ll_barrier_forces", "rb_end_of_step_update", "rb_get_bodies",
+    "rb_channel_wall_overlap", "rb_reset_to_start_of_step", "fcm_em_step",
     "gmres_assemble_rhs", "gmres_saddle_matvec", "gmres_apply_Pinv", "gmres_solve",
     "gmres_get_solution", "fcm_mobility_solve", "fcm_mobility_prepare",
     "fcm_batch_create", "fcm_batch_destroy", "fcm_batch_submit", "fcm_batch_step", "fcm_batch_take",
@@ -115,6 +116,8 @@ def lib():
         L.ll_barrier_forces.argtypes = [H, C.c_double, _dp]
         L.rb_end_of_step_update.argtypes = [H, _dp, C.c_double]
         L.rb_get_bodies.argtypes = [H, _dp, _dp]
+        L.rb_channel_wall_overlap.argtypes = [H, C.POINTER(C.c_int)]
+        L.rb_reset_to_start_of_step.argtypes = [H]
         L.fcm_em_step.argtypes = [H, C.c_uint64, C.c_double, C.POINTER(GmresOpts), C.POINTER(GmresReport)]
         L.gmres_assemble_rhs.argtypes = [H, C.c_int]
         L.gmres_saddle_matvec.argtypes = [H, _dp, _dp, C.c_int]
@@ -332,6 +335,16 @@ class FcmSolver:
     def end_of_step_update(self, V6=None, nu=1.0):
         """Move the bodies by nu*dt*V6 (host [U; Omega] per body, or the device solution)."""
         _chk(self.L.rb_end_of_step_update(self.h, None if V6 is None else _ptr(_f64(V6).ravel()), nu))
+
+    def channel_wall_overlap(self):
+        """True if a blob is on or beyond a slip-channel wall (always False in a periodic box)."""
+        v = C.c_int(0)
+        _chk(self.L.rb_channel_wall_overlap(self.h, C.byref(v)))
+        return bool(v.value)
+
+    def reset_to_start_of_step(self):
+        """Undo the last end-of-step update (bodies, blob positions, Gaussians)."""
+        _chk(self.L.rb_reset_to_start_of_step(self.h))
 
     def bodies(self):
         x = np.zeros(3 * self.NB)

@@ -125,6 +125,7 @@ Sizes sizes_of(const fcm_config* c) {
   const size_t nblk = (s.nmax + 1023) / 1024;
   t += nblk * (2 * (size_t)c->max_krylov + 6) * 8 + 2 * (2 * (size_t)c->max_krylov + 16) * 8;
   t += (size_t)c->max_bodies * 12 * 8 + (size_t)nb * (3 * 8 + 2 * 4);  // time-step buffers
+  t += (size_t)c->max_bodies * 7 * 8 + 4;                                 // bodies before a step, flag
   const size_t m2 = (size_t)(c->max_krylov + 2) * (c->max_krylov + 2);
   t += (2 * m2 + 9 * (size_t)(c->max_krylov + 2) + 16) * 8 + s.ldq * 8;
   s.total = t;
@@ -481,6 +482,9 @@ int fcm_create(const fcm_config* cfg, fcm_handle* out) {
   if (e == hipSuccess) h->d_F6 = dalloc<double>(h, 6 * (size_t)cfg->max_bodies, e);
   if (e == hipSuccess) h->d_F6tot = dalloc<double>(h, 6 * (size_t)cfg->max_bodies, e);
   if (e == hipSuccess) h->d_V6 = dalloc<double>(h, 6 * (size_t)cfg->max_bodies, e);
+  if (e == hipSuccess) h->d_x0 = dalloc<double>(h, 3 * (size_t)cfg->max_bodies, e);
+  if (e == hipSuccess) h->d_q0 = dalloc<double>(h, 4 * (size_t)cfg->max_bodies, e);
+  if (e == hipSuccess) h->d_flag = dalloc<int>(h, 1, e);
   if (e == hipSuccess) h->d_fblob = dalloc<double>(h, 3 * (size_t)nb, e);
   if (e == hipSuccess) h->sw.cell_of = dalloc<int>(h, nb, e);
   if (e == hipSuccess) h->sw.sorted = dalloc<int>(h, nb, e);
@@ -749,6 +753,7 @@ int rb_set_bodies(fcm_handle h, int NB, const double* x, const double* q) {
   int rc = setup_blobs_device(h, NB * h->nbpb);
   if (rc) return rc;
   h->have_bodies = true;
+  h->have_prev = false;
   h->have_rhs = false;
   h->have_blob_forces = false;
   return sync_stream(h);
@@ -1198,10 +1203,39 @@ int rb_end_of_step_update(fcm_handle h, const double* V6, double nu) {
     if (!h->have_sol) return fail(FCM_ERR_STATE, "no solution to take the body velocities from");
     v = h->sol + h->n3();
   }
+  // temp_bodies = bodies (collection_of_rigid_bodies.cpp:761), for reset_to_start_of_step
+  HIPCHK(hipMemcpyAsync(h->d_x0, h->d_x, 3 * (size_t)h->NB * sizeof(double), hipMemcpyDeviceToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(h->d_q0, h->d_q, 4 * (size_t)h->NB * sizeof(double), hipMemcpyDeviceToDevice, h->st));
   HIPCHK(bodies_step(h->g, h->d_x, h->d_q, h->NB, v, nu * h->cfg.dt, h->st));
   BodyData b = h->body();
   HIPCHK(bodies_update(h->g, b, h->st));
   if (int rc = setup_blobs_device(h, h->NB * h->nbpb)) return rc;
+  h->have_prev = true;
+  h->have_blob_forces = false;
+  h->have_rhs = false;
+  return sync_stream(h);
+}
+
+int rb_channel_wall_overlap(fcm_handle h, int* overlap) {
+  if (int rc = check(h)) return rc;
+  if (!overlap) return fail(FCM_ERR_ARG, "null argument");
+  if (!h->have_bodies) return fail(FCM_ERR_STATE, "rb_set_bodies first");
+  *overlap = 0;
+  if (!h->g.channel) return FCM_OK;
+  HIPCHK(wall_overlap(h->dY, h->NB * h->nbpb, 0.5 * h->g.dx * h->g.ny, h->d_flag, h->st));
+  HIPCHK(hipMemcpyAsync(overlap, h->d_flag, sizeof(int), hipMemcpyDeviceToHost, h->st));
+  return sync_stream(h);
+}
+
+int rb_reset_to_start_of_step(fcm_handle h) {
+  if (int rc = check(h)) return rc;
+  if (!h->have_prev) return fail(FCM_ERR_STATE, "no end-of-step update to undo");
+  HIPCHK(hipMemcpyAsync(h->d_x, h->d_x0, 3 * (size_t)h->NB * sizeof(double), hipMemcpyDeviceToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(h->d_q, h->d_q0, 4 * (size_t)h->NB * sizeof(double), hipMemcpyDeviceToDevice, h->st));
+  BodyData b = h->body();
+  HIPCHK(bodies_update(h->g, b, h->st));
+  if (int rc = setup_blobs_device(h, h->NB * h->nbpb)) return rc;
+  h->have_prev = false;
   h->have_blob_forces = false;
   h->have_rhs = false;
   return sync_stream(h);

@@ -54,6 +54,9 @@ struct fcm_handle_s {
   double* d_fblob = nullptr;    // 3 Nb blob forces (barrier forces), blob order
   double* d_F6tot = nullptr;    // 6 NB body forces + K^T blob forces, for the RHS
   double* d_V6 = nullptr;       // 6 NB body velocities for the end-of-step update
+  double *d_x0 = nullptr, *d_q0 = nullptr;  // bodies before the last end-of-step update
+  int* d_flag = nullptr;        // one-word result of the wall-overlap check
+  bool have_prev = false;
   StericWork sw;                // cell list of the barrier forces
   std::vector<void*> lazy;      // cell arrays, grown on demand
   bool have_blob_forces = false;

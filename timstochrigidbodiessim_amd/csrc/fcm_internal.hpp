@@ -192,6 +192,9 @@ struct BodyData {
 };
 
 hipError_t bodies_update(const GridGeom& g, BodyData& b, hipStream_t st);
+// *flag = 1 if any blob sits on the lower wall (y == 0) or at or beyond the
+// upper one (y >= half_ly), else 0
+hipError_t wall_overlap(const double* Y, int nb, double half_ly, int* flag, hipStream_t st);
 // end_of_step_update (collection_of_rigid_bodies.cpp:759-790): body n moves by
 // s [U; Omega] (V6 with stride vstride doubles between bodies' 6-vectors)
 hipError_t bodies_step(const GridGeom& g, double* x, double* q, int NB, const double* V6, double s, hipStream_t st);

@@ -54,6 +54,24 @@ hipError_t bodies_update(const GridGeom& g, BodyData& b, hipStream_t st) {
   return hipGetLastError();
 }
 
+// check_for_channel_wall_overlap (collection_of_rigid_bodies.cpp:207-239): one
+// workgroup strides over the wrapped blob positions and ORs y == 0 || y >= LY/2
+__global__ void __launch_bounds__(1024) wall_overlap_kernel(const double* __restrict__ Y, int nb, double half_ly,
+                                                            int* __restrict__ flag) {
+  int hit = 0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    const double y = Y[3 * (size_t)i + 1];
+    hit |= (y == 0.0 || y >= half_ly) ? 1 : 0;
+  }
+  hit = __syncthreads_or(hit);
+  if (threadIdx.x == 0) *flag = hit ? 1 : 0;
+}
+
+hipError_t wall_overlap(const double* Y, int nb, double half_ly, int* flag, hipStream_t st) {
+  hipLaunchKernelGGL(wall_overlap_kernel, dim3(1), dim3(1024), 0, st, Y, nb, half_ly, flag);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // P^{-1}, gmres_solver.cpp:85-146, evaluated in the body frame: with R the
 // body rotation, Minv_n = (I(x)R) Minv_ref (I(x)R)^T and N_n = diag(R,R) N_ref

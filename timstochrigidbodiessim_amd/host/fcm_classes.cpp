@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <fstream>
+#include <iomanip>
 
 namespace tsrb {
 
@@ -254,7 +256,121 @@ bool collection_of_rigid_bodies::end_of_step_update(const std::vector<double>& V
   }
   all_blob_pos.assign(3 * (size_t)total_num_blobs, 0.0);
   check(rb_get_blob_positions(h_, all_blob_pos.data()));
-  return false;
+  return check_for_channel_wall_overlap();
+}
+
+bool collection_of_rigid_bodies::check_for_channel_wall_overlap() const {
+  int overlap = 0;
+  check(rb_channel_wall_overlap(h_, &overlap));
+  return overlap != 0;
+}
+
+void collection_of_rigid_bodies::reset_to_start_of_step() {
+  check(rb_reset_to_start_of_step(h_));
+  std::vector<double> x(3 * bodies.size()), q(4 * bodies.size());
+  check(rb_get_bodies(h_, x.data(), q.data()));
+  for (size_t n = 0; n < bodies.size(); ++n) {
+    for (int k = 0; k < 3; ++k) bodies[n].x[k] = x[3 * n + k];
+    bodies[n].q = quaternion(q[4 * n], q[4 * n + 1], q[4 * n + 2], q[4 * n + 3]);
+  }
+  all_blob_pos.assign(3 * (size_t)total_num_blobs, 0.0);
+  check(rb_get_blob_positions(h_, all_blob_pos.data()));
+}
+
+namespace {
+void put_sci(std::ofstream& f, double v) { f << v << ' '; }
+
+// the reference's data files are appended to, std::scientific, precision 6
+std::ofstream open_append(const std::string& path) {
+  std::ofstream f(path, std::ios::app);
+  if (!f) throw fcm_error(FCM_ERR_ARG, "cannot open " + path);
+  f << std::scientific << std::setprecision(6);
+  return f;
+}
+}  // namespace
+
+void collection_of_rigid_bodies::write_data_to_file(double t, const std::string& name) const {
+  {
+    std::ofstream f = open_append(name + ".dat");
+    put_sci(f, t);
+    for (const particle& b : bodies) {
+      for (int k = 0; k < 3; ++k) put_sci(f, b.x[k]);
+      put_sci(f, b.q.scalar_part);
+      for (int k = 0; k < 3; ++k) put_sci(f, b.q.vector_part[k]);
+    }
+    f << '\n';
+  }
+  std::ofstream f = open_append(name + "_blob_forces.dat");
+  put_sci(f, t);
+  for (double v : blob_forces) put_sci(f, v);
+  f << '\n';
+}
+
+void collection_of_rigid_bodies::write_blob_references(const std::string& name) const {
+  std::ofstream f(name + "_blob_references.dat");
+  if (!f) throw fcm_error(FCM_ERR_ARG, "cannot open " + name + "_blob_references.dat");
+  f << std::scientific << std::setprecision(6);
+  for (int d = 0; d < 3; ++d) {
+    for (int m = 0; m < blobs_per_body; ++m) put_sci(f, blob_ref_pos[3 * (size_t)m + d]);
+    f << '\n';
+  }
+}
+
+int collection_of_rigid_bodies::read_initial_conditions(const std::string& file, int sample) {
+  const size_t nbody = bodies.size();
+  std::vector<double> state(7 * nbody), chosen;
+  std::ifstream f(file);
+  if (!f) throw fcm_error(FCM_ERR_ARG, "cannot open " + file);
+  // one state: a time, then x y z q0 q1 q2 q3 per body (whitespace separated,
+  // lines do not matter), counted as the reference counts them (:272-292)
+  int n = 0;
+  double t;
+  while (f >> t) {
+    for (size_t i = 0; i < 7 * nbody; ++i) f >> state[i];
+    if (!f) break;
+    ++n;
+    if (sample == 0 || n == sample) chosen = state;
+  }
+  if (n == 0) throw fcm_error(FCM_ERR_ARG, file + " holds no complete state");
+  if (sample < 0 || sample > n) throw fcm_error(FCM_ERR_ARG, "state " + std::to_string(sample) + " of " +
+                                                              std::to_string(n) + " requested from " + file);
+  for (size_t i = 0; i < nbody; ++i) {
+    const double* s = chosen.data() + 7 * i;
+    for (int k = 0; k < 3; ++k) bodies[i].x[k] = s[k];
+    bodies[i].q = quaternion(s[3], s[4], s[5], s[6]);
+  }
+  if (h_) fill_all_blob_pos();  // not yet initialised: initialise uploads them
+  return n;
+}
+
+void write_par_file(const std::string& name, const run_parameters& p) {
+  std::ofstream f(name + ".par");
+  if (!f) throw fcm_error(FCM_ERR_ARG, "cannot open " + name + ".par");
+  f << p.num_bodies << " %% NUM_BODIES\n";
+  f << p.blob_radius << " %% BLOB_RADIUS\n";
+  f << p.fluid_viscosity << " %% FLUID_VISCOSITY\n";
+  f << p.energy_scale << " %% ENERGY_SCALE\n";
+  f << p.LX << " %% LX\n";
+  f << p.LY << " %% LY\n";
+  f << p.LZ << " %% LZ\n";
+  f << p.slip_channel << " %% SLIP_CHANNEL\n";
+  f << p.steps_per_diffusive_time << " %% STEPS_PER_DIFFUSIVE_TIME\n";
+  f << p.num_diffusive_times << " %% NUM_DIFFUSIVE_TIMES\n";
+  f << p.data_saves_per_diffusive_time << " %% DATA_SAVES_PER_DIFFUSIVE_TIME\n";
+  if (p.shearing) {
+    f << p.shear_force_scale << " %% SHEAR_FORCE_SCALE\n";
+    f << p.shear_wavenumber << " %% SHEAR_WAVENUMBER\n";
+  }
+}
+
+void append_velocities(const std::string& name, double t, const std::vector<double>& soln, int total_num_blobs,
+                       int num_bodies) {
+  const size_t off = 3 * (size_t)total_num_blobs, nv = 6 * (size_t)num_bodies;
+  if (soln.size() < off + nv) throw fcm_error(FCM_ERR_ARG, "solution shorter than [lambda; V]");
+  std::ofstream f = open_append(name + "_velocities.dat");
+  put_sci(f, t);
+  for (size_t i = 0; i < nv; ++i) put_sci(f, soln[off + i]);
+  f << '\n';
 }
 
 // ---- gmres_solver -----------------------------------------------------------------------------

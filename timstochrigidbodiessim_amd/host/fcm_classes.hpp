@@ -157,14 +157,52 @@ class collection_of_rigid_bodies {
   std::vector<double> multiply_by_K_at_midstep(const std::vector<double>& in) const;
   std::vector<double> multiply_by_KT_at_midstep(const std::vector<double>& in) const;
   // collection_of_rigid_bodies.cpp:759-790; returns whether the step must be
-  // redone for an overlap (never: SKIP_ON_OVERLAP is false, no channel walls)
+  // redone: a blob on or beyond a slip-channel wall (SKIP_ON_OVERLAP is false,
+  // so blob-blob overlaps never force a redo)
   bool end_of_step_update(const std::vector<double>& V, double nu);
+  // :207-239 (false in a periodic box)
+  bool check_for_channel_wall_overlap() const;
+  // :792-796: the bodies before the last end_of_step_update
+  void reset_to_start_of_step();
+
+  // ---- on-disk formats (collection_of_rigid_bodies.cpp:245-339, 798-834) ----
+  // Appends one line "t x y z q0 q1 q2 q3 ..." (every body) to <name>.dat and one
+  // line "t f_0 ... f_{3Nb-1}" to <name>_blob_forces.dat, "%.6e" numbers each
+  // followed by a space (write_data_to_file, :798-834).
+  void write_data_to_file(double t, const std::string& name) const;
+  // <name>_blob_references.dat: three lines, the x, y and z body-frame offsets
+  // of the blobs (seed_the_bodies, :245-263)
+  void write_blob_references(const std::string& name) const;
+  // Reads a state file in the .dat format (READ_IN_INITIAL_CONDITIONS, :265-339)
+  // into bodies: sample 0 takes the last state (IC_READ_IN_LAST), sample k >= 1
+  // the k-th (IC_READ_IN_RANDOM draws k uniformly from 1..n; the caller draws
+  // it).  Returns the number of states n in the file; throws if it has none or
+  // sample > n.  Blob positions are refreshed on the device.
+  int read_initial_conditions(const std::string& file, int sample);
 
  private:
   fcm_handle h_ = nullptr;
   double L_[3] = {0.0, 0.0, 0.0};
   void upload() const;
 };
+
+// ---- main.cpp output files --------------------------------------------------------
+// The run parameters main.cpp:36-66 writes to <SIMULATION_NAME>.par.
+struct run_parameters {
+  int num_bodies = 0;
+  double blob_radius = 0.0, fluid_viscosity = 0.0, energy_scale = 0.0;
+  double LX = 0.0, LY = 0.0, LZ = 0.0;
+  int slip_channel = 0;
+  double steps_per_diffusive_time = 0.0, num_diffusive_times = 0.0, data_saves_per_diffusive_time = 0.0;
+  bool shearing = false;
+  double shear_force_scale = 0.0, shear_wavenumber = 0.0;
+};
+// "<value> %% NAME" lines, values in the stream's default format (main.cpp:36-66)
+void write_par_file(const std::string& name, const run_parameters& p);
+// appends "t U_0 Omega_0 ... " (the body part of the saddle solution, "%.6e"
+// each followed by a space) to <name>_velocities.dat (main.cpp:343-367)
+void append_velocities(const std::string& name, double t, const std::vector<double>& soln, int total_num_blobs,
+                       int num_bodies);
 
 // ---- gmres_solver.hpp ----------------------------------------------------------------
 class gmres_solver {

@@ -8,6 +8,12 @@
 //       reads a problem, runs its time steps through fcm_fluid_solver,
 //       collection_of_rigid_bodies, linked_list and gmres_solver, and writes
 //       every step's GMRES solution and the final body state
+//   fcm_host_check io <name> <state file> <sample>
+//       writes <name>.dat (two saves), <name>_blob_forces.dat,
+//       <name>_blob_references.dat, <name>.par and <name>_velocities.dat for
+//       three two-sphere bodies with fixed values (no GPU needed), then reads
+//       state <sample> (0: the last) of <state file> into the three bodies and
+//       prints them, one body per line, "%.17g"
 //
 // in.bin (little-endian): int32 n, NB, nbpb, steps, precision; uint64 seed;
 // float64 a, kT, dt, tol, cutoff; float64 ref[3 nbpb], x[3 NB], q[4 NB], F6[6 NB].
@@ -15,7 +21,9 @@
 // float64 x[3 NB], q[4 NB].
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "fcm_classes.hpp"
@@ -154,16 +162,57 @@ int run_steps(const char* in_path, const char* out_path) {
   return 0;
 }
 
+int run_io(const std::string& name, const std::string& state_file, int sample) {
+  collection_of_rigid_bodies c;
+  c.blobs_per_body = 2;
+  c.blob_ref_pos = {-1.86, 0.0, 0.0, 1.86, 0.0, 0.0};
+  c.bodies.resize(3);
+  const double xs[3][3] = {{1.5, 2.25, 3.125}, {-0.0, 1e-300, 123456789.0}, {63.999999951, 0.1, 7.0}};
+  for (int n = 0; n < 3; ++n) {
+    for (int k = 0; k < 3; ++k) c.bodies[n].x[k] = xs[n][k];
+    c.bodies[n].q = quaternion(0.5 + 0.1 * n, -0.5, 0.5, std::sqrt(0.75 - (0.5 + 0.1 * n) * (0.5 + 0.1 * n)));
+  }
+  c.total_num_blobs = 6;
+  c.blob_forces.resize(18);
+  for (int i = 0; i < 18; ++i) c.blob_forces[i] = (i - 9) * 0.3125e-3;
+  c.write_blob_references(name);
+  c.write_data_to_file(0.0, name);
+  c.write_data_to_file(0.20216, name);
+  run_parameters p;
+  p.num_bodies = 3;
+  p.blob_radius = 1.86;
+  p.fluid_viscosity = 1.0;
+  p.energy_scale = 1.0;
+  p.LX = 64.0;
+  p.LY = 128.0;
+  p.LZ = 64.0;
+  p.slip_channel = 1;
+  p.steps_per_diffusive_time = 100;
+  p.num_diffusive_times = 100;
+  p.data_saves_per_diffusive_time = 10;
+  write_par_file(name, p);
+  std::vector<double> soln(3 * 6 + 6 * 3);
+  for (size_t i = 0; i < soln.size(); ++i) soln[i] = 1.0 / (1.0 + (double)i);
+  append_velocities(name, 0.20216, soln, 6, 3);
+  const int n = c.read_initial_conditions(state_file, sample);
+  std::printf("states %d\n", n);
+  for (const particle& b : c.bodies)
+    std::printf("%.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", b.x[0], b.x[1], b.x[2], b.q.scalar_part,
+                b.q.vector_part[0], b.q.vector_part[1], b.q.vector_part[2]);
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
   try {
     if (argc >= 2 && std::strcmp(argv[1], "math") == 0) return run_math();
     if (argc >= 4 && std::strcmp(argv[1], "steps") == 0) return run_steps(argv[2], argv[3]);
+    if (argc >= 5 && std::strcmp(argv[1], "io") == 0) return run_io(argv[2], argv[3], std::atoi(argv[4]));
   } catch (const fcm_error& e) {
     std::fprintf(stderr, "error %d: %s\n", e.code, e.what());
     return 3;
   }
-  std::fprintf(stderr, "usage: %s math | steps <in.bin> <out.bin>\n", argv[0]);
+  std::fprintf(stderr, "usage: %s math | steps <in.bin> <out.bin> | io <name> <state file> <sample>\n", argv[0]);
   return 2;
 }
