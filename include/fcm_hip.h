@@ -150,6 +150,25 @@ int ll_barrier_forces(fcm_handle h, double cutoff, double* F3);
 int rb_end_of_step_update(fcm_handle h, const double* V6, double nu);
 /* current body positions (3 NB) and quaternions (4 NB); either may be NULL */
 int rb_get_bodies(fcm_handle h, double* x, double* q);
+/* ---- shear forcing (SHEARING_FORCES, config.hpp:32) ------------------------ */
+/* produce_shear_velocity_field (fcm_fluid_solver.cpp:703-840): the steady
+ * flow of the force density fx = scale kT sin(2 pi kappa y / ny) / a (images
+ * in a slip channel), solved like produce_deterministic_velocity_field and kept
+ * on the device (3 FP64 fields, allocated by the first call; the working
+ * fields are overwritten, as the reference's fx..uz are).  While it is set,
+ * every RHS (gmres_assemble_rhs, fcm_mobility_solve/_prepare, fcm_em_step)
+ * subtracts its blob velocities at the current positions
+ * (gmres_solver.cpp:62-64).  scale = 0 switches it off. */
+int fcm_shear_field(fcm_handle h, double shear_force_scale, double shear_wavenumber);
+/* calculate_shear_blob_velocities (fcm_fluid_solver.cpp:488-550): the shear
+ * field interpolated at the current blobs (3 Nb, host; NULL: device only) */
+int fcm_shear_interp(fcm_handle h, double* V);
+/* main.cpp:371-433: the x velocity averaged over x and z on each of the ny rows
+ * (prof[ny], host), of the flow of the blob forces lambda (3 Nb host, or NULL
+ * for the lambda part of the last solution) plus the shear field if set; the
+ * blob forces are spread at the current positions (images in a channel). */
+int fcm_x_velocity_profile(fcm_handle h, const double* lambda, double* prof);
+
 /* check_for_channel_wall_overlap (collection_of_rigid_bodies.cpp:207-239): in a
  * slip channel *overlap = 1 if any blob is at y == 0 or y >= LY/2 (wrapped
  * positions), else 0; always 0 in a periodic box.  end_of_step_update returns

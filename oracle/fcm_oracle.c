@@ -293,7 +293,7 @@ struct or_fcm {
   or_cfg c;
   int pad;
   size_t total;
-  double* fld[15];
+  double* fld[18];
   double *qx, *qxsq, *qy, *qysq, *qpad, *qpadsq;
   int nb, max_blobs;
   double *gaussx, *gaussy, *gaussz;
@@ -305,7 +305,7 @@ or_fcm* or_fcm_create(const or_cfg* c, int max_blobs) {
   f->c = *c;
   f->pad = pad_of(c);
   f->total = (size_t)c->nx * c->ny * f->pad;
-  for (int i = 0; i < 15; ++i) f->fld[i] = (double*)calloc(f->total, sizeof(double));
+  for (int i = 0; i < 18; ++i) f->fld[i] = (double*)calloc(f->total, sizeof(double));
   f->max_blobs = max_blobs > 0 ? max_blobs : 1;
   const size_t gsz = (size_t)f->max_blobs * c->ng;
   f->gaussx = (double*)calloc(gsz, sizeof(double));
@@ -346,7 +346,7 @@ or_fcm* or_fcm_create(const or_cfg* c, int max_blobs) {
 
 void or_fcm_destroy(or_fcm* f) {
   if (!f) return;
-  for (int i = 0; i < 15; ++i) free(f->fld[i]);
+  for (int i = 0; i < 18; ++i) free(f->fld[i]);
   free(f->qx); free(f->qxsq); free(f->qy); free(f->qysq); free(f->qpad); free(f->qpadsq);
   free(f->gaussx); free(f->gaussy); free(f->gaussz);
   free(f->indx); free(f->indy); free(f->indz);
@@ -355,7 +355,7 @@ void or_fcm_destroy(or_fcm* f) {
 
 int or_fcm_pad(const or_fcm* f) { return f->pad; }
 size_t or_fcm_total(const or_fcm* f) { return f->total; }
-double* or_fcm_field(or_fcm* f, int which) { return (which >= 0 && which < 15) ? f->fld[which] : NULL; }
+double* or_fcm_field(or_fcm* f, int which) { return (which >= 0 && which < 18) ? f->fld[which] : NULL; }
 
 void or_fcm_tables(or_fcm* f, double* qx, double* qy, double* qpad) {
   memcpy(qx, f->qx, sizeof(double) * f->c.nx);
@@ -378,6 +378,9 @@ void or_fcm_tables(or_fcm* f, double* qx, double* qy, double* qpad) {
 #define UXR f->fld[12]
 #define UYR f->fld[13]
 #define UZR f->fld[14]
+#define UXS f->fld[15]
+#define UYS f->fld[16]
+#define UZS f->fld[17]
 
 static int wrap_index(int g, int n) { return g - n * (int)floor((double)g / (double)n); }
 
@@ -569,6 +572,8 @@ static void interp_from(or_fcm* f, const double* u1, const double* u2, const dou
 
 void or_fcm_interp(or_fcm* f, double* V) { interp_from(f, UX, UY, UZ, V); }
 void or_fcm_random_interp(or_fcm* f, double* V) { interp_from(f, UXR, UYR, UZR, V); }
+/* calculate_shear_blob_velocities, fcm_fluid_solver.cpp:488-550 */
+void or_fcm_shear_interp(or_fcm* f, double* V) { interp_from(f, UXS, UYS, UZS, V); }
 
 void or_fcm_mobility(or_fcm* f, const double* F, double* V) {
   or_fcm_spread(f, F);
@@ -659,6 +664,46 @@ void or_fcm_random_field(or_fcm* f) {
   memcpy(UXR, UX, f->total * sizeof(double));
   memcpy(UYR, UY, f->total * sizeof(double));
   memcpy(UZR, UZ, f->total * sizeof(double));
+}
+
+/* produce_shear_velocity_field, fcm_fluid_solver.cpp:703-840: the force
+ * fx = scale kT sin(2 pi kappa j / ny) / a on every real node (padding zero),
+ * fy = fz = 0, the channel images, then the Stokes solve of
+ * produce_deterministic_velocity_field; the result is kept in ux..uz_shear
+ * (the force and velocity fields are left as the solve leaves them). */
+void or_fcm_shear_field(or_fcm* f, double scale, double kappa) {
+  const or_cfg* c = &f->c;
+  const int ny = c->ny, pad = f->pad;
+  memset(FX, 0, f->total * sizeof(double));
+  memset(FY, 0, f->total * sizeof(double));
+  memset(FZ, 0, f->total * sizeof(double));
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < c->nx; ++i)
+    for (int j = 0; j < ny; ++j) {
+      const double v = scale * c->kT * sin(kappa * 2.0 * OR_PI * (double)j / (double)ny) / c->a;
+      double* row = FX + ((size_t)i * ny + j) * pad;
+      for (int k = 0; k < c->nz; ++k) row[k] = v;
+    }
+  if (c->slip_channel) channel_image(f);
+  or_fcm_stokes(f);
+  memcpy(UXS, UX, f->total * sizeof(double));
+  memcpy(UYS, UY, f->total * sizeof(double));
+  memcpy(UZS, UZ, f->total * sizeof(double));
+}
+
+/* main.cpp:371-433: the x velocity averaged over x and z on every y row, of
+ * the deterministic field currently in ux plus the shear field */
+void or_fcm_x_velocity_profile(or_fcm* f, double* prof) {
+  const or_cfg* c = &f->c;
+  const int ny = c->ny, pad = f->pad;
+  for (int j = 0; j < ny; ++j) {
+    double u = 0.0;
+    for (int i = 0; i < c->nx; ++i) {
+      const size_t row = ((size_t)i * ny + j) * pad;
+      for (int k = 0; k < c->nz; ++k) u += UX[row + k] + UXS[row + k];
+    }
+    prof[j] = u / (double)(c->nx * c->nz);
+  }
 }
 
 /* ========================================================================= */

@@ -54,7 +54,8 @@ or_fcm* or_fcm_create(const or_cfg* c, int max_blobs);
 void    or_fcm_destroy(or_fcm* f);
 int     or_fcm_pad(const or_fcm* f);
 size_t  or_fcm_total(const or_fcm* f);
-double* or_fcm_field(or_fcm* f, int which); /* 0..11 ux uy uz fx fy fz sxx syy szz sxy sxz syz; 12..14 u*_rand */
+double* or_fcm_field(or_fcm* f, int which); /* 0..11 ux uy uz fx fy fz sxx syy szz sxy sxz syz; 12..14 u*_rand;
+                                               15..17 u*_shear */
 void    or_fcm_tables(or_fcm* f, double* qx, double* qy, double* qpad);
 void    or_fcm_set_up_gaussians(or_fcm* f, const double* Y, int nb);
 void    or_fcm_get_gaussians(const or_fcm* f, double* gx, double* gy, double* gz, int* ix, int* iy, int* iz);
@@ -64,6 +65,9 @@ void    or_fcm_interp(or_fcm* f, double* V);
 void    or_fcm_fill_stress(or_fcm* f, uint64_t seed);
 void    or_fcm_random_field(or_fcm* f);
 void    or_fcm_random_interp(or_fcm* f, double* V);
+void    or_fcm_shear_field(or_fcm* f, double scale, double kappa);
+void    or_fcm_shear_interp(or_fcm* f, double* V);
+void    or_fcm_x_velocity_profile(or_fcm* f, double* prof);
 void    or_fcm_mobility(or_fcm* f, const double* F, double* V);
 
 /* ---- rigid bodies (collection_of_rigid_bodies.cpp) ---------------------- */

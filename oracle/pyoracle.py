@@ -70,6 +70,9 @@ def lib():
         L.or_fcm_fill_stress.argtypes = [C.c_void_p, C.c_uint64]
         L.or_fcm_random_field.argtypes = [C.c_void_p]
         L.or_fcm_random_interp.argtypes = [C.c_void_p, _dp]
+        L.or_fcm_shear_field.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.or_fcm_shear_interp.argtypes = [C.c_void_p, _dp]
+        L.or_fcm_x_velocity_profile.argtypes = [C.c_void_p, _dp]
         L.or_fcm_mobility.argtypes = [C.c_void_p, _dp, _dp]
         L.or_quat_matrix.argtypes = [_dp, _dp]
         L.or_fill_blob_pos.argtypes = [C.POINTER(OrCfg), C.c_int, C.c_int, _dp, _dp, _dp, _dp]
@@ -245,7 +248,7 @@ class Fcm:
     """Wrapper of the oracle's fluid solver object."""
 
     FIELDS = ["ux", "uy", "uz", "fx", "fy", "fz", "sxx", "syy", "szz", "sxy", "sxz", "syz",
-              "ux_rand", "uy_rand", "uz_rand"]
+              "ux_rand", "uy_rand", "uz_rand", "ux_shear", "uy_shear", "uz_shear"]
 
     def __init__(self, c, max_blobs=1):
         self.c = c
@@ -310,6 +313,19 @@ class Fcm:
         V = np.zeros(3 * self.nb)
         self.L.or_fcm_random_interp(self.h, V)
         return V
+
+    def shear_field(self, scale, kappa):
+        self.L.or_fcm_shear_field(self.h, scale, kappa)
+
+    def shear_interp(self):
+        V = np.zeros(3 * self.nb)
+        self.L.or_fcm_shear_interp(self.h, V)
+        return V
+
+    def x_velocity_profile(self):
+        p = np.zeros(self.c.ny)
+        self.L.or_fcm_x_velocity_profile(self.h, p)
+        return p
 
     def mobility(self, F):
         V = np.zeros(3 * self.nb)

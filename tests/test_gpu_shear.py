@@ -1,0 +1,89 @@
+"""GPU parity of the shear forcing (SHEARING_FORCES, config.hpp:32) against the
+oracle, through the C ABI: the steady shear flow (fcm_fluid_solver.cpp:
+703-840), its blob velocities (:488-550), the RHS term (gmres_solver.cpp:
+62-64) in a fluctuating solve, and the x-velocity profile (main.cpp:
+371-433); periodic and slip-channel.  Tolerances as for the other FP64
+operators (1e-12) and solves (1e-10)."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+import timstochrigidbodiessim_amd as fa
+from _common import pair_cfg, rel, rod
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("dims,channel,kappa", [((16, 24, 16), False, 1.0), ((32, 32, 16), True, 2.0),
+                                                ((16, 32, 32), True, 1.5)])
+def test_shear_field_and_blob_velocities_match_oracle(dims, channel, kappa):
+    say("shear field", dims, channel, kappa)
+    gc, oc = pair_cfg(*dims, dx=0.9, a=1.5, mu=1.2, kT=1.3, max_blobs=60, slip_channel=channel)
+    o = po.Fcm(oc, 60)
+    o.shear_field(1.7, kappa)
+    ref = np.stack([po.from_padded(oc, o.field(n)) for n in ("ux_shear", "uy_shear", "uz_shear")])
+    rng = np.random.default_rng(41)
+    Y = rng.uniform(0, 1, (60, 3)) * np.array(dims)[None, :] * 0.9
+    if channel:
+        Y[:, 1] *= 0.5
+    o.set_up_gaussians(Y)
+    refV = o.shear_interp()
+    with fa.FcmSolver(gc) as s:
+        s.shear_field(1.7, kappa)
+        got = np.stack([po.from_padded(oc, s.get_field(c)) for c in range(3)])
+        assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+        s.set_up_gaussians(Y)
+        V = s.shear_interp()
+        assert rel(V, refV) < 1e-12, rel(V, refV)
+
+
+def test_channel_fluctuating_solve_with_shear_and_profile_match_oracle():
+    say("shear solve")
+    n, NB, nbpb = 32, 8, 3
+    gc, oc = pair_cfg(n, kT=1.0, dt=0.20216, max_blobs=NB * nbpb, max_bodies=NB, max_blobs_per_body=nbpb,
+                      max_krylov=200, slip_channel=True)
+    rng = np.random.default_rng(42)
+    ref = rod(nbpb, 1.86)
+    x = rng.uniform(0, n, (NB, 3))
+    x[:, 1] = rng.uniform(6.0, 10.0, NB)
+    q = np.tile([1.0, 0.0, 0.0, 0.0], (NB, 1))
+    q[:, 1:] = 0.1 * rng.standard_normal((NB, 3))
+    q /= np.linalg.norm(q, axis=1)[:, None]
+    F6 = np.zeros((NB, 6))
+    o = po.Fcm(oc, NB * nbpb)
+    st, M, Minv, N = o.ref_matrices(ref)
+    o.shear_field(2.0, 1.0)
+    o.set_up_gaussians(po.fill_blob_pos(oc, ref, x, q))
+    o.fill_stress(93)
+    o.random_field()
+    Vr = o.random_interp()
+    Vs = o.shear_interp()
+    assert np.abs(Vs).max() > 1e-2
+    b = po.Bodies(ref, x, q, Minv, N)
+    rhs = np.concatenate([-Vr - Vs, -F6.ravel()])
+    st, sol_o, it_o, _ = po.gmres(o, b, rhs, tol=1e-10, max_iter=200)
+    assert st == 0
+    with fa.FcmSolver(gc) as s:
+        s.set_geometry(ref)
+        s.set_bodies(x, q)
+        s.set_ref_matrices(Minv, N)
+        s.set_body_forces(F6)
+        s.shear_field(2.0, 1.0)
+        rep = s.mobility_solve(93, tol=1e-10, max_iter=200)
+        say("oracle iters", it_o, "gpu", rep)
+        assert rep["status"] == "converged"
+        sol = s.solution()
+        assert rel(sol, sol_o) < 1e-10, rel(sol, sol_o)
+        prof = s.x_velocity_profile()
+        o.spread(sol_o[:3 * NB * nbpb])
+        o.stokes()
+        prof_o = o.x_velocity_profile()
+        assert np.abs(prof - prof_o).max() < 1e-11 * np.abs(prof_o).max()
+        s.shear_field(0.0, 0.0)  # off: the RHS loses the shear term
+        rep = s.mobility_solve(93, tol=1e-10, max_iter=200)
+        assert rel(s.solution(), sol_o) > 1e-3
+    o.close()

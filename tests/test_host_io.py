@@ -2,8 +2,9 @@
 (timstochrigidbodiessim_amd/host): <name>.dat and <name>_blob_forces.dat
 (write_data_to_file, collection_of_rigid_bodies.cpp:798-834),
 <name>_blob_references.dat (seed_the_bodies, :245-263), the initial-condition
-reader (:265-339), <name>.par (main.cpp:36-66) and <name>_velocities.dat
-(main.cpp:343-367).  No GPU is used.
+reader (:265-339), <name>.par (main.cpp:36-66), <name>_velocities.dat
+(main.cpp:343-367) and <name>_fluid_x_velocity_in_y.dat (main.cpp:371-433).
+No GPU is used.
 
 The reference writes with C++ streams (std::scientific, precision 6, each
 number followed by a space; .par in the stream's default format).  The files
@@ -85,6 +86,9 @@ def test_data_force_reference_and_velocity_files(tmp_path):
     vel = open(name + "_velocities.dat").read().split("\n")
     assert len(vel) == 2
     _same(_tokens(vel[0]), [0.20216] + SOLN[18:])
+    # the fluid velocity profile is written in the stream's default format
+    prof = open(name + "_fluid_x_velocity_in_y.dat").read()
+    assert prof == "".join("%g " % v for v in [0.20216, 0.5, -1.25e-7, 3.0, 123456.789, -0.0]) + "\n"
 
 
 def test_par_file(tmp_path):

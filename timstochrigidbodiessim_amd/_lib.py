@@ -26,6 +26,7 @@ EXPORTS = [
     "fft_transform", "fft_inverse_transform", "fcm_get_kspace", "fcm_set_kspace",
     "rb_set_geometry", "rb_set_bodies", "rb_set_body_forces", "rb_get_blob_positions",
     "rb_mul_K", "rb_mul_KT", "ll_barrier_forces", "rb_end_of_step_update", "rb_get_bodies",
+    "fcm_shear_field", "fcm_shear_interp", "fcm_x_velocity_profile",
     "rb_channel_wall_overlap", "rb_reset_to_start_of_step", "fcm_em_step",
     "gmres_assemble_rhs", "gmres_saddle_matvec", "gmres_apply_Pinv", "gmres_solve",
     "gmres_get_solution", "fcm_mobility_solve", "fcm_mobility_prepare",
@@ -117,6 +118,9 @@ def lib():
         L.rb_end_of_step_update.argtypes = [H, _dp, C.c_double]
         L.rb_get_bodies.argtypes = [H, _dp, _dp]
         L.rb_channel_wall_overlap.argtypes = [H, C.POINTER(C.c_int)]
+        L.fcm_shear_field.argtypes = [H, C.c_double, C.c_double]
+        L.fcm_shear_interp.argtypes = [H, _dp]
+        L.fcm_x_velocity_profile.argtypes = [H, _dp, _dp]
         L.rb_reset_to_start_of_step.argtypes = [H]
         L.fcm_em_step.argtypes = [H, C.c_uint64, C.c_double, C.POINTER(GmresOpts), C.POINTER(GmresReport)]
         L.gmres_assemble_rhs.argtypes = [H, C.c_int]
@@ -335,6 +339,21 @@ class FcmSolver:
     def end_of_step_update(self, V6=None, nu=1.0):
         """Move the bodies by nu*dt*V6 (host [U; Omega] per body, or the device solution)."""
         _chk(self.L.rb_end_of_step_update(self.h, None if V6 is None else _ptr(_f64(V6).ravel()), nu))
+
+    def shear_field(self, scale, wavenumber):
+        """Set the steady shear flow (scale 0: off); its blob velocities join every RHS."""
+        _chk(self.L.fcm_shear_field(self.h, scale, wavenumber))
+
+    def shear_interp(self):
+        V = np.zeros(3 * self.nb)
+        _chk(self.L.fcm_shear_interp(self.h, _ptr(V)))
+        return V
+
+    def x_velocity_profile(self, lam=None):
+        """Mean x velocity per y row of the flow of blob forces lam (None: the last solution) plus the shear flow."""
+        p = np.zeros(self.cfg.ny)
+        _chk(self.L.fcm_x_velocity_profile(self.h, None if lam is None else _ptr(_f64(lam).ravel()), _ptr(p)))
+        return p
 
     def channel_wall_overlap(self):
         """True if a blob is on or beyond a slip-channel wall (always False in a periodic box)."""

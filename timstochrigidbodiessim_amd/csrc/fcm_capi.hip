@@ -294,6 +294,18 @@ int body_forces_dev(fcm_handle h, const double** out) {
   return FCM_OK;
 }
 
+// the shear field's blob velocities at the current positions
+// (calculate_shear_blob_velocities, fcm_fluid_solver.cpp:488-550) when a shear
+// field is set, for the RHS (gmres_solver.cpp:62-64); else null
+int shear_velocities_dev(fcm_handle h, const double** out) {
+  *out = nullptr;
+  if (!h->have_shear) return FCM_OK;
+  if (!h->have_blobs) return fail(FCM_ERR_STATE, "no blob positions set");
+  HIPCHK(interp<double>(h->g, h->bb, h->bd, h->fpshear(), h->Vshear, KTerm{}, h->st));
+  *out = h->Vshear;
+  return FCM_OK;
+}
+
 int setup_blobs_device(fcm_handle h, int nb) {
   h->bd.nb = nb;
   h->bd.Y = h->dY;
@@ -865,8 +877,9 @@ int gmres_assemble_rhs(fcm_handle h, int use_random) {
   if (!h->have_bodies) return fail(FCM_ERR_STATE, "rb_set_bodies first");
   const double* F6 = nullptr;
   if (int rc = body_forces_dev(h, &F6)) return rc;
-  HIPCHK(rhs_assemble(use_random ? h->Vrand : nullptr, F6, h->rhs, h->n3(),
-                      6 * (size_t)h->NB, h->st));
+  const double* Vs = nullptr;
+  if (int rc = shear_velocities_dev(h, &Vs)) return rc;
+  HIPCHK(rhs_assemble(use_random ? h->Vrand : nullptr, Vs, F6, h->rhs, h->n3(), 6 * (size_t)h->NB, h->st));
   h->have_rhs = true;
   return sync_stream(h);
 }
@@ -1148,8 +1161,9 @@ int fcm_mobility_prepare(fcm_handle h, uint64_t seed) {
   }
   const double* F6 = nullptr;
   if (int rc = body_forces_dev(h, &F6)) return rc;
-  HIPCHK(rhs_assemble(fluct ? h->Vrand : nullptr, F6, h->rhs, h->n3(),
-                      6 * (size_t)h->NB, h->st));
+  const double* Vs = nullptr;
+  if (int rc = shear_velocities_dev(h, &Vs)) return rc;
+  HIPCHK(rhs_assemble(fluct ? h->Vrand : nullptr, Vs, F6, h->rhs, h->n3(), 6 * (size_t)h->NB, h->st));
   h->have_rhs = true;
   return FCM_OK;
 }
@@ -1213,6 +1227,63 @@ int rb_end_of_step_update(fcm_handle h, const double* V6, double nu) {
   h->have_prev = true;
   h->have_blob_forces = false;
   h->have_rhs = false;
+  return sync_stream(h);
+}
+
+int fcm_shear_field(fcm_handle h, double shear_force_scale, double shear_wavenumber) {
+  if (int rc = check(h)) return rc;
+  if (shear_force_scale == 0.0) {
+    h->have_shear = false;
+    return FCM_OK;
+  }
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < 3 && e == hipSuccess; ++i)
+    if (!h->ushear[i]) h->ushear[i] = dalloc<c2<double>>(h, h->g.field_words(), e);
+  if (e == hipSuccess && !h->Vshear) h->Vshear = dalloc<double>(h, 3 * (size_t)h->cfg.max_blobs, e);
+  if (e != hipSuccess) return fail(FCM_ERR_NOMEM, "shear field allocation failed");
+  const FieldPtrs<double> fp = h->fp64();
+  HIPCHK(shear_force(h->g, fp, shear_force_scale * h->cfg.kT, h->cfg.a, shear_wavenumber, h->st));
+  HIPCHK(channel_images<double>(h->g, fp, h->st));
+  HIPCHK(stokes_solve<double>(h->g, fp, h->pp, h->tw64, h->st));
+  for (int i = 0; i < 3; ++i)
+    HIPCHK(hipMemcpyAsync(h->ushear[i], h->f64[i], h->g.field_words() * sizeof(double2), hipMemcpyDeviceToDevice,
+                          h->st));
+  h->have_shear = true;
+  h->have_rhs = false;
+  return sync_stream(h);
+}
+
+int fcm_shear_interp(fcm_handle h, double* V) {
+  if (int rc = check(h)) return rc;
+  if (!h->have_shear) return fail(FCM_ERR_STATE, "no shear field (fcm_shear_field)");
+  const double* Vs = nullptr;
+  if (int rc = shear_velocities_dev(h, &Vs)) return rc;
+  if (V) HIPCHK(hipMemcpyAsync(V, Vs, h->n3() * sizeof(double), hipMemcpyDeviceToHost, h->st));
+  return sync_stream(h);
+}
+
+int fcm_x_velocity_profile(fcm_handle h, const double* lambda, double* prof) {
+  if (int rc = check(h)) return rc;
+  if (!prof) return fail(FCM_ERR_ARG, "null argument");
+  if (!h->have_blobs) return fail(FCM_ERR_STATE, "no blob positions set");
+  const double* F = nullptr;
+  if (lambda) {
+    HIPCHK(hipMemcpyAsync(h->tmp, lambda, h->n3() * sizeof(double), hipMemcpyHostToDevice, h->st));
+    F = h->tmp;
+  } else {
+    if (!h->have_sol) return fail(FCM_ERR_STATE, "no solution to take the blob forces from");
+    F = h->sol;
+  }
+  if (!h->d_prof) {
+    hipError_t e = hipSuccess;
+    h->d_prof = dalloc<double>(h, h->g.ny, e);
+    if (e != hipSuccess) return fail(FCM_ERR_NOMEM, "profile buffer");
+  }
+  const FieldPtrs<double> fp = h->fp64();
+  HIPCHK(spread<double>(h->g, h->bb, h->bd, F, fp, h->st));
+  HIPCHK(stokes_solve<double>(h->g, fp, h->pp, h->tw64, h->st));
+  HIPCHK(x_velocity_profile(h->g, fp, h->have_shear ? h->fpshear() : FieldPtrs<double>{}, h->d_prof, h->st));
+  HIPCHK(hipMemcpyAsync(prof, h->d_prof, h->g.ny * sizeof(double), hipMemcpyDeviceToHost, h->st));
   return sync_stream(h);
 }
 

@@ -36,6 +36,12 @@ struct fcm_handle_s {
   c2<double>* f64[3] = {nullptr, nullptr, nullptr};
   c2<float>* f32[3] = {nullptr, nullptr, nullptr};
   c2<double>* stress[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  // steady shear velocity field (produce_shear_velocity_field), allocated by
+  // the first fcm_shear_field; its blob velocities join every RHS while set
+  c2<double>* ushear[3] = {nullptr, nullptr, nullptr};
+  double* Vshear = nullptr;
+  double* d_prof = nullptr;
+  bool have_shear = false;
   bool have_urand = false;
   void* twbuf64 = nullptr;
   void* twbuf32 = nullptr;
@@ -123,6 +129,11 @@ struct fcm_handle_s {
   FieldPtrs<double> fprand() const {
     FieldPtrs<double> p{};
     for (int i = 0; i < 3; ++i) p.p[i] = stress[i];
+    return p;
+  }
+  FieldPtrs<double> fpshear() const {
+    FieldPtrs<double> p{};
+    for (int i = 0; i < 3; ++i) p.p[i] = ushear[i];
     return p;
   }
 };

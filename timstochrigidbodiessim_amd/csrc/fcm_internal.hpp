@@ -195,6 +195,13 @@ hipError_t bodies_update(const GridGeom& g, BodyData& b, hipStream_t st);
 // *flag = 1 if any blob sits on the lower wall (y == 0) or at or beyond the
 // upper one (y >= half_ly), else 0
 hipError_t wall_overlap(const double* Y, int nb, double half_ly, int* flag, hipStream_t st);
+// the shear forcing fx = (scale kT) sin(2 pi kappa y / ny) / a on every real node,
+// fy = fz = 0 (fcm_fluid_solver.cpp:705-729)
+hipError_t shear_force(const GridGeom& g, const FieldPtrs<double>& fp, double scale_kT, double a, double kappa,
+                       hipStream_t st);
+// prof[y] = mean over x and z of u[0] (+ add[0] if set) on every y row of real fields
+hipError_t x_velocity_profile(const GridGeom& g, const FieldPtrs<double>& u, const FieldPtrs<double>& add,
+                              double* prof, hipStream_t st);
 // end_of_step_update (collection_of_rigid_bodies.cpp:759-790): body n moves by
 // s [U; Omega] (V6 with stride vstride doubles between bodies' 6-vectors)
 hipError_t bodies_step(const GridGeom& g, double* x, double* q, int NB, const double* V6, double s, hipStream_t st);
@@ -255,7 +262,9 @@ hipError_t combine(const double* Q, size_t ldq, int ncols, const double* c, doub
 hipError_t scale_vec(const double* x, double* y, size_t n, const double* a, bool invsqrt, hipStream_t st);
 hipError_t copy16(const void* src, void* dst, size_t bytes, hipStream_t st);
 hipError_t axpby(double a, const double* x, double b, double* y, size_t n, hipStream_t st);  // y = a x + b y
-hipError_t rhs_assemble(const double* Vrand, const double* F6, double* b, size_t n3, size_t n6, hipStream_t st);
+// b = [-Vrand - Vshear; -F6] (gmres_solver.cpp:58-70); null inputs count as zero
+hipError_t rhs_assemble(const double* Vrand, const double* Vshear, const double* F6, double* b, size_t n3, size_t n6,
+                        hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // device-resident GMRES control (fcm_gmres.hip)

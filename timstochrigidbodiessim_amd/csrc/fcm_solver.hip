@@ -1,5 +1,7 @@
 // fcm_solver.hip -- rigid-body operators, the block-diagonal preconditioner
 // and the Krylov vector kernels of the saddle-point solve (FP64 throughout).
+#include <algorithm>
+
 #include "fcm_internal.hpp"
 
 namespace fcm {
@@ -613,11 +615,16 @@ __global__ void axpby_kernel(double a, const double* x, double b, double* y, siz
   if (r < n) y[r] = (a == 0.0 ? 0.0 : a * x[r]) + (b == 0.0 ? 0.0 : b * y[r]);
 }
 
-__global__ void rhs_kernel(const double* __restrict__ Vrand, const double* __restrict__ F6, double* __restrict__ b,
-                           size_t n3, size_t n6) {
+__global__ void rhs_kernel(const double* __restrict__ Vrand, const double* __restrict__ Vshear,
+                           const double* __restrict__ F6, double* __restrict__ b, size_t n3, size_t n6) {
   const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < n3) b[r] = Vrand ? -Vrand[r] : 0.0;
-  else if (r < n3 + n6) b[r] = F6 ? -F6[r - n3] : 0.0;
+  if (r < n3) {
+    double v = Vrand ? -Vrand[r] : 0.0;
+    if (Vshear) v -= Vshear[r];  // RHS.subvec(0, 3 Nb - 1) -= fcm.Vshear (gmres_solver.cpp:62-64)
+    b[r] = v;
+  } else if (r < n3 + n6) {
+    b[r] = F6 ? -F6[r - n3] : 0.0;
+  }
 }
 
 hipError_t dots(const double* Q, size_t ldq, int ncols, const double* w, size_t n, KrylovWork& kw, double* red_out,
@@ -685,9 +692,68 @@ hipError_t copy16(const void* src, void* dst, size_t bytes, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t rhs_assemble(const double* Vrand, const double* F6, double* b, size_t n3, size_t n6, hipStream_t st) {
+hipError_t rhs_assemble(const double* Vrand, const double* Vshear, const double* F6, double* b, size_t n3, size_t n6,
+                        hipStream_t st) {
   const size_t n = n3 + n6;
-  hipLaunchKernelGGL(rhs_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, Vrand, F6, b, n3, n6);
+  hipLaunchKernelGGL(rhs_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, Vrand, Vshear, F6, b, n3, n6);
+  return hipGetLastError();
+}
+
+// produce_shear_velocity_field's forcing (fcm_fluid_solver.cpp:705-729):
+// fx = amp sin(kappa 2 pi y / ny) on the nz real nodes of each row, the padding
+// and fy, fz zero; amp = SHEAR_FORCE_SCALE kT / a is applied as the reference
+// orders it, (scale kT) sin(...) / a.  One thread per real word.
+__global__ void shear_force_kernel(FieldPtrs<double> fp, GridGeom g, double sk, double a, double kappa) {
+  constexpr double PI = 3.14159265358979323846264338327950288;  // config.hpp:86
+  const size_t rw = 2 * (size_t)g.cpitch;  // real words per row
+  const size_t n = (size_t)g.nx * g.ny * rw;
+  for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < n; w += (size_t)gridDim.x * blockDim.x) {
+    const int k = (int)(w % rw);
+    const int y = (int)((w / rw) % g.ny);
+    double* fx = reinterpret_cast<double*>(fp.p[0]);
+    double* fy = reinterpret_cast<double*>(fp.p[1]);
+    double* fz = reinterpret_cast<double*>(fp.p[2]);
+    fx[w] = k < g.nz ? sk * sin(kappa * 2.0 * PI * (double)y / (double)g.ny) / a : 0.0;
+    fy[w] = 0.0;
+    fz[w] = 0.0;
+  }
+}
+
+hipError_t shear_force(const GridGeom& g, const FieldPtrs<double>& fp, double amp_sk, double a, double kappa,
+                       hipStream_t st) {
+  const size_t n = (size_t)g.nx * g.ny * 2 * g.cpitch;
+  const int blocks = (int)std::min<size_t>((n + 255) / 256, 16384);
+  hipLaunchKernelGGL(shear_force_kernel, dim3(blocks), dim3(256), 0, st, fp, g, amp_sk, a, kappa);
+  return hipGetLastError();
+}
+
+// main.cpp:371-433: one workgroup per y row sums u_x (+ the shear u_x) over x
+// and the nz real nodes of z, then divides by nx nz
+__global__ void __launch_bounds__(256) x_profile_kernel(FieldPtrs<double> u, FieldPtrs<double> add, GridGeom g,
+                                                        double* __restrict__ prof) {
+  __shared__ double red[256];
+  const int y = blockIdx.x;
+  const size_t rw = 2 * (size_t)g.cpitch;
+  const double* ux = reinterpret_cast<const double*>(u.p[0]);
+  const double* us = reinterpret_cast<const double*>(add.p[0]);
+  double s = 0.0;
+  for (int w = threadIdx.x; w < g.nx * g.nz; w += blockDim.x) {
+    const int x = w / g.nz, k = w - (w / g.nz) * g.nz;
+    const size_t i = ((size_t)x * g.ny + y) * rw + k;
+    s += ux[i] + (us ? us[i] : 0.0);
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) prof[y] = red[0] / (double)(g.nx * g.nz);
+}
+
+hipError_t x_velocity_profile(const GridGeom& g, const FieldPtrs<double>& u, const FieldPtrs<double>& add,
+                              double* prof, hipStream_t st) {
+  hipLaunchKernelGGL(x_profile_kernel, dim3(g.ny), dim3(256), 0, st, u, add, g, prof);
   return hipGetLastError();
 }
 

@@ -168,6 +168,21 @@ void fcm_fluid_solver::calculate_random_blob_velocities() {
   check(fcm_random_interp(h_, V.data()));
 }
 
+void fcm_fluid_solver::produce_shear_velocity_field(double shear_force_scale, double shear_wavenumber) {
+  check(fcm_shear_field(h_, shear_force_scale, shear_wavenumber));
+}
+
+void fcm_fluid_solver::calculate_shear_blob_velocities() {
+  Vshear.resize(3 * (size_t)num_blobs_);
+  check(fcm_shear_interp(h_, Vshear.data()));
+}
+
+std::vector<double> fcm_fluid_solver::x_velocity_profile(const std::vector<double>& lambda) {
+  std::vector<double> p(cfg_.ny);
+  check(fcm_x_velocity_profile(h_, lambda.data(), p.data()));
+  return p;
+}
+
 void fcm_fluid_solver::produce_reference_matrices(collection_of_rigid_bodies& all_bodies) {
   const size_t n3 = 3 * (size_t)all_bodies.blobs_per_body;
   ref_Minv.assign(n3 * n3, 0.0);
@@ -363,6 +378,14 @@ void write_par_file(const std::string& name, const run_parameters& p) {
   }
 }
 
+void append_fluid_x_velocity(const std::string& name, double t, const std::vector<double>& profile) {
+  std::ofstream f(name + "_fluid_x_velocity_in_y.dat", std::ios::app);
+  if (!f) throw fcm_error(FCM_ERR_ARG, "cannot open " + name + "_fluid_x_velocity_in_y.dat");
+  f << t << ' ';
+  for (double u : profile) f << u << ' ';
+  f << '\n';
+}
+
 void append_velocities(const std::string& name, double t, const std::vector<double>& soln, int total_num_blobs,
                        int num_bodies) {
   const size_t off = 3 * (size_t)total_num_blobs, nv = 6 * (size_t)num_bodies;
@@ -385,6 +408,8 @@ void gmres_solver::assemble_RHS(const collection_of_rigid_bodies& all_bodies, co
   const std::vector<double> KTf = all_bodies.multiply_by_KT_at_midstep(all_bodies.blob_forces);
   const size_t n3 = 3 * (size_t)num_blobs;
   for (size_t i = 0; i < n3; ++i) RHS[i] = -fcm.V[i];
+  if (fcm.Vshear.size() == n3)  // SHEARING_FORCES (gmres_solver.cpp:62-64)
+    for (size_t i = 0; i < n3; ++i) RHS[i] -= fcm.Vshear[i];
   for (size_t i = 0; i < KTf.size(); ++i) RHS[n3 + i] = -(KTf[i] + all_bodies.body_forces[i]);
 }
 

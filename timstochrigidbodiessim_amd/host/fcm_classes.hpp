@@ -93,6 +93,7 @@ class collection_of_rigid_bodies;
 class fcm_fluid_solver {
  public:
   std::vector<double> V;         // blob velocities of the last interpolation (3 Nb)
+  std::vector<double> Vshear;    // shear-flow blob velocities (SHEARING_FORCES), 3 Nb
   std::vector<double> ref_Minv;  // (3 n_b)^2, row-major
   std::vector<double> ref_N;     // 6 x 6
   fft_solver fft;
@@ -111,6 +112,13 @@ class fcm_fluid_solver {
   void produce_random_velocity_field(uint64_t seed);
   void calculate_random_blob_velocities();
   void produce_reference_matrices(collection_of_rigid_bodies& all_bodies);
+  // SHEARING_FORCES (fcm_fluid_solver.cpp:488-550, 703-840): the steady shear
+  // flow once, then its blob velocities at the current positions each step
+  void produce_shear_velocity_field(double shear_force_scale, double shear_wavenumber);
+  void calculate_shear_blob_velocities();
+  // main.cpp:371-433: mean x velocity per y row of the flow of the blob forces
+  // lambda plus the shear flow
+  std::vector<double> x_velocity_profile(const std::vector<double>& lambda);
 
   // real-space field `which` (0..2 ux,uy,uz / fx,fy,fz after a spread; 3..5 the
   // random velocity) in the reference layout [nx][ny][pad]
@@ -203,6 +211,9 @@ void write_par_file(const std::string& name, const run_parameters& p);
 // each followed by a space) to <name>_velocities.dat (main.cpp:343-367)
 void append_velocities(const std::string& name, double t, const std::vector<double>& soln, int total_num_blobs,
                        int num_bodies);
+// appends "t u_0 ... u_{ny-1} " in the stream's default format to
+// <name>_fluid_x_velocity_in_y.dat (main.cpp:371-433)
+void append_fluid_x_velocity(const std::string& name, double t, const std::vector<double>& profile);
 
 // ---- gmres_solver.hpp ----------------------------------------------------------------
 class gmres_solver {

@@ -10,7 +10,8 @@
 //       every step's GMRES solution and the final body state
 //   fcm_host_check io <name> <state file> <sample>
 //       writes <name>.dat (two saves), <name>_blob_forces.dat,
-//       <name>_blob_references.dat, <name>.par and <name>_velocities.dat for
+//       <name>_blob_references.dat, <name>.par, <name>_velocities.dat and
+//       <name>_fluid_x_velocity_in_y.dat for
 //       three two-sphere bodies with fixed values (no GPU needed), then reads
 //       state <sample> (0: the last) of <state file> into the three bodies and
 //       prints them, one body per line, "%.17g"
@@ -194,6 +195,7 @@ int run_io(const std::string& name, const std::string& state_file, int sample) {
   std::vector<double> soln(3 * 6 + 6 * 3);
   for (size_t i = 0; i < soln.size(); ++i) soln[i] = 1.0 / (1.0 + (double)i);
   append_velocities(name, 0.20216, soln, 6, 3);
+  append_fluid_x_velocity(name, 0.20216, {0.5, -1.25e-7, 3.0, 123456.789, -0.0});
   const int n = c.read_initial_conditions(state_file, sample);
   std::printf("states %d\n", n);
   for (const particle& b : c.bodies)
