@@ -230,16 +230,28 @@ def bench_serve(args):
 def bench_timestep(args):
     """Config-5 time steps: 512^3 grid, 150,000 rods of 12 spheres (1.8M
     blobs), kT = 1, steric cutoff 2.2a, Euler-Maruyama steps each with a full
-    fluctuating mobility solve (FP32+refine, tol 1e-8) and the body update."""
+    fluctuating mobility solve (FP32+refine, tol 1e-8) and the body update.
+    With --slip-channel: walls at y = 0 and y = grid/2, half the rods (the same
+    volume fraction), wall springs, and a step that leaves a blob on or beyond a
+    wall is undone and redone with a new seed (main.cpp:304-326)."""
     import timstochrigidbodiessim_amd as fa
+    chan = args.slip_channel
+    if args.ts_rods is None:
+        args.ts_rods = 75000 if chan else 150000
     n, NB, nbpb, a = args.ts_grid, args.ts_rods, 12, 1.86
+    L, margin = [float(n)] * 3, 0.0
+    if chan:
+        margin = a * (nbpb - 1) + 0.5  # every sphere between the walls
+        L[1] = n / 2 - 2 * margin
     t0 = time.perf_counter()
-    ref, x, q, _, nov = fa.synth_rods(5, NB, nbpb, a, [float(n)] * 3, max_attempts=args.place_attempts)
+    ref, x, q, _, nov = fa.synth_rods(5, NB, nbpb, a, L, max_attempts=args.place_attempts)
+    x[:, 1] += margin
     nb = NB * nbpb
-    phi = nb * 4.0 / 3.0 * np.pi * a ** 3 / n ** 3
+    phi = nb * 4.0 / 3.0 * np.pi * a ** 3 / (n ** 3 / (2 if chan else 1))
     log(f"[bench] timestep geometry {time.perf_counter() - t0:.1f} s: {NB} rods, phi {phi:.3f}, {nov} overlapping")
     cfg = fa.make_config(n, ng=7, dx=1.0, a=a, mu=1.0, kT=1.0, dt=args.dt, max_blobs=nb, max_bodies=NB,
-                         max_blobs_per_body=nbpb, max_krylov=args.max_iter, fp32=True, random_field=True)
+                         max_blobs_per_body=nbpb, max_krylov=args.max_iter, fp32=True, random_field=True,
+                         slip_channel=chan)
     log(f"[bench] footprint {fa.footprint(cfg) / 1e9:.1f} GB")
     s = fa.FcmSolver(cfg)
     s.set_geometry(ref)
@@ -247,10 +259,13 @@ def bench_timestep(args):
     s.set_bodies(x, q)
     s.set_body_forces(np.zeros((NB, 6)))
     cutoff = 2.2 * a
-    times, reps = [], []
+    times, reps, redone = [], [], 0
     for k in range(args.warmup + args.steps):
         t1 = time.perf_counter()
         rep = s.em_step(1 + k, cutoff, tol=args.tol, max_iter=args.max_iter, precision=fa.FCM_PREC_FP32)
+        if chan and s.channel_wall_overlap():  # the step is undone and, as in main.cpp, taken again
+            s.reset_to_start_of_step()
+            redone += 1
         dt = time.perf_counter() - t1
         log(f"[bench] step {k}: {dt:.3f} s {rep}")
         if k >= args.warmup:
@@ -259,13 +274,16 @@ def bench_timestep(args):
     s.close()
     ms = 1e3 * statistics.mean(times)
     line = {
-        "metric": "Brownian time steps/s at 512^3 with 1.8M spheres (FP32+refine GMRES at 1e-8, sterics, EM)",
+        "metric": "Brownian time steps/s at 512^3 with 1.8M spheres (FP32+refine GMRES at 1e-8, sterics, EM)"
+                  + (" (slip channel)" if chan else ""),
         "value": round(1e3 / ms, 4), "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms, 1), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 grid passes + f64 Krylov/refinement/constraints/quaternions",
         "data": "synthetic (seeded rods and stress stream)",
         "config": {"workload": f"{n}^3 grid, {NB} rods x {nbpb} spheres ({nb} blobs), a=1.86, kT=1, steric "
-                               f"cutoff 2.2a, Euler-Maruyama, dt {args.dt:.5f}, volume fraction {phi:.3f}",
+                               f"cutoff 2.2a, Euler-Maruyama, dt {args.dt:.5f}, volume fraction {phi:.3f}"
+                               + (f", slip channel (walls at y = 0 and {n // 2}), {redone} step(s) undone for a "
+                                  f"wall overlap" if chan else ""),
                    "parallelism": "single GPU"},
         "iterations": [r["iterations"] for r in reps], "status": sorted(set(r["status"] for r in reps)),
     }
@@ -279,7 +297,8 @@ def main():
                          "batch; serve: a Poisson stream of requests with continuous batching; timestep: "
                          "Euler-Maruyama steps at 512^3 with 1.8M spheres")
     ap.add_argument("--ts-grid", type=int, default=512)
-    ap.add_argument("--ts-rods", type=int, default=150000)
+    ap.add_argument("--ts-rods", type=int, default=None, help="rods of the timestep workload: 150000, or 75000 "
+                    "with --slip-channel")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--steps-per-round", type=int, default=4)
@@ -305,7 +324,8 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--slip-channel", action="store_true",
-                    help="solve256 in the slip-channel geometry: walls at y = 0 and y = grid/2, rods inside")
+                    help="solve256 / timestep in the slip-channel geometry: walls at y = 0 and y = grid/2, rods "
+                         "inside")
     args = ap.parse_args()
 
     if args.max_iter is None:

@@ -58,22 +58,40 @@ __device__ __forceinline__ unsigned long long sm_mix(unsigned long long z) {
   return z ^ (z >> 31);
 }
 
-// One N(0,1) draw for (component, node): SplitMix64 stream keyed by the seed,
-// read at position (comp*nnodes + node)*64 + t, Marsaglia polar method, first
-// variate (parallel_random_numbers.cpp:8-19).
-__device__ double stress_normal(unsigned long long seed, int comp, unsigned long long node,
-                                unsigned long long nnodes) {
+// The N(0,1) draws of (component, node) and (component, node + 1): for each,
+// the SplitMix64 stream keyed by the seed is read at position
+// (comp*nnodes + node)*64 + t, Marsaglia polar method, first variate, at most
+// 32 tries, else 0 (parallel_random_numbers.cpp:8-19).  One rejection loop
+// serves both draws (the second starts when the first is accepted), so a wave
+// runs as many passes as its busiest lane needs for the pair, and the
+// transform x1 sqrt(-2 ln w / w) runs once per draw after the loop instead of
+// inside it under every lane's acceptance.  Same values as one draw at a time.
+__device__ void stress_normal_pair(unsigned long long seed, int comp, unsigned long long node,
+                                   unsigned long long nnodes, double& a, double& b) {
   const unsigned long long G = 0x9E3779B97F4A7C15ULL;
   const unsigned long long key = sm_mix(seed + G);
-  const unsigned long long base = ((unsigned long long)comp * nnodes + node) * 64ULL;
-  for (int t = 0; t < 32; ++t) {
+  const unsigned long long base0 = ((unsigned long long)comp * nnodes + node) * 64ULL;
+  double xa = 0.0, wa = 0.0, xb = 0.0, wb = 0.0;  // w = 0: no acceptance (the draw is 0)
+  bool second = false;
+  int t = 0;
+  for (int pass = 0; pass < 64; ++pass) {
+    const unsigned long long base = second ? base0 + 64ULL : base0;
     const double u1 = (double)(sm_mix(key + G * (base + 2ULL * t + 1ULL)) >> 11) * (1.0 / 9007199254740992.0);
     const double u2 = (double)(sm_mix(key + G * (base + 2ULL * t + 2ULL)) >> 11) * (1.0 / 9007199254740992.0);
     const double x1 = 2.0 * u1 - 1.0, x2 = 2.0 * u2 - 1.0;
     const double w = x1 * x1 + x2 * x2;
-    if (w < 1.0 && w > 0.0) return x1 * sqrt((-2.0 * log(w)) / w);
+    const bool ok = w < 1.0 && w > 0.0;
+    if (!second) {
+      if (ok) { xa = x1; wa = w; }
+      if (ok || t == 31) { second = true; t = 0; continue; }
+    } else {
+      if (ok) { xb = x1; wb = w; }
+      if (ok || t == 31) break;
+    }
+    ++t;
   }
-  return 0.0;
+  a = wa > 0.0 ? xa * sqrt((-2.0 * log(wa)) / wa) : 0.0;
+  b = wb > 0.0 ? xb * sqrt((-2.0 * log(wb)) / wb) : 0.0;
 }
 
 template <typename T, int M, bool GEN>
@@ -117,8 +135,10 @@ __global__ void __launch_bounds__(256) zfwd_kernel(FieldPtrs<T> fp, int rows, in
     for (int s = 0; s < P; ++s) {
       const int e = fft_in_index<M, false>(tcol, s);
       const unsigned long long node = (unsigned long long)srow * nz + 2 * e;
-      const double a = draw ? sc * stress_normal(sg.seed, comp, node, nnodes) : 0.0;
-      const double b = draw ? sc * stress_normal(sg.seed, comp, node + 1, nnodes) : 0.0;
+      double na = 0.0, nb = 0.0;
+      if (draw) stress_normal_pair(sg.seed, comp, node, nnodes, na, nb);
+      const double a = draw ? sc * na : 0.0;
+      const double b = draw ? sc * nb : 0.0;
       v[s] = mk<T>(T(a), T(b));
     }
   } else {
