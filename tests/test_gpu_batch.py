@@ -118,3 +118,43 @@ def test_batch_zero_rhs_and_iteration_cap():
     assert reps[id(s2)]["status"] == "max_iter" and reps[id(s2)]["iterations"] == 7
     s1.close()
     s2.close()
+
+
+def make_channel_instance(seed, n, NB, nbpb, max_krylov=200):
+    """Rods with every sphere between slip walls at y = 0 and y = n/2."""
+    cfg = fa.make_config(n, ng=7, dx=1.0, a=1.86, mu=1.0, kT=1.0, dt=0.20216, max_blobs=NB * nbpb, max_bodies=NB,
+                         max_blobs_per_body=nbpb, max_krylov=max_krylov, fp32=True, random_field=True,
+                         slip_channel=True)
+    margin = 1.86 * (nbpb - 1) + 0.5
+    ref, x, q, F6, _ = fa.synth_rods(seed, NB, nbpb, 1.86, [float(n), n / 2 - 2 * margin, float(n)], max_attempts=50)
+    x[:, 1] += margin
+    s = fa.FcmSolver(cfg)
+    s.set_geometry(ref)
+    _, Minv, N = s.ref_matrices()
+    s.set_bodies(x, q)
+    s.set_ref_matrices(Minv, N)
+    s.set_body_forces(F6)
+    return s
+
+
+def test_batch_mixes_channel_and_periodic_instances():
+    """A batch advances each handle with its own geometry: slip-channel and
+    periodic instances together give bitwise their solves alone."""
+    say("batch, channel and periodic")
+    insts = [(make_channel_instance(51, 64, 30, 6), fa.FCM_PREC_FP32), (make_instance(52, 32, 20, 6), fa.FCM_PREC_FP64),
+             (make_channel_instance(53, 48, 20, 4), fa.FCM_PREC_FP64), (make_instance(54, 48, 40, 8), fa.FCM_PREC_FP32)]
+    alone = _alone(insts)
+    for sol, rep in alone:
+        assert rep["status"] == "converged", rep
+    with fa.FcmBatch(max_jobs=8, worker_streams=2, steps_per_round=3) as b:
+        for i, (s, prec) in enumerate(insts):
+            s.mobility_prepare(100 + i)
+            b.submit(s, tol=1e-8, max_iter=200, precision=prec)
+        b.run()
+        done = b.take()
+    by_solver = {id(s): rep for s, rep in done}
+    for (s, prec), (sol, rep) in zip(insts, alone):
+        assert by_solver[id(s)]["iterations"] == rep["iterations"]
+        assert np.array_equal(s.solution(), sol)
+    for s, _ in insts:
+        s.close()
