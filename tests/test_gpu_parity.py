@@ -208,6 +208,34 @@ def test_ref_matrices_and_Pinv_match_oracle():
         assert rel(s.saddle_matvec(v), po.saddle_matvec(o, b, v)) < FP64_TOL
 
 
+@pytest.mark.parametrize("nbpb,NB", [(1, 40), (2, 30), (3, 25), (5, 12), (24, 5), (30, 3)])
+def test_Pinv_every_body_size_matches_oracle(nbpb, NB):
+    """P^-1 on bodies of 1 to 30 spheres: the packed kernel (2 to 24 spheres,
+    whose row threads also form the six body rows) and the one-workgroup-per-body
+    kernel (1 sphere, whose three rows could not, and more than 24)."""
+    say("Pinv nbpb", nbpb, NB)
+    n = 48
+    gc, oc = pair_cfg(n, max_blobs=NB * nbpb, max_bodies=NB, max_blobs_per_body=nbpb)
+    # spheres 2a apart on a 3 x 2 x k lattice: compact and well conditioned
+    i = np.arange(nbpb)
+    ref = 2 * 1.86 * np.stack([i % 3, (i // 3) % 2, i // 6], axis=1).astype(float)
+    ref -= ref.mean(axis=0)
+    o = po.Fcm(oc, NB * nbpb)
+    st, M, Minv, N = o.ref_matrices(ref)
+    rng = np.random.default_rng(nbpb)
+    x = rng.uniform(0, n, (NB, 3))
+    q = random_quats(rng, NB)
+    with fa.FcmSolver(gc) as s:
+        s.set_geometry(ref)
+        s.set_bodies(x, q)
+        s.set_ref_matrices(Minv, N)
+        b = po.Bodies(ref, x, q, Minv, N)
+        v = rng.standard_normal(b.n)
+        assert rel(s.apply_Pinv(v), po.apply_Pinv(b, v)) < 1e-12, rel(s.apply_Pinv(v), po.apply_Pinv(b, v))
+        o.set_up_gaussians(po.fill_blob_pos(oc, ref, x, q))
+        assert rel(s.saddle_matvec(v), po.saddle_matvec(o, b, v)) < FP64_TOL
+
+
 @pytest.mark.parametrize("dims,ng,nb", [((32, 24, 16), 7, 200), ((64, 64, 64), 7, 1000), ((16, 8, 8), 7, 30),
                                         ((48, 32, 96), 7, 300), ((32, 16, 16), 8, 80), ((24, 24, 48), 5, 100),
                                         ((16, 16, 16), 7, 60), ((32, 48, 32), 8, 400)])

@@ -190,7 +190,8 @@ __global__ void __launch_bounds__(128) pinv_kernel(BodyData b, const double* __r
 // body read consecutive words; copied from the transposed matrix kept in HBM,
 // a contiguous read) and ref_N are staged in LDS once.  Every sum
 // runs over k in the order of pinv_kernel, so the results are the same bit for
-// bit.  Used when 3 n_b <= 72 (the LDS stays under 64 KiB).
+// bit.  Used when 6 <= 3 n_b <= 72 (the threads of a body's rows also compute
+// its six body rows, and the LDS stays under 64 KiB).
 __global__ void __launch_bounds__(256) pinv_pack_kernel(BodyData b, const double* __restrict__ in,
                                                         double* __restrict__ out, double* __restrict__ wKT) {
   extern __shared__ double sm[];
@@ -319,7 +320,8 @@ __global__ void __launch_bounds__(256) pinv_pack_kernel(BodyData b, const double
 hipError_t apply_pinv(const BodyData& b, const double* in, double* out, double* wKT, hipStream_t st) {
   if (b.NB == 0) return hipSuccess;
   const int n3 = 3 * b.nbpb;
-  if (n3 <= 72) {
+  // the packed kernel's rows double as the six body rows: it needs 3 n_b >= 6
+  if (n3 >= 6 && n3 <= 72) {
     const int BP = 256 / n3;
     const size_t lds = sizeof(double) * ((size_t)n3 * n3 + 36 + n3 + (size_t)BP * (4 * n3 + 12));
     const int groups = (b.NB + BP - 1) / BP;
