@@ -29,13 +29,15 @@ def test_host_math_classes():
     assert "FAIL" not in r.stdout
 
 
-def _write_problem(path, n, NB, nbpb, steps, prec, seed, a, kT, dt, tol, cutoff, ref, x, q, F6):
+def _write_problem(path, n, NB, nbpb, steps, prec, seed, a, kT, dt, tol, cutoff, ref, x, q, F6, channel=None):
     with open(path, "wb") as f:
         f.write(struct.pack("<5i", n, NB, nbpb, steps, prec))
         f.write(struct.pack("<Q", seed))
         f.write(struct.pack("<5d", a, kT, dt, tol, cutoff))
         for arr in (ref, x, q, F6):
             f.write(np.ascontiguousarray(arr, dtype="<f8").tobytes())
+        if channel is not None:  # (slip_channel, shear_force_scale, shear_wavenumber)
+            f.write(struct.pack("<i2d", *channel))
 
 
 def _read_result(path, nsys, NB, steps):
@@ -90,6 +92,59 @@ def test_host_classes_time_steps(tmp_path):
         s.set_body_forces(F6)
         for k in range(steps):
             rep = s.em_step(seed + k, cutoff, tol=tol, max_iter=200)
+            assert rep["iterations"] == its[k]
+        xd, qd = s.bodies()
+    assert np.array_equal(xd, xh) and np.array_equal(qd, qh)
+
+
+@pytest.mark.gpu
+def test_host_classes_channel_shear_time_steps(tmp_path):
+    """The host-class time loop in a slip channel with shear forcing (main.cpp:
+    98-102, 270-274, 304-326): first solve against the oracle, every step
+    bitwise equal to the device-resident step with the same shear field."""
+    _need_exe()
+    n, NB, nbpb, steps, seed, a, kT, dt, tol, cutoff = 32, 8, 3, 3, 91, 1.86, 1.0, 0.20216, 1e-10, 2.2 * 1.86
+    scale, kappa = 2.0, 1.0
+    margin = a * (nbpb - 1) + 0.5
+    ref, x, q, F6, _ = fa.synth_rods(6, NB, nbpb, a, [float(n), n / 2 - 2 * margin, float(n)], max_attempts=20)
+    x[:, 1] += margin
+    inp, out = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    _write_problem(inp, n, NB, nbpb, steps, fa.FCM_PREC_FP64, seed, a, kT, dt, tol, cutoff, ref, x, q, F6,
+                   channel=(1, scale, kappa))
+    r = subprocess.run([EXE, "steps", inp, out], capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    assert "redone" not in r.stdout  # the rods stay clear of the walls over these steps
+    nb = NB * nbpb
+    sols, its, xh, qh = _read_result(out, 3 * nb + 6 * NB, NB, steps)
+    oc = po.cfg(n, ng=7, dx=1.0, a=a, mu=1.0, kT=kT, dt=dt, slip_channel=True)
+    o = po.Fcm(oc, nb)
+    st, M, Minv, N = o.ref_matrices(ref)
+    o.shear_field(scale, kappa)
+    Y = po.fill_blob_pos(oc, ref, x, q)
+    o.set_up_gaussians(Y)
+    o.fill_stress(seed)
+    o.random_field()
+    Vr = o.random_interp()
+    Vs = o.shear_interp()
+    f = po.barrier_forces(oc, Y, nbpb, cutoff)
+    rhs = np.concatenate([-Vr - Vs, -(po.mul_KT(ref, q, f) + F6.ravel())])
+    st, sol, it, _ = po.gmres(o, po.Bodies(ref, x, q, Minv, N), rhs, tol=tol, max_iter=200)
+    o.close()
+    err = np.linalg.norm(sols[0] - sol) / np.linalg.norm(sol)
+    print("host-class first channel+shear solve vs oracle", err, "iterations", its[0], it)
+    assert err < 1e-10
+    cfg = fa.make_config(n, ng=7, dx=1.0, a=a, mu=1.0, kT=kT, dt=dt, max_blobs=nb, max_bodies=NB,
+                         max_blobs_per_body=nbpb, max_krylov=200, fp32=False, random_field=True, slip_channel=True)
+    with fa.FcmSolver(cfg) as s:
+        s.set_geometry(ref)
+        s.ref_matrices()
+        s.shear_field(scale, kappa)
+        s.set_bodies(x, q)
+        s.set_body_forces(F6)
+        for k in range(steps):
+            rep = s.em_step(seed + k, cutoff, tol=tol, max_iter=200)
+            assert not s.channel_wall_overlap()
             assert rep["iterations"] == its[k]
         xd, qd = s.bodies()
     assert np.array_equal(xd, xh) and np.array_equal(qd, qh)

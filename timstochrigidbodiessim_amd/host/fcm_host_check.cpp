@@ -17,7 +17,8 @@
 //       prints them, one body per line, "%.17g"
 //
 // in.bin (little-endian): int32 n, NB, nbpb, steps, precision; uint64 seed;
-// float64 a, kT, dt, tol, cutoff; float64 ref[3 nbpb], x[3 NB], q[4 NB], F6[6 NB].
+// float64 a, kT, dt, tol, cutoff; float64 ref[3 nbpb], x[3 NB], q[4 NB], F6[6 NB];
+// optionally int32 slip_channel, float64 shear_force_scale, shear_wavenumber.
 // out.bin: per step float64 soln[3 Nb + 6 NB] and int32 iterations; then
 // float64 x[3 NB], q[4 NB].
 #include <cmath>
@@ -116,10 +117,14 @@ int run_steps(const char* in_path, const char* out_path) {
   if (!rd(f, ref.data(), ref.size()) || !rd(f, x.data(), x.size()) || !rd(f, q.data(), q.size()) ||
       !rd(f, F6.data(), F6.size()))
     return 2;
+  // optional trailer: int32 slip_channel; float64 shear_force_scale, shear_wavenumber
+  int chan = 0;
+  double shear[2] = {0.0, 0.0};
+  if (rd(f, &chan, 1) && !rd(f, shear, 2)) return 2;
   std::fclose(f);
   const int nb = NB * nbpb;
 
-  fcm_config cfg{n, n, n, 7, 1.0, a, 1.0, kT, dt, nb, NB, nbpb, 200, precision == FCM_PREC_FP32, 1};
+  fcm_config cfg{n, n, n, 7, 1.0, a, 1.0, kT, dt, nb, NB, nbpb, 200, precision == FCM_PREC_FP32, 1, chan};
   fcm_fluid_solver fcm;
   fcm.initialise(cfg, nb);
   std::vector<particle> init(NB);
@@ -131,28 +136,38 @@ int run_steps(const char* in_path, const char* out_path) {
   all_bodies.initialise(fcm, ref, init, cutoff);
   all_bodies.body_forces = F6;
   fcm.produce_reference_matrices(all_bodies);
+  if (shear[0] != 0.0) fcm.produce_shear_velocity_field(shear[0], shear[1]);  // main.cpp:98-102
   gmres_solver gmres;
   gmres.initialise(nb, NB, gmres_opts{tol, 200, precision, 0.0, 0});
 
   std::FILE* o = std::fopen(out_path, "wb");
   if (!o) return 2;
   double t = 0;
-  for (int step = 0; step < steps; ++step) {
-    // main.cpp:230-233
+  int redone = 0;
+  for (int step = 0; step < steps;) {
+    // main.cpp:230-233; a redone step draws from a seed of its own
+    const uint64_t s = seed + step + 1000 * (uint64_t)redone;
     all_bodies.fill_all_blob_pos();
-    fcm.produce_random_velocity_field(seed + step);
+    fcm.produce_random_velocity_field(s);
     fcm.set_up_gaussians(all_bodies.all_blob_pos);
     fcm.calculate_random_blob_velocities();
+    if (shear[0] != 0.0) fcm.calculate_shear_blob_velocities();  // main.cpp:270-274
     // main.cpp:285, 305-310
     all_bodies.calculate_forces();
     gmres.assemble_RHS(all_bodies, fcm);
     std::vector<double> soln = gmres.invert_saddle_point_system(all_bodies, fcm, 0, t);
+    std::vector<double> V(soln.begin() + 3 * nb, soln.end());
+    if (all_bodies.end_of_step_update(V, 1.0)) {  // main.cpp:314-326
+      std::printf("step %d: redone (a blob reached a channel wall)\n", step);
+      all_bodies.reset_to_start_of_step();
+      ++redone;
+      continue;
+    }
     wr(o, soln.data(), soln.size());
     wr(o, &gmres.report.iterations, 1);
     std::printf("step %d: %d iterations, status %d\n", step, gmres.report.iterations, gmres.report.status);
-    std::vector<double> V(soln.begin() + 3 * nb, soln.end());
-    all_bodies.end_of_step_update(V, 1.0);
     t += dt;
+    ++step;
   }
   for (const particle& b : all_bodies.bodies) wr(o, b.x, 3);
   for (const particle& b : all_bodies.bodies) {
