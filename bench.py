@@ -262,7 +262,12 @@ def bench_timestep(args):
     times, reps, redone = [], [], 0
     for k in range(args.warmup + args.steps):
         t1 = time.perf_counter()
-        rep = s.em_step(1 + k, cutoff, tol=args.tol, max_iter=args.max_iter, precision=fa.FCM_PREC_FP32)
+        if args.gdc:
+            rep = s.gdc_step(1 + k, cutoff, fd_delta=1e-5, tol=args.tol, max_iter=args.max_iter,
+                             precision=fa.FCM_PREC_FP32)
+            redone += int(rep["redo"])
+        else:
+            rep = s.em_step(1 + k, cutoff, tol=args.tol, max_iter=args.max_iter, precision=fa.FCM_PREC_FP32)
         if chan and s.channel_wall_overlap():  # the step is undone and, as in main.cpp, taken again
             s.reset_to_start_of_step()
             redone += 1
@@ -274,18 +279,19 @@ def bench_timestep(args):
     s.close()
     ms = 1e3 * statistics.mean(times)
     line = {
-        "metric": "Brownian time steps/s at 512^3 with 1.8M spheres (FP32+refine GMRES at 1e-8, sterics, EM)"
-                  + (" (slip channel)" if chan else ""),
+        "metric": "Brownian time steps/s at 512^3 with 1.8M spheres (FP32+refine GMRES at 1e-8, sterics, "
+                  + ("drifter-corrector)" if args.gdc else "EM)") + (" (slip channel)" if chan else ""),
         "value": round(1e3 / ms, 4), "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms, 1), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 grid passes + f64 Krylov/refinement/constraints/quaternions",
         "data": "synthetic (seeded rods and stress stream)",
         "config": {"workload": f"{n}^3 grid, {NB} rods x {nbpb} spheres ({nb} blobs), a=1.86, kT=1, steric "
-                               f"cutoff 2.2a, Euler-Maruyama, dt {args.dt:.5f}, volume fraction {phi:.3f}"
+                               f"cutoff 2.2a, {'generalised drifter-corrector (divU by 6 finite differences)' if args.gdc else 'Euler-Maruyama'}, dt {args.dt:.5f}, volume fraction {phi:.3f}"
                                + (f", slip channel (walls at y = 0 and {n // 2}), {redone} step(s) undone for a "
                                   f"wall overlap" if chan else ""),
                    "parallelism": "single GPU"},
         "iterations": [r["iterations"] for r in reps], "status": sorted(set(r["status"] for r in reps)),
+        "divU": [r["divU"] for r in reps] if args.gdc else None,
     }
     print(json.dumps(line), flush=True)
 
@@ -323,6 +329,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--gdc", action="store_true",
+                    help="timestep: the generalised drifter-corrector step instead of Euler-Maruyama")
     ap.add_argument("--slip-channel", action="store_true",
                     help="solve256 / timestep in the slip-channel geometry: walls at y = 0 and y = grid/2, rods "
                          "inside")

@@ -232,6 +232,26 @@ int fcm_mobility_prepare(fcm_handle h, uint64_t seed);
  * and, on 1, calls rb_reset_to_start_of_step and steps again with a new seed
  * (main.cpp:304-326). */
 int fcm_em_step(fcm_handle h, uint64_t seed, double cutoff, const gmres_opts* opts, gmres_report* rep);
+/* one generalised drifter-corrector step (main.cpp:222-312 with
+ * GEN_DRIFTER_CORRECTOR), on the device: the seeded fluctuating field and its
+ * blob velocities V; advance_to_midstep (collection_of_rigid_bodies.cpp:
+ * 566-621): U0 = the rigid fit of V per body, (K_ref^T K_ref)^-1 in the body
+ * frame, and the mid-step bodies = start bodies moved by DT/2 U0;
+ * calculate_divU (:869-953): for each of the six directions every body is
+ * displaced by e (fd_delta * a for translations, fd_delta for rotations;
+ * FINITE_DIFFERENCE_DELTA_FACTOR, config.hpp:73, is 1e-5), the same field is
+ * interpolated and divU += sum_n (U_n - U0_n)_i / e; then at the mid-step the
+ * blob velocities of the same field, the barrier forces (cutoff > 0), the RHS
+ * and the saddle-point solve, and the start bodies move by nu DT V with
+ * nu = 1 + DT divU / 2.  divU (may be NULL) receives divU.  In a slip channel,
+ * *redo = 1 (may be NULL) means the mid-step put a blob on or beyond a wall:
+ * nothing moved and the step is to be taken again with a new seed
+ * (main.cpp:237-246); after the step, rb_channel_wall_overlap and
+ * rb_reset_to_start_of_step apply as for fcm_em_step.  A body whose K^T K is
+ * singular (collinear blobs) uses its pseudo-inverse where the reference
+ * inverts. */
+int fcm_gdc_step(fcm_handle h, uint64_t seed, double cutoff, double fd_delta, const gmres_opts* opts,
+                 gmres_report* rep, double* divU, int* redo);
 
 /* ---- batches of independent solves (no counterpart in the reference) ------ */
 /* The reference solves one system per process (main.cpp:308-310).  A batch

@@ -90,3 +90,86 @@ def test_euler_maruyama_steps_match_oracle():
             assert dx.max() < 1e-9
             assert np.abs(qg.ravel() - qo).max() < 1e-9
     o.close()
+
+
+def _gdc_pair(n, NB, nbpb, channel, rng):
+    import _gdc
+    a = 1.86
+    ref = _gdc.reference_rods(nbpb, a)
+    gc, oc = pair_cfg(n, kT=1.0, dt=0.20216, max_blobs=NB * nbpb, max_bodies=NB, max_blobs_per_body=nbpb,
+                      max_krylov=300, slip_channel=channel)
+    o = po.Fcm(oc, NB * nbpb)
+    st, M, Minv, N = o.ref_matrices(ref)
+    return _gdc, ref, gc, oc, o, Minv, N
+
+
+def test_gdc_steps_match_oracle():
+    """Generalised drifter-corrector steps (main.cpp:235-274, 308-312):
+    divU from the six finite differences and the positions after each step."""
+    n, NB, nbpb, cutoff, tol = 32, 8, 6, 2.2 * 1.86, 1e-10
+    rng = np.random.default_rng(3)
+    _gdc, ref, gc, oc, o, Minv, N = _gdc_pair(n, NB, nbpb, False, rng)
+    x = rng.uniform(0, n, (NB, 3))
+    q = random_quats(rng, NB)
+    xo, qo = x.ravel().copy(), q.ravel().copy()
+    with fa.FcmSolver(gc) as s:
+        s.set_geometry(ref)
+        s.set_bodies(x, q)
+        s.set_ref_matrices(Minv, N)
+        s.set_body_forces(np.zeros((NB, 6)))
+        for step in range(2):
+            seed = 11 + step
+            xo, qo, divU, sol, _ = _gdc.gdc_step(o, oc, ref, xo, qo, Minv, N, seed, cutoff, 1e-5, tol)
+            rep = s.gdc_step(seed, cutoff, fd_delta=1e-5, tol=tol, max_iter=300)
+            xg, qg = s.bodies()
+            d = np.abs(xg.ravel() - xo)
+            d = np.minimum(d, n - d)
+            say("gdc step", step, rep, "oracle divU", divU, "max |dx|", d.max())
+            assert rep["status"] == "converged" and not rep["redo"]
+            assert abs(rep["divU"] - divU) < 1e-6 * abs(divU)
+            assert d.max() < 1e-9
+            assert np.abs(qg.ravel() - qo).max() < 1e-9
+    o.close()
+
+
+def test_gdc_channel_midstep_wall_overlap_matches_oracle():
+    """In a slip channel the step is taken again when the mid-step puts a blob
+    on or beyond a wall (main.cpp:237-246): the GPU's verdict matches the
+    oracle's mid-step for each seed, a refused step leaves the bodies as they
+    were, an accepted one matches the oracle's step."""
+    n, NB, nbpb, cutoff, tol = 32, 8, 6, 2.2 * 1.86, 1e-10
+    rng = np.random.default_rng(3)
+    _gdc, ref, gc, oc, o, Minv, N = _gdc_pair(n, NB, nbpb, True, rng)
+    x = rng.uniform(0, n, (NB, 3))
+    x[:, 1] = rng.uniform(5, 11, NB)
+    x[0, 1] = 1.86 + 1e-3  # two spheres of body 0 sit 1e-3 above the lower wall
+    q = np.tile([1.0, 0.0, 0.0, 0.0], (NB, 1))
+    seen = set()
+    with fa.FcmSolver(gc) as s:
+        s.set_geometry(ref)
+        for seed in (20, 21, 22, 23):
+            s.set_bodies(x, q)
+            s.set_ref_matrices(Minv, N)
+            s.set_body_forces(np.zeros((NB, 6)))
+            x1, q1, divU, sol, Ym = _gdc.gdc_step(o, oc, ref, x, q, Minv, N, seed, cutoff, 1e-5, tol)
+            # the oracle's mid-step configuration decides
+            o.set_up_gaussians(po.fill_blob_pos(oc, ref, x, q))
+            o.fill_stress(seed)
+            o.random_field()
+            U0 = _gdc.rigid_fit(ref, q, o.random_interp(), _gdc.ktk_inv(ref))
+            xm, qm = po.end_of_step_update(oc, U0.ravel(), 0.5 * oc.dt, x.ravel(), q.ravel())
+            ym = po.fill_blob_pos(oc, ref, xm, qm).reshape(-1, 3)[:, 1]
+            expect = bool(((ym == 0.0) | (ym >= n / 2)).any())
+            rep = s.gdc_step(seed, cutoff, fd_delta=1e-5, tol=tol, max_iter=300)
+            say("seed", seed, "redo", rep["redo"], "oracle", expect)
+            assert rep["redo"] == expect
+            seen.add(expect)
+            xg, qg = s.bodies()
+            if expect:
+                assert np.array_equal(xg, x) and np.array_equal(qg, q)
+            else:
+                d = np.abs(xg.ravel() - x1)
+                d = np.minimum(d, n - d)
+                assert d.max() < 1e-9 and np.abs(qg.ravel() - q1).max() < 1e-9
+    assert seen == {True, False}
+    o.close()

@@ -27,7 +27,7 @@ EXPORTS = [
     "rb_set_geometry", "rb_set_bodies", "rb_set_body_forces", "rb_get_blob_positions",
     "rb_mul_K", "rb_mul_KT", "ll_barrier_forces", "rb_end_of_step_update", "rb_get_bodies",
     "fcm_shear_field", "fcm_shear_interp", "fcm_x_velocity_profile",
-    "rb_channel_wall_overlap", "rb_reset_to_start_of_step", "fcm_em_step",
+    "rb_channel_wall_overlap", "rb_reset_to_start_of_step", "fcm_em_step", "fcm_gdc_step",
     "gmres_assemble_rhs", "gmres_saddle_matvec", "gmres_apply_Pinv", "gmres_solve",
     "gmres_get_solution", "fcm_mobility_solve", "fcm_mobility_prepare",
     "fcm_batch_create", "fcm_batch_destroy", "fcm_batch_submit", "fcm_batch_step", "fcm_batch_take",
@@ -123,6 +123,8 @@ def lib():
         L.fcm_x_velocity_profile.argtypes = [H, _dp, _dp]
         L.rb_reset_to_start_of_step.argtypes = [H]
         L.fcm_em_step.argtypes = [H, C.c_uint64, C.c_double, C.POINTER(GmresOpts), C.POINTER(GmresReport)]
+        L.fcm_gdc_step.argtypes = [H, C.c_uint64, C.c_double, C.c_double, C.POINTER(GmresOpts),
+                                   C.POINTER(GmresReport), C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.gmres_assemble_rhs.argtypes = [H, C.c_int]
         L.gmres_saddle_matvec.argtypes = [H, _dp, _dp, C.c_int]
         L.gmres_apply_Pinv.argtypes = [H, _dp, _dp]
@@ -377,6 +379,20 @@ class FcmSolver:
         r = GmresReport()
         _chk(self.L.fcm_em_step(self.h, seed, cutoff, C.byref(o), C.byref(r)))
         return r.as_dict()
+
+    def gdc_step(self, seed, cutoff, fd_delta=1e-5, tol=1e-8, max_iter=200, precision=FCM_PREC_FP64, inner_tol=0.0,
+                 max_refine=0):
+        """One generalised drifter-corrector step on the device; returns the solve
+        report with divU and redo (a mid-step wall overlap: nothing moved)."""
+        o = GmresOpts(tol, max_iter, precision, inner_tol, max_refine)
+        r = GmresReport()
+        d = C.c_double(0.0)
+        redo = C.c_int(0)
+        _chk(self.L.fcm_gdc_step(self.h, seed, cutoff, fd_delta, C.byref(o), C.byref(r), C.byref(d), C.byref(redo)))
+        out = r.as_dict()
+        out["divU"] = d.value
+        out["redo"] = bool(redo.value)
+        return out
 
     def assemble_rhs(self, use_random=True):
         _chk(self.L.gmres_assemble_rhs(self.h, int(use_random)))

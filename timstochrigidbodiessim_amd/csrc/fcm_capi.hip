@@ -126,6 +126,7 @@ Sizes sizes_of(const fcm_config* c) {
   t += nblk * (2 * (size_t)c->max_krylov + 6) * 8 + 2 * (2 * (size_t)c->max_krylov + 16) * 8;
   t += (size_t)c->max_bodies * 12 * 8 + (size_t)nb * (3 * 8 + 2 * 4);  // time-step buffers
   t += (size_t)c->max_bodies * 7 * 8 + 4;                                 // bodies before a step, flag
+  t += (size_t)c->max_bodies * 32 * 8 + 42 * 8;                           // drifter-corrector buffers
   const size_t m2 = (size_t)(c->max_krylov + 2) * (c->max_krylov + 2);
   t += (2 * m2 + 9 * (size_t)(c->max_krylov + 2) + 16) * 8 + s.ldq * 8;
   s.total = t;
@@ -497,6 +498,18 @@ int fcm_create(const fcm_config* cfg, fcm_handle* out) {
   if (e == hipSuccess) h->d_x0 = dalloc<double>(h, 3 * (size_t)cfg->max_bodies, e);
   if (e == hipSuccess) h->d_q0 = dalloc<double>(h, 4 * (size_t)cfg->max_bodies, e);
   if (e == hipSuccess) h->d_flag = dalloc<int>(h, 1, e);
+  {
+    const size_t nbd = (size_t)cfg->max_bodies;
+    if (e == hipSuccess) h->d_xs = dalloc<double>(h, 3 * nbd, e);
+    if (e == hipSuccess) h->d_qs = dalloc<double>(h, 4 * nbd, e);
+    if (e == hipSuccess) h->d_xm = dalloc<double>(h, 3 * nbd, e);
+    if (e == hipSuccess) h->d_qm = dalloc<double>(h, 4 * nbd, e);
+    if (e == hipSuccess) h->d_U0 = dalloc<double>(h, 6 * nbd, e);
+    if (e == hipSuccess) h->d_Ud = dalloc<double>(h, 6 * nbd, e);
+    if (e == hipSuccess) h->d_b6 = dalloc<double>(h, 6 * nbd, e);
+    if (e == hipSuccess) h->d_KTKinv = dalloc<double>(h, 36, e);
+    if (e == hipSuccess) h->d_divu = dalloc<double>(h, 6, e);
+  }
   if (e == hipSuccess) h->d_fblob = dalloc<double>(h, 3 * (size_t)nb, e);
   if (e == hipSuccess) h->sw.cell_of = dalloc<int>(h, nb, e);
   if (e == hipSuccess) h->sw.sorted = dalloc<int>(h, nb, e);
@@ -744,6 +757,23 @@ int rb_set_geometry(fcm_handle h, int nbpb, const double* blob_ref) {
   h->nbpb = nbpb;
   h->h_ref.assign(blob_ref, blob_ref + 3 * (size_t)nbpb);
   HIPCHK(hipMemcpyAsync(h->d_ref, blob_ref, 3 * (size_t)nbpb * sizeof(double), hipMemcpyHostToDevice, h->st));
+  {
+    // ref_KTKinv = inv(K^T K), K = [I, rcross(r_m)] over the blobs
+    // (collection_of_rigid_bodies.cpp:117-128); a body whose blobs are collinear
+    // (K^T K singular about the axis) gets the pseudo-inverse
+    double KTK[36] = {0.0};
+    for (int m = 0; m < nbpb; ++m) {
+      const double* r = blob_ref + 3 * m;
+      // rows of K for this blob: K(3m + c, .) = [e_c, (rcross(r))_c.], rcross(r) v = v x r
+      const double Kb[3][6] = {{1, 0, 0, 0, r[2], -r[1]}, {0, 1, 0, -r[2], 0, r[0]}, {0, 0, 1, r[1], -r[0], 0}};
+      for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j)
+          for (int c = 0; c < 3; ++c) KTK[6 * i + j] += Kb[c][i] * Kb[c][j];
+    }
+    double inv[36];
+    if (dense::inv_spd(6, KTK, inv) != 0) dense::pinv_sym(6, KTK, inv, 1e-9);
+    HIPCHK(hipMemcpyAsync(h->d_KTKinv, inv, 36 * sizeof(double), hipMemcpyHostToDevice, h->st));
+  }
   h->have_geometry = true;
   ++h->epoch;
   h->have_bodies = false;
@@ -1285,6 +1315,96 @@ int fcm_x_velocity_profile(fcm_handle h, const double* lambda, double* prof) {
   HIPCHK(x_velocity_profile(h->g, fp, h->have_shear ? h->fpshear() : FieldPtrs<double>{}, h->d_prof, h->st));
   HIPCHK(hipMemcpyAsync(prof, h->d_prof, h->g.ny * sizeof(double), hipMemcpyDeviceToHost, h->st));
   return sync_stream(h);
+}
+
+int fcm_gdc_step(fcm_handle h, uint64_t seed, double cutoff, double fd_delta, const gmres_opts* opts,
+                 gmres_report* rep, double* divU_out, int* redo) {
+  if (int rc = check(h)) return rc;
+  if (!h->have_bodies || !h->have_ref) return fail(FCM_ERR_STATE, "bodies and reference matrices must be set");
+  if (!h->stress[0] || !(h->cfg.kT > 0.0)) return fail(FCM_ERR_STATE, "the drifter-corrector step needs kT > 0 and the random field");
+  if (!(fd_delta > 0.0)) return fail(FCM_ERR_ARG, "fd_delta must be positive");
+  const fcm_config& c = h->cfg;
+  const int NB = h->NB, nb = NB * h->nbpb;
+  const double dt = c.dt;
+  if (redo) *redo = 0;
+  auto copy_bodies = [&](double* xd, double* qd, const double* xsrc, const double* qsrc) -> hipError_t {
+    hipError_t e = hipMemcpyAsync(xd, xsrc, 3 * (size_t)NB * sizeof(double), hipMemcpyDeviceToDevice, h->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(qd, qsrc, 4 * (size_t)NB * sizeof(double), hipMemcpyDeviceToDevice, h->st);
+    return e;
+  };
+  auto place = [&]() -> int {  // blob positions, rotations and Gaussians of the bodies in d_x, d_q
+    BodyData b = h->body();
+    HIPCHK(bodies_update(h->g, b, h->st));
+    return setup_blobs_device(h, nb);
+  };
+  // main.cpp:230-233: the fluctuating field and its blob velocities at the start
+  const double var1 = std::sqrt(2.0 * c.kT * c.mu / (c.dx * c.dx * c.dx * c.dt));
+  const double var2 = std::sqrt(2.0) * var1;
+  const double scale[6] = {var2, var2, var2, var1, var1, var1};
+  HIPCHK(copy_bodies(h->d_xs, h->d_qs, h->d_x, h->d_q));
+  HIPCHK(random_velocity<double>(h->g, h->fpstress(), true, (unsigned long long)seed, scale, h->pp, h->tw64, h->st));
+  h->have_urand = true;
+  HIPCHK(interp<double>(h->g, h->bb, h->bd, h->fprand(), h->Vrand, KTerm{}, h->st));
+  // advance_to_midstep (collection_of_rigid_bodies.cpp:566-621): U0 from the
+  // blob velocities, every body moved by DT/2 U0
+  BodyData b = h->body();
+  HIPCHK(mul_KT(b, h->Vrand, h->d_b6, h->st));
+  HIPCHK(body_fit(b, h->d_b6, h->d_KTKinv, h->d_U0, h->st));
+  HIPCHK(bodies_step(h->g, h->d_x, h->d_q, NB, h->d_U0, 0.5 * dt, h->st));
+  HIPCHK(copy_bodies(h->d_xm, h->d_qm, h->d_x, h->d_q));
+  if (int rc = place()) return rc;
+  if (h->g.channel) {
+    int overlap = 0;
+    HIPCHK(wall_overlap(h->dY, nb, 0.5 * h->g.dx * h->g.ny, h->d_flag, h->st));
+    HIPCHK(hipMemcpyAsync(&overlap, h->d_flag, sizeof(int), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (overlap) {  // main.cpp:237-246: the step is restarted; the bodies stay
+      HIPCHK(copy_bodies(h->d_x, h->d_q, h->d_xs, h->d_qs));
+      if (int rc = place()) return rc;
+      h->have_rhs = false;
+      h->have_blob_forces = false;
+      if (redo) *redo = 1;
+      return sync_stream(h);
+    }
+  }
+  // calculate_divU (collection_of_rigid_bodies.cpp:869-953): every body displaced
+  // by e_i from its start, the same fluctuating field interpolated, the rigid fit
+  for (int i = 0; i < 6; ++i) {
+    const double e = i < 3 ? fd_delta * c.a : fd_delta;
+    HIPCHK(copy_bodies(h->d_x, h->d_q, h->d_xs, h->d_qs));
+    HIPCHK(fill_body6(h->d_b6, NB, i, e, h->st));
+    HIPCHK(bodies_step(h->g, h->d_x, h->d_q, NB, h->d_b6, 1.0, h->st));
+    if (int rc = place()) return rc;
+    HIPCHK(interp<double>(h->g, h->bb, h->bd, h->fprand(), h->Vtmp, KTerm{}, h->st));
+    BodyData bd = h->body();
+    HIPCHK(mul_KT(bd, h->Vtmp, h->d_b6, h->st));
+    HIPCHK(body_fit(bd, h->d_b6, h->d_KTKinv, h->d_Ud, h->st));
+    HIPCHK(divu_sum(h->d_Ud, h->d_U0, NB, i, e, h->d_divu, h->st));
+  }
+  double acc[6];
+  HIPCHK(hipMemcpyAsync(acc, h->d_divu, 6 * sizeof(double), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  double divU = 0.0;
+  for (int i = 0; i < 6; ++i) divU += acc[i];
+  const double nu = 1.0 + 0.5 * dt * divU;  // main.cpp:254
+  if (divU_out) *divU_out = divU;
+  // main.cpp:262-266, 278, 308-312 at the mid-step: blob velocities of the same
+  // field, forces, the RHS and the solve; then the start bodies move by nu DT V
+  HIPCHK(copy_bodies(h->d_x, h->d_q, h->d_xm, h->d_qm));
+  if (int rc = place()) return rc;
+  HIPCHK(interp<double>(h->g, h->bb, h->bd, h->fprand(), h->Vrand, KTerm{}, h->st));
+  h->have_blob_forces = false;
+  if (cutoff > 0.0)
+    if (int rc = ll_barrier_forces(h, cutoff, nullptr)) return rc;
+  const double* F6 = nullptr;
+  if (int rc = body_forces_dev(h, &F6)) return rc;
+  const double* Vs = nullptr;
+  if (int rc = shear_velocities_dev(h, &Vs)) return rc;
+  HIPCHK(rhs_assemble(h->Vrand, Vs, F6, h->rhs, h->n3(), 6 * (size_t)NB, h->st));
+  h->have_rhs = true;
+  if (int rc = solve_single(h, opts, rep, nullptr)) return rc;
+  HIPCHK(copy_bodies(h->d_x, h->d_q, h->d_xs, h->d_qs));
+  return rb_end_of_step_update(h, nullptr, nu);
 }
 
 int rb_channel_wall_overlap(fcm_handle h, int* overlap) {

@@ -61,6 +61,11 @@ struct fcm_handle_s {
   double* d_F6tot = nullptr;    // 6 NB body forces + K^T blob forces, for the RHS
   double* d_V6 = nullptr;       // 6 NB body velocities for the end-of-step update
   double *d_x0 = nullptr, *d_q0 = nullptr;  // bodies before the last end-of-step update
+  // generalised drifter-corrector step: start-of-step and mid-step bodies, the
+  // rigid fits U0 and U (6 NB each), a 6 NB scratch, (K^T K)^-1 of the
+  // reference body and the six finite-difference sums
+  double *d_xs = nullptr, *d_qs = nullptr, *d_xm = nullptr, *d_qm = nullptr;
+  double *d_U0 = nullptr, *d_Ud = nullptr, *d_b6 = nullptr, *d_KTKinv = nullptr, *d_divu = nullptr;
   int* d_flag = nullptr;        // one-word result of the wall-overlap check
   bool have_prev = false;
   StericWork sw;                // cell list of the barrier forces

@@ -195,6 +195,13 @@ hipError_t bodies_update(const GridGeom& g, BodyData& b, hipStream_t st);
 // *flag = 1 if any blob sits on the lower wall (y == 0) or at or beyond the
 // upper one (y >= half_ly), else 0
 hipError_t wall_overlap(const double* Y, int nb, double half_ly, int* flag, hipStream_t st);
+// U = diag(R,R) KTKinv diag(R^T,R^T) KTv per body (the body-frame rigid fit of
+// advance_to_midstep / calculate_divU)
+hipError_t body_fit(const BodyData& b, const double* KTv, const double* KTKinv, double* U, hipStream_t st);
+// V6[6n + k] = (k == i) ? e : 0 for every body n
+hipError_t fill_body6(double* V6, int NB, int i, double e, hipStream_t st);
+// acc[i] = sum_n (U[6n+i] - U0[6n+i]) / e (one workgroup, fixed order)
+hipError_t divu_sum(const double* U, const double* U0, int NB, int i, double e, double* acc, hipStream_t st);
 // the shear forcing fx = (scale kT) sin(2 pi kappa y / ny) / a on every real node,
 // fy = fz = 0 (fcm_fluid_solver.cpp:705-729)
 hipError_t shear_force(const GridGeom& g, const FieldPtrs<double>& fp, double scale_kT, double a, double kappa,

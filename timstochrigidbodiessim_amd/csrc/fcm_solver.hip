@@ -56,6 +56,71 @@ hipError_t bodies_update(const GridGeom& g, BodyData& b, hipStream_t st) {
   return hipGetLastError();
 }
 
+// The rigid-body velocity that best fits blob velocities in the body frame
+// (collection_of_rigid_bodies.cpp:583-599, 925-944): from b = K^T V (the
+// world-frame sums [sum v; sum (R r_m) x v]), U = diag(R, R) KTKinv
+// diag(R^T, R^T) b with KTKinv = (K_ref^T K_ref)^-1.  One thread per body.
+__global__ void body_fit_kernel(BodyData b, const double* __restrict__ KTv, const double* __restrict__ KTKinv,
+                                double* __restrict__ U) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= b.NB) return;
+  const double* R = b.R + 9 * (size_t)n;
+  const double* v = KTv + 6 * (size_t)n;
+  double bb[6], u[6];
+  for (int h = 0; h < 2; ++h)
+    for (int i = 0; i < 3; ++i)  // (R^T b)_i
+      bb[3 * h + i] = R[i] * v[3 * h] + R[3 + i] * v[3 * h + 1] + R[6 + i] * v[3 * h + 2];
+  for (int i = 0; i < 6; ++i) {
+    double s = 0.0;
+    for (int k = 0; k < 6; ++k) s += KTKinv[6 * i + k] * bb[k];
+    u[i] = s;
+  }
+  for (int h = 0; h < 2; ++h)
+    for (int i = 0; i < 3; ++i)
+      U[6 * (size_t)n + 3 * h + i] = R[3 * i] * u[3 * h] + R[3 * i + 1] * u[3 * h + 1] + R[3 * i + 2] * u[3 * h + 2];
+}
+
+hipError_t body_fit(const BodyData& b, const double* KTv, const double* KTKinv, double* U, hipStream_t st) {
+  if (b.NB <= 0) return hipSuccess;
+  hipLaunchKernelGGL(body_fit_kernel, dim3((b.NB + 255) / 256), dim3(256), 0, st, b, KTv, KTKinv, U);
+  return hipGetLastError();
+}
+
+// V6 of every body = e (the common displacement of calculate_divU's finite
+// difference, collection_of_rigid_bodies.cpp:877-897)
+__global__ void fill_body6_kernel(double* __restrict__ V6, int NB, int i, double e) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < 6 * NB) V6[r] = (r % 6 == i) ? e : 0.0;
+}
+
+hipError_t fill_body6(double* V6, int NB, int i, double e, hipStream_t st) {
+  if (NB <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_body6_kernel, dim3((6 * NB + 255) / 256), dim3(256), 0, st, V6, NB, i, e);
+  return hipGetLastError();
+}
+
+// acc[i] = sum over bodies of (U_n(i) - U0_n(i)) / e, one workgroup, fixed
+// order (strided partial sums, then a tree): one direction of calculate_divU
+// (collection_of_rigid_bodies.cpp:946)
+__global__ void __launch_bounds__(256) divu_kernel(const double* __restrict__ U, const double* __restrict__ U0, int NB,
+                                                   int i, double e, double* __restrict__ acc) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int n = threadIdx.x; n < NB; n += 256) s += (U[6 * (size_t)n + i] - U0[6 * (size_t)n + i]) / e;
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) acc[i] = red[0];
+}
+
+hipError_t divu_sum(const double* U, const double* U0, int NB, int i, double e, double* acc, hipStream_t st) {
+  hipLaunchKernelGGL(divu_kernel, dim3(1), dim3(256), 0, st, U, U0, NB, i, e, acc);
+  return hipGetLastError();
+}
+
 // check_for_channel_wall_overlap (collection_of_rigid_bodies.cpp:207-239): one
 // workgroup strides over the wrapped blob positions and ORs y == 0 || y >= LY/2
 __global__ void __launch_bounds__(1024) wall_overlap_kernel(const double* __restrict__ Y, int nb, double half_ly,
