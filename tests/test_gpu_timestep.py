@@ -173,3 +173,49 @@ def test_gdc_channel_midstep_wall_overlap_matches_oracle():
                 assert d.max() < 1e-9 and np.abs(qg.ravel() - q1).max() < 1e-9
     assert seen == {True, False}
     o.close()
+
+
+def test_blob_mode_steps_match_oracle():
+    """SIMULATING_BLOBS (RIGID_BODY_TYPE 0, main.cpp:290-304): free spheres, no
+    saddle-point solve.  Each step composes the entry points: the fluctuating
+    field's blob velocities, the barrier forces, the deterministic mobility
+    product of those forces, and every sphere moved by DT (V_rand + M F)
+    (a one-sphere body per blob, rotation zero)."""
+    n, NB, dt, cutoff = 32, 60, 0.20216, 2.2 * 1.86
+    gc, oc = pair_cfg(n, kT=1.0, dt=dt, max_blobs=NB, max_bodies=NB, max_blobs_per_body=1)
+    rng = np.random.default_rng(17)
+    x = rng.uniform(0, n, (NB, 3))
+    q = np.tile([1.0, 0.0, 0.0, 0.0], (NB, 1))
+    ref = np.zeros((1, 3))
+    o = po.Fcm(oc, NB)
+    xo, qo = x.ravel().copy(), q.ravel().copy()
+    with fa.FcmSolver(gc) as s:
+        s.set_geometry(ref)
+        s.set_bodies(x, q)
+        for step in range(3):
+            seed = 40 + step
+            Y = po.fill_blob_pos(oc, ref, xo, qo)
+            o.set_up_gaussians(Y)
+            o.fill_stress(seed)
+            o.random_field()
+            Vr = o.random_interp()
+            F = po.barrier_forces(oc, Y, 1, cutoff)
+            V = Vr + o.mobility(F)
+            V6 = np.zeros((NB, 6))
+            V6[:, :3] = V.reshape(-1, 3)
+            xo, qo = po.end_of_step_update(oc, V6.ravel(), dt, xo, qo)
+
+            s.random_field(None, seed)
+            Vg = s.random_interp()
+            Fg = s.barrier_forces(cutoff)
+            Vg = Vg + s.mobility(Fg.ravel())
+            V6g = np.zeros((NB, 6))
+            V6g[:, :3] = Vg.reshape(-1, 3)
+            s.end_of_step_update(V6g, nu=1.0)
+            xg, qg = s.bodies()
+            d = np.abs(xg.ravel() - xo)
+            d = np.minimum(d, n - d)
+            say("blob step", step, "max |dx|", d.max())
+            assert d.max() < 1e-11
+            assert np.array_equal(qg.ravel(), qo)  # no rotation
+    o.close()
