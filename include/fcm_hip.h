@@ -252,6 +252,20 @@ int fcm_em_step(fcm_handle h, uint64_t seed, double cutoff, const gmres_opts* op
  * inverts. */
 int fcm_gdc_step(fcm_handle h, uint64_t seed, double cutoff, double fd_delta, const gmres_opts* opts,
                  gmres_report* rep, double* divU, int* redo);
+/* The pieces of that step for the reference's own loop.
+ * advance_to_midstep (collection_of_rigid_bodies.cpp:566-621): from the blob
+ * velocities V (3 Nb host, or NULL for those of the last fcm_random_interp),
+ * U0 per body and the mid-step bodies = bodies moved by DT/2 U0; the device's
+ * blobs (positions, Gaussians, K) become the mid-step ones, the bodies stay
+ * the start-of-step ones (the next rb_end_of_step_update moves them).  In a
+ * slip channel *overlap = 1 (may be NULL) when the mid-step touches a wall:
+ * then nothing changes and the step is begun again. */
+int rb_advance_to_midstep(fcm_handle h, const double* V, int* overlap);
+/* calculate_divU (:869-953) after rb_advance_to_midstep, with the random
+ * field in place: six displaced copies of the start bodies, the field's blob
+ * velocities and rigid fits; the blobs are the mid-step ones again afterwards
+ * (fill_all_blob_pos_at_midstep, main.cpp:263). */
+int rb_calculate_divU(fcm_handle h, double fd_delta, double* divU);
 
 /* ---- batches of independent solves (no counterpart in the reference) ------ */
 /* The reference solves one system per process (main.cpp:308-310).  A batch

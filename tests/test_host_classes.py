@@ -36,8 +36,10 @@ def _write_problem(path, n, NB, nbpb, steps, prec, seed, a, kT, dt, tol, cutoff,
         f.write(struct.pack("<5d", a, kT, dt, tol, cutoff))
         for arr in (ref, x, q, F6):
             f.write(np.ascontiguousarray(arr, dtype="<f8").tobytes())
-        if channel is not None:  # (slip_channel, shear_force_scale, shear_wavenumber)
-            f.write(struct.pack("<i2d", *channel))
+        if channel is not None:  # (slip_channel, shear_force_scale, shear_wavenumber[, algorithm])
+            f.write(struct.pack("<i2d", *channel[:3]))
+            if len(channel) > 3:
+                f.write(struct.pack("<i", channel[3]))
 
 
 def _read_result(path, nsys, NB, steps):
@@ -146,5 +148,42 @@ def test_host_classes_channel_shear_time_steps(tmp_path):
             rep = s.em_step(seed + k, cutoff, tol=tol, max_iter=200)
             assert not s.channel_wall_overlap()
             assert rep["iterations"] == its[k]
+        xd, qd = s.bodies()
+    assert np.array_equal(xd, xh) and np.array_equal(qd, qh)
+
+
+@pytest.mark.gpu
+def test_host_classes_drifter_corrector_steps(tmp_path):
+    """The host-class loop's GEN_DRIFTER_CORRECTOR branch (main.cpp:235-268:
+    advance_to_midstep, calculate_divU, the mid-step blob velocities) gives
+    bitwise the device-resident drifter-corrector steps."""
+    _need_exe()
+    import _gdc
+    n, NB, nbpb, steps, seed, a, kT, dt, tol, cutoff = 32, 8, 6, 2, 57, 1.86, 1.0, 0.20216, 1e-10, 2.2 * 1.86
+    ref = _gdc.reference_rods(nbpb, a)
+    rng = np.random.default_rng(8)
+    x = rng.uniform(0, n, (NB, 3))
+    q = rng.standard_normal((NB, 4))
+    q /= np.linalg.norm(q, axis=1)[:, None]
+    F6 = np.zeros((NB, 6))
+    inp, out = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    _write_problem(inp, n, NB, nbpb, steps, fa.FCM_PREC_FP64, seed, a, kT, dt, tol, cutoff, ref, x, q, F6,
+                   channel=(0, 0.0, 0.0, 1))
+    r = subprocess.run([EXE, "steps", inp, out], capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0 and "redone" not in r.stdout
+    nb = NB * nbpb
+    sols, its, xh, qh = _read_result(out, 3 * nb + 6 * NB, NB, steps)
+    cfg = fa.make_config(n, ng=7, dx=1.0, a=a, mu=1.0, kT=kT, dt=dt, max_blobs=nb, max_bodies=NB,
+                         max_blobs_per_body=nbpb, max_krylov=200, fp32=False, random_field=True)
+    with fa.FcmSolver(cfg) as s:
+        s.set_geometry(ref)
+        s.ref_matrices()
+        s.set_bodies(x, q)
+        s.set_body_forces(F6)
+        for k in range(steps):
+            rep = s.gdc_step(seed + k, cutoff, fd_delta=1e-5, tol=tol, max_iter=200)
+            assert not rep["redo"] and rep["iterations"] == its[k]
+            assert np.array_equal(s.solution(), sols[k])
         xd, qd = s.bodies()
     assert np.array_equal(xd, xh) and np.array_equal(qd, qh)

@@ -28,6 +28,7 @@ EXPORTS = [
     "rb_mul_K", "rb_mul_KT", "ll_barrier_forces", "rb_end_of_step_update", "rb_get_bodies",
     "fcm_shear_field", "fcm_shear_interp", "fcm_x_velocity_profile",
     "rb_channel_wall_overlap", "rb_reset_to_start_of_step", "fcm_em_step", "fcm_gdc_step",
+    "rb_advance_to_midstep", "rb_calculate_divU",
     "gmres_assemble_rhs", "gmres_saddle_matvec", "gmres_apply_Pinv", "gmres_solve",
     "gmres_get_solution", "fcm_mobility_solve", "fcm_mobility_prepare",
     "fcm_batch_create", "fcm_batch_destroy", "fcm_batch_submit", "fcm_batch_step", "fcm_batch_take",
@@ -125,6 +126,8 @@ def lib():
         L.fcm_em_step.argtypes = [H, C.c_uint64, C.c_double, C.POINTER(GmresOpts), C.POINTER(GmresReport)]
         L.fcm_gdc_step.argtypes = [H, C.c_uint64, C.c_double, C.c_double, C.POINTER(GmresOpts),
                                    C.POINTER(GmresReport), C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.rb_advance_to_midstep.argtypes = [H, _dp, C.POINTER(C.c_int)]
+        L.rb_calculate_divU.argtypes = [H, C.c_double, C.POINTER(C.c_double)]
         L.gmres_assemble_rhs.argtypes = [H, C.c_int]
         L.gmres_saddle_matvec.argtypes = [H, _dp, _dp, C.c_int]
         L.gmres_apply_Pinv.argtypes = [H, _dp, _dp]

@@ -796,6 +796,7 @@ int rb_set_bodies(fcm_handle h, int NB, const double* x, const double* q) {
   if (rc) return rc;
   h->have_bodies = true;
   h->have_prev = false;
+  h->gdc_mid = false;
   h->have_rhs = false;
   h->have_blob_forces = false;
   return sync_stream(h);
@@ -1236,6 +1237,23 @@ int ll_barrier_forces(fcm_handle h, double cutoff, double* F3) {
   return sync_stream(h);
 }
 
+namespace {
+
+hipError_t copy_bodies(fcm_handle h, double* xd, double* qd, const double* xsrc, const double* qsrc) {
+  hipError_t e = hipMemcpyAsync(xd, xsrc, 3 * (size_t)h->NB * sizeof(double), hipMemcpyDeviceToDevice, h->st);
+  if (e == hipSuccess) e = hipMemcpyAsync(qd, qsrc, 4 * (size_t)h->NB * sizeof(double), hipMemcpyDeviceToDevice, h->st);
+  return e;
+}
+
+// blob positions, rotations and Gaussians of the bodies in d_x, d_q
+int place_bodies(fcm_handle h) {
+  BodyData b = h->body();
+  HIPCHK(bodies_update(h->g, b, h->st));
+  return setup_blobs_device(h, h->NB * h->nbpb);
+}
+
+}  // namespace
+
 int rb_end_of_step_update(fcm_handle h, const double* V6, double nu) {
   if (int rc = check(h)) return rc;
   if (!h->have_bodies) return fail(FCM_ERR_STATE, "rb_set_bodies first");
@@ -1246,6 +1264,12 @@ int rb_end_of_step_update(fcm_handle h, const double* V6, double nu) {
   } else {
     if (!h->have_sol) return fail(FCM_ERR_STATE, "no solution to take the body velocities from");
     v = h->sol + h->n3();
+  }
+  // after a drifter-corrector mid-step the update applies to the start-of-step
+  // bodies (bodies vs midstep_bodies, collection_of_rigid_bodies.cpp:566-621, 759-790)
+  if (h->gdc_mid) {
+    HIPCHK(copy_bodies(h, h->d_x, h->d_q, h->d_xs, h->d_qs));
+    h->gdc_mid = false;
   }
   // temp_bodies = bodies (collection_of_rigid_bodies.cpp:761), for reset_to_start_of_step
   HIPCHK(hipMemcpyAsync(h->d_x0, h->d_x, 3 * (size_t)h->NB * sizeof(double), hipMemcpyDeviceToDevice, h->st));
@@ -1317,64 +1341,53 @@ int fcm_x_velocity_profile(fcm_handle h, const double* lambda, double* prof) {
   return sync_stream(h);
 }
 
-int fcm_gdc_step(fcm_handle h, uint64_t seed, double cutoff, double fd_delta, const gmres_opts* opts,
-                 gmres_report* rep, double* divU_out, int* redo) {
+int rb_advance_to_midstep(fcm_handle h, const double* V, int* overlap) {
   if (int rc = check(h)) return rc;
-  if (!h->have_bodies || !h->have_ref) return fail(FCM_ERR_STATE, "bodies and reference matrices must be set");
-  if (!h->stress[0] || !(h->cfg.kT > 0.0)) return fail(FCM_ERR_STATE, "the drifter-corrector step needs kT > 0 and the random field");
-  if (!(fd_delta > 0.0)) return fail(FCM_ERR_ARG, "fd_delta must be positive");
-  const fcm_config& c = h->cfg;
-  const int NB = h->NB, nb = NB * h->nbpb;
-  const double dt = c.dt;
-  if (redo) *redo = 0;
-  auto copy_bodies = [&](double* xd, double* qd, const double* xsrc, const double* qsrc) -> hipError_t {
-    hipError_t e = hipMemcpyAsync(xd, xsrc, 3 * (size_t)NB * sizeof(double), hipMemcpyDeviceToDevice, h->st);
-    if (e == hipSuccess) e = hipMemcpyAsync(qd, qsrc, 4 * (size_t)NB * sizeof(double), hipMemcpyDeviceToDevice, h->st);
-    return e;
-  };
-  auto place = [&]() -> int {  // blob positions, rotations and Gaussians of the bodies in d_x, d_q
-    BodyData b = h->body();
-    HIPCHK(bodies_update(h->g, b, h->st));
-    return setup_blobs_device(h, nb);
-  };
-  // main.cpp:230-233: the fluctuating field and its blob velocities at the start
-  const double var1 = std::sqrt(2.0 * c.kT * c.mu / (c.dx * c.dx * c.dx * c.dt));
-  const double var2 = std::sqrt(2.0) * var1;
-  const double scale[6] = {var2, var2, var2, var1, var1, var1};
-  HIPCHK(copy_bodies(h->d_xs, h->d_qs, h->d_x, h->d_q));
-  HIPCHK(random_velocity<double>(h->g, h->fpstress(), true, (unsigned long long)seed, scale, h->pp, h->tw64, h->st));
-  h->have_urand = true;
-  HIPCHK(interp<double>(h->g, h->bb, h->bd, h->fprand(), h->Vrand, KTerm{}, h->st));
-  // advance_to_midstep (collection_of_rigid_bodies.cpp:566-621): U0 from the
-  // blob velocities, every body moved by DT/2 U0
-  BodyData b = h->body();
-  HIPCHK(mul_KT(b, h->Vrand, h->d_b6, h->st));
-  HIPCHK(body_fit(b, h->d_b6, h->d_KTKinv, h->d_U0, h->st));
-  HIPCHK(bodies_step(h->g, h->d_x, h->d_q, NB, h->d_U0, 0.5 * dt, h->st));
-  HIPCHK(copy_bodies(h->d_xm, h->d_qm, h->d_x, h->d_q));
-  if (int rc = place()) return rc;
-  if (h->g.channel) {
-    int overlap = 0;
-    HIPCHK(wall_overlap(h->dY, nb, 0.5 * h->g.dx * h->g.ny, h->d_flag, h->st));
-    HIPCHK(hipMemcpyAsync(&overlap, h->d_flag, sizeof(int), hipMemcpyDeviceToHost, h->st));
-    HIPCHK(hipStreamSynchronize(h->st));
-    if (overlap) {  // main.cpp:237-246: the step is restarted; the bodies stay
-      HIPCHK(copy_bodies(h->d_x, h->d_q, h->d_xs, h->d_qs));
-      if (int rc = place()) return rc;
-      h->have_rhs = false;
-      h->have_blob_forces = false;
-      if (redo) *redo = 1;
-      return sync_stream(h);
-    }
+  if (!h->have_bodies) return fail(FCM_ERR_STATE, "rb_set_bodies first");
+  const double* v = h->Vrand;
+  if (V) {
+    HIPCHK(hipMemcpyAsync(h->Vtmp, V, h->n3() * sizeof(double), hipMemcpyHostToDevice, h->st));
+    v = h->Vtmp;
+  } else if (!h->have_urand) {
+    return fail(FCM_ERR_STATE, "no fluctuating blob velocities (fcm_random_field, fcm_random_interp)");
   }
-  // calculate_divU (collection_of_rigid_bodies.cpp:869-953): every body displaced
-  // by e_i from its start, the same fluctuating field interpolated, the rigid fit
+  if (h->gdc_mid) HIPCHK(copy_bodies(h, h->d_x, h->d_q, h->d_xs, h->d_qs));  // a step begun again
+  else HIPCHK(copy_bodies(h, h->d_xs, h->d_qs, h->d_x, h->d_q));
+  BodyData b = h->body();
+  HIPCHK(mul_KT(b, v, h->d_b6, h->st));
+  HIPCHK(body_fit(b, h->d_b6, h->d_KTKinv, h->d_U0, h->st));
+  HIPCHK(bodies_step(h->g, h->d_x, h->d_q, h->NB, h->d_U0, 0.5 * h->cfg.dt, h->st));
+  HIPCHK(copy_bodies(h, h->d_xm, h->d_qm, h->d_x, h->d_q));
+  if (int rc = place_bodies(h)) return rc;
+  h->gdc_mid = true;
+  h->have_blob_forces = false;
+  h->have_rhs = false;
+  int ov = 0;
+  if (h->g.channel) {
+    HIPCHK(wall_overlap(h->dY, h->NB * h->nbpb, 0.5 * h->g.dx * h->g.ny, h->d_flag, h->st));
+    HIPCHK(hipMemcpyAsync(&ov, h->d_flag, sizeof(int), hipMemcpyDeviceToHost, h->st));
+  }
+  HIPCHK(hipStreamSynchronize(h->st));
+  if (ov) {  // main.cpp:237-246: the step is begun again, the bodies stay where they were
+    HIPCHK(copy_bodies(h, h->d_x, h->d_q, h->d_xs, h->d_qs));
+    if (int rc = place_bodies(h)) return rc;
+    h->gdc_mid = false;
+  }
+  if (overlap) *overlap = ov;
+  return sync_stream(h);
+}
+
+int rb_calculate_divU(fcm_handle h, double fd_delta, double* divU) {
+  if (int rc = check(h)) return rc;
+  if (!h->gdc_mid) return fail(FCM_ERR_STATE, "rb_advance_to_midstep first");
+  if (!(fd_delta > 0.0)) return fail(FCM_ERR_ARG, "fd_delta must be positive");
+  const int NB = h->NB;
   for (int i = 0; i < 6; ++i) {
-    const double e = i < 3 ? fd_delta * c.a : fd_delta;
-    HIPCHK(copy_bodies(h->d_x, h->d_q, h->d_xs, h->d_qs));
+    const double e = i < 3 ? fd_delta * h->cfg.a : fd_delta;
+    HIPCHK(copy_bodies(h, h->d_x, h->d_q, h->d_xs, h->d_qs));
     HIPCHK(fill_body6(h->d_b6, NB, i, e, h->st));
     HIPCHK(bodies_step(h->g, h->d_x, h->d_q, NB, h->d_b6, 1.0, h->st));
-    if (int rc = place()) return rc;
+    if (int rc = place_bodies(h)) return rc;
     HIPCHK(interp<double>(h->g, h->bb, h->bd, h->fprand(), h->Vtmp, KTerm{}, h->st));
     BodyData bd = h->body();
     HIPCHK(mul_KT(bd, h->Vtmp, h->d_b6, h->st));
@@ -1383,15 +1396,43 @@ int fcm_gdc_step(fcm_handle h, uint64_t seed, double cutoff, double fd_delta, co
   }
   double acc[6];
   HIPCHK(hipMemcpyAsync(acc, h->d_divu, 6 * sizeof(double), hipMemcpyDeviceToHost, h->st));
+  // back to the mid-step configuration (fill_all_blob_pos_at_midstep, main.cpp:263)
+  HIPCHK(copy_bodies(h, h->d_x, h->d_q, h->d_xm, h->d_qm));
+  if (int rc = place_bodies(h)) return rc;
   HIPCHK(hipStreamSynchronize(h->st));
+  double d = 0.0;
+  for (int i = 0; i < 6; ++i) d += acc[i];
+  if (divU) *divU = d;
+  return FCM_OK;
+}
+
+int fcm_gdc_step(fcm_handle h, uint64_t seed, double cutoff, double fd_delta, const gmres_opts* opts,
+                 gmres_report* rep, double* divU_out, int* redo) {
+  if (int rc = check(h)) return rc;
+  if (!h->have_bodies || !h->have_ref) return fail(FCM_ERR_STATE, "bodies and reference matrices must be set");
+  if (!h->stress[0] || !(h->cfg.kT > 0.0)) return fail(FCM_ERR_STATE, "the drifter-corrector step needs kT > 0 and the random field");
+  if (!(fd_delta > 0.0)) return fail(FCM_ERR_ARG, "fd_delta must be positive");
+  const fcm_config& c = h->cfg;
+  if (redo) *redo = 0;
+  // main.cpp:230-233: the fluctuating field and its blob velocities at the start
+  const double var1 = std::sqrt(2.0 * c.kT * c.mu / (c.dx * c.dx * c.dx * c.dt));
+  const double var2 = std::sqrt(2.0) * var1;
+  const double scale[6] = {var2, var2, var2, var1, var1, var1};
+  HIPCHK(random_velocity<double>(h->g, h->fpstress(), true, (unsigned long long)seed, scale, h->pp, h->tw64, h->st));
+  h->have_urand = true;
+  HIPCHK(interp<double>(h->g, h->bb, h->bd, h->fprand(), h->Vrand, KTerm{}, h->st));
+  int overlap = 0;
+  if (int rc = rb_advance_to_midstep(h, nullptr, &overlap)) return rc;
+  if (overlap) {
+    if (redo) *redo = 1;
+    return FCM_OK;
+  }
   double divU = 0.0;
-  for (int i = 0; i < 6; ++i) divU += acc[i];
-  const double nu = 1.0 + 0.5 * dt * divU;  // main.cpp:254
+  if (int rc = rb_calculate_divU(h, fd_delta, &divU)) return rc;
+  const double nu = 1.0 + 0.5 * c.dt * divU;  // main.cpp:254
   if (divU_out) *divU_out = divU;
   // main.cpp:262-266, 278, 308-312 at the mid-step: blob velocities of the same
   // field, forces, the RHS and the solve; then the start bodies move by nu DT V
-  HIPCHK(copy_bodies(h->d_x, h->d_q, h->d_xm, h->d_qm));
-  if (int rc = place()) return rc;
   HIPCHK(interp<double>(h->g, h->bb, h->bd, h->fprand(), h->Vrand, KTerm{}, h->st));
   h->have_blob_forces = false;
   if (cutoff > 0.0)
@@ -1400,10 +1441,9 @@ int fcm_gdc_step(fcm_handle h, uint64_t seed, double cutoff, double fd_delta, co
   if (int rc = body_forces_dev(h, &F6)) return rc;
   const double* Vs = nullptr;
   if (int rc = shear_velocities_dev(h, &Vs)) return rc;
-  HIPCHK(rhs_assemble(h->Vrand, Vs, F6, h->rhs, h->n3(), 6 * (size_t)NB, h->st));
+  HIPCHK(rhs_assemble(h->Vrand, Vs, F6, h->rhs, h->n3(), 6 * (size_t)h->NB, h->st));
   h->have_rhs = true;
   if (int rc = solve_single(h, opts, rep, nullptr)) return rc;
-  HIPCHK(copy_bodies(h->d_x, h->d_q, h->d_xs, h->d_qs));
   return rb_end_of_step_update(h, nullptr, nu);
 }
 

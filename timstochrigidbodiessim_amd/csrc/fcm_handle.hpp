@@ -66,6 +66,7 @@ struct fcm_handle_s {
   // reference body and the six finite-difference sums
   double *d_xs = nullptr, *d_qs = nullptr, *d_xm = nullptr, *d_qm = nullptr;
   double *d_U0 = nullptr, *d_Ud = nullptr, *d_b6 = nullptr, *d_KTKinv = nullptr, *d_divu = nullptr;
+  bool gdc_mid = false;  // the device bodies are a drifter-corrector mid-step (start in d_xs)
   int* d_flag = nullptr;        // one-word result of the wall-overlap check
   bool have_prev = false;
   StericWork sw;                // cell list of the barrier forces

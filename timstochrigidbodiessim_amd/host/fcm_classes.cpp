@@ -274,6 +274,23 @@ bool collection_of_rigid_bodies::end_of_step_update(const std::vector<double>& V
   return check_for_channel_wall_overlap();
 }
 
+bool collection_of_rigid_bodies::advance_to_midstep(const std::vector<double>& V) {
+  int overlap = 0;
+  check(rb_advance_to_midstep(h_, V.data(), &overlap));
+  return overlap != 0;
+}
+
+double collection_of_rigid_bodies::calculate_divU(fcm_fluid_solver&, double delta) {
+  double d = 0.0;
+  check(rb_calculate_divU(h_, delta, &d));
+  return d;
+}
+
+void collection_of_rigid_bodies::fill_all_blob_pos_at_midstep() {
+  all_blob_pos.assign(3 * (size_t)total_num_blobs, 0.0);
+  check(rb_get_blob_positions(h_, all_blob_pos.data()));
+}
+
 bool collection_of_rigid_bodies::check_for_channel_wall_overlap() const {
   int overlap = 0;
   check(rb_channel_wall_overlap(h_, &overlap));

@@ -170,6 +170,13 @@ class collection_of_rigid_bodies {
   bool end_of_step_update(const std::vector<double>& V, double nu);
   // :207-239 (false in a periodic box)
   bool check_for_channel_wall_overlap() const;
+  // GEN_DRIFTER_CORRECTOR (:566-621, 869-953): the mid-step from the blob
+  // velocities V (true: a wall overlap, nothing moved), divU by finite
+  // differences of step delta (FINITE_DIFFERENCE_DELTA_FACTOR, config.hpp:73),
+  // and the mid-step blob positions (:fill_all_blob_pos_at_midstep)
+  bool advance_to_midstep(const std::vector<double>& V);
+  double calculate_divU(fcm_fluid_solver& fcm, double delta = 1e-5);
+  void fill_all_blob_pos_at_midstep();
   // :792-796: the bodies before the last end_of_step_update
   void reset_to_start_of_step();
 

@@ -18,7 +18,8 @@
 //
 // in.bin (little-endian): int32 n, NB, nbpb, steps, precision; uint64 seed;
 // float64 a, kT, dt, tol, cutoff; float64 ref[3 nbpb], x[3 NB], q[4 NB], F6[6 NB];
-// optionally int32 slip_channel, float64 shear_force_scale, shear_wavenumber.
+// optionally int32 slip_channel, float64 shear_force_scale, shear_wavenumber
+// [, int32 algorithm: 0 Euler-Maruyama, 1 generalised drifter-corrector].
 // out.bin: per step float64 soln[3 Nb + 6 NB] and int32 iterations; then
 // float64 x[3 NB], q[4 NB].
 #include <cmath>
@@ -117,10 +118,12 @@ int run_steps(const char* in_path, const char* out_path) {
   if (!rd(f, ref.data(), ref.size()) || !rd(f, x.data(), x.size()) || !rd(f, q.data(), q.size()) ||
       !rd(f, F6.data(), F6.size()))
     return 2;
-  // optional trailer: int32 slip_channel; float64 shear_force_scale, shear_wavenumber
-  int chan = 0;
+  // optional trailer: int32 slip_channel; float64 shear_force_scale, shear_wavenumber;
+  // int32 algorithm (0 Euler-Maruyama, 1 generalised drifter-corrector)
+  int chan = 0, gdc = 0;
   double shear[2] = {0.0, 0.0};
   if (rd(f, &chan, 1) && !rd(f, shear, 2)) return 2;
+  if (!rd(f, &gdc, 1)) gdc = 0;
   std::fclose(f);
   const int nb = NB * nbpb;
 
@@ -151,13 +154,27 @@ int run_steps(const char* in_path, const char* out_path) {
     fcm.produce_random_velocity_field(s);
     fcm.set_up_gaussians(all_bodies.all_blob_pos);
     fcm.calculate_random_blob_velocities();
+    double nu = 1.0;
+    if (gdc) {  // main.cpp:235-268
+      if (all_bodies.advance_to_midstep(fcm.V)) {
+        std::printf("step %d: redone (the mid-step reached a channel wall)\n", step);
+        ++redone;
+        continue;
+      }
+      const double divU = all_bodies.calculate_divU(fcm);
+      nu = 1.0 + 0.5 * dt * divU;
+      std::printf("step %d: divU %.17g\n", step, divU);
+      all_bodies.fill_all_blob_pos_at_midstep();
+      fcm.set_up_gaussians(all_bodies.all_blob_pos);
+      fcm.calculate_random_blob_velocities();
+    }
     if (shear[0] != 0.0) fcm.calculate_shear_blob_velocities();  // main.cpp:270-274
     // main.cpp:285, 305-310
     all_bodies.calculate_forces();
     gmres.assemble_RHS(all_bodies, fcm);
     std::vector<double> soln = gmres.invert_saddle_point_system(all_bodies, fcm, 0, t);
     std::vector<double> V(soln.begin() + 3 * nb, soln.end());
-    if (all_bodies.end_of_step_update(V, 1.0)) {  // main.cpp:314-326
+    if (all_bodies.end_of_step_update(V, nu)) {  // main.cpp:314-326
       std::printf("step %d: redone (a blob reached a channel wall)\n", step);
       all_bodies.reset_to_start_of_step();
       ++redone;
