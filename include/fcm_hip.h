@@ -47,7 +47,8 @@ typedef struct fcm_handle_s* fcm_handle;
 
 /* Runtime replacement of the compile-time parameters of config.hpp. */
 typedef struct {
-  int nx, ny, nz;      /* NUM_FFT_GRID_POINTS_{X,Y,Z} (config.hpp:65-67); even, 2^a 3^b   */
+  int nx, ny, nz;      /* NUM_FFT_GRID_POINTS_{X,Y,Z} (config.hpp:65-67): nx, ny and nz/2
+                          each 2^a or 3 2^a between 4 and 512 (nx, ny >= 8)            */
   int ng;              /* NUM_FCM_GAUSSIAN_POINTS (config.hpp:69), <= 16                 */
   double dx;           /* DX (config.hpp:88), decoupled from the blob radius             */
   double a;            /* BLOB_RADIUS (config.hpp:48)                                    */

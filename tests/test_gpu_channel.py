@@ -72,7 +72,8 @@ def test_channel_spread_interp_variants_match_oracle(dims, ng, nb):
 
 
 @pytest.mark.parametrize("dims,ng,nb", [((32, 32, 32), 7, 300), ((16, 16, 64), 7, 600), ((32, 16, 48), 8, 200),
-                                        ((16, 16, 256), 7, 200)])
+                                        ((16, 16, 256), 7, 200), ((16, 16, 512), 8, 200), ((32, 24, 32), 7, 200),
+                                        ((32, 12, 32), 5, 100)])
 def test_channel_mobility_column_and_separate_match_oracle(dims, ng, nb):
     say("channel mobility", dims, ng, nb)
     dx = 0.9
@@ -91,8 +92,9 @@ def test_channel_mobility_column_and_separate_match_oracle(dims, ng, nb):
     with fa.FcmSolver(gc) as s:
         s.set_up_gaussians(Y)
         t = s.tuning()
-        assert t["column"] == [1, 1], t
-        for column in ((1, 1), (0, 0)):
+        # (32, 12, 32) is too short in y for the column path: separate passes only
+        assert t["column"] == ([0, 0] if dims[1] < 16 else [1, 1]), t
+        for column in (((1, 1), (0, 0)) if t["column"] == [1, 1] else ((0, 0),)):
             s.set_tuning(t["spread_variant"], t["interp_izb"], column=column)
             for prec, tol in ((fa.FCM_PREC_FP64, FP64_TOL), (fa.FCM_PREC_FP32, FP32_TOL)):
                 V = s.mobility(F, prec)
