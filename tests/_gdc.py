@@ -40,9 +40,11 @@ def rigid_fit(ref, q, V, KTKi):
     return U
 
 
-def gdc_step(o, oc, ref, x, q, Minv, N, seed, cutoff, delta, tol, F6=None):
+def gdc_step(o, oc, ref, x, q, Minv, N, seed, cutoff, delta, tol, F6=None, shear=False):
     """Returns (x1, q1, divU, sol, midstep blob positions) for one step from
-    bodies (x, q); o is an oracle Fcm of the same grid."""
+    bodies (x, q); o is an oracle Fcm of the same grid (with its shear field
+    set when shear is true: the mid-step RHS then subtracts its blob
+    velocities, main.cpp:270-274 and gmres_solver.cpp:62-64)."""
     x = np.asarray(x, dtype=float).ravel()
     q = np.asarray(q, dtype=float).ravel()
     NB = x.size // 3
@@ -66,6 +68,8 @@ def gdc_step(o, oc, ref, x, q, Minv, N, seed, cutoff, delta, tol, F6=None):
     Ym = po.fill_blob_pos(oc, ref, xm, qm)
     o.set_up_gaussians(Ym)
     Vm = o.random_interp()
+    if shear:
+        Vm = Vm + o.shear_interp()
     Fb = po.mul_KT(ref, qm, po.barrier_forces(oc, Ym, len(ref), cutoff)) if cutoff > 0 else np.zeros(6 * NB)
     if F6 is not None:
         Fb = Fb + np.asarray(F6).ravel()

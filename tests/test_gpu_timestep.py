@@ -219,3 +219,31 @@ def test_blob_mode_steps_match_oracle():
             assert d.max() < 1e-11
             assert np.array_equal(qg.ravel(), qo)  # no rotation
     o.close()
+
+
+def test_gdc_channel_shear_steps_match_oracle():
+    """A drifter-corrector step in a slip channel with shear forcing: the
+    mid-step RHS carries the shear blob velocities at the mid-step positions."""
+    n, NB, nbpb, cutoff, tol = 32, 6, 6, 2.2 * 1.86, 1e-10
+    rng = np.random.default_rng(5)
+    _gdc, ref, gc, oc, o, Minv, N = _gdc_pair(n, NB, nbpb, True, rng)
+    o.shear_field(1.5, 1.0)
+    x = rng.uniform(0, n, (NB, 3))
+    x[:, 1] = rng.uniform(6.0, 10.0, NB)
+    q = np.tile([1.0, 0.0, 0.0, 0.0], (NB, 1))
+    with fa.FcmSolver(gc) as s:
+        s.set_geometry(ref)
+        s.set_bodies(x, q)
+        s.set_ref_matrices(Minv, N)
+        s.set_body_forces(np.zeros((NB, 6)))
+        s.shear_field(1.5, 1.0)
+        x1, q1, divU, sol, _ = _gdc.gdc_step(o, oc, ref, x, q, Minv, N, 31, cutoff, 1e-5, tol, shear=True)
+        rep = s.gdc_step(31, cutoff, fd_delta=1e-5, tol=tol, max_iter=300)
+        assert not rep["redo"] and rep["status"] == "converged"
+        assert abs(rep["divU"] - divU) < 1e-6 * abs(divU)
+        assert rel(s.solution(), sol) < 1e-10, rel(s.solution(), sol)
+        xg, qg = s.bodies()
+        d = np.abs(xg.ravel() - x1)
+        d = np.minimum(d, n - d)
+        assert d.max() < 1e-9 and np.abs(qg.ravel() - q1).max() < 1e-9
+    o.close()
