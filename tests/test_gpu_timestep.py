@@ -247,3 +247,40 @@ def test_gdc_channel_shear_steps_match_oracle():
         d = np.minimum(d, n - d)
         assert d.max() < 1e-9 and np.abs(qg.ravel() - q1).max() < 1e-9
     o.close()
+
+
+@pytest.mark.parametrize("channel", [False, True])
+def test_fp32_refined_steps_track_fp64_steps(channel):
+    """Time steps with the FP32+refine solve (the benchmark's form) stay with the
+    FP64 steps over three Euler-Maruyama steps, periodic and in a slip channel.
+    Both solves stop at a relative residual of 1e-8, so their solutions differ
+    by up to the conditioning times that (about 1e-6 relative here): the body
+    displacements must agree to 1e-5 of their size."""
+    n, NB, nbpb, dt, cutoff = 32, 10, 6, 0.20216, 2.2 * 1.86
+    import _gdc
+    ref = _gdc.reference_rods(nbpb, 1.86)
+    rng = np.random.default_rng(23)
+    x = rng.uniform(0, n, (NB, 3))
+    if channel:
+        x[:, 1] = rng.uniform(6.0, 10.0, NB)
+    q = random_quats(rng, NB)
+    out = {}
+    for prec in (fa.FCM_PREC_FP64, fa.FCM_PREC_FP32):
+        gc, _ = pair_cfg(n, kT=1.0, dt=dt, max_blobs=NB * nbpb, max_bodies=NB, max_blobs_per_body=nbpb,
+                         max_krylov=300, slip_channel=channel)
+        with fa.FcmSolver(gc) as s:
+            s.set_geometry(ref)
+            s.ref_matrices()
+            s.set_bodies(x, q)
+            s.set_body_forces(np.zeros((NB, 6)))
+            for k in range(3):
+                rep = s.em_step(60 + k, cutoff, tol=1e-8, max_iter=300, precision=prec)
+                assert rep["status"] == "converged", rep
+            out[prec] = s.bodies()
+    d64 = out[fa.FCM_PREC_FP64][0] - x
+    d32 = out[fa.FCM_PREC_FP32][0] - x
+    d64 -= n * np.round(d64 / n)
+    d32 -= n * np.round(d32 / n)
+    say("fp32 vs fp64 steps", channel, np.abs(d32 - d64).max(), np.abs(d64).max())
+    assert np.abs(d32 - d64).max() < 1e-5 * np.abs(d64).max()
+    assert np.abs(out[fa.FCM_PREC_FP32][1] - out[fa.FCM_PREC_FP64][1]).max() < 1e-5
