@@ -212,7 +212,9 @@ int gmres_apply_Pinv(fcm_handle h, const double* in, double* out);
 /* invert_saddle_point_system, gmres_solver.cpp:148-306.  rhs: host vector or
  * NULL (use the device RHS from gmres_assemble_rhs).  soln: host output or
  * NULL (keep it on the device; read with gmres_get_solution).  resid_hist may
- * be NULL or hold max_iter entries (|beta_j|/||RHS||). */
+ * be NULL or hold max_iter entries (|beta_j|/||RHS||, the inner solves' histories
+ * one after another in the FP32 form); it receives the first
+ * min(iterations, max_krylov + 2) of them. */
 int gmres_solve(fcm_handle h, const double* rhs, double* soln, const gmres_opts* opts, gmres_report* rep,
                 double* resid_hist);
 int gmres_get_solution(fcm_handle h, double* soln);

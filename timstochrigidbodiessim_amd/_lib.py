@@ -420,7 +420,7 @@ class FcmSolver:
         _chk(self.L.gmres_solve(self.h, _ptr(rr), _ptr(soln), C.byref(o), C.byref(r), _ptr(hh)))
         d = r.as_dict()
         if hist:
-            d["hist"] = hh[:r.iterations].copy()
+            d["hist"] = hh[:min(r.iterations, self.cfg.max_krylov + 2)].copy()
         return soln, d
 
     def mobility_solve(self, seed, tol=1e-8, max_iter=200, precision=FCM_PREC_FP64, inner_tol=0.0, max_refine=0):

@@ -168,6 +168,9 @@ void batch_free(fcm_batch_s* B) {
   delete B;
 }
 
+// entries of the residual history a handle stores (its hist_all buffer)
+int hist_capacity(const fcm_handle h) { return h->cfg.max_krylov + 2; }
+
 int inner_max_iter(const Job* j) {
   const fcm_handle h = j->h;
   const long long left = (long long)j->max_iter - j->rep.iterations;
@@ -229,8 +232,10 @@ int job_inner_done(fcm_batch_s* B, Job* j, const double* ctl) {
   HIPCHK(gm_launch_finish(h->gm, h->Q, h->ldq, h->tmp, n, st));
   BodyData bdy = h->body();
   HIPCHK(apply_pinv(bdy, h->tmp, h->sol, nullptr, st));
-  if (nc > 0 && j->rep.iterations + nc <= h->cfg.max_krylov + 2)
-    HIPCHK(hipMemcpyAsync(h->hist_all + j->rep.iterations, h->gm.hist, (size_t)nc * sizeof(double),
+  // hist_all keeps the first hist_capacity(h) entries of the whole solve's history
+  const int keep = std::min(nc, hist_capacity(h) - j->rep.iterations);
+  if (keep > 0)
+    HIPCHK(hipMemcpyAsync(h->hist_all + j->rep.iterations, h->gm.hist, (size_t)keep * sizeof(double),
                           hipMemcpyDeviceToDevice, st));
   j->rep.iterations += nc;
   j->rep.rel_residual = rel;
@@ -503,8 +508,8 @@ int solve_single(fcm_handle h, const gmres_opts* o, gmres_report* rep, double* h
   const gmres_report r = j->rep;
   delete j;
   HIPCHK(hipStreamSynchronize(h->st));
-  if (hist && r.iterations > 0)
-    HIPCHK(hipMemcpy(hist, h->hist_all, (size_t)r.iterations * sizeof(double), hipMemcpyDeviceToHost));
+  const int nh = std::min(r.iterations, hist_capacity(h));
+  if (hist && nh > 0) HIPCHK(hipMemcpy(hist, h->hist_all, (size_t)nh * sizeof(double), hipMemcpyDeviceToHost));
   if (rep) *rep = r;
   return FCM_OK;
 }
