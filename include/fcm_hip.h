@@ -49,7 +49,9 @@ typedef struct fcm_handle_s* fcm_handle;
 typedef struct {
   int nx, ny, nz;      /* NUM_FFT_GRID_POINTS_{X,Y,Z} (config.hpp:65-67): nx, ny and nz/2
                           each 2^a or 3 2^a between 4 and 512 (nx, ny >= 8)            */
-  int ng;              /* NUM_FCM_GAUSSIAN_POINTS (config.hpp:69), <= 16                 */
+  int ng;              /* NUM_FCM_GAUSSIAN_POINTS (config.hpp:69), <= 20 (the reference's
+                          default); the column kernels serve ng <= 8, wider stencils
+                          run the separate passes                                      */
   double dx;           /* DX (config.hpp:88), decoupled from the blob radius             */
   double a;            /* BLOB_RADIUS (config.hpp:48)                                    */
   double mu;           /* FLUID_VISCOSITY (config.hpp:49)                                */

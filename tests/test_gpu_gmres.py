@@ -150,3 +150,26 @@ def test_long_vector_solve_residual():
     say("fp32 vs fp64: all", rel(sol32, sol64), "body velocities", rel(sol32[nl:], sol64[nl:]))
     assert rel(sol32, sol64) < 1e-4
     assert rel(sol32[nl:], sol64[nl:]) < 1e-5
+
+
+def test_fp32_history_longer_than_krylov_capacity():
+    """An FP32+refine solve may take more iterations in all than the Krylov
+    capacity (each inner solve restarts the basis); the residual history keeps
+    the first max_krylov + 2 entries and the solve is unaffected."""
+    say("fp32 history beyond capacity")
+    NB, nbpb, cap = 40, 8, 12
+    gc, oc = pair_cfg(48, max_blobs=NB * nbpb, max_bodies=NB, max_blobs_per_body=nbpb, max_krylov=cap)
+    ref, x, q, F6, _ = fa.synth_rods(8, NB, nbpb, 1.86, [48.0] * 3, max_attempts=200)
+    rhs = np.concatenate([np.zeros(3 * NB * nbpb), -F6.ravel()])
+    with fa.FcmSolver(gc) as s:
+        s.set_geometry(ref)
+        s.ref_matrices()
+        s.set_bodies(x, q)
+        sol, rep = s.solve(rhs, tol=1e-8, max_iter=400, precision=fa.FCM_PREC_FP32, max_refine=60, hist=True)
+        say(rep["iterations"], rep["refinements"], rep["status"], len(rep["hist"]))
+        assert rep["status"] == "converged" and rep["iterations"] > cap + 2
+        h = rep["hist"]
+        assert len(h) == cap + 2
+        assert np.all(np.isfinite(h)) and np.all(h > 0) and np.all(h <= 1.0 + 1e-12)
+        sol2, rep2 = s.solve(rhs, tol=1e-8, max_iter=400, precision=fa.FCM_PREC_FP32, max_refine=60)
+        assert np.array_equal(sol, sol2) and rep2["iterations"] == rep["iterations"]

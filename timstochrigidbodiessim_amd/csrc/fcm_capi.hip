@@ -60,7 +60,7 @@ bool valid_cfg(const fcm_config* c, std::string& why) {
     if (d < 2 && !fft_len_supported(n[d])) { why = "unsupported FFT length " + std::to_string(n[d]); return false; }
   }
   if (!fft_len_supported(c->nz / 2)) { why = "unsupported z length " + std::to_string(c->nz); return false; }
-  if (c->ng < 1 || c->ng > 16) { why = "ng must be in 1..16"; return false; }
+  if (c->ng < 1 || c->ng > 20) { why = "ng must be in 1..20"; return false; }
   if (c->slip_channel != 0 && c->slip_channel != 1) { why = "slip_channel must be 0 or 1"; return false; }
   if (c->ng > std::min(c->nx, std::min(c->ny, c->nz))) { why = "ng exceeds the grid"; return false; }
   if (!(c->dx > 0) || !(c->a > 0) || !(c->mu > 0)) { why = "dx, a and mu must be positive"; return false; }
@@ -418,6 +418,10 @@ int fcm_create(const fcm_config* cfg, fcm_handle* out) {
   bb.sz = (bb.nbz % 2 == 0 && 2 * bb.bz <= 16) ? 2 : 1;
   while (bb.sy * bb.by * bb.sz * bb.bz > 256) {
     if (bb.sz > 1) bb.sz = 1; else bb.sy = 1;
+  }
+  if (!spread_fast_ok(h->g, bb, cfg->ng) && !spread_generic_ok(h->g, bb, cfg->ng)) {
+    delete h;
+    return fail(FCM_ERR_ARG, "stencil too wide for the spreading tiles of this grid");
   }
   const double LX = cfg->dx * cfg->nx, LY = cfg->dx * cfg->ny, LZ = cfg->dx * cfg->nz;
   h->pp.facx = 2.0 * PI / LX;

@@ -209,7 +209,7 @@ template hipError_t channel_images<float>(const GridGeom&, const FieldPtrs<float
 // spreading
 // ---------------------------------------------------------------------------
 static constexpr int SP_TXMAX = 8;    // tile x extent held in registers
-static constexpr int SP_NGMAX = 16;   // largest stencil supported
+static constexpr int SP_NGMAX = 20;   // largest stencil supported (NUM_FCM_GAUSSIAN_POINTS, config.hpp:69)
 static constexpr int SP_BATCH = 128;  // candidate blobs staged per round
 static constexpr int SP_MAXC = 512;   // candidate bins per tile
 
@@ -881,6 +881,19 @@ __global__ void __launch_bounds__(256) spread_bits_kernel(SpreadListArgs A, Fiel
   }
 }
 
+// the generic spreading kernel's limits: stencil, tile shape, and the bins
+// whose blobs can reach a tile (a tile of ext nodes meets at most
+// (ext + ng - 2) / b + 2 bins of b nodes per axis), which it lists in LDS
+bool spread_generic_ok(const GridGeom& g, const BlobBins& bb, int ng) {
+  if (ng > SP_NGMAX || bb.sx * bb.bx > SP_TXMAX || bb.sy * bb.by * bb.sz * bb.bz > 256) return false;
+  const int ext[3] = {bb.sx * bb.bx, bb.sy * bb.by, bb.sz * bb.bz}, b[3] = {bb.bx, bb.by, bb.bz},
+            nb[3] = {bb.nbx, bb.nby, bb.nbz};
+  long long cand = 1;
+  for (int d = 0; d < 3; ++d) cand *= std::min(nb[d], (ext[d] + ng - 2) / b[d] + 2);
+  (void)g;
+  return cand <= SP_MAXC;
+}
+
 bool spread_fast_ok(const GridGeom& g, const BlobBins& bb, int ng) {
   return ng <= 8 && g.nx >= 16 && g.nx % 8 == 0 && g.ny % 8 == 0 && g.nz % 8 == 0 && bb.bx == 4 && bb.by == 8 &&
          bb.bz == 8;
@@ -919,8 +932,7 @@ hipError_t spread(const GridGeom& g, const BlobBins& bb, const BlobData& bd, con
       hipLaunchKernelGGL((spread_fast_kernel<T>), dim3(8 * ((nt + 7) / 8)), dim3(256), 0, st, A, fp);
   } else {
     SpreadArgs A{g, bb, bd.ng, bd.bin_start, bd.rec, bd.w64, bd.w32, F};
-    if (bd.ng > SP_NGMAX || bb.sx * bb.bx > SP_TXMAX || bb.sy * bb.by * bb.sz * bb.bz > 256)
-      return hipErrorInvalidValue;
+    if (!spread_generic_ok(g, bb, bd.ng)) return hipErrorInvalidValue;
     const int ntiles = (g.nx / (bb.sx * bb.bx)) * (g.ny / (bb.sy * bb.by)) * (g.nz / (bb.sz * bb.bz));
     hipLaunchKernelGGL((spread_kernel<T>), dim3(ntiles), dim3(256), 0, st, A, fp);
   }

@@ -108,6 +108,7 @@ struct BlobData {
 
 // spreading tiles of the main path (0 if the grid does not qualify)
 bool spread_fast_ok(const GridGeom& g, const BlobBins& bb, int ng);
+bool spread_generic_ok(const GridGeom& g, const BlobBins& bb, int ng);
 int spread_tiles(const GridGeom& g);
 // slip-channel image rule on the three force fields after spreading (no-op
 // unless g.channel)
