@@ -145,6 +145,17 @@ int rb_mul_KT(fcm_handle h, const double* in3, double* out6);
  * F3 (3 Nb, blob order) receives a copy unless NULL.  Coincident pairs are
  * skipped (the reference divides by r = 0). */
 int ll_barrier_forces(fcm_handle h, double cutoff, double* F3);
+/* The reference's own interaction set for every later barrier-force
+ * evaluation of the handle (ll_barrier_forces, fcm_em_step, fcm_gdc_step):
+ * linked_list::initialise/populate (linked_list.cpp:23-102) on a
+ * cells_per_dim^3 grid (NUM_LINKED_LIST_CELLS_PER_DIM, config.hpp:71, is 6),
+ * cell floor(n Y/L) per axis, and every pair of blobs of different bodies in
+ * the same or neighbouring cells interacting at any distance (the cell and
+ * its 13-cell half shell, apply_barrier_forces :192-281); `cutoff` then only
+ * switches the forces on (> 0).  0 (the default) restores pairs closer than
+ * the cutoff on cells at least the cutoff wide.  cells_per_dim is 0 or 3..1024
+ * (with fewer than 3 cells the reference's half shell names one cell twice). */
+int ll_set_cells(fcm_handle h, int cells_per_dim);
 /* end_of_step_update (collection_of_rigid_bodies.cpp:759-790): every body moves
  * by nu*DT*[U; Omega] with particle::update (particle.cpp:45-56): x += ..., q =
  * lie_exp(...) * q (quaternion.cpp:128-145), x wrapped into the box; then the

@@ -828,6 +828,91 @@ void or_barrier_forces(const or_cfg* c, int Nb, int nbpb, const double* Y, doubl
   }
 }
 
+/* linked_list::initialise / populate / apply_barrier_forces
+ * (linked_list.cpp:23-70, 80-102, 192-281): an ncell^3 cell grid, cell of a
+ * blob floor(ncell (Y/L)) per axis (:74-78), head/list chains filled in blob
+ * order, then for every cell each of its blobs against the later blobs of
+ * its own chain (raw separation, :205-227) and against every blob of the 13
+ * half-shell cells (nearest image, :231-275), at any distance; each pair's
+ * force is added to the first blob and subtracted from the second, in the
+ * reference's loop order.  Wall springs as or_barrier_forces. */
+static int ll_cell_id(int i, int j, int k, int n) {
+  const int im = (int)fmod(i + n, n), jm = (int)fmod(j + n, n), km = (int)fmod(k + n, n);
+  return im + jm * n + km * n * n;  /* get_cell_id, :14-22 */
+}
+
+void or_barrier_forces_cells(const or_cfg* c, int Nb, int nbpb, const double* Y, int ncell, double* F) {
+  const double L[3] = {c->dx * c->nx, c->dx * c->ny, c->dx * c->nz};
+  const double U0 = 20.0 * c->kT, b = 0.5 * c->a, two_a = 2.0 * c->a;
+  const int n = ncell, ncells = n * n * n;
+  int* head = (int*)malloc(sizeof(int) * ncells);
+  int* list = (int*)malloc(sizeof(int) * (Nb > 0 ? Nb : 1));
+  int* map = (int*)malloc(sizeof(int) * 13 * ncells);
+  /* the half-shell map, :44-68 */
+  static const int off[13][3] = {{1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {-1, 1, 0}, {1, 0, -1}, {1, 1, -1}, {0, 1, -1},
+                                 {-1, 1, -1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}, {-1, 1, 1}, {0, 0, 1}};
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j)
+      for (int k = 0; k < n; ++k) {
+        const int me = ll_cell_id(i, j, k, n);
+        for (int m = 0; m < 13; ++m) map[13 * me + m] = ll_cell_id(i + off[m][0], j + off[m][1], k + off[m][2], n);
+      }
+  for (int i = 0; i < ncells; ++i) head[i] = -1;
+  for (int i = 0; i < Nb; ++i) {
+    const int cx = (int)floor(n * (Y[3 * i] / L[0]));
+    const int cy = (int)floor(n * (Y[3 * i + 1] / L[1]));
+    const int cz = (int)floor(n * (Y[3 * i + 2] / L[2]));
+    const int cell = cx + cy * n + cz * n * n;  /* find_containing_cell, :74-78 */
+    list[i] = head[cell];
+    head[cell] = i;
+  }
+  for (int i = 0; i < 3 * Nb; ++i) F[i] = 0.0;
+  for (int cell = 0; cell < ncells; ++cell) {
+    for (int b1 = head[cell]; b1 != -1; b1 = list[b1]) {
+      const int body1 = b1 / nbpb;
+      for (int b2 = list[b1]; b2 != -1; b2 = list[b2]) {
+        if (b2 / nbpb == body1) continue;
+        double d[3];
+        for (int k = 0; k < 3; ++k) d[k] = Y[3 * b1 + k] - Y[3 * b2 + k];
+        const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        if (r == 0.0) continue;
+        const double mag = r < two_a ? U0 / b : U0 * exp((two_a - r) / b) / b;
+        for (int k = 0; k < 3; ++k) {
+          F[3 * b1 + k] += mag * (d[k] / r);
+          F[3 * b2 + k] -= mag * (d[k] / r);
+        }
+      }
+      for (int m = 0; m < 13; ++m)
+        for (int b2 = head[map[13 * cell + m]]; b2 != -1; b2 = list[b2]) {
+          if (b2 / nbpb == body1) continue;
+          double d[3];
+          for (int k = 0; k < 3; ++k) {
+            const double dd = Y[3 * b1 + k] - Y[3 * b2 + k];
+            d[k] = dd - L[k] * trunc(2.0 * dd / L[k]);
+          }
+          const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+          if (r == 0.0) continue;
+          const double mag = r < two_a ? U0 / b : U0 * exp((two_a - r) / b) / b;
+          for (int k = 0; k < 3; ++k) {
+            F[3 * b1 + k] += mag * (d[k] / r);
+            F[3 * b2 + k] -= mag * (d[k] / r);
+          }
+        }
+    }
+  }
+  if (c->slip_channel) {
+    const double Lw = 2.2 * c->a, Kw = (40.0 * c->kT / c->a) / Lw;
+    for (int i = 0; i < Nb; ++i) {
+      const double y = Y[3 * i + 1];
+      if (y <= Lw) F[3 * i + 1] += Kw * (Lw - y);
+      else if ((0.5 * L[1] - y <= Lw) && (y <= 0.5 * L[1])) F[3 * i + 1] -= Kw * (Lw - 0.5 * L[1] + y);
+    }
+  }
+  free(head);
+  free(list);
+  free(map);
+}
+
 void or_lie_exp(const double u[3], double q[4]) {
   const double theta = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
   if (theta < 1e-10) {

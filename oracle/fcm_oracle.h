@@ -88,6 +88,11 @@ void or_mul_KT(int NB, int nbpb, const double* blob_ref, const double* q,
  * `cutoff` (the reference's cell traversal bounds the range instead).
  * Brute force over all pairs, in blob order; coincident pairs are skipped. */
 void or_barrier_forces(const or_cfg* c, int Nb, int nbpb, const double* Y, double cutoff, double* F);
+/* The reference's own interaction set (linked_list.cpp:23-102,192-281):
+ * every pair of blobs of different bodies in one cell of an ncell^3 grid or
+ * in neighbouring cells (its 13-cell half shell), at any distance, in the
+ * reference's head/list traversal order. */
+void or_barrier_forces_cells(const or_cfg* c, int Nb, int nbpb, const double* Y, int ncell, double* F);
 /* quaternion.cpp:128-145: exp map of a rotation vector (identity below 1e-10) */
 void or_lie_exp(const double u[3], double q[4]);
 /* p * q (quaternion.hpp operator*) */

@@ -79,6 +79,7 @@ def lib():
         L.or_mul_K.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp]
         L.or_mul_KT.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp]
         L.or_barrier_forces.argtypes = [C.POINTER(OrCfg), C.c_int, C.c_int, _dp, C.c_double, _dp]
+        L.or_barrier_forces_cells.argtypes = [C.POINTER(OrCfg), C.c_int, C.c_int, _dp, C.c_int, _dp]
         L.or_lie_exp.argtypes = [_dp, _dp]
         L.or_quat_mul.argtypes = [_dp, _dp, _dp]
         L.or_end_of_step_update.argtypes = [C.POINTER(OrCfg), C.c_int, _dp, C.c_double, _dp, _dp]
@@ -207,6 +208,15 @@ def barrier_forces(c, Y, nbpb, cutoff):
     Y = _f64(Y).ravel()
     F = np.zeros_like(Y)
     lib().or_barrier_forces(C.byref(c), Y.size // 3, nbpb, Y, cutoff, F)
+    return F
+
+
+def barrier_forces_cells(c, Y, nbpb, ncell):
+    """The reference's cell-list interaction set (linked_list.cpp:23-102,192-281)
+    on an ncell^3 grid: neighbouring-cell pairs at any distance."""
+    Y = _f64(Y).ravel()
+    F = np.zeros_like(Y)
+    lib().or_barrier_forces_cells(C.byref(c), Y.size // 3, nbpb, Y, ncell, F)
     return F
 
 

@@ -284,3 +284,73 @@ def test_fp32_refined_steps_track_fp64_steps(channel):
     say("fp32 vs fp64 steps", channel, np.abs(d32 - d64).max(), np.abs(d64).max())
     assert np.abs(d32 - d64).max() < 1e-5 * np.abs(d64).max()
     assert np.abs(out[fa.FCM_PREC_FP32][1] - out[fa.FCM_PREC_FP64][1]).max() < 1e-5
+
+
+@pytest.mark.parametrize("n,NB,nbpb,cells,channel", [(32, 60, 6, 6, False), (24, 40, 4, 3, False),
+                                                     (48, 200, 8, 6, False), (32, 30, 4, 6, True),
+                                                     (48, 80, 5, 7, False)])
+def test_reference_cell_interaction_set_matches_oracle(n, NB, nbpb, cells, channel):
+    """ll_set_cells: the reference's interaction set on a cells^3 grid, every
+    pair in the same or neighbouring cells at any distance (linked_list.cpp:
+    23-102,192-281), against the oracle's restatement of that traversal."""
+    say("reference cells", n, NB, nbpb, cells, channel)
+    a = 1.86
+    gc, oc = pair_cfg(n, a=a, kT=1.0, max_blobs=NB * nbpb, max_bodies=NB, max_blobs_per_body=nbpb,
+                      slip_channel=channel)
+    rng = np.random.default_rng(n + NB + cells)
+    ref = rod(nbpb, a)
+    x = rng.uniform(0, n, (NB, 3))
+    if channel:
+        x[:, 1] = rng.uniform(a + 0.5, n / 2 - a - 0.5, NB)
+        th = rng.uniform(0, 2 * np.pi, NB)  # rods turned about y keep their spheres at the centre's y
+        q = np.stack([np.cos(th / 2), np.zeros(NB), np.sin(th / 2), np.zeros(NB)], axis=1)
+    else:
+        q = random_quats(rng, NB)
+    with fa.FcmSolver(gc) as s:
+        s.set_geometry(ref)
+        s.set_bodies(x, q)
+        Y = po.fill_blob_pos(oc, ref, x, q)
+        Fo = po.barrier_forces_cells(oc, Y, nbpb, cells).reshape(-1, 3)
+        s.set_steric_cells(cells)
+        F = s.barrier_forces(1.0)  # the cutoff only switches the forces on
+        say("  rel", rel(F, Fo))
+        assert np.abs(Fo).max() > 0
+        assert rel(F, Fo) < 1e-12, rel(F, Fo)
+        assert np.array_equal(F, s.barrier_forces(1.0))
+        s.set_steric_cells(0)
+        Fc = s.barrier_forces(2.2 * a)
+        assert rel(Fc, po.barrier_forces(oc, Y, nbpb, 2.2 * a).reshape(-1, 3)) < 1e-12
+
+
+def test_euler_maruyama_step_with_reference_cells_matches_oracle():
+    n, NB, nbpb, dt, tol, cells = 32, 12, 6, 0.20216, 1e-10, 6
+    gc, oc = pair_cfg(n, kT=1.0, dt=dt, max_blobs=NB * nbpb, max_bodies=NB, max_blobs_per_body=nbpb, max_krylov=150)
+    ref, x, q, _, _ = fa.synth_rods(23, NB, nbpb, 1.86, [float(n)] * 3, max_attempts=20)
+    o = po.Fcm(oc, NB * nbpb)
+    st, M, Minv, N = o.ref_matrices(ref)
+    xo, qo = x.ravel().copy(), q.ravel().copy()
+    with fa.FcmSolver(gc) as s:
+        s.set_geometry(ref)
+        s.set_bodies(x, q)
+        s.set_ref_matrices(Minv, N)
+        s.set_body_forces(np.zeros((NB, 6)))
+        s.set_steric_cells(cells)
+        for step in range(2):
+            seed = 700 + step
+            Y = po.fill_blob_pos(oc, ref, xo, qo)
+            o.set_up_gaussians(Y)
+            o.fill_stress(seed)
+            o.random_field()
+            rhs = np.concatenate([-o.random_interp(),
+                                  -po.mul_KT(ref, qo, po.barrier_forces_cells(oc, Y, nbpb, cells))])
+            st, sol, it, _ = po.gmres(o, po.Bodies(ref, xo, qo, Minv, N), rhs, tol=tol, max_iter=150)
+            assert st == 0
+            xo, qo = po.end_of_step_update(oc, sol[3 * NB * nbpb:], dt, xo, qo)
+            rep = s.em_step(seed, 1.0, tol=tol, max_iter=150)
+            xg, qg = s.bodies()
+            dx = np.abs(xg.ravel() - xo)
+            dx = np.minimum(dx, n - dx)
+            say("step", step, rep, "max |dx|", dx.max())
+            assert rep["status"] == "converged"
+            assert dx.max() < 1e-9 and np.abs(qg.ravel() - qo).max() < 1e-9
+    o.close()

@@ -37,7 +37,10 @@ def test_query_footprint_and_rejects_bad_configs():
     with pytest.raises(fa.FcmError):
         fa.footprint(fa.make_config(14))          # 7 = nz/2 is not 2^a 3^b
     with pytest.raises(fa.FcmError):
-        fa.footprint(fa.make_config(16, ng=20))   # stencil wider than the grid / than 16
+        fa.footprint(fa.make_config(16, ng=20))   # stencil wider than the grid
+    with pytest.raises(fa.FcmError):
+        fa.footprint(fa.make_config(64, ng=21))   # wider than the reference's 20 points
+    assert fa.footprint(fa.make_config(64, ng=20)) > 0  # NUM_FCM_GAUSSIAN_POINTS 20 (config.hpp:69)
 
 
 def test_synth_rods_geometry_and_forces():

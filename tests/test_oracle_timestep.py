@@ -87,3 +87,57 @@ def test_end_of_step_update_matches_scipy_composition():
         R0 = Rotation.from_quat(np.r_[q[n, 1:], q[n, 0]]).as_matrix()
         R1 = Rotation.from_quat(np.r_[q1[4 * n + 1:4 * n + 4], q1[4 * n]]).as_matrix()
         assert np.allclose(R1, Rotation.from_rotvec(s * V6[n, 3:]).as_matrix() @ R0, atol=1e-13)
+
+
+def _adjacent(ci, cj, n):
+    """Cells of an n^3 periodic grid that are the same or neighbours."""
+    d = np.abs(ci - cj)
+    d = np.minimum(d, n - d)
+    return np.all(d <= 1, axis=-1)
+
+
+def test_reference_cell_traversal_covers_every_pair_once_on_three_cells():
+    """On a 3^3 cell grid every cell neighbours every other one, so the
+    reference's traversal (each cell against itself and its 13 half-shell
+    cells, linked_list.cpp:192-281) must give every pair exactly once: the
+    all-pairs brute force with no cutoff."""
+    c = _cfg(n=24, a=1.0)
+    rng = np.random.default_rng(3)
+    Y = rng.uniform(0, 24, (90, 3))
+    F = po.barrier_forces_cells(c, Y, 3, 3).reshape(-1, 3)
+    G = po.barrier_forces(c, Y, 3, 1e9).reshape(-1, 3)
+    assert np.abs(G).max() > 0
+    assert np.allclose(F, G, rtol=1e-12, atol=1e-14 * np.abs(G).max())
+
+
+def test_reference_cell_traversal_is_the_sum_over_neighbouring_cell_pairs():
+    """For n >= 4 the interaction set is the pairs of blobs of different bodies
+    whose cells floor(n Y/L) are the same or neighbours (periodically), at any
+    distance: the result is the superposition of the two-blob forces over
+    exactly those pairs.  Channel wall springs come on top."""
+    for n_cells, channel, seed in ((4, False, 5), (6, False, 6), (5, True, 7)):
+        c = po.cfg(20, ng=7, dx=1.0, a=1.0, mu=1.0, kT=1.3, dt=0.1, slip_channel=channel)
+        rng = np.random.default_rng(seed)
+        nbpb = 2
+        Y = rng.uniform(0, 20, (48, 3))
+        if channel:
+            Y[:, 1] = rng.uniform(0.5, 9.5, 48)
+        F = po.barrier_forces_cells(c, Y, nbpb, n_cells).reshape(-1, 3)
+        cells = np.floor(n_cells * (Y / 20.0)).astype(int)
+        # wall springs alone: each blob as its own body, far from every other
+        W = np.stack([po.barrier_forces(c, Y[i:i + 1], 1, 1e-9).reshape(3) for i in range(len(Y))])
+        S = W.copy()
+        c0 = po.cfg(20, ng=7, dx=1.0, a=1.0, mu=1.0, kT=1.3, dt=0.1)
+        far = 0
+        for i in range(len(Y)):
+            for j in range(len(Y)):
+                if i // nbpb == j // nbpb or not _adjacent(cells[i], cells[j], n_cells):
+                    continue
+                S[i] += po.barrier_forces(c0, Y[[i, j]], 1, 1e9).reshape(2, 3)[0]
+                d = Y[i] - Y[j]
+                d -= 20.0 * np.round(d / 20.0)
+                far += np.linalg.norm(d) > 20.0 / n_cells
+        assert far > 0, "some interacting pairs must lie farther apart than one cell width"
+        assert np.allclose(F, S, rtol=1e-12, atol=1e-13 * np.abs(S).max()), (n_cells, channel)
+        if not channel:
+            assert np.abs(F.sum(axis=0)).max() < 1e-12 * np.abs(F).max()

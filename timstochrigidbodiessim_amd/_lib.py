@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(HERE, "libfcm_hip.so")
 
 FCM_PREC_FP64 = 0
 FCM_PREC_FP32 = 1
+FCM_ERR_ARG = 1
+FCM_ERR_CAPACITY = 6
 
 GMRES_STATUS = {0: "converged", 1: "max_iter", 2: "zero_rhs", 3: "breakdown"}
 
@@ -25,7 +27,7 @@ EXPORTS = [
     "fcm_ref_matrices", "fcm_set_ref_matrices",
     "fft_transform", "fft_inverse_transform", "fcm_get_kspace", "fcm_set_kspace",
     "rb_set_geometry", "rb_set_bodies", "rb_set_body_forces", "rb_get_blob_positions",
-    "rb_mul_K", "rb_mul_KT", "ll_barrier_forces", "rb_end_of_step_update", "rb_get_bodies",
+    "rb_mul_K", "rb_mul_KT", "ll_barrier_forces", "ll_set_cells", "rb_end_of_step_update", "rb_get_bodies",
     "fcm_shear_field", "fcm_shear_interp", "fcm_x_velocity_profile",
     "rb_channel_wall_overlap", "rb_reset_to_start_of_step", "fcm_em_step", "fcm_gdc_step",
     "rb_advance_to_midstep", "rb_calculate_divU",
@@ -116,6 +118,7 @@ def lib():
         L.rb_mul_K.argtypes = [H, _dp, _dp]
         L.rb_mul_KT.argtypes = [H, _dp, _dp]
         L.ll_barrier_forces.argtypes = [H, C.c_double, _dp]
+        L.ll_set_cells.argtypes = [H, C.c_int]
         L.rb_end_of_step_update.argtypes = [H, _dp, C.c_double]
         L.rb_get_bodies.argtypes = [H, _dp, _dp]
         L.rb_channel_wall_overlap.argtypes = [H, C.POINTER(C.c_int)]
@@ -340,6 +343,10 @@ class FcmSolver:
         F = np.zeros(3 * self.NB * self.nbpb)
         _chk(self.L.ll_barrier_forces(self.h, cutoff, _ptr(F)))
         return F.reshape(-1, 3)
+
+    def set_steric_cells(self, cells_per_dim):
+        """The reference's interaction set on a cells_per_dim^3 grid (no cutoff), or 0 for the cutoff rule."""
+        _chk(self.L.ll_set_cells(self.h, int(cells_per_dim)))
 
     def end_of_step_update(self, V6=None, nu=1.0):
         """Move the bodies by nu*dt*V6 (host [U; Omega] per body, or the device solution)."""

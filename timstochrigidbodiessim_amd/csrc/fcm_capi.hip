@@ -1203,19 +1203,24 @@ int fcm_mobility_prepare(fcm_handle h, uint64_t seed) {
   return FCM_OK;
 }
 
+int ll_set_cells(fcm_handle h, int cells_per_dim) {
+  if (int rc = check(h)) return rc;
+  if (cells_per_dim != 0 && (cells_per_dim < 3 || cells_per_dim > 1024))
+    return fail(FCM_ERR_ARG, "cells_per_dim must be 0 or in 3..1024");
+  h->ll_cells = cells_per_dim;
+  h->have_blob_forces = false;
+  h->have_rhs = false;
+  return FCM_OK;
+}
+
 int ll_barrier_forces(fcm_handle h, double cutoff, double* F3) {
   if (int rc = check(h)) return rc;
   if (!h->have_bodies) return fail(FCM_ERR_STATE, "rb_set_bodies first");
   if (!(cutoff > 0.0)) return fail(FCM_ERR_ARG, "cutoff must be positive");
   const fcm_config& c = h->cfg;
-  StericCells cg;
-  const int n[3] = {c.nx, c.ny, c.nz};
-  size_t ncells = 1;
-  for (int d = 0; d < 3; ++d) {
-    cg.L[d] = c.dx * n[d];
-    cg.nc[d] = std::max(1, std::min(1024, (int)std::floor(cg.L[d] / cutoff)));
-    ncells *= (size_t)cg.nc[d];
-  }
+  const double L[3] = {c.dx * c.nx, c.dx * c.ny, c.dx * c.nz};
+  const StericCells cg = steric_cells(L, cutoff, h->ll_cells);
+  const size_t ncells = (size_t)cg.nc[0] * cg.nc[1] * cg.nc[2];
   if ((int)ncells > h->sw.cap_cells) {
     HIPCHK(hipStreamSynchronize(h->st));
     for (void* p : h->lazy) (void)hipFree(p);
@@ -1234,7 +1239,8 @@ int ll_barrier_forces(fcm_handle h, double cutoff, double* F3) {
     h->sw.cap_cells = (int)ncells;
   }
   const int nb = h->NB * h->nbpb;
-  HIPCHK(barrier_forces(h->dY, nb, h->nbpb, cg, cutoff, c.kT, c.a, c.slip_channel != 0, h->sw, h->d_fblob, h->st));
+  HIPCHK(barrier_forces(h->dY, nb, h->nbpb, cg, cutoff, h->ll_cells > 0, c.kT, c.a, c.slip_channel != 0, h->sw,
+                        h->d_fblob, h->st));
   h->have_blob_forces = true;
   h->have_rhs = false;
   if (F3) HIPCHK(hipMemcpyAsync(F3, h->d_fblob, 3 * (size_t)nb * sizeof(double), hipMemcpyDeviceToHost, h->st));

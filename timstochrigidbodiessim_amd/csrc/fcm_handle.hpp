@@ -70,6 +70,7 @@ struct fcm_handle_s {
   int* d_flag = nullptr;        // one-word result of the wall-overlap check
   bool have_prev = false;
   StericWork sw;                // cell list of the barrier forces
+  int ll_cells = 0;             // > 0: the reference's cells^3 interaction set (ll_set_cells)
   std::vector<void*> lazy;      // cell arrays, grown on demand
   bool have_blob_forces = false;
   bool have_geometry = false, have_bodies = false, have_ref = false, have_forces = false;

@@ -228,6 +228,24 @@ struct StericCells {
   int nc[3];
   double L[3];
 };
+// The cell grid of a barrier-force evaluation.  cells_per_dim > 0: the
+// reference's NUM_LINKED_LIST_CELLS_PER_DIM^3 grid (config.hpp:71), every pair
+// in neighbouring cells interacting at any distance (linked_list.cpp:23-102);
+// else cells at least `cutoff` wide (at most 1024 per axis), pairs closer than
+// the cutoff.
+inline StericCells steric_cells(const double L[3], double cutoff, int cells_per_dim) {
+  StericCells cg;
+  for (int d = 0; d < 3; ++d) {
+    cg.L[d] = L[d];
+    if (cells_per_dim > 0) {
+      cg.nc[d] = cells_per_dim;
+    } else {
+      const double w = L[d] / cutoff;
+      cg.nc[d] = w >= 1024.0 ? 1024 : (w < 1.0 ? 1 : (int)w);
+    }
+  }
+  return cg;
+}
 struct StericWork {
   int* cell_of = nullptr;  // nb
   int* sorted = nullptr;   // nb
@@ -236,8 +254,9 @@ struct StericWork {
   int* cursor = nullptr;   // ncells
   int cap_cells = 0;
 };
-hipError_t barrier_forces(const double* Y, int nb, int nbpb, const StericCells& cg, double cutoff, double kT, double a,
-                          bool channel, StericWork& w, double* F, hipStream_t st);
+// all_pairs: every pair in neighbouring cells (no cutoff), else pairs closer than cutoff
+hipError_t barrier_forces(const double* Y, int nb, int nbpb, const StericCells& cg, double cutoff, bool all_pairs,
+                          double kT, double a, bool channel, StericWork& w, double* F, hipStream_t st);
 // out = P^{-1} in; if wKT != null also writes wKT = -K^T out_lambda (6 per body)
 hipError_t apply_pinv(const BodyData& b, const double* in, double* out, double* wKT, hipStream_t st);
 hipError_t mul_K(const BodyData& b, const double* in6, double* out3, hipStream_t st);

@@ -4,18 +4,22 @@
 // linked_list.cpp builds a 6^3 cell list on the host (populate, :80-102) and
 // walks each cell against itself and 13 neighbours, adding the soft-sphere
 // force of every pair of blobs in different bodies to both blobs
-// (apply_barrier_forces, :192-281).  Here the cells are at least the cutoff
-// wide, blobs are counting-sorted by cell (ascending blob index inside a
-// cell), and every blob GATHERS the forces of the blobs in its 27 surrounding
-// cells in a fixed order: no atomics on the forces, bitwise repeatable, and
-// the same per-pair terms as the reference (action and reaction are computed
-// by the two blobs separately and are exact negatives).
+// (apply_barrier_forces, :192-281).  Here blobs are counting-sorted by cell
+// (ascending blob index inside a cell), and every blob GATHERS the forces of
+// the blobs in its 27 surrounding cells in a fixed order: no atomics on the
+// forces, bitwise repeatable, and the same per-pair terms as the reference
+// (action and reaction are computed by the two blobs separately and are exact
+// negatives).  Two interaction sets: the reference's (an N^3 grid, every pair
+// in the same or neighbouring cells at any distance: for N >= 3 the 27 cells
+// around a blob are the cell, its 13 half-shell cells and theirs mirrored) or
+// pairs closer than a cutoff, on cells at least the cutoff wide.
 #include "fcm_internal.hpp"
 
 namespace fcm {
 
+// find_containing_cell (linked_list.cpp:74-78): floor(nc (y / L)), kept in range
 __device__ __forceinline__ int cell_coord(double y, double L, int nc) {
-  int c = (int)(y * nc / L);
+  int c = (int)floor(nc * (y / L));
   return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
 }
 
@@ -90,8 +94,8 @@ __global__ void __launch_bounds__(256) ll_force_kernel(const double* __restrict_
   F[3 * (size_t)i + 2] = fz;
 }
 
-hipError_t barrier_forces(const double* Y, int nb, int nbpb, const StericCells& cg, double cutoff, double kT, double a,
-                          bool channel, StericWork& w, double* F, hipStream_t st) {
+hipError_t barrier_forces(const double* Y, int nb, int nbpb, const StericCells& cg, double cutoff, bool all_pairs,
+                          double kT, double a, bool channel, StericWork& w, double* F, hipStream_t st) {
   if (nb <= 0) return hipSuccess;
   const int ncells = cg.nc[0] * cg.nc[1] * cg.nc[2];
   if (ncells > w.cap_cells) return hipErrorInvalidValue;
@@ -103,7 +107,7 @@ hipError_t barrier_forces(const double* Y, int nb, int nbpb, const StericCells& 
   const double U0 = 20.0 * kT, bl = 0.5 * a;  // linked_list.cpp:367-368
   const double wall_L = channel ? 2.2 * a : 0.0, wall_K = channel ? (40.0 * kT / a) / wall_L : 0.0;
   hipLaunchKernelGGL(ll_force_kernel, dim3((nb + 255) / 256), dim3(256), 0, st, Y, nb, nbpb, cg, w.start, w.sorted,
-                     cutoff * cutoff, U0, bl, 2.0 * a, wall_L, wall_K, F);
+                     all_pairs ? HUGE_VAL : cutoff * cutoff, U0, bl, 2.0 * a, wall_L, wall_K, F);
   return hipGetLastError();
 }
 
