@@ -349,6 +349,26 @@ int fcm_profile_matvec(fcm_handle h, int precision, int reps, double* ms_per_pas
  * and for the achievable copy bandwidth.  *ms_per_copy = mean device time. */
 int fcm_profile_copy(fcm_handle h, size_t bytes, int reps, double* ms_per_copy);
 
+/* ---- an entry point that allocates nothing -------------------------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the blob forces of ll_barrier_forces, bitwise the same,
+ * with the cell list in a static device array of FCM_NOALLOC_SCRATCH_BYTES on
+ * every device the library runs on (held until the process ends) instead of
+ * memory the library allocates.  Y (nb x 3, wrapped into [0, L)) and F (3 nb)
+ * are DEVICE pointers; body of blob i = i / nbpb; L the box (dx n per axis);
+ * a, kT as fcm_config; the interaction set is ll_set_cells's (cells_per_dim 0:
+ * pairs closer than `cutoff`; 3..1024: the reference's cells, `cutoff` > 0 is
+ * then only a switch); slip_channel adds the wall springs.  Work is enqueued
+ * on `stream` (a hipStream_t; NULL: the null stream) and not waited for.
+ * FCM_ERR_CAPACITY if the cell list (2 nb + 3 cells + 1 ints) exceeds the
+ * array.  Calls that overlap on one device share the array and race.  The
+ * first call on a device resolves the array's address there (make it before
+ * any graph capture). */
+#define FCM_NOALLOC_SCRATCH_BYTES (48u << 20)
+int ll_barrier_forces_noalloc(const double* Y, int nb, int nbpb, const double L[3], double a, double kT,
+                              double cutoff, int cells_per_dim, int slip_channel, double* F, void* stream);
+
 /* ---- synthetic inputs (host only; no GPU needed) ---------------------------- */
 /* Rods of nbpb spheres spaced 2a on the body x axis, centres uniform in the box
  * with rejection of inter-body blob pairs closer than 2a (minimum image), at

@@ -16,6 +16,7 @@ FCM_PREC_FP64 = 0
 FCM_PREC_FP32 = 1
 FCM_ERR_ARG = 1
 FCM_ERR_CAPACITY = 6
+FCM_NOALLOC_SCRATCH_BYTES = 48 << 20  # include/fcm_hip.h
 
 GMRES_STATUS = {0: "converged", 1: "max_iter", 2: "zero_rhs", 3: "breakdown"}
 
@@ -36,7 +37,7 @@ EXPORTS = [
     "fcm_batch_create", "fcm_batch_destroy", "fcm_batch_submit", "fcm_batch_step", "fcm_batch_take",
     "fcm_batch_run",
     "fcm_get_tuning", "fcm_set_tuning", "fcm_autotune",
-    "fcm_profile_matvec", "fcm_profile_copy", "fcm_synth_rods",
+    "fcm_profile_matvec", "fcm_profile_copy", "ll_barrier_forces_noalloc", "fcm_synth_rods",
 ]
 
 PASS_NAMES = ["spread", "z_forward", "y_forward", "x_project", "y_inverse", "z_inverse", "interp", "KT",
@@ -119,6 +120,8 @@ def lib():
         L.rb_mul_KT.argtypes = [H, _dp, _dp]
         L.ll_barrier_forces.argtypes = [H, C.c_double, _dp]
         L.ll_set_cells.argtypes = [H, C.c_int]
+        L.ll_barrier_forces_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_double, C.c_double,
+                                                C.c_double, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
         L.rb_end_of_step_update.argtypes = [H, _dp, C.c_double]
         L.rb_get_bodies.argtypes = [H, _dp, _dp]
         L.rb_channel_wall_overlap.argtypes = [H, C.POINTER(C.c_int)]
