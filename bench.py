@@ -87,10 +87,13 @@ def measured_traffic(kernel, grid, nb):
     return None
 
 
-def cpu_baseline(args, ref, x, q, F6, fp64_iters):
+def cpu_baseline(args, ref, x, q, F6, fp64_iters, Minv, N):
     """Times the CPU oracle (OpenMP over x slabs) on a bounded sample of the
-    same workload: one fluctuating-stress field and one saddle-point mat-vec
-    at full size, extrapolated to a solve with the GPU FP64 iteration count."""
+    same workload and sums a solve from the measured parts: one fluctuating
+    stress field, one saddle-point mat-vec and one P^-1 at full size per
+    iteration, and the modified Gram-Schmidt step of each iteration
+    (gmres_solver.cpp:198-210), measured against 1, it/2 and it basis vectors
+    and summed over the GPU FP64 iteration count `it` (linear in j)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
@@ -99,7 +102,7 @@ def cpu_baseline(args, ref, x, q, F6, fp64_iters):
     o = po.Fcm(oc, x.shape[0] * ref.shape[0])
     Y = po.fill_blob_pos(oc, ref, x, q)
     o.set_up_gaussians(Y)
-    b = po.Bodies(ref, x, q, np.eye(3 * ref.shape[0]), np.eye(6))
+    b = po.Bodies(ref, x, q, Minv, N)
     t0 = time.perf_counter()
     o.fill_stress(1)
     o.random_field()
@@ -109,17 +112,42 @@ def cpu_baseline(args, ref, x, q, F6, fp64_iters):
     t0 = time.perf_counter()
     po.saddle_matvec(o, b, v)
     t_mv = time.perf_counter() - t0
-    t_solve = t_rand + (fp64_iters + 1) * t_mv
+    t0 = time.perf_counter()
+    po.apply_Pinv(b, v)
+    t_pinv = time.perf_counter() - t0
     o.close()
+    it = max(2, fp64_iters)
+    js = (1, it // 2, it)
+    t_orth = [statistics.median(po.gmres_orth_seconds(b.n, j) for _ in range(3)) for j in js]
+    slope = (t_orth[2] - t_orth[0]) / (it - 1)
+    mgs = sum(t_orth[0] + slope * (j - 1) for j in range(1, it + 1))
+    t_solve = t_rand + it * (t_mv + t_pinv) + mgs + t_pinv
     return {
         "value": 1.0 / t_solve,
         "unit": "solves/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"oracle (C restatement, FP64, {threads} OpenMP threads) at {n}^3 with {Y.size // 3} blobs: "
-                   f"1 fluctuating field {t_rand:.2f} s + 1 saddle mat-vec {t_mv:.2f} s, extrapolated to "
-                   f"{fp64_iters} GMRES iterations + the final P^-1 product (MGS dot products not counted)"),
+        "sample": (f"oracle (C restatement, FP64, {threads} OpenMP threads) at {n}^3 with {Y.size // 3} blobs, "
+                   f"a solve summed from measured parts: 1 fluctuating field {t_rand:.2f} s + {it} iterations x "
+                   f"(saddle mat-vec {t_mv:.2f} s + P^-1 {t_pinv * 1e3:.1f} ms) + modified Gram-Schmidt "
+                   f"(gmres_solver.cpp:198-210) measured at j = {js[0]}, {js[1]}, {js[2]} basis vectors "
+                   f"({', '.join(f'{t * 1e3:.1f}' for t in t_orth)} ms) and summed over j = 1..{it}: "
+                   f"{mgs:.2f} s; total {t_solve:.1f} s"),
     }
+
+
+def fp64_true_residuals(s, seeds, args, F6, fa):
+    """||b - A x||/||b|| of the FP64 solves of `seeds` with the FP64 operator
+    (the plain solve stops on the Givens estimate): each seed's solve is run
+    again after the timed region (bitwise the same solve) and one FP64
+    saddle-point product is applied to its solution."""
+    out = []
+    for sd in seeds:
+        rep = s.mobility_solve(sd, tol=args.tol, max_iter=args.max_iter, precision=fa.FCM_PREC_FP64)
+        sol = s.solution()
+        rhs = np.concatenate([-s.random_interp(), -np.asarray(F6).ravel()])
+        out.append((rep["iterations"], float(np.linalg.norm(rhs - s.saddle_matvec(sol)) / np.linalg.norm(rhs))))
+    return out
 
 
 def bench_batch(args):
@@ -376,7 +404,7 @@ def main():
     s = fa.FcmSolver(cfg)
     s.set_geometry(ref)
     t0 = time.perf_counter()
-    s.ref_matrices()
+    _, Minv, Nref = s.ref_matrices()
     log(f"[bench] reference matrices: {time.perf_counter() - t0:.2f} s")
     s.set_bodies(x, q)
     s.set_body_forces(F6)
@@ -406,6 +434,12 @@ def main():
         s.L.fcm_sync(s.h)
         total = time.perf_counter() - t_start
         log(f"[bench] {name}: {args.steps} solves in {total:.3f} s; last {reps[-1]}")
+        if prec == fa.FCM_PREC_FP64:
+            tr = fp64_true_residuals(s, range(1, args.steps + 1), args, F6, fa)
+            assert [i for i, _ in tr] == [r["iterations"] for r in reps], "FP64 re-run differs from the timed solve"
+            for r, (_, res) in zip(reps, tr):
+                r["true_rel_residual"] = res
+            log(f"[bench] fp64 true residuals (untimed re-runs): max {max(t for _, t in tr):.3e}")
         results[name] = {
             "solves_per_s": args.steps / total,
             "ms_per_step": 1e3 * total / args.steps,
@@ -458,7 +492,7 @@ def main():
         try:
             it64 = int(np.median(results["fp64"]["iterations"])) if "fp64" in results else 50
             t0 = time.perf_counter()
-            cpu = cpu_baseline(args, ref, x, q, F6, it64)
+            cpu = cpu_baseline(args, ref, x, q, F6, it64, Minv, Nref)
             log(f"[bench] cpu baseline ({time.perf_counter() - t0:.1f} s): {cpu}")
         except Exception as e:  # the baseline is reported, never required
             log(f"[bench] cpu baseline failed: {e!r}")

@@ -169,13 +169,18 @@ int rb_get_bodies(fcm_handle h, double* x, double* q);
  * flow of the force density fx = scale kT sin(2 pi kappa y / ny) / a (images
  * in a slip channel), solved like produce_deterministic_velocity_field and kept
  * on the device (3 FP64 fields, allocated by the first call; the working
- * fields are overwritten, as the reference's fx..uz are).  While it is set,
- * every RHS (gmres_assemble_rhs, fcm_mobility_solve/_prepare, fcm_em_step)
- * subtracts its blob velocities at the current positions
- * (gmres_solver.cpp:62-64).  scale = 0 switches it off. */
+ * fields are overwritten, as the reference's fx..uz are).  As in the
+ * reference, assemble_RHS (gmres_assemble_rhs, fcm_mobility_solve/_prepare,
+ * fcm_em_step) subtracts the STORED shear blob velocities Vshear
+ * (gmres_solver.cpp:62-64), which only fcm_shear_interp
+ * (calculate_shear_blob_velocities) and the drifter-corrector mid-step of
+ * fcm_gdc_step (main.cpp:270-274) fill; main.cpp's Euler-Maruyama loop never
+ * fills them, so its steps carry no shear term.  A new field, or scale = 0
+ * (off), drops the stored velocities. */
 int fcm_shear_field(fcm_handle h, double shear_force_scale, double shear_wavenumber);
 /* calculate_shear_blob_velocities (fcm_fluid_solver.cpp:488-550): the shear
- * field interpolated at the current blobs (3 Nb, host; NULL: device only) */
+ * field interpolated at the current blobs, stored as the Vshear every later
+ * RHS subtracts, and copied to V (3 Nb, host; NULL: device only) */
 int fcm_shear_interp(fcm_handle h, double* V);
 /* main.cpp:371-433: the x velocity averaged over x and z on each of the ny rows
  * (prof[ny], host), of the flow of the blob forces lambda (3 Nb host, or NULL

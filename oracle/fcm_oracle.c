@@ -1275,6 +1275,42 @@ void or_apply_Pinv(const or_bodies* b, const double* in, double* out) {
   }
 }
 
+/* gmres_solver.cpp:198-210: modified Gram-Schmidt of the new vector qn against
+ * the iter basis vectors in Q (n each), h_i = <qn, q_i> then qn -= h_i q_i in
+ * turn, hcol[iter] = ||qn|| and qn normalised when that is > 0. */
+static double mgs_orth(size_t n, const double* Q, double* qn, int iter, double* hcol) {
+  for (int i = 0; i < iter; ++i) {
+    const double h = dotp(n, qn, Q + n * i);
+    hcol[i] = h;
+    const double* qi = Q + n * i;
+#pragma omp parallel for schedule(static)
+    for (long r = 0; r < (long)n; ++r) qn[r] -= h * qi[r];
+  }
+  const double q_norm = sqrt(dotp(n, qn, qn));
+  hcol[iter] = q_norm;
+  if (q_norm > 0.0) {
+#pragma omp parallel for schedule(static)
+    for (long r = 0; r < (long)n; ++r) qn[r] /= q_norm;
+  }
+  return q_norm;
+}
+
+double or_gmres_orth_seconds(size_t n, int iter) {
+  double* Q = (double*)malloc(sizeof(double) * n * (size_t)iter);
+  double* qn = (double*)malloc(sizeof(double) * n);
+  double* hcol = (double*)malloc(sizeof(double) * (iter + 1));
+#pragma omp parallel for schedule(static)
+  for (long r = 0; r < (long)n; ++r) {
+    for (int i = 0; i < iter; ++i) Q[(size_t)i * n + r] = sin(0.37 * (double)r + 1.3 * i);
+    qn[r] = cos(0.11 * (double)r);
+  }
+  const double t0 = omp_get_wtime();
+  mgs_orth(n, Q, qn, iter, hcol);
+  const double t1 = omp_get_wtime();
+  free(Q); free(qn); free(hcol);
+  return t1 - t0;
+}
+
 /* gmres_solver.cpp:148-306: unrestarted, right-preconditioned GMRES with
  * modified Gram-Schmidt, x0 = 0.  Returns 0 converged, 1 iteration cap hit,
  * 2 zero right-hand side (solution 0), 3 exact breakdown (solution exact).
@@ -1304,21 +1340,11 @@ int or_gmres(or_fcm* f, const or_bodies* b, const double* rhs, double* soln,
     double* qn = Q + n * iter;
     or_apply_Pinv(b, Q + n * (iter - 1), z);
     or_saddle_matvec(f, b, z, qn);
-    for (int i = 0; i < iter; ++i) {
-      const double h = dotp(n, qn, Q + n * i);
-      HH(i, iter - 1) = h;
-      const double* qi = Q + n * i;
-#pragma omp parallel for schedule(static)
-      for (long r = 0; r < (long)n; ++r) qn[r] -= h * qi[r];
-    }
-    const double q_norm = sqrt(dotp(n, qn, qn));
-    HH(iter, iter - 1) = q_norm;
-    if (q_norm > 0.0) {
-#pragma omp parallel for schedule(static)
-      for (long r = 0; r < (long)n; ++r) qn[r] /= q_norm;
-    } else {
-      breakdown = 1;
-    }
+    double* hcol = (double*)malloc(sizeof(double) * (iter + 1));
+    const double q_norm = mgs_orth(n, Q, qn, iter, hcol);
+    for (int i = 0; i <= iter; ++i) HH(i, iter - 1) = hcol[i];
+    free(hcol);
+    if (!(q_norm > 0.0)) breakdown = 1;
     for (int i = 1; i < iter; ++i) {
       const double temp = CS[i - 1] * HH(i - 1, iter - 1) + SN[i - 1] * HH(i, iter - 1);
       HH(i, iter - 1) = CS[i - 1] * HH(i, iter - 1) - SN[i - 1] * HH(i - 1, iter - 1);

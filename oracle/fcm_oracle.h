@@ -121,6 +121,10 @@ typedef struct {
   const double* N;          /* 36 row-major                             */
 } or_bodies;
 
+/* Wall seconds of one modified Gram-Schmidt step of or_gmres (gmres_solver.cpp:
+ * 198-210) against `iter` basis vectors of length n, on synthetic vectors: the
+ * orthogonalisation share of a CPU solve for the bench's baseline. */
+double or_gmres_orth_seconds(size_t n, int iter);
 void or_saddle_matvec(or_fcm* f, const or_bodies* b, const double* in, double* out);
 void or_apply_Pinv(const or_bodies* b, const double* in, double* out);
 int  or_gmres(or_fcm* f, const or_bodies* b, const double* rhs, double* soln,

@@ -89,6 +89,8 @@ def lib():
         L.or_ref_matrices.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp]
         L.or_saddle_matvec.argtypes = [C.c_void_p, C.POINTER(OrBodies), _dp, _dp]
         L.or_apply_Pinv.argtypes = [C.POINTER(OrBodies), _dp, _dp]
+        L.or_gmres_orth_seconds.argtypes = [C.c_size_t, C.c_int]
+        L.or_gmres_orth_seconds.restype = C.c_double
         L.or_gmres.argtypes = [C.c_void_p, C.POINTER(OrBodies), _dp, _dp, C.c_double, C.c_int,
                                C.POINTER(C.c_int), _dp]
         _lib = L
@@ -381,6 +383,11 @@ def apply_Pinv(bodies, v):
     out = np.zeros(bodies.n)
     lib().or_apply_Pinv(C.byref(bodies.s), _f64(v), out)
     return out
+
+
+def gmres_orth_seconds(n, iters):
+    """Seconds of one MGS step of the oracle's GMRES against `iters` vectors of length n."""
+    return lib().or_gmres_orth_seconds(n, iters)
 
 
 def gmres(fcm, bodies, rhs, tol=1e-8, max_iter=200):

@@ -73,6 +73,10 @@ def test_channel_fluctuating_solve_with_shear_and_profile_match_oracle():
         s.set_ref_matrices(Minv, N)
         s.set_body_forces(F6)
         s.shear_field(2.0, 1.0)
+        assert s.mobility_solve(93, tol=1e-10, max_iter=200)["status"] == "converged"
+        # no Vshear stored yet: the RHS has no shear term (main.cpp's EM loop)
+        assert rel(s.solution(), sol_o) > 1e-3
+        s.shear_interp()  # calculate_shear_blob_velocities, then assemble_RHS subtracts it
         rep = s.mobility_solve(93, tol=1e-10, max_iter=200)
         say("oracle iters", it_o, "gpu", rep)
         assert rep["status"] == "converged"

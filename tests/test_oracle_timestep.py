@@ -141,3 +141,105 @@ def test_reference_cell_traversal_is_the_sum_over_neighbouring_cell_pairs():
         assert np.allclose(F, S, rtol=1e-12, atol=1e-13 * np.abs(S).max()), (n_cells, channel)
         if not channel:
             assert np.abs(F.sum(axis=0)).max() < 1e-12 * np.abs(F).max()
+
+
+def _interp_and_gradient(o, oc, Y, names=("ux_rand", "uy_rand", "uz_rand")):
+    """Blob velocities V_m = dx^3 sum g_x g_y g_z u of the oracle's field and
+    their analytic derivatives dV_m/dY_m (3 x 3 per blob): the weight
+    g(Y) = exp(-(xg dx - Y)^2 / (2 a^2/pi)) / (sqrt 2 a) of node xg = int(Y/dx)
+    - ng/2 + i + 1 (fcm_fluid_solver.cpp:131-165) has dg/dY = g 2 (xg dx - Y) /
+    (2 a^2/pi); the stencil does not move under a displacement that keeps
+    int(Y/dx)."""
+    ng, dx, a = oc.ng, oc.dx, oc.a
+    n = (oc.nx, oc.ny, oc.nz)
+    pad = po.pad_of(oc)
+    u = [o.field(c).reshape(oc.nx, oc.ny, pad)[:, :, :oc.nz] for c in names]
+    Y = np.asarray(Y).reshape(-1, 3)
+    V = np.zeros((len(Y), 3))
+    G = np.zeros((len(Y), 3, 3))  # [blob, component, derivative axis]
+    s2 = 2 * a * a / np.pi
+    for m, y in enumerate(Y):
+        w, dw, idx = [], [], []
+        for d in range(3):
+            xg = int(y[d] / dx) - ng // 2 + np.arange(1, ng + 1)
+            g = np.exp(-(xg * dx - y[d]) ** 2 / s2) / (np.sqrt(2.0) * a)
+            w.append(g)
+            dw.append(g * 2 * (xg * dx - y[d]) / s2)
+            idx.append(np.mod(xg, n[d]))
+        for c in range(3):
+            box = u[c][np.ix_(idx[0], idx[1], idx[2])]
+            V[m, c] = dx ** 3 * np.einsum("i,j,k,ijk", w[0], w[1], w[2], box)
+            G[m, c, 0] = dx ** 3 * np.einsum("i,j,k,ijk", dw[0], w[1], w[2], box)
+            G[m, c, 1] = dx ** 3 * np.einsum("i,j,k,ijk", w[0], dw[1], w[2], box)
+            G[m, c, 2] = dx ** 3 * np.einsum("i,j,k,ijk", w[0], w[1], dw[2], box)
+    return V, G
+
+
+def _skew(e):
+    return np.array([[0.0, -e[2], e[1]], [e[2], 0.0, -e[0]], [-e[1], e[0], 0.0]])
+
+
+def test_gdc_divU_checker_matches_the_analytic_derivative():
+    """tests/_gdc.py's divU (calculate_divU, collection_of_rigid_bodies.cpp:
+    869-953: forward differences of the rigid fit of the interpolated random
+    field under six body displacements) against the derivative taken
+    analytically: d/de of U_n = diag(R,R) (K^T K)^-1 diag(R^T,R^T) [sum v;
+    sum (R r_m) x v] with v = V(Y(e)) from the analytic gradient of the
+    Gaussian interpolation and, for rotations, dR/de = [e_i]x R.  The
+    finite-difference error must be first order in the step: 100 times
+    smaller at step 1e-5 than at 1e-3, and below 1e-4 relative.  A dropped
+    term (the fit's dependence on orientation, a blob's own displacement)
+    or a wrong sign leaves an error that does not shrink with the step."""
+    import _gdc
+    n, NB, nbpb, a = 24, 3, 6, 1.86
+    oc = po.cfg(n, ng=7, dx=1.0, a=a, mu=1.0, kT=1.0, dt=0.2)
+    ref = _gdc.reference_rods(nbpb, a)
+    rng = np.random.default_rng(12)
+    x = rng.uniform(0, n, (NB, 3))
+    q = rng.standard_normal((NB, 4))
+    q /= np.linalg.norm(q, axis=1)[:, None]
+    o = po.Fcm(oc, NB * nbpb)
+    st, M, Minv, N = o.ref_matrices(ref)
+    fd = {}
+    for delta in (1e-3, 1e-5):
+        fd[delta] = _gdc.gdc_step(o, oc, ref, x, q, Minv, N, seed=31, cutoff=0.0, delta=delta, tol=1e-10)[2]
+    # the field the checker differentiated (same seed), and its analytic derivative
+    Y = po.fill_blob_pos(oc, ref, x, q).reshape(-1, 3)
+    o.set_up_gaussians(Y.ravel())
+    o.fill_stress(31)
+    o.random_field()
+    V, G = _interp_and_gradient(o, oc, Y)
+    assert np.allclose(V.ravel(), o.random_interp(), rtol=1e-12, atol=1e-14 * np.abs(V).max())
+    KTKi = _gdc.ktk_inv(ref)
+    an = 0.0
+    for b in range(NB):
+        R = po.quat_matrix(q[b]).reshape(3, 3)
+        d = (R @ ref.T).T  # R r_m
+        vb, Gb = V[b * nbpb:(b + 1) * nbpb], G[b * nbpb:(b + 1) * nbpb]
+
+        def fit(Rm, v):
+            dd = (Rm @ ref.T).T
+            w = np.concatenate([v.sum(0), np.cross(dd, v).sum(0)])
+            B = np.kron(np.eye(2), Rm)
+            return B @ KTKi @ B.T @ w
+
+        for i in range(6):
+            e = np.eye(3)[i % 3]
+            if i < 3:
+                dv = Gb @ e  # every blob moves by e
+                dU = fit(R, dv)
+            else:
+                dY = np.cross(e, d)
+                dv = np.einsum("mcd,md->mc", Gb, dY)
+                dR = _skew(e) @ R
+                w = np.concatenate([vb.sum(0), np.cross(d, vb).sum(0)])
+                dw = np.concatenate([dv.sum(0), (np.cross(dY, vb) + np.cross(d, dv)).sum(0)])
+                B, dB = np.kron(np.eye(2), R), np.kron(np.eye(2), dR)
+                dU = dB @ KTKi @ B.T @ w + B @ KTKi @ dB.T @ w + B @ KTKi @ B.T @ dw
+            an += dU[i]
+    e3, e5 = abs(fd[1e-3] - an), abs(fd[1e-5] - an)
+    print("divU analytic", an, "fd 1e-3", fd[1e-3], "fd 1e-5", fd[1e-5])
+    assert abs(an) > 1e-3
+    # forward differences: the error falls with the step, 100x for 100x
+    assert e5 < 1e-4 * abs(an) and 0.005 < e5 / e3 < 0.02, (e3, e5)
+    o.close()

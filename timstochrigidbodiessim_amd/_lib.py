@@ -356,10 +356,12 @@ class FcmSolver:
         _chk(self.L.rb_end_of_step_update(self.h, None if V6 is None else _ptr(_f64(V6).ravel()), nu))
 
     def shear_field(self, scale, wavenumber):
-        """Set the steady shear flow (scale 0: off); its blob velocities join every RHS."""
+        """Set the steady shear flow (scale 0: off).  The RHS subtracts the blob
+        velocities stored by shear_interp() (or a drifter-corrector mid-step)."""
         _chk(self.L.fcm_shear_field(self.h, scale, wavenumber))
 
     def shear_interp(self):
+        """calculate_shear_blob_velocities: computes, stores (for the RHS) and returns Vshear."""
         V = np.zeros(3 * self.nb)
         _chk(self.L.fcm_shear_interp(self.h, _ptr(V)))
         return V

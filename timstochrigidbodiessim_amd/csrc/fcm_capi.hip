@@ -295,15 +295,22 @@ int body_forces_dev(fcm_handle h, const double** out) {
   return FCM_OK;
 }
 
-// the shear field's blob velocities at the current positions
-// (calculate_shear_blob_velocities, fcm_fluid_solver.cpp:488-550) when a shear
-// field is set, for the RHS (gmres_solver.cpp:62-64); else null
-int shear_velocities_dev(fcm_handle h, const double** out) {
-  *out = nullptr;
+// calculate_shear_blob_velocities (fcm_fluid_solver.cpp:488-550): the shear
+// field interpolated at the current blobs, stored as the handle's Vshear
+int compute_shear_velocities(fcm_handle h) {
   if (!h->have_shear) return FCM_OK;
   if (!h->have_blobs) return fail(FCM_ERR_STATE, "no blob positions set");
   HIPCHK(interp<double>(h->g, h->bb, h->bd, h->fpshear(), h->Vshear, KTerm{}, h->st));
-  *out = h->Vshear;
+  h->have_vshear = true;
+  h->vshear_nb = h->bd.nb;
+  return FCM_OK;
+}
+
+// the Vshear that assemble_RHS subtracts (gmres_solver.cpp:62-64): the stored
+// vector of the last calculate_shear_blob_velocities, which main.cpp fills only
+// at the drifter-corrector mid-step (main.cpp:270-274); null if none is stored
+int shear_velocities_dev(fcm_handle h, const double** out) {
+  *out = (h->have_shear && h->have_vshear && h->vshear_nb == h->bd.nb) ? h->Vshear : nullptr;
   return FCM_OK;
 }
 
@@ -1296,6 +1303,8 @@ int rb_end_of_step_update(fcm_handle h, const double* V6, double nu) {
 
 int fcm_shear_field(fcm_handle h, double shear_force_scale, double shear_wavenumber) {
   if (int rc = check(h)) return rc;
+  h->have_vshear = false;  // the stored blob velocities belonged to the previous field
+  h->have_rhs = false;
   if (shear_force_scale == 0.0) {
     h->have_shear = false;
     return FCM_OK;
@@ -1320,9 +1329,9 @@ int fcm_shear_field(fcm_handle h, double shear_force_scale, double shear_wavenum
 int fcm_shear_interp(fcm_handle h, double* V) {
   if (int rc = check(h)) return rc;
   if (!h->have_shear) return fail(FCM_ERR_STATE, "no shear field (fcm_shear_field)");
-  const double* Vs = nullptr;
-  if (int rc = shear_velocities_dev(h, &Vs)) return rc;
-  if (V) HIPCHK(hipMemcpyAsync(V, Vs, h->n3() * sizeof(double), hipMemcpyDeviceToHost, h->st));
+  if (int rc = compute_shear_velocities(h)) return rc;
+  h->have_rhs = false;
+  if (V) HIPCHK(hipMemcpyAsync(V, h->Vshear, h->n3() * sizeof(double), hipMemcpyDeviceToHost, h->st));
   return sync_stream(h);
 }
 
@@ -1449,6 +1458,7 @@ int fcm_gdc_step(fcm_handle h, uint64_t seed, double cutoff, double fd_delta, co
     if (int rc = ll_barrier_forces(h, cutoff, nullptr)) return rc;
   const double* F6 = nullptr;
   if (int rc = body_forces_dev(h, &F6)) return rc;
+  if (int rc = compute_shear_velocities(h)) return rc;  // main.cpp:270-274
   const double* Vs = nullptr;
   if (int rc = shear_velocities_dev(h, &Vs)) return rc;
   HIPCHK(rhs_assemble(h->Vrand, Vs, F6, h->rhs, h->n3(), 6 * (size_t)h->NB, h->st));

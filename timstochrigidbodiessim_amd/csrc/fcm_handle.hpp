@@ -42,6 +42,8 @@ struct fcm_handle_s {
   double* Vshear = nullptr;
   double* d_prof = nullptr;
   bool have_shear = false;
+  bool have_vshear = false;     // Vshear holds calculate_shear_blob_velocities' result
+  int vshear_nb = 0;            // for this many blobs
   bool have_urand = false;
   void* twbuf64 = nullptr;
   void* twbuf32 = nullptr;

@@ -167,8 +167,9 @@ int run_steps(const char* in_path, const char* out_path) {
       all_bodies.fill_all_blob_pos_at_midstep();
       fcm.set_up_gaussians(all_bodies.all_blob_pos);
       fcm.calculate_random_blob_velocities();
+      // main.cpp:270-274; the Euler-Maruyama branch never computes Vshear
+      if (shear[0] != 0.0) fcm.calculate_shear_blob_velocities();
     }
-    if (shear[0] != 0.0) fcm.calculate_shear_blob_velocities();  // main.cpp:270-274
     // main.cpp:285, 305-310
     all_bodies.calculate_forces();
     gmres.assemble_RHS(all_bodies, fcm);
