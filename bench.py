@@ -60,15 +60,20 @@ def pass_bytes(nx, ny, nz, nb, word, column=False):
     }
 
 
-# kernel behind each profiled pass of the FP32 mobility product (column path, separate passes)
-KERNEL_OF_PASS = {
-    True: {"spread": "col_spread_kernel<float, 256, 4, 8>", "interp": "col_interp_kernel<float, 256, 4, 8>",
-           "y_forward": "cols_kernel<float, 256, -1>", "x_project": "xproj_kernel<float, 256>",
-           "y_inverse": "cols_kernel<float, 256, 1>", "interp_reduce": "col_reduce_kernel<float, 4>"},
-    False: {"z_forward": "zfwd_kernel<float, 128, false>", "z_inverse": "zinv_kernel<float, 128>",
-            "y_forward": "cols_kernel<float, 256, -1>", "x_project": "xproj_kernel<float, 256>",
-            "y_inverse": "cols_kernel<float, 256, 1>"},
-}
+def kernel_of_pass(t, n, dom):
+    """Kernel behind profiled pass `dom` of the FP32 mobility product on an n^3
+    grid under tuning `t` (as the library reports it: shapes resolved)."""
+    p = 1  # FP32 slot
+    cx, cp, yp, xp = t["col_cx"], t["col_points"][p], t["y_points"][p], t["x_points"][p]
+    names = {"y_forward": f"cols_kernel<float, {n}, -1, false, {yp}>", "x_project": f"xproj_kernel<float, {n}, {xp}>",
+             "y_inverse": f"cols_kernel<float, {n}, 1, false, {yp}>"}
+    if t["column"][p] == 1:
+        names.update({"spread": f"col_spread_kernel<float, {n}, {cx}, {cp}>",
+                      "interp": f"col_interp_kernel<float, {n}, {cx}, {cp}>",
+                      "interp_reduce": f"col_reduce_kernel<float, {cx}>"})
+    else:
+        names.update({"z_forward": f"zfwd_kernel<float, {n // 2}, false>", "z_inverse": f"zinv_kernel<float, {n // 2}>"})
+    return names.get(dom, dom)
 
 
 def measured_traffic(kernel, grid, nb):
@@ -157,7 +162,7 @@ def bench_batch(args):
     import math
     import timstochrigidbodiessim_amd as fa
     rng = np.random.default_rng(3)
-    insts = []
+    insts, tuned = [], []
     t0 = time.perf_counter()
     for i in range(args.instances):
         n = int(rng.choice([32, 48, 64, 96, 128]))
@@ -171,11 +176,18 @@ def bench_batch(args):
         s.set_geometry(ref)
         s.ref_matrices()
         s.set_bodies(x, q)
+        if not args.no_autotune:  # one timing per shape class, the cached choice for the rest
+            for prec in (fa.FCM_PREC_FP32, fa.FCM_PREC_FP64):  # FP32 solves refine with FP64 products
+                t, hit = s.autotune_cached(prec, reps=3)
+            if not hit:
+                tuned.append({"grid": n, "blobs": NB * nbpb, "tuning": t})
         s.set_body_forces(F6)
         s.assemble_rhs(use_random=False)
         insts.append((s, n, NB, nbpb))
     log(f"[bench] batch set-up {time.perf_counter() - t0:.1f} s: {len(insts)} instances, "
-        f"{sum(i[2] * i[3] for i in insts)} blobs in total")
+        f"{sum(i[2] * i[3] for i in insts)} blobs in total, {len(tuned)} shape classes tuned")
+    for tt in tuned:
+        log(f"[bench] tuned {tt['grid']}^3, {tt['blobs']} blobs: {tt['tuning']}")
     prec = fa.FCM_PREC_FP32
     # one at a time
     t0 = time.perf_counter()
@@ -210,6 +222,7 @@ def bench_batch(args):
         "one_at_a_time_solves_per_s": round(len(insts) / t_alone, 3),
         "batch_speedup_vs_one_at_a_time": round(t_alone / t_batch, 3),
         "converged": conv, "iteration_cap": args.max_iter, "iterations_total": sum(r["iterations"] for r in reps),
+        "tuned_shape_classes": tuned,
     }
     for s, *_ in insts:
         s.close()
@@ -232,10 +245,12 @@ def bench_serve(args):
             r.make_geometry()
         log(f"[bench] {len(reqs)} requests generated ({time.perf_counter() - t0:.1f} s)")
         with serve.MobilityServer(max_jobs=args.max_jobs, worker_streams=args.workers,
-                                  steps_per_round=args.steps_per_round, max_krylov=args.max_iter) as srv:
+                                  steps_per_round=args.steps_per_round, max_krylov=args.max_iter,
+                                  autotune=not args.no_autotune) as srv:
             srv.prepare_refs(reqs)
             st = srv.run(reqs)
             st["handles_created"] = srv.created
+            st["tunings_run"] = srv.tunings_run
             st["max_active"] = srv.max_active
         log(f"[bench] serve {name}: {st}")
         out[name] = st
@@ -478,7 +493,7 @@ def main():
     head = prof["fp32"]
     dom = max((k for k in head if k != "KT"), key=lambda k: head[k]["ms"])
     achieved = head[dom]["GBps"]
-    kernel = KERNEL_OF_PASS[column[fa.FCM_PREC_FP32] == 1].get(dom, dom)
+    kernel = kernel_of_pass(s.tuning(), n, dom)
     mv_bytes = sum(pass_bytes(n, n, n, nb, 4, column[fa.FCM_PREC_FP32] == 1).values())
     mv_ms = sum(v["ms"] for v in head.values())
     roofline = {"bound": "hbm", "kernel": kernel, "pass": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

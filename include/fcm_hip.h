@@ -330,13 +330,42 @@ typedef struct {
   int spread_variant[2];
   int interp_izb[2];
   int column[2];
+  /* z-pass points per thread of the column kernels: 8 or 16 (0: the plan that
+     pads the patch rows least) */
+  int col_points[2];
+  /* y FFT passes of the mobility product: 16 or 8 points per thread (0: 16),
+     with as many columns per workgroup as fit 256 threads and 64 KiB of LDS
+     (8 only where ny >= 64) */
+  int y_points[2];
+  /* fused x pass (forward FFT, projection, inverse): 16 or 8 in FP32 (0: 16;
+     8 only where nx >= 64); FP64 keeps 8 (0 or 8) */
+  int x_points[2];
+  /* patch x extent of the column path, 4 or 2 (0 where the grid has none);
+     one per handle, as the patch lists serve both precisions */
+  int col_cx;
 } fcm_tuning;
+/* the shapes the passes run with: a default (0) is reported as the shape it
+   resolves to, so the result can be passed back to fcm_set_tuning */
 int fcm_get_tuning(fcm_handle h, fcm_tuning* t);
+/* FCM_ERR_ARG for a tuning the handle cannot run (the message says which field) */
 int fcm_set_tuning(fcm_handle h, const fcm_tuning* t);
-/* Times every candidate of one precision on the handle's current blobs (`reps`
- * launches each, HIP events on the handle's stream), keeps the fastest for each
- * pass and returns the handle's tuning. */
+/* Times the candidates of one precision on the handle's current blobs (`reps`
+ * launches each, HIP events on the handle's stream) and keeps the fastest:
+ * spreading and interpolation tiles of the separate passes, the y and x pass
+ * shapes, then for the column path each patch width (only when tuning FP32,
+ * or FP64 on a handle without the FP32 path) and z-pass points per thread, and
+ * the column path against the separate passes.  Returns the handle's tuning
+ * and stores it in a process-wide cache under the handle's shape class (grid,
+ * ng, slip channel, the power of two at or above the blob count); a handle
+ * whose tuning was never set takes its class's cached tuning when its blobs
+ * are set up. */
 int fcm_autotune(fcm_handle h, int precision, int reps, fcm_tuning* chosen);
+/* As fcm_autotune, but a shape class already in the cache is applied without
+ * timing anything (*from_cache = 1; may be NULL).  For batches and serving,
+ * where many handles share a class. */
+int fcm_autotune_cached(fcm_handle h, int precision, int reps, fcm_tuning* chosen, int* from_cache);
+/* empty the shape-class cache (handles keep the tunings they have) */
+int fcm_clear_tuning_cache(void);
 
 /* ---- measurement hook (no counterpart in the reference) ----------------------- */
 /* Runs one saddle-point mat-vec `reps` times (after one untimed warm-up) on a

@@ -36,7 +36,7 @@ EXPORTS = [
     "gmres_get_solution", "fcm_mobility_solve", "fcm_mobility_prepare",
     "fcm_batch_create", "fcm_batch_destroy", "fcm_batch_submit", "fcm_batch_step", "fcm_batch_take",
     "fcm_batch_run",
-    "fcm_get_tuning", "fcm_set_tuning", "fcm_autotune",
+    "fcm_get_tuning", "fcm_set_tuning", "fcm_autotune", "fcm_autotune_cached", "fcm_clear_tuning_cache",
     "fcm_profile_matvec", "fcm_profile_copy", "ll_barrier_forces_noalloc", "fcm_synth_rods",
 ]
 
@@ -44,12 +44,24 @@ PASS_NAMES = ["spread", "z_forward", "y_forward", "x_project", "y_inverse", "z_i
               "interp_reduce"]
 
 
+TUNING_PAIRS = ("spread_variant", "interp_izb", "column", "col_points", "y_points", "x_points")
+
+
 class FcmTuning(C.Structure):
-    _fields_ = [("spread_variant", C.c_int * 2), ("interp_izb", C.c_int * 2), ("column", C.c_int * 2)]
+    _fields_ = [(k, C.c_int * 2) for k in TUNING_PAIRS] + [("col_cx", C.c_int)]
 
     def as_dict(self):
-        return {"spread_variant": list(self.spread_variant), "interp_izb": list(self.interp_izb),
-                "column": list(self.column)}
+        d = {k: list(getattr(self, k)) for k in TUNING_PAIRS}
+        d["col_cx"] = int(self.col_cx)
+        return d
+
+    @classmethod
+    def from_dict(cls, d):
+        t = cls()
+        for k in TUNING_PAIRS:
+            setattr(t, k, (C.c_int * 2)(*d[k]))
+        t.col_cx = int(d["col_cx"])
+        return t
 
 
 class FcmConfig(C.Structure):
@@ -150,6 +162,8 @@ def lib():
         L.fcm_get_tuning.argtypes = [H, C.POINTER(FcmTuning)]
         L.fcm_set_tuning.argtypes = [H, C.POINTER(FcmTuning)]
         L.fcm_autotune.argtypes = [H, C.c_int, C.c_int, C.POINTER(FcmTuning)]
+        L.fcm_clear_tuning_cache.argtypes = []
+        L.fcm_autotune_cached.argtypes = [H, C.c_int, C.c_int, C.POINTER(FcmTuning), C.POINTER(C.c_int)]
         L.fcm_profile_matvec.argtypes = [H, C.c_int, C.c_int, _dp]
         L.fcm_profile_copy.argtypes = [H, C.c_size_t, C.c_int, _dp]
         L.fcm_synth_rods.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_double, _dp, C.c_int, _dp, _dp,
@@ -187,6 +201,11 @@ def synth_rods(seed, NB, nbpb, a, L, max_attempts=50, forces=True):
     _chk(lib().fcm_synth_rods(seed, NB, nbpb, a, _ptr(Lh), max_attempts, _ptr(ref), _ptr(x), _ptr(q),
                               _ptr(F6), C.byref(nov)))
     return ref.reshape(nbpb, 3), x.reshape(NB, 3), q.reshape(NB, 4), (F6.reshape(NB, 6) if forces else None), nov.value
+
+
+def clear_tuning_cache():
+    """Empty the library's shape-class cache of tuned launch shapes."""
+    _chk(lib().fcm_clear_tuning_cache())
 
 
 def device_memory():
@@ -442,21 +461,35 @@ class FcmSolver:
         return r.as_dict()
 
     def tuning(self):
-        """{'spread_variant': [fp64, fp32], 'interp_izb': [fp64, fp32]}"""
+        """The handle's launch shapes: per precision [FP64, FP32] spread_variant,
+        interp_izb, column, col_points, y_points, x_points, and col_cx."""
         t = FcmTuning()
         _chk(self.L.fcm_get_tuning(self.h, C.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, spread_variant=(0, 0), interp_izb=(0, 0), column=None):
-        if column is None:
-            column = self.tuning()["column"]
-        t = FcmTuning((C.c_int * 2)(*spread_variant), (C.c_int * 2)(*interp_izb), (C.c_int * 2)(*column))
-        _chk(self.L.fcm_set_tuning(self.h, C.byref(t)))
+    def set_tuning(self, spread_variant=None, interp_izb=None, column=None, **shapes):
+        """Set launch shapes; fields not given keep their current values."""
+        d = self.tuning()
+        for k, v in (("spread_variant", spread_variant), ("interp_izb", interp_izb), ("column", column)):
+            if v is not None:
+                d[k] = list(v)
+        for k, v in shapes.items():
+            if k not in d:
+                raise KeyError(k)
+            d[k] = int(v) if k == "col_cx" else list(v)
+        _chk(self.L.fcm_set_tuning(self.h, C.byref(FcmTuning.from_dict(d))))
 
     def autotune(self, precision=FCM_PREC_FP32, reps=5):
         t = FcmTuning()
         _chk(self.L.fcm_autotune(self.h, precision, reps, C.byref(t)))
         return t.as_dict()
+
+    def autotune_cached(self, precision=FCM_PREC_FP32, reps=5):
+        """(tuning, from_cache): the shape class's cached tuning, or a fresh autotune."""
+        t = FcmTuning()
+        hit = C.c_int(0)
+        _chk(self.L.fcm_autotune_cached(self.h, precision, reps, C.byref(t), C.byref(hit)))
+        return t.as_dict(), bool(hit.value)
 
     def mobility_prepare(self, seed):
         """Seeded fluctuating field + RHS on the device, not waited for."""

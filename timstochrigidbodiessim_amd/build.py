@@ -18,7 +18,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfcm_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["fcm_fft.hip", "fcm_grid.hip", "fcm_solver.hip", "fcm_gmres.hip", "fcm_capi.hip", "fcm_batch.hip", "fcm_steric.hip", "fcm_column.hip", "fcm_noalloc.hip", "fcm_synth.cpp"]
+SOURCES = ["fcm_fft.hip", "fcm_grid.hip", "fcm_solver.hip", "fcm_gmres.hip", "fcm_capi.hip", "fcm_batch.hip", "fcm_steric.hip", "fcm_column.hip", "fcm_column32.hip", "fcm_noalloc.hip", "fcm_synth.cpp"]
 ARCH = os.environ.get("FCM_OFFLOAD_ARCH", "gfx950")
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result", "-Wno-unused-function"]

@@ -12,6 +12,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -114,9 +117,9 @@ Sizes sizes_of(const fcm_config* c) {
     bb2.bx = largest_divisor_le(c->nx, 4);
     bb2.by = largest_divisor_le(c->ny, 8);
     bb2.bz = largest_divisor_le(c->nz, 8);
-    const int cx = col_patch_cx(g2, bb2, c->ng);
-    if (cx > 0)
-      t += (3 * (size_t)col_groups(g2, cx) + 1) * 4 + col_list_capacity(nb) * 20 + col_zst_words(g2, cx) * 4 +
+    // the column buffers are sized for the narrower of the two patch widths
+    if (col_patch_cx(g2, bb2, c->ng) > 0)
+      t += (3 * (size_t)col_groups(g2, 2) + 1) * 4 + col_list_capacity(nb) * 20 + col_zst_words(g2, 2) * 4 +
            (size_t)nb * COL_SLOTS_MAX * 3 * 8;
   }
   const int n3b = 3 * c->max_blobs_per_body;
@@ -281,6 +284,10 @@ int launch_arnoldi_step(fcm_handle h, int precision, hipStream_t st) {
 
 }  // namespace fcm_host
 
+namespace fcm_host {
+int apply_cached_tuning(fcm_handle h);
+}  // namespace fcm_host
+
 namespace {
 
 // body forces for the RHS: F6 (rb_set_body_forces) plus K^T of the blob forces
@@ -315,11 +322,17 @@ int shear_velocities_dev(fcm_handle h, const double** out) {
 }
 
 int setup_blobs_device(fcm_handle h, int nb) {
+  const int nb_before = h->bd.nb;
   h->bd.nb = nb;
   h->bd.Y = h->dY;
   ++h->epoch;
   HIPCHK(setup_gaussians(h->g, h->bb, h->bd, h->cfg.a, h->st));
   h->have_blobs = true;
+  // a handle's first blobs (or a new blob count) select its shape class
+  if (!h->tuning_set && (nb != nb_before || !h->cache_checked)) {
+    h->cache_checked = true;
+    return fcm_host::apply_cached_tuning(h);
+  }
   return FCM_OK;
 }
 
@@ -483,13 +496,13 @@ int fcm_create(const fcm_config* cfg, fcm_handle* out) {
   h->bd.ng = cfg->ng;
   h->bd.col_cx = col_patch_cx(h->g, bb, cfg->ng);
   if (h->bd.col_cx > 0) {
-    const int ng_ = col_groups(h->g, h->bd.col_cx);
+    const int ng_ = col_groups(h->g, 2);  // sized for either patch width
     if (e == hipSuccess) h->bd.col_count = dalloc<int>(h, ng_, e);
     if (e == hipSuccess) h->bd.col_start = dalloc<int>(h, (size_t)ng_ + 1, e);
     if (e == hipSuccess) h->bd.col_cursor = dalloc<int>(h, ng_, e);
     if (e == hipSuccess) h->bd.col_list = dalloc<int>(h, col_list_capacity(nb), e);
     if (e == hipSuccess) h->bd.col_ent = dalloc<int4>(h, col_list_capacity(nb), e);
-    if (e == hipSuccess) h->bd.col_zst = dalloc<int>(h, col_zst_words(h->g, h->bd.col_cx), e);
+    if (e == hipSuccess) h->bd.col_zst = dalloc<int>(h, col_zst_words(h->g, 2), e);
     if (e == hipSuccess) h->bd.col_part = dalloc<double>(h, (size_t)nb * COL_SLOTS_MAX * 3, e);
   } else {
     h->bd.col_fused[0] = h->bd.col_fused[1] = 0;
@@ -1045,34 +1058,154 @@ int fcm_profile_matvec(fcm_handle h, int precision, int reps, double* ms_per_pas
 int fcm_get_tuning(fcm_handle h, fcm_tuning* t) {
   if (int rc = check(h)) return rc;
   if (!t) return fail(FCM_ERR_ARG, "null tuning");
+  // the shapes the passes run with: a default (0) is reported as what it resolves to
   for (int p = 0; p < 2; ++p) {
     t->spread_variant[p] = h->bd.spread_variant[p];
     t->interp_izb[p] = h->bd.interp_izb[p];
     t->column[p] = h->bd.col_fused[p];
+    t->col_points[p] =
+        h->bd.col_cx > 0 ? col_shape_pm(p == FCM_PREC_FP32 ? 4 : 8, h->g.nz, h->bd.col_cx, h->bd.col_pm[p]) : 0;
+    t->y_points[p] = h->g.ypm[p] ? h->g.ypm[p] : 16;
+    t->x_points[p] = h->g.xpm[p] ? h->g.xpm[p] : (p == FCM_PREC_FP32 ? 16 : 8);
   }
+  t->col_cx = h->bd.col_cx;
   return FCM_OK;
 }
 
-int fcm_set_tuning(fcm_handle h, const fcm_tuning* t) {
-  if (int rc = check(h)) return rc;
-  if (!t) return fail(FCM_ERR_ARG, "null tuning");
-  for (int p = 0; p < 2; ++p)
-    if (t->spread_variant[p] < 0 || t->spread_variant[p] > 2 || t->interp_izb[p] < 0 || t->interp_izb[p] > 3 ||
-        t->column[p] < 0 || t->column[p] > 1 || (t->column[p] == 1 && h->bd.col_cx <= 0))
-      return fail(FCM_ERR_ARG, "bad tuning");
+}  // extern "C"
+
+namespace {
+
+bool pass_points_ok(int n, int pm) { return pm == 0 || pm == 16 || (pm == 8 && n >= 64); }
+
+// why `t` is not a tuning this handle can run, or null
+const char* tuning_problem(const fcm_handle h, const fcm_tuning* t) {
+  const bool colgrid = h->bd.col_cx > 0;
+  if (colgrid) {
+    if (t->col_cx != 2 && t->col_cx != 4) return "col_cx must be 2 or 4 on this grid";
+    if (h->g.nx % t->col_cx != 0) return "col_cx must divide nx";
+  } else if (t->col_cx != 0) {
+    return "the grid has no column path (col_cx must be 0)";
+  }
+  for (int p = 0; p < 2; ++p) {
+    if (t->spread_variant[p] < 0 || t->spread_variant[p] > 2 || t->interp_izb[p] < 0 || t->interp_izb[p] > 3)
+      return "spread_variant must be 0..2 and interp_izb 0..3";
+    if (t->column[p] < 0 || t->column[p] > 1 || (t->column[p] == 1 && !colgrid)) return "column must be 0 or 1";
+    if (t->col_points[p] != 0 && t->col_points[p] != 8 && t->col_points[p] != 16) return "col_points must be 0, 8 or 16";
+    if (colgrid && !col_shape_valid(p == FCM_PREC_FP32 ? 4 : 8, h->g.nz, t->col_cx, t->col_points[p]))
+      return "no column kernels for this patch width and z-pass points per thread";
+    if (!pass_points_ok(h->g.ny, t->y_points[p]) || t->y_points[p] < 0) return "y_points must be 0, 16 or 8 (ny >= 64)";
+    if (p == FCM_PREC_FP64 ? (t->x_points[p] != 0 && t->x_points[p] != 8)
+                           : (t->x_points[p] < 0 || !pass_points_ok(h->g.nx, t->x_points[p])))
+      return "x_points must be 0, 16 or 8 (nx >= 64); FP64 keeps 8 (0 or 8)";
+  }
+  return nullptr;
+}
+
+int apply_tuning(fcm_handle h, const fcm_tuning* t) {
   HIPCHK(hipStreamSynchronize(h->st));
   for (int p = 0; p < 2; ++p) {
     h->bd.spread_variant[p] = t->spread_variant[p];
     h->bd.interp_izb[p] = t->interp_izb[p];
     h->bd.col_fused[p] = t->column[p];
+    h->bd.col_pm[p] = t->col_points[p];
+    h->g.ypm[p] = t->y_points[p];
+    h->g.xpm[p] = t->x_points[p];
+  }
+  if (t->col_cx != h->bd.col_cx) {
+    h->bd.col_cx = t->col_cx;
+    // the patch lists follow the patch width
+    if (h->have_blobs) HIPCHK(build_col_lists(h->g, h->bb, h->bd, h->st));
   }
   ++h->epoch;
-  return FCM_OK;
+  return sync_stream(h);
 }
+
+// Tuned launch shapes cached by shape class: grid, stencil, channel and the
+// power of two at or above the blob count.  fcm_autotune stores its choice;
+// fcm_autotune_cached and the blob set-up of a handle whose tuning was never
+// set explicitly take it.  One cache per process, guarded by a mutex.
+struct ShapeClass {
+  int nx, ny, nz, ng, channel, nb2;
+  bool operator<(const ShapeClass& o) const {
+    return std::tie(nx, ny, nz, ng, channel, nb2) < std::tie(o.nx, o.ny, o.nz, o.ng, o.channel, o.nb2);
+  }
+};
+struct CachedTuning {
+  fcm_tuning t{};
+  bool have[2] = {false, false};
+};
+std::mutex g_tune_mu;
+std::map<ShapeClass, CachedTuning> g_tune_cache;
+
+ShapeClass shape_class(const fcm_handle h) {
+  int nb2 = 1;
+  while (nb2 < h->bd.nb) nb2 *= 2;
+  return ShapeClass{h->cfg.nx, h->cfg.ny, h->cfg.nz, h->cfg.ng, h->cfg.slip_channel, nb2};
+}
+
+// the cached choice for precision p of h's class into t (the rest of t kept)
+bool cache_lookup(const fcm_handle h, int p, fcm_tuning* t) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  const auto it = g_tune_cache.find(shape_class(h));
+  if (it == g_tune_cache.end() || !it->second.have[p]) return false;
+  const fcm_tuning& c = it->second.t;
+  t->spread_variant[p] = c.spread_variant[p];
+  t->interp_izb[p] = c.interp_izb[p];
+  t->column[p] = c.column[p];
+  t->col_points[p] = c.col_points[p];
+  t->y_points[p] = c.y_points[p];
+  t->x_points[p] = c.x_points[p];
+  // the patch width is shared by both precisions; the FP32 choice wins
+  if (p == FCM_PREC_FP32 || !it->second.have[FCM_PREC_FP32]) t->col_cx = c.col_cx;
+  return true;
+}
+
+void cache_store(const fcm_handle h, int p, const fcm_tuning* t) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  CachedTuning& c = g_tune_cache[shape_class(h)];
+  c.t.spread_variant[p] = t->spread_variant[p];
+  c.t.interp_izb[p] = t->interp_izb[p];
+  c.t.column[p] = t->column[p];
+  c.t.col_points[p] = t->col_points[p];
+  c.t.y_points[p] = t->y_points[p];
+  c.t.x_points[p] = t->x_points[p];
+  if (p == FCM_PREC_FP32 || !c.have[FCM_PREC_FP32]) c.t.col_cx = t->col_cx;
+  c.have[p] = true;
+}
+
+}  // namespace
+
+int fcm_set_tuning(fcm_handle h, const fcm_tuning* t) {
+  if (int rc = check(h)) return rc;
+  if (!t) return fail(FCM_ERR_ARG, "null tuning");
+  if (const char* why = tuning_problem(h, t)) return fail(FCM_ERR_ARG, std::string("bad tuning: ") + why);
+  h->tuning_set = true;
+  return apply_tuning(h, t);
+}
+
+namespace fcm_host {
+
+// at a handle's blob set-up: the cached tuning of its shape class, unless the
+// handle's tuning was set explicitly
+int apply_cached_tuning(fcm_handle h) {
+  if (h->tuning_set) return FCM_OK;
+  fcm_tuning t;
+  if (int rc = fcm_get_tuning(h, &t)) return rc;
+  bool any = false;
+  for (int p = 0; p < 2; ++p) any = cache_lookup(h, p, &t) || any;
+  if (!any || tuning_problem(h, &t)) return FCM_OK;
+  return apply_tuning(h, &t);
+}
+
+}  // namespace fcm_host
+
+extern "C" {
 
 int fcm_autotune(fcm_handle h, int precision, int reps, fcm_tuning* chosen) {
   if (int rc = check(h)) return rc;
   if (!h->have_blobs) return fail(FCM_ERR_STATE, "no blob positions set");
+  if (precision != FCM_PREC_FP32 && precision != FCM_PREC_FP64) return fail(FCM_ERR_ARG, "bad precision");
   if (precision == FCM_PREC_FP32 && !h->f32[0]) return fail(FCM_ERR_STATE, "FP32 path not allocated");
   reps = std::max(1, reps);
   hipEvent_t e0, e1;
@@ -1091,6 +1224,7 @@ int fcm_autotune(fcm_handle h, int precision, int reps, fcm_tuning* chosen) {
     return FCM_OK;
   };
   const KTerm kt{};
+  const int pi = precision == FCM_PREC_FP32 ? 1 : 0;
   auto spread_once = [&]() -> int {
     if (precision == FCM_PREC_FP32) HIPCHK(spread<float>(h->g, h->bb, h->bd, h->tmp, h->fp32(), h->st, false));
     else HIPCHK(spread<double>(h->g, h->bb, h->bd, h->tmp, h->fp64(), h->st, false));
@@ -1101,55 +1235,107 @@ int fcm_autotune(fcm_handle h, int precision, int reps, fcm_tuning* chosen) {
     else HIPCHK(interp<double>(h->g, h->bb, h->bd, h->fp64(), h->Vtmp, kt, h->st));
     return FCM_OK;
   };
-  const int pi = precision == FCM_PREC_FP32 ? 1 : 0;
+  auto product_once = [&]() -> int {
+    if (precision == FCM_PREC_FP32)
+      return mobility_passes<float>(h, h->fp32(), h->tw32, h->bd.col_fused[1] != 0, h->tmp, h->Vtmp, kt, h->st);
+    return mobility_passes<double>(h, h->fp64(), h->tw64, h->bd.col_fused[0] != 0, h->tmp, h->Vtmp, kt, h->st);
+  };
+  // the fastest value of one choice, by timing `once` for each candidate
   int rc = FCM_OK;
-  float best = 0;
-  int best_v = h->bd.spread_variant[pi];
-  for (int v = 0; v <= 2 && !rc; ++v) {
-    h->bd.spread_variant[pi] = v;
-    float ms = 0;
-    rc = time_it(spread_once, ms);
-    if (!rc && (v == 0 || ms < best)) {
-      best = ms;
-      best_v = v;
+  auto pick = [&](int* knob, std::initializer_list<int> cands, auto&& once, auto&& valid) {
+    float best = 0.f;
+    int best_v = *knob;
+    bool first = true;
+    for (int v : cands) {
+      if (rc || !valid(v)) continue;
+      *knob = v;
+      float ms = 0.f;
+      rc = time_it(once, ms);
+      if (!rc && (first || ms < best)) {
+        best = ms;
+        best_v = v;
+        first = false;
+      }
     }
-  }
-  h->bd.spread_variant[pi] = best_v;
-  int best_z = h->bd.interp_izb[pi];
-  float best_i = 0;
-  for (int z = 1; z <= 3 && !rc; ++z) {
-    h->bd.interp_izb[pi] = z;
-    float ms = 0;
-    rc = time_it(interp_once, ms);
-    if (!rc && (z == 1 || ms < best_i)) {
-      best_i = ms;
-      best_z = z;
-    }
-  }
-  h->bd.interp_izb[pi] = best_z;
+    *knob = best_v;
+  };
+  auto any = [](int) { return true; };
+  // separate-path spreading and interpolation tiles
+  pick(&h->bd.spread_variant[pi], {0, 1, 2}, spread_once, any);
+  pick(&h->bd.interp_izb[pi], {1, 2, 3}, interp_once, any);
+  // FFT pass shapes (points per thread, and with them the columns per workgroup)
+  pick(&h->g.ypm[pi], {16, 8}, product_once, [&](int v) { return pass_points_ok(h->g.ny, v); });
+  if (precision == FCM_PREC_FP32)
+    pick(&h->g.xpm[pi], {16, 8}, product_once, [&](int v) { return pass_points_ok(h->g.nx, v); });
   if (!rc && h->bd.col_cx > 0) {
-    // the column path against the separate passes, whole mobility products
-    auto product_once = [&]() -> int {
-      if (precision == FCM_PREC_FP32)
-        return mobility_passes<float>(h, h->fp32(), h->tw32, h->bd.col_fused[1] != 0, h->tmp, h->Vtmp, kt, h->st);
-      return mobility_passes<double>(h, h->fp64(), h->tw64, h->bd.col_fused[0] != 0, h->tmp, h->Vtmp, kt, h->st);
-    };
-    float ms_col = 0, ms_sep = 0;
+    // the column path: patch width (shared by both precisions, so only the FP32
+    // tuning moves it when the handle has an FP32 path) and z-pass points per
+    // thread, then the column path against the separate passes
+    const int cx0 = h->bd.col_cx;
     h->bd.col_fused[pi] = 1;
-    rc = time_it(product_once, ms_col);
-    if (!rc) {
-      h->bd.col_fused[pi] = 0;
-      rc = time_it(product_once, ms_sep);
+    const bool tune_cx = precision == FCM_PREC_FP32 || !h->f32[0];
+    int best_cx = cx0, best_pm = h->bd.col_pm[pi];
+    float best = 0.f;
+    bool first = true;
+    for (int cx : {4, 2}) {
+      if (rc || (!tune_cx && cx != cx0) || h->g.nx % cx != 0) continue;
+      if (cx != h->bd.col_cx) {
+        h->bd.col_cx = cx;
+        if (hipError_t e = build_col_lists(h->g, h->bb, h->bd, h->st)) rc = fail(FCM_ERR_HIP, hipGetErrorString(e));
+      }
+      for (int pm : {8, 16}) {
+        if (rc || !col_shape_valid(precision == FCM_PREC_FP32 ? 4 : 8, h->g.nz, cx, pm)) continue;
+        h->bd.col_pm[pi] = pm;
+        float ms = 0.f;
+        rc = time_it(product_once, ms);
+        if (!rc && (first || ms < best)) {
+          best = ms;
+          best_cx = cx;
+          best_pm = pm;
+          first = false;
+        }
+      }
     }
-    h->bd.col_fused[pi] = (!rc && ms_col <= ms_sep) ? 1 : 0;
+    if (!rc && best_cx != h->bd.col_cx) {
+      h->bd.col_cx = best_cx;
+      if (hipError_t e = build_col_lists(h->g, h->bb, h->bd, h->st)) rc = fail(FCM_ERR_HIP, hipGetErrorString(e));
+    }
+    h->bd.col_pm[pi] = best_pm;
+    pick(&h->bd.col_fused[pi], {1, 0}, product_once, any);
   }
   ++h->epoch;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
-  if (rc == FCM_OK && chosen) rc = fcm_get_tuning(h, chosen);
-  if (rc) return rc;
+  fcm_tuning t;
+  if (int r2 = fcm_get_tuning(h, &t)) return r2;
+  cache_store(h, precision, &t);
+  h->tuning_set = true;
+  if (chosen) *chosen = t;
   return sync_stream(h);
+}
+
+int fcm_clear_tuning_cache(void) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tune_cache.clear();
+  return FCM_OK;
+}
+
+int fcm_autotune_cached(fcm_handle h, int precision, int reps, fcm_tuning* chosen, int* from_cache) {
+  if (int rc = check(h)) return rc;
+  if (!h->have_blobs) return fail(FCM_ERR_STATE, "no blob positions set");
+  if (precision != FCM_PREC_FP32 && precision != FCM_PREC_FP64) return fail(FCM_ERR_ARG, "bad precision");
+  fcm_tuning t;
+  if (int rc = fcm_get_tuning(h, &t)) return rc;
+  if (cache_lookup(h, precision, &t) && !tuning_problem(h, &t)) {
+    if (from_cache) *from_cache = 1;
+    h->tuning_set = true;
+    if (int rc = apply_tuning(h, &t)) return rc;
+    if (chosen) *chosen = t;
+    return FCM_OK;
+  }
+  if (from_cache) *from_cache = 0;
+  return fcm_autotune(h, precision, reps, chosen);
 }
 
 int fcm_profile_copy(fcm_handle h, size_t bytes, int reps, double* ms_per_copy) {

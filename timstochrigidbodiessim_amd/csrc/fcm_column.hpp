@@ -1,0 +1,755 @@
+// fcm_column.hpp -- the column path's kernels and their launchers (internal to
+// libfcm_hip.so; fcm_column.hip builds the FP64 and list-building side,
+// fcm_column32.hip the FP32 kernels).
+#pragma once
+// spreading and interpolation of the mobility product, each
+// fused with the z pass of the FFT that touches the same grid rows.
+//
+// A patch is cx x 4 grid rows (x, y) with all of z.  For every patch the blobs
+// whose x-y stencil touches it are listed once per set of positions, ordered by
+// the z of their first stencil node ("stencil start"), with a prefix table
+// over z.  One workgroup owns a patch and one thread owns one z:
+//
+//   spread + z R2C (col_spread_kernel): the thread sums F gx gy gz over exactly
+//   the blobs whose z stencil covers its z -- a contiguous range of the ordered
+//   list -- for every node of the patch rows, in registers.  The rows go
+//   through LDS into the nz/2-point complex FFT and the R2C untangling, and
+//   only k-space rows are written (fcm_fluid_solver.cpp:552-600 followed by the
+//   z stage of fft_solver.cpp:32-36).
+//
+//   z C2R + interpolation (col_interp_kernel): the k-space rows are brought
+//   back to real space in LDS (imaginary parts of kz = 0 and nz/2 ignored, as
+//   FFTW's halfcomplex format stores none), each thread keeps its z column of
+//   the patch in registers and, for every blob covering its z, forms
+//   gz sum_xy gx gy u.  The ng values of a blob are summed in a fixed order and
+//   the patch's partial sum goes to the blob's slot for this patch;
+//   col_reduce_kernel adds the (at most 5 x 3) slots of a blob in a fixed
+//   order and writes dx^3 J u - K V (fcm_fluid_solver.cpp:363-424 preceded by
+//   the z stage of fft_solver.cpp:38-42).
+//
+// Nothing is accumulated with atomics, so both are bitwise repeatable.  Per
+// product the grid is written once and read once in k space, instead of real
+// space being written, read back, transformed and written again on each side.
+#include "fcm_internal.hpp"
+
+#include <type_traits>
+
+namespace fcm {
+
+__device__ __forceinline__ int col_wrapi(int v, int n) {
+  const int r = v % n;
+  return r < 0 ? r + n : r;
+}
+__device__ __forceinline__ int col_fdiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+__device__ __forceinline__ int col_wrap_half(int v, int n) {
+  if (v < -(n / 2)) v += n; else if (v >= n - n / 2) v -= n;
+  return v;
+}
+// blocks are spread over the 8 XCDs so that each works on a contiguous range
+// of patches (neighbouring patches share blobs in that XCD's L2)
+__device__ __forceinline__ int col_xcd(int b, int n) {
+  const int per = (n + 7) / 8;
+  return (b % 8) * per + b / 8;
+}
+
+template <typename T> __device__ __forceinline__ const T* col_w(const double* w64, const float* w32);
+template <> __device__ __forceinline__ const double* col_w<double>(const double* w64, const float*) { return w64; }
+template <> __device__ __forceinline__ const float* col_w<float>(const double*, const float* w32) { return w32; }
+
+struct ColArgs {
+  GridGeom g;
+  int ng, npx, npy;
+  const int4* ent;  // entry records of the patch lists (col_entry)
+  const int* zst;
+  const double* w64;
+  const float* w32;
+};
+
+// One entry of a patch list: (sorted blob slot, blob id, packed offsets, index
+// of the blob's partial-sum slot for this patch: slot-major, dst nb + blob, so
+// that the reduction reads consecutive blobs).  The packed word holds the
+// stencil start relative to the patch in x and y (+8, 5 bits each) and the
+// wrapped stencil start in z (bits 10 and up).
+__device__ __forceinline__ int col_rx(int pk) { return (pk & 31) - 8; }
+__device__ __forceinline__ int col_ry(int pk) { return ((pk >> 5) & 31) - 8; }
+__device__ __forceinline__ int col_sz(int pk) { return pk >> 10; }
+
+// ---------------------------------------------------------------------------
+// launch shapes
+// ---------------------------------------------------------------------------
+constexpr int col_threads(int nz) { return (nz + 63) / 64 * 64; }  // one thread per z
+constexpr int col_max(int a, int b) { return a > b ? a : b; }
+constexpr int col_min(int a, int b) { return a < b ? a : b; }
+// largest odd number <= v (LDS planes of an odd number of words are not
+// aligned with one another in the 64 banks)
+constexpr int col_odd_le(int v) { return (v % 2 == 1) ? v : v - 1; }
+
+// points per thread of the z pass: the plan whose rounds pad the 3 cx 4 rows
+// least, then the one with the smaller LDS scratch
+constexpr int col_pm(int NZ, int CX) {
+  const int nt = col_threads(NZ), rows = 3 * CX * COL_CY;
+  int best = 16, best_pad = 1 << 30, best_lds = 1 << 30;
+  for (int pm : {16, 8}) {
+    const FftPlan pl = make_plan(NZ / 2, pm);
+    if (!pl.ok || nt % pl.T != 0) continue;
+    const int rr = nt / pl.T;
+    const int pad = (rows + rr - 1) / rr * rr;
+    const int lds = (NZ / 2 + 1) * (rr + 1);
+    if (pad < best_pad || (pad == best_pad && lds < best_lds)) {
+      best_pad = pad;
+      best_lds = lds;
+      best = pm;
+    }
+  }
+  return best;
+}
+
+// LDS layout of the z pass scratch: element e of FFT column col at
+// e ls + (e >> sh) sb + col.
+// Chosen per shape with a model of the 64 LDS banks (a b64 access is served
+// 32 lanes at a time, a b128 access 16 at a time) over every exchange of the
+// transform, within the LDS the kernel has anyway at its occupancy; ls = 0
+// keeps the plain layout (ls = rows + 1, no skew).  At nz = 256 in FP32 this
+// takes the bank cycles of the exchanges of one round from 228 to 124 per
+// wavefront (forward) and from 208 to 80 (inverse).
+struct ColLayout {
+  int ls, sh, sb;
+};
+constexpr ColLayout col_layout(int tbytes, int nz) {
+  if (tbytes == 4) {
+    switch (nz) {
+      case 32: return {72, 2, 8};
+      case 48: return {48, 2, 8};
+      case 64: return {24, 3, 8};
+      case 96: return {48, 1, 4};
+      case 128: return {20, 3, 4};
+      case 192: return {14, 1, 0};
+      case 256: return {18, 4, 2};
+      case 512: return {17, 5, 3};
+      default: return {0, 0, 0};
+    }
+  }
+  switch (nz) {
+    case 32: return {40, 3, 8};
+    case 48: return {20, 1, 0};
+    case 64: return {20, 3, 4};
+    case 96: return {18, 1, 0};
+    case 128: return {18, 3, 2};
+    case 256: return {17, 4, 1};
+    default: return {0, 0, 0};  // nz = 512: the skew costs the inverse kernel a wave per SIMD
+  }
+}
+
+// Lane pairing of the C2R input.  The point k of the inverse z transform needs
+// X[k] and X[M - k]; with the first radix of the inverse plan equal to 2, the
+// word X[M - k] of thread tcol's slot s is thread (TC - tcol)'s slot P - 1 - s,
+// and for tcol = 0 it is the thread's own slot c2r_self_slot(s) (none for k = 0,
+// whose partner is the Nyquist bin X[M]).  Checked at compile time over every
+// thread; where it does not hold, both words are loaded.
+constexpr int c2r_in_index(const FftPlan& pl, int tcol, int s) {
+  return tcol + (s / pl.r[0]) * pl.T + (s % pl.r[0]) * (pl.N / pl.r[0]);
+}
+constexpr int c2r_self_slot(const FftPlan& pl, int s) {
+  const int want = (pl.N - c2r_in_index(pl, 0, s)) % pl.N;
+  if (want == 0) return -1;
+  for (int q = 0; q < pl.P; ++q)
+    if (c2r_in_index(pl, 0, q) == want) return q;
+  return -2;
+}
+constexpr bool c2r_lane_pairs(const FftPlan& pl) {
+  if (pl.T > 64 || 64 % pl.T != 0) return false;
+  for (int tcol = 1; tcol < pl.T; ++tcol)
+    for (int s = 0; s < pl.P; ++s)
+      if (c2r_in_index(pl, pl.T - tcol, pl.P - 1 - s) != pl.N - c2r_in_index(pl, tcol, s)) return false;
+  for (int s = 0; s < pl.P; ++s)
+    if (c2r_self_slot(pl, s) == -2 || (c2r_self_slot(pl, s) == -1) != (c2r_in_index(pl, 0, s) == 0)) return false;
+  return true;
+}
+
+// patch x extent per nz by default: FP64 rows of a 4 x 4 patch exceed the LDS at nz = 512
+constexpr int col_cx_for(int nz) { return nz <= 256 ? 4 : 2; }
+
+template <typename T, int NZ, int CX, int PM>
+struct ColumnShape {
+  static constexpr int NT = col_threads(NZ);
+  static constexpr int M = NZ / 2;                // complex FFT length of the z pass
+  static constexpr FftPlan PL = make_plan(M, PM);
+  static constexpr bool PLAN_OK = PL.ok && PL.T > 0 && NT % PL.T == 0 && (PM == 8 || PM == 16);
+  static constexpr int P = PL.P, TC = PLAN_OK ? PL.T : 1;
+  static constexpr int RR = NT / TC;              // rows transformed at once (one round)
+  static constexpr int NROW = 3 * CX * COL_CY;    // rows of a patch (3 components)
+  static constexpr int NRND = (NROW + RR - 1) / RR;
+  // LDS for staging the blob lists (about 38 KB per 256 threads in FP32, 52 KB
+  // in FP64: four or two workgroups per CU at nz = 256)
+  static constexpr int STAGE = col_min((sizeof(T) == 4 ? 38912 : 53248) / 256 * NT, 65536);
+  static constexpr int plain_words(int n) { return col_max((M - 1) * (n * RR + 1) + n * RR, n * RR * (M + 1)); }
+  // all rounds transformed together (one set of barriers) when their scratch
+  // fits in the staging LDS (FP32 only), else one round at a time: a larger
+  // scratch would cost workgroups per CU (at nz = 256 in FP32, three rounds at
+  // once held three workgroups per CU instead of four and took 15% longer)
+  static constexpr int NCB = (sizeof(T) == 4 && plain_words(NRND) * 8 <= STAGE) ? NRND : 1;
+  // the bank-skewed layouts are fitted to the default patch width and plan;
+  // other shapes take the plain layout
+  static constexpr bool DEFAULT_SHAPE = CX == col_cx_for(NZ) && PM == col_pm(NZ, CX);
+  static constexpr ColLayout LAY = DEFAULT_SHAPE ? col_layout((int)sizeof(T), NZ) : ColLayout{0, 0, 0};
+  // FFT scratch: element e of row r at e LS + (e >> SSH) SB + r
+  static constexpr int LS = LAY.ls > 0 ? LAY.ls : NCB * RR + 1;
+  static constexpr int SSH = LAY.ls > 0 ? LAY.sh : 0, SB = LAY.ls > 0 ? LAY.sb : 0;
+  static constexpr int RS = M + 1;                // row-major rows: complex words per row
+  static constexpr int FFT_WORDS =
+      col_max((M - 1) * LS + ((M - 1) >> SSH) * SB + NCB * RR, NCB * RR * RS);
+  static constexpr int FFT_BYTES = FFT_WORDS * 2 * (int)sizeof(T);
+  static __device__ __forceinline__ int slot(int e, int col) { return fft_slot<SSH, SB>(e, LS, col); }
+  static __device__ __forceinline__ int row0(int r) { return r * RS; }
+  // staging of the blob lists: budget per workgroup, then planes of CAP entries
+  static constexpr int BUDGET = col_max(STAGE, FFT_BYTES);
+  static constexpr int S_ENT = 19 * (int)sizeof(T) + 4;   // gx 4, gy 4, F 3, gz 8 | stencil-start z
+  static constexpr int S_CAP = col_odd_le(BUDGET / S_ENT);
+  static constexpr int S_LDS = col_max(FFT_BYTES, S_CAP * S_ENT);
+  static constexpr int I_ENT = 40 * (int)sizeof(T) + 8;   // gx 4, gy 4, gz 8, slots 3 x 8 | z, partial slot
+  // FP64 interpolation at 256 threads fits 168 VGPRs (three waves per SIMD), so
+  // its staging is trimmed to let three workgroups share a CU's LDS
+  static constexpr int I_BUDGET = (sizeof(T) == 8 && NT == 256) ? col_max(col_min(BUDGET, 49664), FFT_BYTES) : BUDGET;
+  static constexpr int I_CAP = col_odd_le(I_BUDGET / I_ENT);
+  static constexpr int I_LDS = col_max(FFT_BYTES, I_CAP * I_ENT);
+  // a shape the kernels can be built for: a z plan that tiles the threads, FFT
+  // columns that do not overlap, room to stage 15 entries, and the LDS of a CU
+  static constexpr bool ok = PLAN_OK && LS >= NCB * RR && S_CAP >= 15 && I_CAP >= 15 &&
+                             S_LDS + (M + M / 2 + 1) * 2 * (int)sizeof(T) <= 163840 &&
+                             I_LDS + M * 2 * (int)sizeof(T) <= 163840;
+};
+
+// the range [a1, b1) u [a2, b2) of the patch list holds exactly the blobs
+// whose stencil covers z (stencil start in [z - ng + 1, z] modulo nz)
+template <int NZ>
+__device__ __forceinline__ void col_ranges(const int* __restrict__ zs, int z, int ng, int& a1, int& b1, int& a2, int& b2) {
+  a1 = b1 = a2 = b2 = 0;
+  if (z >= NZ) return;
+  const int lo = z - ng + 1;
+  if (lo >= 0) {
+    a1 = zs[lo];
+    b1 = zs[z + 1];
+  } else {
+    a1 = zs[NZ + lo];
+    b1 = zs[NZ];
+    a2 = zs[0];
+    b2 = zs[z + 1];
+  }
+}
+
+// x and y weights of a blob over the patch (zero outside the stencil)
+template <typename T, int CX>
+__device__ __forceinline__ void col_xy_weights(const T* ws, int ng, int rx, int ry, T* gx, T* gy) {
+#pragma unroll
+  for (int ii = 0; ii < 4; ++ii) {
+    const int o = ii - rx;
+    gx[ii] = (ii < CX && o >= 0 && o < ng) ? ws[o] : T(0);
+  }
+#pragma unroll
+  for (int jj = 0; jj < 4; ++jj) {
+    const int o = jj - ry;
+    gy[jj] = (o >= 0 && o < ng) ? ws[ng + o] : T(0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// spreading + forward z R2C
+//
+// LDS while accumulating: planes of S_CAP entries (entry i of field f at
+// f S_CAP + i), so that the lanes of a wavefront, which read neighbouring
+// entries of the list, read neighbouring banks.  Then, per round of RR rows:
+// the rows go row-major through LDS into registers, the FFT runs with a
+// padded scratch stride, and the untangled k-space rows are stored.
+// ---------------------------------------------------------------------------
+template <typename T, int NZ, int CX, int PM>
+__global__ void __launch_bounds__(col_threads(NZ))
+    col_spread_kernel(ColArgs A, const double* __restrict__ F, FieldPtrs<T> fp, const c2<T>* __restrict__ tw,
+                      const c2<T>* __restrict__ twz) {
+  using SH = ColumnShape<T, NZ, CX, PM>;
+  static_assert(SH::ok, "column shape");
+  using V = c2<T>;
+  constexpr int NT = SH::NT, M = SH::M, P = SH::P, TC = SH::TC, RR = SH::RR, NROW = SH::NROW, NRND = SH::NRND;
+  constexpr int CY = COL_CY, CAP = SH::S_CAP, LS = SH::LS, NCB = SH::NCB;
+  __shared__ __align__(16) unsigned char smem[SH::S_LDS];
+  __shared__ V s_tw[M], s_twz[M / 2 + 1];  // untangle twiddles W^k, k <= M/2
+  if (fp.gate && *fp.gate) return;
+  const GridGeom& g = A.g;
+  const int npatch = A.npx * A.npy;
+  const int p = col_xcd(blockIdx.x, npatch);
+  if (p >= npatch) return;
+  const int px = p / A.npy, py = p - (p / A.npy) * A.npy;
+  const int x0 = CX * px, y0 = CY * py;
+  // slip channel: rows above ny/2 are images, made from their mirror rows when
+  // they are read, so a patch wholly above ny/2 has nothing to write
+  if (g.channel && y0 > g.ny / 2) return;
+  const int t = threadIdx.x;
+  const int ng = A.ng, omin = 1 - ng / 2;
+  for (int i = t; i < M; i += NT) {
+    s_tw[i] = tw[i];
+    if (i <= M / 2) s_twz[i] = twz[i];
+  }
+  __syncthreads();
+  // this patch's z prefix table, read from HBM (a few words per thread)
+  const int* zs = A.zst + (size_t)p * (NZ + 1);
+  const int L0 = zs[0], L1 = zs[NZ];
+  int a1, b1, a2, b2;
+  col_ranges<NZ>(zs, t, ng, a1, b1, a2, b2);
+
+  T* sw = reinterpret_cast<T*>(smem);  // planes: gx 0-3, gy 4-7, F 8-10, gz 11-18
+  int* ssz = reinterpret_cast<int*>(sw + 19 * CAP);
+  const T* W = col_w<T>(A.w64, A.w32);
+  T acc[NROW];  // row (c CX + ii) CY + jj of this thread's z
+#pragma unroll
+  for (int r = 0; r < NROW; ++r) acc[r] = T(0);
+
+  for (int c0 = L0; c0 < L1; c0 += CAP) {
+    const int cnt = min(CAP, L1 - c0);
+    __syncthreads();
+    // two entries per thread per step, all of their loads in flight together
+    for (int i0 = 0; i0 < cnt; i0 += 2 * NT) {
+      int4 e[2];
+      T gx[2][4], gy[2][4], gz[2][8], f[2][3];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int i = i0 + t + u * NT;
+        e[u] = i < cnt ? A.ent[c0 + i] : make_int4(0, 0, 8 | (8 << 5), 0);
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const T* ws = W + (size_t)e[u].x * 3 * ng;
+        col_xy_weights<T, CX>(ws, ng, col_rx(e[u].z), col_ry(e[u].z), gx[u], gy[u]);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) gz[u][k] = k < ng ? ws[2 * ng + k] : T(0);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) f[u][c] = T(F[3 * (size_t)e[u].y + c]);
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int i = i0 + t + u * NT;
+        if (i < cnt) {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            sw[k * CAP + i] = gx[u][k];
+            sw[(4 + k) * CAP + i] = gy[u][k];
+          }
+#pragma unroll
+          for (int c = 0; c < 3; ++c) sw[(8 + c) * CAP + i] = f[u][c];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) sw[(11 + k) * CAP + i] = gz[u][k];
+          ssz[i] = col_sz(e[u].z);
+        }
+      }
+    }
+    __syncthreads();
+    const int e1a = max(a1, c0), e1b = min(b1, c0 + cnt);
+    const int e2a = max(a2, c0), e2b = min(b2, c0 + cnt);
+    const int n1 = max(0, e1b - e1a), n2 = max(0, e2b - e2a);
+    for (int k = 0; k < n1 + n2; ++k) {
+      const int i = (k < n1 ? e1a + k : e2a + (k - n1)) - c0;
+      int o = t - ssz[i];
+      if (o < 0) o += NZ;
+      const T w = sw[(11 + o) * CAP + i];
+      T gx[CX], gy[CY];
+#pragma unroll
+      for (int q = 0; q < CX; ++q) gx[q] = sw[q * CAP + i];
+#pragma unroll
+      for (int q = 0; q < CY; ++q) gy[q] = sw[(4 + q) * CAP + i];
+      const T wf0 = w * sw[8 * CAP + i], wf1 = w * sw[9 * CAP + i], wf2 = w * sw[10 * CAP + i];
+#pragma unroll
+      for (int jj = 0; jj < CY; ++jj) {
+        const T wy0 = wf0 * gy[jj], wy1 = wf1 * gy[jj], wy2 = wf2 * gy[jj];
+#pragma unroll
+        for (int ii = 0; ii < CX; ++ii) {
+          acc[(0 * CX + ii) * CY + jj] += wy0 * gx[ii];
+          acc[(1 * CX + ii) * CY + jj] += wy1 * gx[ii];
+          acc[(2 * CX + ii) * CY + jj] += wy2 * gx[ii];
+        }
+      }
+    }
+  }
+  V* area = reinterpret_cast<V*>(smem);
+  T* rowsT = reinterpret_cast<T*>(smem);
+  const int rloc = t / TC, tcol = t - (t / TC) * TC;
+#pragma unroll
+  for (int qg = 0; qg < NRND; qg += NCB) {
+    __syncthreads();  // staging (first group) or the previous group's scratch is free
+    if (t < NZ) {
+#pragma unroll
+      for (int r = 0; r < NCB * RR; ++r)
+        if (qg * RR + r < NROW) rowsT[2 * SH::row0(r) + t] = acc[qg * RR + r];
+    }
+    __syncthreads();
+    V v[NCB][P];
+#pragma unroll
+    for (int c = 0; c < NCB; ++c)
+#pragma unroll
+      for (int s = 0; s < P; ++s) v[c][s] = area[SH::row0(rloc + c * RR) + fft_in_index<M, false, PM>(tcol, s)];
+    __syncthreads();
+    fft_columns<T, M, -1, false, PM, NCB, SH::SSH, SH::SB>(v, area, LS, rloc, RR, tcol, s_tw);
+#pragma unroll
+    for (int c = 0; c < NCB; ++c)
+#pragma unroll
+      for (int s = 0; s < P; ++s) area[SH::slot(fft_out_index<M, false, PM>(tcol, s), rloc + c * RR)] = v[c][s];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NCB; ++c) {
+      const int row = (qg + c) * RR + rloc;
+      if (row < NROW) {
+        const int cc = row / (CX * CY), rem = row - (row / (CX * CY)) * (CX * CY);
+        const int ii = rem / CY, jj = rem - (rem / CY) * CY;
+        V* out = fp.p[cc] + ((size_t)(x0 + ii) * g.ny + (y0 + jj)) * g.cpitch;
+        // untangle, one bin pair (k, M - k) per step
+        const int zc = rloc + c * RR;
+        for (int k = tcol; k <= M / 2; k += TC) {
+          V xk, xmk;
+          r2c_untangle_pair<T>(area[SH::slot(k, zc)], area[SH::slot(k == 0 ? 0 : M - k, zc)], s_twz[k], xk, xmk);
+          out[k] = xk;
+          if (M - k != k) out[M - k] = xmk;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// inverse z C2R + interpolation partial sums
+//
+// Per group of rounds: each thread loads the k-space words its FFT points
+// need (X[k] and X[M-k]) from HBM, the rows are transformed through the padded
+// LDS scratch and read back row-major into this thread's z column
+// (registers).  Then the blob lists are staged in planes as for spreading,
+// and each (blob, stencil z) value lands in its own LDS slot, so the ng
+// values of a blob are added in a fixed order.
+// ---------------------------------------------------------------------------
+template <typename T, int NZ, int CX, int PM>
+__global__ void __launch_bounds__(col_threads(NZ))
+    col_interp_kernel(ColArgs A, FieldPtrs<T> fp, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ twz,
+                      T* __restrict__ part) {
+  using SH = ColumnShape<T, NZ, CX, PM>;
+  static_assert(SH::ok, "column shape");
+  using V = c2<T>;
+  constexpr int NT = SH::NT, M = SH::M, P = SH::P, TC = SH::TC, RR = SH::RR, NROW = SH::NROW, NRND = SH::NRND;
+  constexpr int CY = COL_CY, CAP = SH::I_CAP, LS = SH::LS, NCB = SH::NCB;
+  __shared__ __align__(16) unsigned char smem[SH::I_LDS];
+  __shared__ V s_tw[M], s_twz[M];  // FFT twiddles, C2R twiddles W^k
+  if (fp.gate && *fp.gate) return;
+  const GridGeom& g = A.g;
+  const int npatch = A.npx * A.npy;
+  const int p = col_xcd(blockIdx.x, npatch);
+  if (p >= npatch) return;
+  const int px = p / A.npy, py = p - (p / A.npy) * A.npy;
+  const int x0 = CX * px, y0 = CY * py;
+  const int t = threadIdx.x;
+  const int ng = A.ng, omin = 1 - ng / 2;
+  for (int i = t; i < M; i += NT) {
+    s_tw[i] = tw[i];
+    s_twz[i] = twz[i];
+  }
+  __syncthreads();
+
+  V* area = reinterpret_cast<V*>(smem);
+  const T* rowsT = reinterpret_cast<const T*>(smem);
+  const int rloc = t / TC, tcol = t - (t / TC) * TC;
+  T u[NROW];  // row (c CX + ii) CY + jj at this thread's z
+#pragma unroll
+  for (int qg = 0; qg < NRND; qg += NCB) {
+    // C2R input of this thread's FFT points, straight from HBM: each point k
+    // needs X[k] and X[M - k] (imaginary parts of k = 0 and M ignored); the
+    // second word comes from the partner lane where the plan allows
+    constexpr FftPlan IPL = reverse_plan(make_plan(M, PM));
+    constexpr bool PAIRS = c2r_lane_pairs(IPL);
+    const int partner = (t & 63) - tcol + ((TC - tcol) & (TC - 1));  // lane of thread TC - tcol of this row
+    V v[NCB][P];
+#pragma unroll
+    for (int c = 0; c < NCB; ++c) {
+      const int row = (qg + c) * RR + rloc;
+      const bool valid = row < NROW;
+      const int cc = row / (CX * CY), rem = row - (row / (CX * CY)) * (CX * CY);
+      const int ii = rem / CY, jj = rem - (rem / CY) * CY;
+      const V* in = fp.p[valid ? cc : 0] +
+                    ((size_t)(x0 + (valid ? ii : 0)) * g.ny + (y0 + (valid ? jj : 0))) * g.cpitch;
+      V xk[P];
+#pragma unroll
+      for (int s = 0; s < P; ++s) xk[s] = valid ? in[fft_in_index<M, true, PM>(tcol, s)] : mk<T>(T(0), T(0));
+      V xn = mk<T>(T(0), T(0));  // X[M], the partner of k = 0 (thread 0 of the row)
+      if (PAIRS && valid && tcol == 0) xn = in[M];
+#pragma unroll
+      for (int s = 0; s < P; ++s) {
+        const int k = fft_in_index<M, true, PM>(tcol, s);
+        V xm;
+        if constexpr (PAIRS) {
+          const V sh = mk<T>(__shfl(xk[P - 1 - s].x, partner), __shfl(xk[P - 1 - s].y, partner));
+          const int self = c2r_self_slot(IPL, s);  // a constant once unrolled
+          const V own = self >= 0 ? xk[self >= 0 ? self : 0] : xn;
+          xm = tcol == 0 ? own : sh;
+        } else {
+          xm = valid ? in[M - k] : mk<T>(T(0), T(0));
+        }
+        V xs = xk[s];
+        if (k == 0) { xs.y = T(0); xm.y = T(0); }
+        const V sum = cadd(xs, cconj(xm));
+        const V dif = csub(xs, cconj(xm));
+        const V rot = cmulc(dif, s_twz[k]);      // W^{-k} (X[k] - conj X[M-k])
+        v[c][s] = cadd(sum, mk<T>(-rot.y, rot.x));  // + i * rot
+      }
+    }
+    __syncthreads();  // the previous group's rows have been read (and the twiddles are in LDS)
+    fft_columns<T, M, +1, true, PM, NCB, SH::SSH, SH::SB>(v, area, LS, rloc, RR, tcol, s_tw);
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NCB; ++c)
+#pragma unroll
+      for (int s = 0; s < P; ++s) area[SH::row0(rloc + c * RR) + fft_out_index<M, true, PM>(tcol, s)] = v[c][s];
+    __syncthreads();
+    if (t < NZ) {
+#pragma unroll
+      for (int r = 0; r < NCB * RR; ++r)
+        if (qg * RR + r < NROW) u[qg * RR + r] = rowsT[2 * SH::row0(r) + t];
+    }
+  }
+  // this patch's z prefix table, read from HBM (a few words per thread)
+  const int* zs = A.zst + (size_t)p * (NZ + 1);
+  const int L0 = zs[0], L1 = zs[NZ];
+  int a1, b1, a2, b2;
+  col_ranges<NZ>(zs, t, ng, a1, b1, a2, b2);
+  T* sw = reinterpret_cast<T*>(smem);  // planes: gx 0-3, gy 4-7, gz 8-15, slot (c, o) 16 + 8 c + o
+  int* ssz = reinterpret_cast<int*>(sw + 40 * CAP);
+  int* sout = ssz + CAP;  // partial-sum slot of the blob for this patch
+  const T* W = col_w<T>(A.w64, A.w32);
+  for (int c0 = L0; c0 < L1; c0 += CAP) {
+    const int cnt = min(CAP, L1 - c0);
+    __syncthreads();
+    for (int i0 = 0; i0 < cnt; i0 += 2 * NT) {
+      int4 e[2];
+      T gx[2][4], gy[2][4], gz[2][8];
+#pragma unroll
+      for (int uu = 0; uu < 2; ++uu) {
+        const int i = i0 + t + uu * NT;
+        e[uu] = i < cnt ? A.ent[c0 + i] : make_int4(0, 0, 8 | (8 << 5), 0);
+      }
+#pragma unroll
+      for (int uu = 0; uu < 2; ++uu) {
+        const T* ws = W + (size_t)e[uu].x * 3 * ng;
+        col_xy_weights<T, CX>(ws, ng, col_rx(e[uu].z), col_ry(e[uu].z), gx[uu], gy[uu]);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) gz[uu][k] = k < ng ? ws[2 * ng + k] : T(0);
+      }
+#pragma unroll
+      for (int uu = 0; uu < 2; ++uu) {
+        const int i = i0 + t + uu * NT;
+        if (i < cnt) {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            sw[k * CAP + i] = gx[uu][k];
+            sw[(4 + k) * CAP + i] = gy[uu][k];
+          }
+#pragma unroll
+          for (int k = 0; k < 8; ++k) sw[(8 + k) * CAP + i] = gz[uu][k];
+          ssz[i] = col_sz(e[uu].z);
+          sout[i] = e[uu].w;
+        }
+      }
+    }
+    __syncthreads();
+    const int e1a = max(a1, c0), e1b = min(b1, c0 + cnt);
+    const int e2a = max(a2, c0), e2b = min(b2, c0 + cnt);
+    const int n1 = max(0, e1b - e1a), n2 = max(0, e2b - e2a);
+    for (int k = 0; k < n1 + n2; ++k) {
+      const int i = (k < n1 ? e1a + k : e2a + (k - n1)) - c0;
+      int o = t - ssz[i];
+      if (o < 0) o += NZ;
+      const T w = sw[(8 + o) * CAP + i];
+      T gx[CX], gy[CY];
+#pragma unroll
+      for (int q = 0; q < CX; ++q) gx[q] = sw[q * CAP + i];
+#pragma unroll
+      for (int q = 0; q < CY; ++q) gy[q] = sw[(4 + q) * CAP + i];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        T a = T(0);
+#pragma unroll
+        for (int ii = 0; ii < CX; ++ii) {
+          T b = T(0);
+#pragma unroll
+          for (int jj = 0; jj < CY; ++jj) b += gy[jj] * u[(c * CX + ii) * CY + jj];
+          a += gx[ii] * b;
+        }
+        sw[(16 + 8 * c + o) * CAP + i] = w * a;
+      }
+    }
+    __syncthreads();
+    for (int i = t; i < cnt; i += NT) {
+      T s0 = T(0), s1 = T(0), s2 = T(0);
+      for (int o = 0; o < ng; ++o) {
+        s0 += sw[(16 + o) * CAP + i];
+        s1 += sw[(24 + o) * CAP + i];
+        s2 += sw[(32 + o) * CAP + i];
+      }
+      T* out = part + (size_t)sout[i] * 3;
+      out[0] = s0;
+      out[1] = s1;
+      out[2] = s2;
+    }
+  }
+}
+
+// sum of a blob's patch partials in a fixed order, then dx^3 J u - K V.  The
+// loads of every slot a blob can have (XM x 3 for patches CX wide, ng <= 8)
+// are issued before the record that says which of them are its own arrives;
+// the others are read but not added.
+template <typename T, int CX>
+__global__ void __launch_bounds__(256) col_reduce_kernel(GridGeom g, int ng, int nb, const int4* __restrict__ rec,
+                                                         const T* __restrict__ part, double* V, KTerm kt, double dx3,
+                                                         const int* gate) {
+  constexpr int XM = (CX - 1 + 7) / CX + 1, YM = (COL_CY - 1 + 7) / COL_CY + 1;
+  static_assert(XM * YM <= COL_SLOTS_MAX, "partial-sum slots");
+  if (gate && *gate) return;
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= nb) return;
+  T pv[XM][YM][3];
+#pragma unroll
+  for (int dx = 0; dx < XM; ++dx)
+#pragma unroll
+    for (int dy = 0; dy < YM; ++dy) {
+      const T* pp = part + ((size_t)(dx * 3 + dy) * nb + s) * 3;
+      pv[dx][dy][0] = pp[0];
+      pv[dx][dy][1] = pp[1];
+      pv[dx][dy][2] = pp[2];
+    }
+  const int omin = 1 - ng / 2;
+  const int4 r = rec[s];
+  const int sxw = col_wrapi(r.x + omin, g.nx), syw = col_wrapi(r.y + omin, g.ny);
+  const int nxp = (sxw % CX + ng - 1) / CX + 1, nyp = (syw % COL_CY + ng - 1) / COL_CY + 1;
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+#pragma unroll
+  for (int dx = 0; dx < XM; ++dx)
+#pragma unroll
+    for (int dy = 0; dy < YM; ++dy)
+      if (dx < nxp && dy < nyp) {
+        v0 += (double)pv[dx][dy][0];
+        v1 += (double)pv[dx][dy][1];
+        v2 += (double)pv[dx][dy][2];
+      }
+  store_blob_velocity(V, kt, dx3, r.w, v0, v1, v2);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+#define FCM_COL_NZ(X) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(512)
+
+namespace {
+
+ColArgs col_args(const GridGeom& g, const BlobData& bd, bool interp) {
+  return ColArgs{g,      bd.ng, g.nx / bd.col_cx, g.ny / COL_CY, bd.col_ent, bd.col_zst, interp ? bd.iw64 : bd.w64,
+                 interp ? bd.iw32 : bd.w32};
+}
+
+// points per thread of the z pass: the tuned request (8 or 16) or the default
+// plan of the shape; 0 if the shape cannot be built
+template <typename T, int NZ, int CX>
+constexpr int col_pm_pick(int req) {
+  constexpr int PD = col_pm(NZ, CX);
+  if (req == 8 && ColumnShape<T, NZ, CX, 8>::ok) return 8;
+  if (req == 16 && ColumnShape<T, NZ, CX, 16>::ok) return 16;
+  if (req != 0 && req != PD) return 0;
+  return ColumnShape<T, NZ, CX, PD>::ok ? PD : 0;
+}
+
+template <typename T, int NZ, int CX, int PM>
+hipError_t launch_spread(const ColArgs& A, const double* F, const FieldPtrs<T>& fp, const TwiddleSet<T>& tw,
+                         hipStream_t st) {
+  if constexpr (!ColumnShape<T, NZ, CX, PM>::ok) {
+    return hipErrorInvalidValue;
+  } else {
+    using SH = ColumnShape<T, NZ, CX, PM>;
+    const int npatch = A.npx * A.npy;
+    hipLaunchKernelGGL((col_spread_kernel<T, NZ, CX, PM>), dim3(8 * ((npatch + 7) / 8)), dim3(SH::NT), 0, st, A, F,
+                       fp, tw.z, tw.zu);
+    return hipGetLastError();
+  }
+}
+
+template <typename T, int NZ, int CX, int PM>
+hipError_t launch_interp(const ColArgs& A, const FieldPtrs<T>& fp, const TwiddleSet<T>& tw, T* part, hipStream_t st) {
+  if constexpr (!ColumnShape<T, NZ, CX, PM>::ok) {
+    return hipErrorInvalidValue;
+  } else {
+    using SH = ColumnShape<T, NZ, CX, PM>;
+    const int npatch = A.npx * A.npy;
+    hipLaunchKernelGGL((col_interp_kernel<T, NZ, CX, PM>), dim3(8 * ((npatch + 7) / 8)), dim3(SH::NT), 0, st, A, fp,
+                       tw.z, tw.zu, part);
+    return hipGetLastError();
+  }
+}
+
+// the kernel pair of (nz, cx, pm) for one side of the product
+template <typename T, int NZ>
+hipError_t col_launch(bool interp, const ColArgs& A, const BlobData& bd, const double* F, const FieldPtrs<T>& fp,
+                      const TwiddleSet<T>& tw, T* part, hipStream_t st) {
+  const int req = bd.col_pm[sizeof(T) == 4 ? 1 : 0];
+  auto go = [&](auto cxc) -> hipError_t {
+    constexpr int CX = decltype(cxc)::value;
+    switch (col_pm_pick<T, NZ, CX>(req)) {
+      case 8:
+        return interp ? launch_interp<T, NZ, CX, 8>(A, fp, tw, part, st) : launch_spread<T, NZ, CX, 8>(A, F, fp, tw, st);
+      case 16:
+        return interp ? launch_interp<T, NZ, CX, 16>(A, fp, tw, part, st)
+                      : launch_spread<T, NZ, CX, 16>(A, F, fp, tw, st);
+      default:
+        return hipErrorInvalidValue;
+    }
+  };
+  if (bd.col_cx == 4) return go(std::integral_constant<int, 4>{});
+  if (bd.col_cx == 2) return go(std::integral_constant<int, 2>{});
+  return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+template <typename T>
+hipError_t spread_zfwd(const GridGeom& g, const BlobData& bd, const double* F, const FieldPtrs<T>& fp,
+                       const TwiddleSet<T>& tw, hipStream_t st, bool images) {
+  if (bd.col_cx <= 0) return hipErrorInvalidValue;
+  const ColArgs A = col_args(g, bd, false);
+  hipError_t e = hipErrorInvalidValue;
+  switch (g.nz) {
+#define FCM_COL_SPREAD(v) \
+  case v: e = col_launch<T, v>(false, A, bd, F, fp, tw, nullptr, st); break;
+    FCM_COL_NZ(FCM_COL_SPREAD)
+#undef FCM_COL_SPREAD
+    default:
+      break;
+  }
+  return e != hipSuccess || !images ? e : channel_images<T>(g, fp, st);
+}
+
+template <typename T>
+hipError_t zinv_interp(const GridGeom& g, const BlobData& bd, const FieldPtrs<T>& fp, const TwiddleSet<T>& tw,
+                       double* V, const KTerm& kt, hipStream_t st, hipEvent_t between) {
+  if (bd.col_cx <= 0) return hipErrorInvalidValue;
+  if (bd.nb == 0) return hipSuccess;
+  const ColArgs A = col_args(g, bd, true);
+  T* part = static_cast<T*>(bd.col_part);
+  hipError_t e = hipErrorInvalidValue;
+  switch (g.nz) {
+#define FCM_COL_INTERP(v) \
+  case v: e = col_launch<T, v>(true, A, bd, nullptr, fp, tw, part, st); break;
+    FCM_COL_NZ(FCM_COL_INTERP)
+#undef FCM_COL_INTERP
+    default:
+      break;
+  }
+  if (e != hipSuccess) return e;
+  if (between && (e = hipEventRecord(between, st)) != hipSuccess) return e;
+  const dim3 grid((bd.nb + 255) / 256);
+  if (bd.col_cx == 4)
+    hipLaunchKernelGGL((col_reduce_kernel<T, 4>), grid, dim3(256), 0, st, g, bd.ng, bd.nb, bd.rec, (const T*)part, V,
+                       kt, g.dx * g.dx * g.dx, fp.gate);
+  else
+    hipLaunchKernelGGL((col_reduce_kernel<T, 2>), grid, dim3(256), 0, st, g, bd.ng, bd.nb, bd.rec, (const T*)part, V,
+                       kt, g.dx * g.dx * g.dx, fp.gate);
+  return hipGetLastError();
+}
+
+}  // namespace fcm

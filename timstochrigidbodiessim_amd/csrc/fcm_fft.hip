@@ -216,13 +216,13 @@ __global__ void __launch_bounds__(256) zinv_kernel(FieldPtrs<T> fp, int rows, in
 // need not write it: element y > N/2 is read from N - y, field 1 (fy) negated
 // there and zero at y = 0 and y = N/2.  On a field that already holds the
 // images the read returns what is stored, so the rule may be applied twice.
-template <typename T, int N, int S, bool MIRROR = false>
+template <typename T, int N, int S, bool MIRROR = false, int PM = 16>
 __global__ void __launch_bounds__(256) cols_kernel(FieldPtrs<T> fp, size_t plane_stride, size_t estride, int nzc,
                                                    const c2<T>* __restrict__ tw) {
   if (fp.gate && *fp.gate) return;
   using V = c2<T>;
-  using SH = ColShape<T, N>;
-  constexpr FftPlan PL = make_plan(N);
+  using SH = ColShape<T, N, PM>;
+  constexpr FftPlan PL = make_plan(N, PM);
   constexpr int P = PL.P, TC = SH::TC, C = SH::C;
   constexpr bool REV = (S > 0);
   __shared__ V lds[N * C];
@@ -245,7 +245,7 @@ __global__ void __launch_bounds__(256) cols_kernel(FieldPtrs<T> fp, size_t plane
     const bool fy = blockIdx.z == 1;
 #pragma unroll
     for (int s = 0; s < P; ++s) {
-      const int e = fft_in_index<N, REV>(tcol, s);
+      const int e = fft_in_index<N, REV, PM>(tcol, s);
       const bool up = e > N / 2;
       V x = valid ? *reinterpret_cast<const V*>(base + (colb + (unsigned)(up ? N - e : e) * eb)) : mk<T>(T(0), T(0));
       if (fy) {
@@ -257,15 +257,15 @@ __global__ void __launch_bounds__(256) cols_kernel(FieldPtrs<T> fp, size_t plane
   } else {
 #pragma unroll
     for (int s = 0; s < P; ++s) {
-      const int e = fft_in_index<N, REV>(tcol, s);
+      const int e = fft_in_index<N, REV, PM>(tcol, s);
       v[s] = valid ? *reinterpret_cast<const V*>(base + (colb + (unsigned)e * eb)) : mk<T>(T(0), T(0));
     }
   }
-  fft_column<T, N, S, REV>(v, lds, C, col, tcol, s_tw);
+  fft_column<T, N, S, REV, PM>(v, lds, C, col, tcol, s_tw);
   if (valid) {
 #pragma unroll
     for (int s = 0; s < P; ++s)
-      *reinterpret_cast<V*>(base + (colb + (unsigned)fft_out_index<N, REV>(tcol, s) * eb)) = v[s];
+      *reinterpret_cast<V*>(base + (colb + (unsigned)fft_out_index<N, REV, PM>(tcol, s) * eb)) = v[s];
   }
 }
 
@@ -303,12 +303,11 @@ __device__ __forceinline__ int signed_k(int k, int n) { return k <= n / 2 ? k : 
 template <typename T> struct XPM { static constexpr int value = sizeof(T) == 8 ? 8 : 16; };
 template <typename T> struct XTMAX { static constexpr int value = 256; };
 
-template <typename T, int N>
+template <typename T, int N, int PM = XPM<T>::value>
 __global__ void __launch_bounds__(XTMAX<T>::value) xproj_kernel(FieldPtrs<T> fp, int ny, int cpitch, int nzc,
                                                     ProjParams pp, const c2<T>* __restrict__ tw) {
   if (fp.gate && *fp.gate) return;
   using V = c2<T>;
-  constexpr int PM = XPM<T>::value;
   using SH = ColShape<T, N, PM, XTMAX<T>::value>;
   constexpr FftPlan PL = make_plan(N, PM);
   constexpr int P = PL.P, TC = SH::TC, C = SH::C;
@@ -469,6 +468,23 @@ static hipError_t launch_z(const GridGeom& g, const FieldPtrs<T>& fp, int nf, bo
   return hipGetLastError();
 }
 
+template <typename T, int N, int PM>
+static hipError_t cols_launch(const GridGeom& g, const FieldPtrs<T>& fp, int nf, int nplanes, size_t plane_stride,
+                              size_t estride, bool fwd, bool mirror, const c2<T>* tw, hipStream_t st) {
+  using SH = ColShape<T, N, PM>;
+  const dim3 grid((g.nzc + SH::C - 1) / SH::C, nplanes, nf);
+  if (mirror)
+    hipLaunchKernelGGL((cols_kernel<T, N, -1, true, PM>), grid, dim3(SH::THREADS), 0, st, fp, plane_stride, estride,
+                       g.nzc, tw);
+  else if (fwd)
+    hipLaunchKernelGGL((cols_kernel<T, N, -1, false, PM>), grid, dim3(SH::THREADS), 0, st, fp, plane_stride, estride,
+                       g.nzc, tw);
+  else
+    hipLaunchKernelGGL((cols_kernel<T, N, 1, false, PM>), grid, dim3(SH::THREADS), 0, st, fp, plane_stride, estride,
+                       g.nzc, tw);
+  return hipGetLastError();
+}
+
 template <typename T>
 static hipError_t launch_cols(const GridGeom& g, const FieldPtrs<T>& fp, int nf, int axis, bool fwd,
                               const TwiddleSet<T>& tws, hipStream_t st, bool mirror = false) {
@@ -478,40 +494,44 @@ static hipError_t launch_cols(const GridGeom& g, const FieldPtrs<T>& fp, int nf,
   const size_t plane_stride = axis == 1 ? (size_t)g.ny * g.cpitch : (size_t)g.cpitch;
   const size_t estride = axis == 1 ? (size_t)g.cpitch : (size_t)g.ny * g.cpitch;
   const c2<T>* tw = axis == 1 ? tws.y : tws.x;
+  // the tuned shape applies to the y passes; x passes of the stand-alone
+  // transforms keep the default
+  const int pm = axis == 1 ? g.ypm[sizeof(T) == 4 ? 1 : 0] : 0;
   switch (N) {
 #define FCM_CASE(v)                                                                                              \
   case v: {                                                                                                      \
-    using SH = ColShape<T, v>;                                                                                   \
-    dim3 grid((g.nzc + SH::C - 1) / SH::C, nplanes, nf);                                                         \
-    if (mirror)                                                                                                  \
-      hipLaunchKernelGGL((cols_kernel<T, v, -1, true>), grid, dim3(SH::THREADS), 0, st, fp, plane_stride, estride,  \
-                         g.nzc, tw);                                                                             \
-    else if (fwd)                                                                                                \
-      hipLaunchKernelGGL((cols_kernel<T, v, -1>), grid, dim3(SH::THREADS), 0, st, fp, plane_stride, estride, g.nzc, tw); \
-    else                                                                                                         \
-      hipLaunchKernelGGL((cols_kernel<T, v, 1>), grid, dim3(SH::THREADS), 0, st, fp, plane_stride, estride, g.nzc, tw); \
-    break;                                                                                                       \
+    if constexpr (v >= 64)                                                                                       \
+      if (pm == 8) return cols_launch<T, v, 8>(g, fp, nf, nplanes, plane_stride, estride, fwd, mirror, tw, st);  \
+    return cols_launch<T, v, 16>(g, fp, nf, nplanes, plane_stride, estride, fwd, mirror, tw, st);               \
   }
     FCM_FOR_EACH_N(FCM_CASE)
 #undef FCM_CASE
     default:
       return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
 template <typename T>
 static hipError_t launch_xproj(const GridGeom& g, const FieldPtrs<T>& fp, bool stress, const ProjParams& pp,
                                const TwiddleSet<T>& tws, hipStream_t st) {
+  const int pm = g.xpm[sizeof(T) == 4 ? 1 : 0];
   switch (g.nx) {
 #define FCM_CASE(v)                                                                                             \
   case v: {                                                                                                     \
     using SH = ColShape<T, v, XPM<T>::value, XTMAX<T>::value>;                                                  \
     dim3 grid((g.nzc + SH::C - 1) / SH::C, g.ny);                                                               \
-    if (stress)                                                                                                 \
+    if (stress) {                                                                                               \
       hipLaunchKernelGGL((xstress_kernel<T, v>), grid, dim3(SH::THREADS), 0, st, fp, g.ny, g.cpitch, g.nzc, pp, tws.x); \
-    else                                                                                                        \
-      hipLaunchKernelGGL((xproj_kernel<T, v>), grid, dim3(SH::THREADS), 0, st, fp, g.ny, g.cpitch, g.nzc, pp, tws.x); \
+      break;                                                                                                    \
+    }                                                                                                           \
+    if constexpr (sizeof(T) == 4 && v >= 64)                                                                    \
+      if (pm == 8) {                                                                                            \
+        using S8 = ColShape<T, v, 8, XTMAX<T>::value>;                                                          \
+        hipLaunchKernelGGL((xproj_kernel<T, v, 8>), dim3((g.nzc + S8::C - 1) / S8::C, g.ny), dim3(S8::THREADS), \
+                           0, st, fp, g.ny, g.cpitch, g.nzc, pp, tws.x);                                        \
+        break;                                                                                                  \
+      }                                                                                                         \
+    hipLaunchKernelGGL((xproj_kernel<T, v>), grid, dim3(SH::THREADS), 0, st, fp, g.ny, g.cpitch, g.nzc, pp, tws.x); \
     break;                                                                                                      \
   }
     FCM_FOR_EACH_N(FCM_CASE)

@@ -98,6 +98,8 @@ struct fcm_handle_s {
   double* w = nullptr;        // A P^-1 q_k of the current step (fixed address for graph capture)
   double* hist_all = nullptr; // residual history of the whole (refined) solve, max_krylov + 2
   unsigned long long epoch = 0; // bumped whenever kernel arguments of an Arnoldi step change
+  bool tuning_set = false;      // launch shapes set explicitly (fcm_set_tuning, fcm_autotune*)
+  bool cache_checked = false;   // the shape-class cache has been consulted
   fcm_batch_s* solo = nullptr;  // single-solve driver on this handle's stream
   std::vector<fcm_batch_s*> batches;  // batches whose captured graphs may reference this handle
   bool have_rhs = false, have_sol = false;

@@ -12,6 +12,11 @@ struct GridGeom {
   int nx = 0, ny = 0, nz = 0, nzc = 0, cpitch = 0;
   int channel = 0;  // slip-channel image system in y (walls at rows 0 and ny/2)
   double dx = 1.0;
+  // launch shapes of the y and x FFT passes of the mobility product per
+  // precision (0 FP64, 1 FP32): points per thread, 8 or 16 (0: the default,
+  // 16; the x pass keeps 8 in FP64), with as many columns per workgroup as fit
+  // 256 threads and 64 KiB of LDS; lengths below 64 always use the default
+  int ypm[2] = {0, 0}, xpm[2] = {0, 0};
   size_t field_words() const { return (size_t)nx * ny * cpitch; }  // complex words per field
 };
 
@@ -94,7 +99,8 @@ struct BlobData {
   // column path of the mobility product (fcm_column.hip): x-y patches of
   // col_cx x COL_CY nodes with all of z; per patch the blobs whose x-y stencil
   // touches it, ordered by stencil-start z, and a z prefix table
-  int col_cx = 0;              // patch x extent (0: the grid does not qualify)
+  int col_cx = 0;              // patch x extent, 4 or 2 (0: the grid does not qualify)
+  int col_pm[2] = {0, 0};      // z-pass points per thread per precision: 8, 16 (0: the shape's default)
   int* col_count = nullptr;    // npatch x nbz: entries per (patch, z group)
   int* col_start = nullptr;    // npatch x nbz + 1 (exclusive scan)
   int* col_cursor = nullptr;   // scan scratch
@@ -160,6 +166,11 @@ constexpr int COL_CY = 4;          // patch y extent
 constexpr int COL_SLOTS_MAX = 15;  // partial sums per blob: up to 5 x 3 patches
 // patch x extent usable on this grid for both precisions (0: none)
 int col_patch_cx(const GridGeom& g, const BlobBins& bb, int ng);
+// the column kernels exist for word size tbytes at this nz, patch width cx and
+// z-pass points per thread pm (0: the default plan)
+bool col_shape_valid(int tbytes, int nz, int cx, int pm);
+// the z-pass points per thread those kernels run with (0: no such kernels)
+int col_shape_pm(int tbytes, int nz, int cx, int pm);
 size_t col_list_capacity(int nb);
 size_t col_zst_words(const GridGeom& g, int cx);
 int col_groups(const GridGeom& g, int cx);  // npatch x nbz

@@ -83,13 +83,20 @@ class MobilityServer:
     """Continuous-batching request loop over one FcmBatch."""
 
     def __init__(self, hbm_budget=None, max_jobs=32, worker_streams=4, steps_per_round=4, max_krylov=400,
-                 keep_solutions=False):
+                 keep_solutions=False, autotune=True, tune_reps=3):
         self.batch = L.FcmBatch(max_jobs=max_jobs, worker_streams=worker_streams, steps_per_round=steps_per_round)
         free, _ = L.device_memory()
         self.budget = int(hbm_budget) if hbm_budget else int(0.85 * free)
         self.max_jobs = max_jobs
         self.max_krylov = max_krylov
         self.keep_solutions = keep_solutions
+        # launch shapes: each shape class is tuned once, by its first request, in
+        # both precisions (an FP32 solve refines with FP64 products); later
+        # requests, and new handles of the class, take the library's cached choice,
+        # so a request runs the same shapes as the same request solved alone
+        self.autotune = autotune
+        self.tune_reps = tune_reps
+        self.tunings_run = 0  # (shape class, precision) pairs timed
         self.used = 0
         self.idle = collections.defaultdict(list)  # shape key -> idle solvers
         self.bytes_of = {}
@@ -161,6 +168,10 @@ class MobilityServer:
             self.refs[rkey] = (Minv, N)
         Minv, N = self.refs[rkey]
         s.set_bodies(x, q)
+        if self.autotune:
+            for prec in (L.FCM_PREC_FP32, L.FCM_PREC_FP64):
+                _, hit = s.autotune_cached(prec, reps=self.tune_reps)
+                self.tunings_run += 0 if hit else 1
         s.set_ref_matrices(Minv, N)
         s.set_body_forces(F6)
         s.mobility_prepare(req.seed)
