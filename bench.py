@@ -214,7 +214,7 @@ def bench_batch(args):
         "metric": "batched FCM mobility solves/s (64 uneven instances, FP32+refine, tol 1e-8)",
         "value": round(len(insts) / t_batch, 3), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(t_batch * 1e3, 2), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32 grid passes + f64 Krylov/refinement",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 grid passes and inner Krylov basis + f64 coefficients/P^-1/refinement",
         "data": "synthetic (seeded rods and forces)",
         "config": {"workload": f"{len(insts)} instances, grids 32..128^3, 50-2000 rods x 6-12 spheres, kT=0",
                    "parallelism": "single GPU", "worker_streams": args.workers,
@@ -259,7 +259,7 @@ def bench_serve(args):
         "metric": "served FCM mobility solves/s (Poisson arrivals, 64..256^3, FP32+refine vs FP64); p50/p99 latency",
         "value": round(h["solves_per_s"], 3), "unit": "solves/s", "n_gpus": 1, "steps": args.requests,
         "warmup": 0, "ms_per_step": round(1e3 * h["wall_s"] / max(1, h["requests"]), 2), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32 grid passes + f64 Krylov/refinement",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 grid passes and inner Krylov basis + f64 coefficients/P^-1/refinement",
         "data": "synthetic (seeded Poisson stream, rods, forces and stress)",
         "config": {"workload": f"{args.requests} requests at {args.rate}/s, grids 64..256^3, 100-20000 rods x 10, "
                                f"kT=1, tol {args.tol:g}, iteration cap {args.max_iter}", "parallelism": "single GPU"},
@@ -326,7 +326,7 @@ def bench_timestep(args):
                   + ("drifter-corrector)" if args.gdc else "EM)") + (" (slip channel)" if chan else ""),
         "value": round(1e3 / ms, 4), "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms, 1), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 grid passes + f64 Krylov/refinement/constraints/quaternions",
+        "dtype": "f32 grid passes and inner Krylov basis + f64 coefficients/P^-1/refinement/constraints/quaternions",
         "data": "synthetic (seeded rods and stress stream)",
         "config": {"workload": f"{n}^3 grid, {NB} rods x {nbpb} spheres ({nb} blobs), a=1.86, kT=1, steric "
                                f"cutoff 2.2a, {'generalised drifter-corrector (divU by 6 finite differences)' if args.gdc else 'Euler-Maruyama'}, dt {args.dt:.5f}, volume fraction {phi:.3f}"
@@ -525,7 +525,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 grid passes + f64 Krylov/refinement",
+        "dtype": "f32 grid passes and inner Krylov basis + f64 coefficients/P^-1/refinement",
         "data": "synthetic (seeded rods, forces and stress stream)",
         "config": {"workload": f"{n}^3 grid, {args.rods} rods x {args.spheres} spheres ({nb} blobs), a=1.86, dx=1, "
                                f"mu=1, kT=1, ng=7, GMRES tol {args.tol:g}, volume fraction {phi:.3f}, "

@@ -203,8 +203,10 @@ int rb_reset_to_start_of_step(fcm_handle h);
 typedef struct {
   double tol;          /* GMRES_TOL (config.hpp:62): stop when |beta_j|/||RHS|| < tol */
   int max_iter;        /* GMRES_MAX_ITER (config.hpp:63)                             */
-  int precision;       /* FCM_PREC_FP64: plain solve; FCM_PREC_FP32: FP32 bulk inner
-                          solves with FP64 iterative refinement on the true residual */
+  int precision;       /* FCM_PREC_FP64: plain solve; FCM_PREC_FP32: inner solves with
+                          the grid passes and the Krylov basis in FP32 (coefficients,
+                          least squares and P^-1 in FP64), and FP64 iterative
+                          refinement on the true residual */
   double inner_tol;    /* FP32 form: relative tolerance of each inner solve
                           (0: max(1e-2, tol ||b|| / (2 ||r||)))                  */
   int max_refine;      /* FP32 form: maximum outer refinements (0: 12)              */

@@ -179,8 +179,10 @@ int inner_max_iter(const Job* j) {
 
 // Each inner FP32 solve reduces the current residual by 1e-2 (or just enough to
 // reach the tolerance): restarting the Krylov space every ~75 iterations keeps
-// the FP64 orthogonalisation, whose cost grows with the square of the basis
-// size, well below that of one long solve, at about the same iteration count.
+// the orthogonalisation, whose cost grows with the square of the basis size,
+// well below that of one long solve, at about the same iteration count.  The
+// inner basis is stored in FP32 (half the bytes per pass over it); the FP64
+// refinement of the true residual removes its rounding from the solution.
 double inner_tol(const Job* j, double bn, double rn) {
   double itol = j->inner_tol > 0 ? j->inner_tol : std::max(0.5 * j->tol * bn / rn, 1e-2);
   return std::min(itol, 0.5);
@@ -197,7 +199,7 @@ int job_begin(fcm_batch_s* B, Job* j) {
     HIPCHK(axpby(1.0, h->rhs, 0.0, h->res, n, st));
     HIPCHK(dots(h->rhs, 0, 1, h->rhs, n, h->kw, h->gm_ctl + 7, st));
     HIPCHK(dots(h->res, 0, 1, h->res, n, h->kw, h->gm.red, st));
-    HIPCHK(gm_launch_start(h->gm, inner_tol(j, 1.0, 1.0), inner_max_iter(j), h->res, h->Q, n, st));
+    HIPCHK(gm_launch_start(h->gm, inner_tol(j, 1.0, 1.0), inner_max_iter(j), h->res, h->Qf(), n, st));
   } else {
     HIPCHK(dots(h->rhs, 0, 1, h->rhs, n, h->kw, h->gm.red, st));
     HIPCHK(gm_launch_start(h->gm, j->tol, inner_max_iter(j), h->rhs, h->Q, n, st));
@@ -229,7 +231,8 @@ int job_inner_done(fcm_batch_s* B, Job* j, const double* ctl) {
     nc = 0;
     rel = 0.0;
   }
-  HIPCHK(gm_launch_finish(h->gm, h->Q, h->ldq, h->tmp, n, st));
+  if (j->prec == FCM_PREC_FP32) HIPCHK(gm_launch_finish(h->gm, h->Qf(), h->ldq, h->tmp, n, st));
+  else HIPCHK(gm_launch_finish(h->gm, h->Q, h->ldq, h->tmp, n, st));
   BodyData bdy = h->body();
   HIPCHK(apply_pinv(bdy, h->tmp, h->sol, nullptr, st));
   // hist_all keeps the first hist_capacity(h) entries of the whole solve's history
@@ -278,7 +281,7 @@ int job_refine_decide(fcm_batch_s* B, Job* j, const double* ctl) {
   }
   const size_t n = h->nsys();
   HIPCHK(dots(h->res, 0, 1, h->res, n, h->kw, h->gm.red, B->master));
-  HIPCHK(gm_launch_start(h->gm, inner_tol(j, bn, rn), inner_max_iter(j), h->res, h->Q, n, B->master));
+  HIPCHK(gm_launch_start(h->gm, inner_tol(j, bn, rn), inner_max_iter(j), h->res, h->Qf(), n, B->master));
   j->phase = Phase::INNER;
   j->valid_after = B->seq + 1;
   return FCM_OK;

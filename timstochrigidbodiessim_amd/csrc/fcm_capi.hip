@@ -272,13 +272,17 @@ int launch_arnoldi_step(fcm_handle h, int precision, hipStream_t st) {
   b.gate = gate;
   b.kcol = h->gm.ictl + GM_K;
   b.ldq = h->ldq;
-  HIPCHK(apply_pinv(b, h->Q, h->z, h->w + n3, st));
+  // the inner solves of FP32+refine keep their basis in FP32
+  const bool fp32 = precision == FCM_PREC_FP32;
+  if (fp32) HIPCHK(apply_pinv(b, h->Qf(), h->z, h->w + n3, st));
+  else HIPCHK(apply_pinv(b, h->Q, h->z, h->w + n3, st));
   KTerm kt;
   kt.zV = h->z + n3;
   kt.dpos = h->d_dpos;
   kt.nbpb = h->nbpb;
   if (int rc = mobility_dev(h, h->z, h->w, precision, kt, st, gate)) return rc;
-  HIPCHK(gm_launch_basis(h->gm, h->Q, h->ldq, h->w, n, h->kw.partial, gm_blocks(n), st));
+  if (fp32) HIPCHK(gm_launch_basis(h->gm, h->Qf(), h->ldq, h->w, n, h->kw.partial, gm_blocks(n), st));
+  else HIPCHK(gm_launch_basis(h->gm, h->Q, h->ldq, h->w, n, h->kw.partial, gm_blocks(n), st));
   return FCM_OK;
 }
 

@@ -27,30 +27,113 @@ static constexpr int KRT = 256;  // threads of the basis kernels
 static constexpr int KRP = 4;    // rows per thread
 static constexpr int KRB = KRT * KRP;
 
-// Rows of the basis passes.  Thread t of block x holds KRP rows: with PAIR,
-// rows x KRB + 2 t + 2 KRT p + j (pair p < KRP/2, j = 0, 1), read and written
-// two at a time (the basis columns are 256-word aligned, so r even is 16-byte
-// aligned) -- faster on long vectors (about 15% on the update pass at n = 840k);
-// otherwise rows x KRB + t + KRT q one at a time -- faster on short ones (the
-// batches of small systems ran 8% faster).  Rows at or past n read as zero and
-// are not written.
-__device__ __forceinline__ size_t gm_row1(int q) { return (size_t)blockIdx.x * KRB + threadIdx.x + (size_t)q * KRT; }
-__device__ __forceinline__ size_t gm_row2(int p) { return (size_t)blockIdx.x * KRB + 2 * threadIdx.x + (size_t)p * 2 * KRT; }
-__device__ __forceinline__ double2 gm_ld2(const double* __restrict__ v, size_t r, size_t n) {
-  if (r + 1 < n) return *reinterpret_cast<const double2*>(v + r);
-  double2 t;
-  t.x = r < n ? v[r] : 0.0;
-  t.y = 0.0;
-  return t;
+// Rows of the basis passes.  Thread t of block x holds KRP rows, element q at
+//   MAP 1: x KRB + t + KRT q, one word at a time -- faster on short vectors
+//          (the batches of small systems ran 8% faster);
+//   MAP 2: x KRB + 2 t + 2 KRT (q / 2) + q % 2, two words at a time (16-byte
+//          loads of an FP64 basis: about 15% faster on the update pass at n = 840k);
+//   MAP 4: x KRB + 4 t + q, four words at a time (16-byte loads of an FP32 basis).
+// The basis columns are 256-word aligned, so every vector access is aligned.
+// Rows at or past n read as zero and are not written.  Every vector the passes
+// touch (basis columns in TQ, the new vector in FP64) uses the same rows.
+template <int MAP> __device__ __forceinline__ size_t gm_row(int q) {
+  const size_t b = (size_t)blockIdx.x * KRB;
+  if constexpr (MAP == 1) return b + threadIdx.x + (size_t)q * KRT;
+  else if constexpr (MAP == 2) return b + 2 * threadIdx.x + (size_t)(q / 2) * 2 * KRT + (q % 2);
+  else return b + 4 * threadIdx.x + q;
 }
-__device__ __forceinline__ void gm_st2(double* __restrict__ v, size_t r, size_t n, double x0, double x1) {
-  if (r + 1 < n) {
-    double2 t;
-    t.x = x0;
-    t.y = x1;
-    *reinterpret_cast<double2*>(v + r) = t;
-  } else if (r < n) {
-    v[r] = x0;
+
+template <typename T> struct Vec2;
+template <> struct Vec2<double> { using type = double2; };
+template <> struct Vec2<float> { using type = float2; };
+
+template <int MAP, typename T>
+__device__ __forceinline__ void gm_load(const T* __restrict__ v, size_t n, double (&o)[KRP]) {
+  if constexpr (MAP == 1) {
+#pragma unroll
+    for (int q = 0; q < KRP; ++q) {
+      const size_t r = gm_row<1>(q);
+      o[q] = r < n ? (double)v[r] : 0.0;
+    }
+  } else if constexpr (MAP == 2) {
+    using V2 = typename Vec2<T>::type;
+#pragma unroll
+    for (int p = 0; p < KRP / 2; ++p) {
+      const size_t r = gm_row<2>(2 * p);
+      if (r + 1 < n) {
+        const V2 x = *reinterpret_cast<const V2*>(v + r);
+        o[2 * p] = x.x;
+        o[2 * p + 1] = x.y;
+      } else {
+        o[2 * p] = r < n ? (double)v[r] : 0.0;
+        o[2 * p + 1] = 0.0;
+      }
+    }
+  } else {
+    static_assert(KRP == 4, "four rows per thread");
+    const size_t r = gm_row<4>(0);
+    if (r + 3 < n) {
+      if constexpr (sizeof(T) == 4) {
+        const float4 x = *reinterpret_cast<const float4*>(v + r);
+        o[0] = x.x;
+        o[1] = x.y;
+        o[2] = x.z;
+        o[3] = x.w;
+      } else {
+        const double2 x = *reinterpret_cast<const double2*>(v + r), y = *reinterpret_cast<const double2*>(v + r + 2);
+        o[0] = x.x;
+        o[1] = x.y;
+        o[2] = y.x;
+        o[3] = y.y;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) o[q] = r + q < n ? (double)v[r + q] : 0.0;
+    }
+  }
+}
+
+template <int MAP, typename T>
+__device__ __forceinline__ void gm_store(T* __restrict__ v, size_t n, const double (&x)[KRP]) {
+  if constexpr (MAP == 1) {
+#pragma unroll
+    for (int q = 0; q < KRP; ++q) {
+      const size_t r = gm_row<1>(q);
+      if (r < n) v[r] = (T)x[q];
+    }
+  } else if constexpr (MAP == 2) {
+    using V2 = typename Vec2<T>::type;
+#pragma unroll
+    for (int p = 0; p < KRP / 2; ++p) {
+      const size_t r = gm_row<2>(2 * p);
+      if (r + 1 < n) {
+        V2 t;
+        t.x = (T)x[2 * p];
+        t.y = (T)x[2 * p + 1];
+        *reinterpret_cast<V2*>(v + r) = t;
+      } else if (r < n) {
+        v[r] = (T)x[2 * p];
+      }
+    }
+  } else {
+    const size_t r = gm_row<4>(0);
+    if (r + 3 < n) {
+      if constexpr (sizeof(T) == 4) {
+        *reinterpret_cast<float4*>(v + r) = make_float4((float)x[0], (float)x[1], (float)x[2], (float)x[3]);
+      } else {
+        double2 a, b;
+        a.x = x[0];
+        a.y = x[1];
+        b.x = x[2];
+        b.y = x[3];
+        *reinterpret_cast<double2*>(v + r) = a;
+        *reinterpret_cast<double2*>(v + r + 2) = b;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (r + q < n) v[r + q] = (T)x[q];
+    }
   }
 }
 
@@ -151,11 +234,12 @@ __global__ void __launch_bounds__(GT) gm_start_kernel(GmDev G, double tol, int m
 }
 
 // Q col 0 = b / ||b||
-__global__ void gm_q0_kernel(GmDev G, const double* __restrict__ b, double* __restrict__ Q, size_t n) {
+template <typename TQ>
+__global__ void gm_q0_kernel(GmDev G, const double* __restrict__ b, TQ* __restrict__ Q, size_t n) {
   if (G.ictl[GM_STATUS] != 0) return;
   const double s = 1.0 / G.dctl[1];
   for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (size_t)gridDim.x * blockDim.x)
-    Q[r] = b[r] * s;
+    Q[r] = (TQ)(b[r] * s);
 }
 
 // ---- basis passes ------------------------------------------------------------
@@ -165,8 +249,8 @@ __global__ void gm_q0_kernel(GmDev G, const double* __restrict__ b, double* __re
 // columns, so that enough waves stream the basis at once.
 static constexpr int DOT_CG = 4;
 
-template <bool PAIR>
-__global__ void __launch_bounds__(KRT) gm_dots_kernel(GmDev G, const double* __restrict__ Q, size_t ldq,
+template <int MAP, typename TQ>
+__global__ void __launch_bounds__(KRT) gm_dots_kernel(GmDev G, const TQ* __restrict__ Q, size_t ldq,
                                                       const double* __restrict__ wbuf, size_t n, int mode,
                                                       double* __restrict__ partial, int nblk) {
   const int status = G.ictl[GM_STATUS];
@@ -175,48 +259,32 @@ __global__ void __launch_bounds__(KRT) gm_dots_kernel(GmDev G, const double* __r
   const bool prov = G.ictl[GM_PROV] != 0;
   const int ncols = mode == 0 ? k + 1 : k + 2;
   if ((int)blockIdx.y * 8 >= ncols) return;
-  const double* v1 = mode == 0 ? (prov ? Q + (size_t)k * ldq : nullptr) : Q + (size_t)(k + 1) * ldq;
+  const TQ* v1 = mode == 0 ? (prov ? Q + (size_t)k * ldq : nullptr) : Q + (size_t)(k + 1) * ldq;
   const double* v2 = mode == 0 ? wbuf : nullptr;
   __shared__ double red[4 * 16];
   double a[KRP], b[KRP];
-  const double2 z2 = {0.0, 0.0};
-  if constexpr (PAIR) {
-#pragma unroll
-    for (int p = 0; p < KRP / 2; ++p) {
-      const double2 x = v1 ? gm_ld2(v1, gm_row2(p), n) : z2;
-      const double2 y = v2 ? gm_ld2(v2, gm_row2(p), n) : z2;
-      a[2 * p] = x.x;
-      a[2 * p + 1] = x.y;
-      b[2 * p] = y.x;
-      b[2 * p + 1] = y.y;
-    }
+  if (v1) {
+    gm_load<MAP>(v1, n, a);
   } else {
 #pragma unroll
-    for (int q = 0; q < KRP; ++q) {
-      const size_t r = gm_row1(q);
-      a[q] = (v1 && r < n) ? v1[r] : 0.0;
-      b[q] = (v2 && r < n) ? v2[r] : 0.0;
-    }
+    for (int q = 0; q < KRP; ++q) a[q] = 0.0;
+  }
+  if (v2) {
+    gm_load<MAP>(v2, n, b);
+  } else {
+#pragma unroll
+    for (int q = 0; q < KRP; ++q) b[q] = 0.0;
   }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int c0 = blockIdx.y * 8; c0 < ncols; c0 += DOT_CG * 8) {
     double qv[8][KRP];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-      const double* q = Q + (size_t)(c0 + c) * ldq;
-      if constexpr (PAIR) {
-#pragma unroll
-        for (int p = 0; p < KRP / 2; ++p) {
-          const double2 x = c0 + c < ncols ? gm_ld2(q, gm_row2(p), n) : z2;
-          qv[c][2 * p] = x.x;
-          qv[c][2 * p + 1] = x.y;
-        }
+      if (c0 + c < ncols) {
+        gm_load<MAP>(Q + (size_t)(c0 + c) * ldq, n, qv[c]);
       } else {
 #pragma unroll
-        for (int qq = 0; qq < KRP; ++qq) {
-          const size_t r = gm_row1(qq);
-          qv[c][qq] = (c0 + c < ncols && r < n) ? q[r] : 0.0;
-        }
+        for (int q = 0; q < KRP; ++q) qv[c][q] = 0.0;
       }
     }
     double acc[16];
@@ -290,15 +358,15 @@ __device__ __forceinline__ void end_iteration(const GmDev& G) {
 // mode 0: q = (Q col k - sum_{i<m} s_i Q_i) / nu written back (if m >= 0 and
 // provisional), w <- w - sum_{i<mt} t_i Q_i - pk q, partial ||w||^2.
 // mode 1: Q col k+1 <- (Q col k+1 - sum_{i<=k} s_i Q_i) / nu (if fin).
-template <bool PAIR>
-__global__ void __launch_bounds__(KRT) gm_update_kernel(GmDev G, double* __restrict__ Q, size_t ldq,
+template <int MAP, typename TQ>
+__global__ void __launch_bounds__(KRT) gm_update_kernel(GmDev G, TQ* __restrict__ Q, size_t ldq,
                                                         double* __restrict__ wbuf, size_t n, int mode,
                                                         double* __restrict__ partial) {
   const int status = G.ictl[GM_STATUS];
   if (status != 0 || (mode == 1 && G.ictl[GM_FIN] == 0)) return;
   const int k = G.ictl[GM_K];
   const int m = G.ictl[GM_M], mt = G.ictl[GM_MT];
-  double* qt = mode == 0 ? (m > 0 ? Q + (size_t)k * ldq : nullptr) : Q + (size_t)(k + 1) * ldq;
+  TQ* qt = mode == 0 ? (m > 0 ? Q + (size_t)k * ldq : nullptr) : Q + (size_t)(k + 1) * ldq;
   double* w = mode == 0 ? wbuf : nullptr;
   __shared__ double cs[2048 + 4];
   __shared__ double red[4];
@@ -312,25 +380,15 @@ __global__ void __launch_bounds__(KRT) gm_update_kernel(GmDev G, double* __restr
   for (int q = 0; q < KRP; ++q) as[q] = at[q] = 0.0;
   const int mm = m > mt ? m : mt;
   constexpr int UC = 4;  // columns whose loads are in flight together
-  const double2 z2 = {0.0, 0.0};
   for (int c0 = 0; c0 < mm; c0 += UC) {
     double qv[UC][KRP];
 #pragma unroll
     for (int u = 0; u < UC; ++u) {
-      const double* q = Q + (size_t)(c0 + u) * ldq;
-      if constexpr (PAIR) {
-#pragma unroll
-        for (int p = 0; p < KRP / 2; ++p) {
-          const double2 x = c0 + u < mm ? gm_ld2(q, gm_row2(p), n) : z2;
-          qv[u][2 * p] = x.x;
-          qv[u][2 * p + 1] = x.y;
-        }
+      if (c0 + u < mm) {
+        gm_load<MAP>(Q + (size_t)(c0 + u) * ldq, n, qv[u]);
       } else {
 #pragma unroll
-        for (int qq = 0; qq < KRP; ++qq) {
-          const size_t r = gm_row1(qq);
-          qv[u][qq] = (c0 + u < mm && r < n) ? q[r] : 0.0;
-        }
+        for (int q = 0; q < KRP; ++q) qv[u][q] = 0.0;
       }
     }
 #pragma unroll
@@ -345,45 +403,26 @@ __global__ void __launch_bounds__(KRT) gm_update_kernel(GmDev G, double* __restr
       }
     }
   }
-  double acc = 0.0;
-  if constexpr (PAIR) {
+  // the finalised previous vector, as the basis stores it (rounded to TQ), and
+  // the new vector; rows at or past n are zero throughout
+  double qk[KRP], acc = 0.0;
 #pragma unroll
-    for (int p = 0; p < KRP / 2; ++p) {
-      const size_t r = gm_row2(p);
-      if (r < n) {
-        double qk0 = 0.0, qk1 = 0.0;
-        if (qt) {
-          const double2 o = gm_ld2(qt, r, n);
-          qk0 = (o.x - as[2 * p]) * inv_nu;
-          qk1 = (o.y - as[2 * p + 1]) * inv_nu;
-          gm_st2(qt, r, n, qk0, qk1);
-        }
-        if (w) {
-          const double2 o = gm_ld2(w, r, n);
-          const double x0 = o.x - at[2 * p] - pk * qk0, x1 = o.y - at[2 * p + 1] - pk * qk1;
-          gm_st2(w, r, n, x0, x1);
-          acc += x0 * x0;
-          if (r + 1 < n) acc += x1 * x1;
-        }
-      }
-    }
-  } else {
+  for (int q = 0; q < KRP; ++q) qk[q] = 0.0;
+  if (qt) {
+    gm_load<MAP>(qt, n, qk);
 #pragma unroll
-    for (int qq = 0; qq < KRP; ++qq) {
-      const size_t r = gm_row1(qq);
-      if (r < n) {
-        double qk = 0.0;
-        if (qt) {
-          qk = (qt[r] - as[qq]) * inv_nu;
-          qt[r] = qk;
-        }
-        if (w) {
-          const double v = w[r] - at[qq] - pk * qk;
-          w[r] = v;
-          acc += v * v;
-        }
-      }
+    for (int q = 0; q < KRP; ++q) qk[q] = (double)(TQ)((qk[q] - as[q]) * inv_nu);
+    gm_store<MAP>(qt, n, qk);
+  }
+  if (w) {
+    double x[KRP];
+    gm_load<MAP>(w, n, x);
+#pragma unroll
+    for (int q = 0; q < KRP; ++q) {
+      x[q] = x[q] - at[q] - pk * qk[q];
+      acc += x[q] * x[q];
     }
+    gm_store<MAP>(w, n, x);
   }
   if (mode == 0) {
     acc = wave_sum(acc);
@@ -532,14 +571,14 @@ __global__ void __launch_bounds__(GT) gm_ctl2_kernel(GmDev G, const double* __re
 }
 
 // Q col k+1 = w / ||w||
-__global__ void gm_scale_kernel(GmDev G, const double* __restrict__ wbuf, double* __restrict__ Q, size_t ldq,
-                                size_t n) {
+template <typename TQ>
+__global__ void gm_scale_kernel(GmDev G, const double* __restrict__ wbuf, TQ* __restrict__ Q, size_t ldq, size_t n) {
   if (G.ictl[GM_STATUS] != 0) return;
   const int k = G.ictl[GM_K];
   const double s = G.dctl[5];
-  double* q = Q + (size_t)(k + 1) * ldq;
+  TQ* q = Q + (size_t)(k + 1) * ldq;
   for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (size_t)gridDim.x * blockDim.x)
-    q[r] = wbuf[r] * s;
+    q[r] = (TQ)(wbuf[r] * s);
 }
 
 // after the finalising dots pass
@@ -613,13 +652,13 @@ __global__ void __launch_bounds__(64) gm_backsub_kernel(GmDev G) {
 }
 
 // x = sum_{i < ncols} y_i Q_i (zero for a zero right-hand side)
-__global__ void gm_combine_kernel(GmDev G, const double* __restrict__ Q, size_t ldq, double* __restrict__ x,
-                                  size_t n) {
+template <typename TQ>
+__global__ void gm_combine_kernel(GmDev G, const TQ* __restrict__ Q, size_t ldq, double* __restrict__ x, size_t n) {
   const int st = G.ictl[GM_STATUS] - 1;
   const int nc = st == 2 ? 0 : G.ictl[GM_NCOLS];
   for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (size_t)gridDim.x * blockDim.x) {
     double s = 0.0;
-    for (int i = 0; i < nc; ++i) s += G.y[i] * Q[(size_t)i * ldq + r];
+    for (int i = 0; i < nc; ++i) s += G.y[i] * (double)Q[(size_t)i * ldq + r];
     x[r] = s;
   }
 }
@@ -652,45 +691,73 @@ hipError_t gm_set_ptr(const double** table, int idx, const double* p, hipStream_
 // ---- host-side launch helpers -------------------------------------------------------
 int gm_blocks(size_t n) { return (int)((n + KRB - 1) / KRB); }
 
-hipError_t gm_launch_start(const GmDev& G, double tol, int max_iter, const double* b, double* Q, size_t n,
-                           hipStream_t st) {
+template <typename TQ>
+static hipError_t gm_launch_start_t(const GmDev& G, double tol, int max_iter, const double* b, TQ* Q, size_t n,
+                                    hipStream_t st) {
   hipLaunchKernelGGL(gm_start_kernel, dim3(1), dim3(GT), 0, st, G, tol, max_iter);
-  hipLaunchKernelGGL(gm_q0_kernel, dim3(1024), dim3(256), 0, st, G, b, Q, n);
+  hipLaunchKernelGGL(gm_q0_kernel<TQ>, dim3(1024), dim3(256), 0, st, G, b, Q, n);
   return hipGetLastError();
 }
+hipError_t gm_launch_start(const GmDev& G, double tol, int max_iter, const double* b, double* Q, size_t n,
+                           hipStream_t st) {
+  return gm_launch_start_t(G, tol, max_iter, b, Q, n, st);
+}
+hipError_t gm_launch_start(const GmDev& G, double tol, int max_iter, const double* b, float* Q, size_t n,
+                           hipStream_t st) {
+  return gm_launch_start_t(G, tol, max_iter, b, Q, n, st);
+}
 
-template <bool PAIR>
-static void gm_launch_basis_t(const GmDev& G, double* Q, size_t ldq, double* wbuf, size_t n, double* partial, int nblk,
+template <int MAP, typename TQ>
+static void gm_launch_basis_t(const GmDev& G, TQ* Q, size_t ldq, double* wbuf, size_t n, double* partial, int nblk,
                               hipStream_t st) {
   // main dots, reduce, control 1, H s, update, norm reduce + control 2, scale
-  hipLaunchKernelGGL(gm_dots_kernel<PAIR>, dim3(nblk, DOT_CG), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 0, partial, nblk);
+  hipLaunchKernelGGL((gm_dots_kernel<MAP, TQ>), dim3(nblk, DOT_CG), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 0, partial,
+                     nblk);
   hipLaunchKernelGGL(gm_reduce_kernel, dim3(256), dim3(256), 0, st, G, partial, nblk, 0);
   hipLaunchKernelGGL(gm_ctl1_kernel, dim3(1), dim3(GT), 0, st, G);
   hipLaunchKernelGGL(gm_hs_kernel, dim3(G.M + 1), dim3(64), 0, st, G);
-  hipLaunchKernelGGL(gm_update_kernel<PAIR>, dim3(nblk), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 0, partial);
+  hipLaunchKernelGGL((gm_update_kernel<MAP, TQ>), dim3(nblk), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 0, partial);
   hipLaunchKernelGGL(gm_ctl2_kernel, dim3(1), dim3(GT), 0, st, G, partial, nblk);
-  hipLaunchKernelGGL(gm_scale_kernel, dim3(1024), dim3(256), 0, st, G, wbuf, Q, ldq, n);
+  hipLaunchKernelGGL(gm_scale_kernel<TQ>, dim3(1024), dim3(256), 0, st, G, wbuf, Q, ldq, n);
   // finalisation of the new vector when convergence is predicted
-  hipLaunchKernelGGL(gm_dots_kernel<PAIR>, dim3(nblk, DOT_CG), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 1, partial, nblk);
+  hipLaunchKernelGGL((gm_dots_kernel<MAP, TQ>), dim3(nblk, DOT_CG), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 1, partial,
+                     nblk);
   hipLaunchKernelGGL(gm_reduce_kernel, dim3(256), dim3(256), 0, st, G, partial, nblk, 1);
   hipLaunchKernelGGL(gm_ctl3_kernel, dim3(1), dim3(GT), 0, st, G);
-  hipLaunchKernelGGL(gm_update_kernel<PAIR>, dim3(nblk), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 1, partial);
+  hipLaunchKernelGGL((gm_update_kernel<MAP, TQ>), dim3(nblk), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 1, partial);
   hipLaunchKernelGGL(gm_end_kernel, dim3(1), dim3(64), 0, st, G);
 }
 
+// 16-byte accesses on long vectors (two FP64 or four FP32 words), single
+// words on short ones
 hipError_t gm_launch_basis(const GmDev& G, double* Q, size_t ldq, double* wbuf, size_t n, double* partial,
                            int nblk, hipStream_t st) {
   if (n >= ((size_t)1 << 18))
-    gm_launch_basis_t<true>(G, Q, ldq, wbuf, n, partial, nblk, st);
+    gm_launch_basis_t<2>(G, Q, ldq, wbuf, n, partial, nblk, st);
   else
-    gm_launch_basis_t<false>(G, Q, ldq, wbuf, n, partial, nblk, st);
+    gm_launch_basis_t<1>(G, Q, ldq, wbuf, n, partial, nblk, st);
+  return hipGetLastError();
+}
+hipError_t gm_launch_basis(const GmDev& G, float* Q, size_t ldq, double* wbuf, size_t n, double* partial,
+                           int nblk, hipStream_t st) {
+  if (n >= ((size_t)1 << 18))
+    gm_launch_basis_t<4>(G, Q, ldq, wbuf, n, partial, nblk, st);
+  else
+    gm_launch_basis_t<1>(G, Q, ldq, wbuf, n, partial, nblk, st);
   return hipGetLastError();
 }
 
-hipError_t gm_launch_finish(const GmDev& G, const double* Q, size_t ldq, double* x, size_t n, hipStream_t st) {
+template <typename TQ>
+static hipError_t gm_launch_finish_t(const GmDev& G, const TQ* Q, size_t ldq, double* x, size_t n, hipStream_t st) {
   hipLaunchKernelGGL(gm_backsub_kernel, dim3(1), dim3(64), 0, st, G);
-  hipLaunchKernelGGL(gm_combine_kernel, dim3(2048), dim3(256), 0, st, G, Q, ldq, x, n);
+  hipLaunchKernelGGL(gm_combine_kernel<TQ>, dim3(2048), dim3(256), 0, st, G, Q, ldq, x, n);
   return hipGetLastError();
+}
+hipError_t gm_launch_finish(const GmDev& G, const double* Q, size_t ldq, double* x, size_t n, hipStream_t st) {
+  return gm_launch_finish_t(G, Q, ldq, x, n, st);
+}
+hipError_t gm_launch_finish(const GmDev& G, const float* Q, size_t ldq, double* x, size_t n, hipStream_t st) {
+  return gm_launch_finish_t(G, Q, ldq, x, n, st);
 }
 
 }  // namespace fcm

@@ -80,7 +80,8 @@ struct fcm_handle_s {
 
   // vectors
   size_t nmax = 0, ldq = 0;
-  double* Q = nullptr;        // (max_krylov + 1) columns
+  double* Q = nullptr;        // (max_krylov + 1) columns of ldq words: FP64, or FP32 in the inner solves
+  float* Qf() const { return reinterpret_cast<float*>(Q); }  // of FP32+refine (the same columns, ldq floats apart)
   double* z = nullptr;        // preconditioned vector
   double* rhs = nullptr;
   double* sol = nullptr;

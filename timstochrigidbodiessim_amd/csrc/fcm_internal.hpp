@@ -270,6 +270,8 @@ hipError_t barrier_forces(const double* Y, int nb, int nbpb, const StericCells& 
                           double kT, double a, bool channel, StericWork& w, double* F, hipStream_t st);
 // out = P^{-1} in; if wKT != null also writes wKT = -K^T out_lambda (6 per body)
 hipError_t apply_pinv(const BodyData& b, const double* in, double* out, double* wKT, hipStream_t st);
+// the same with the input in FP32 (a column of an FP32 Krylov basis)
+hipError_t apply_pinv(const BodyData& b, const float* in, double* out, double* wKT, hipStream_t st);
 hipError_t mul_K(const BodyData& b, const double* in6, double* out3, hipStream_t st);
 hipError_t mul_KT(const BodyData& b, const double* in3, double* out6, hipStream_t st);
 
@@ -323,11 +325,18 @@ struct GmDev {
 };
 
 int gm_blocks(size_t n);
+// the basis in FP64 (FP64 solves) or FP32 (the inner solves of FP32+refine);
+// every coefficient, the new vector and the least squares stay in FP64
 hipError_t gm_launch_start(const GmDev& G, double tol, int max_iter, const double* b, double* Q, size_t n,
+                           hipStream_t st);
+hipError_t gm_launch_start(const GmDev& G, double tol, int max_iter, const double* b, float* Q, size_t n,
                            hipStream_t st);
 hipError_t gm_launch_basis(const GmDev& G, double* Q, size_t ldq, double* wbuf, size_t n, double* partial,
                            int nblk, hipStream_t st);
+hipError_t gm_launch_basis(const GmDev& G, float* Q, size_t ldq, double* wbuf, size_t n, double* partial,
+                           int nblk, hipStream_t st);
 hipError_t gm_launch_finish(const GmDev& G, const double* Q, size_t ldq, double* x, size_t n, hipStream_t st);
+hipError_t gm_launch_finish(const GmDev& G, const float* Q, size_t ldq, double* x, size_t n, hipStream_t st);
 // copy the 16-double control blocks src[0..n) into dst (device memory)
 hipError_t gm_gather_ctl(const double* const* src, double* dst, int n, hipStream_t st);
 // table[idx] = p, stream-ordered

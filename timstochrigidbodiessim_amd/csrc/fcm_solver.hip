@@ -148,7 +148,8 @@ hipError_t wall_overlap(const double* Y, int nb, double half_ly, int* flag, hipS
 //   V' = N_ref s',  out_V = diag(R,R) V',  out_lambda = R (Minv_ref (K_ref V' + b')).
 // One workgroup per body; optionally also -K^T out_lambda for the mat-vec.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(128) pinv_kernel(BodyData b, const double* __restrict__ in, double* __restrict__ out,
+template <typename TI>
+__global__ void __launch_bounds__(128) pinv_kernel(BodyData b, const TI* __restrict__ in, double* __restrict__ out,
                                                    double* __restrict__ wKT) {
   extern __shared__ double sm[];
   const int n3 = 3 * b.nbpb;
@@ -165,15 +166,15 @@ __global__ void __launch_bounds__(128) pinv_kernel(BodyData b, const double* __r
   const int n = blockIdx.x;
   const int t = threadIdx.x;
   const size_t nlam = 3 * (size_t)b.NB * b.nbpb;
-  const double* inl = in + (size_t)n3 * n;
-  const double* inv = in + nlam + 6 * (size_t)n;
+  const TI* inl = in + (size_t)n3 * n;
+  const TI* inv = in + nlam + 6 * (size_t)n;
   if (t < 9) Rs[t] = b.R[9 * (size_t)n + t];
   for (int i = t; i < n3; i += blockDim.x) ref[i] = b.ref[i];
   __syncthreads();
   for (int i = t; i < n3; i += blockDim.x) {
     const int m = i / 3, c = i % 3;
-    const double* v = inl + 3 * m;
-    bp[i] = Rs[c] * v[0] + Rs[3 + c] * v[1] + Rs[6 + c] * v[2];  // (R^T v)_c
+    const TI* v = inl + 3 * m;
+    bp[i] = Rs[c] * (double)v[0] + Rs[3 + c] * (double)v[1] + Rs[6 + c] * (double)v[2];  // (R^T v)_c
   }
   __syncthreads();
   for (int i = t; i < n3; i += blockDim.x) {
@@ -186,8 +187,8 @@ __global__ void __launch_bounds__(128) pinv_kernel(BodyData b, const double* __r
   if (t < 6) {
     const int c = t % 3;
     // (R^T b_V)_c for the translational (t<3) or rotational (t>=3) part
-    const double* v = inv + (t < 3 ? 0 : 3);
-    const double rv = Rs[c] * v[0] + Rs[3 + c] * v[1] + Rs[6 + c] * v[2];
+    const TI* v = inv + (t < 3 ? 0 : 3);
+    const double rv = Rs[c] * (double)v[0] + Rs[3 + c] * (double)v[1] + Rs[6 + c] * (double)v[2];
     double k = 0.0;
     if (t < 3) {
       for (int m = 0; m < b.nbpb; ++m) k += tt[3 * m + c];
@@ -257,7 +258,8 @@ __global__ void __launch_bounds__(128) pinv_kernel(BodyData b, const double* __r
 // runs over k in the order of pinv_kernel, so the results are the same bit for
 // bit.  Used when 6 <= 3 n_b <= 72 (the threads of a body's rows also compute
 // its six body rows, and the LDS stays under 64 KiB).
-__global__ void __launch_bounds__(256) pinv_pack_kernel(BodyData b, const double* __restrict__ in,
+template <typename TI>
+__global__ void __launch_bounds__(256) pinv_pack_kernel(BodyData b, const TI* __restrict__ in,
                                                         double* __restrict__ out, double* __restrict__ wKT) {
   extern __shared__ double sm[];
   if (b.gate && *b.gate) return;
@@ -296,10 +298,10 @@ __global__ void __launch_bounds__(256) pinv_pack_kernel(BodyData b, const double
 #pragma unroll
     for (int r = 0; r < 9; ++r) R[r] = live ? b.R[9 * (size_t)n + r] : 0.0;
     if (live) {
-      const double* v = in + (size_t)n3 * n + 3 * m;
+      const TI* v = in + (size_t)n3 * n + 3 * m;
       v3[0] = v[0]; v3[1] = v[1]; v3[2] = v[2];
       if (i < 6) {
-        const double* vb = in + nlam + 6 * (size_t)n + (i < 3 ? 0 : 3);
+        const TI* vb = in + nlam + 6 * (size_t)n + (i < 3 ? 0 : 3);
         w3[0] = vb[0]; w3[1] = vb[1]; w3[2] = vb[2];
       }
       if (wKT) {
@@ -382,7 +384,8 @@ __global__ void __launch_bounds__(256) pinv_pack_kernel(BodyData b, const double
   }
 }
 
-hipError_t apply_pinv(const BodyData& b, const double* in, double* out, double* wKT, hipStream_t st) {
+template <typename TI>
+static hipError_t apply_pinv_t(const BodyData& b, const TI* in, double* out, double* wKT, hipStream_t st) {
   if (b.NB == 0) return hipSuccess;
   const int n3 = 3 * b.nbpb;
   // the packed kernel's rows double as the six body rows: it needs 3 n_b >= 6
@@ -390,12 +393,19 @@ hipError_t apply_pinv(const BodyData& b, const double* in, double* out, double* 
     const int BP = 256 / n3;
     const size_t lds = sizeof(double) * ((size_t)n3 * n3 + 36 + n3 + (size_t)BP * (4 * n3 + 12));
     const int groups = (b.NB + BP - 1) / BP;
-    hipLaunchKernelGGL(pinv_pack_kernel, dim3(groups < 2048 ? groups : 2048), dim3(256), lds, st, b, in, out, wKT);
+    hipLaunchKernelGGL(pinv_pack_kernel<TI>, dim3(groups < 2048 ? groups : 2048), dim3(256), lds, st, b, in, out,
+                       wKT);
     return hipGetLastError();
   }
   const size_t lds = sizeof(double) * (5 * (size_t)n3 + 21);
-  hipLaunchKernelGGL(pinv_kernel, dim3(b.NB), dim3(128), lds, st, b, in, out, wKT);
+  hipLaunchKernelGGL(pinv_kernel<TI>, dim3(b.NB), dim3(128), lds, st, b, in, out, wKT);
   return hipGetLastError();
+}
+hipError_t apply_pinv(const BodyData& b, const double* in, double* out, double* wKT, hipStream_t st) {
+  return apply_pinv_t(b, in, out, wKT, st);
+}
+hipError_t apply_pinv(const BodyData& b, const float* in, double* out, double* wKT, hipStream_t st) {
+  return apply_pinv_t(b, in, out, wKT, st);
 }
 
 // K [U;W] per blob: U + W x (R r_m)   (collection_of_rigid_bodies.cpp:707-731)
