@@ -228,6 +228,7 @@ __global__ void __launch_bounds__(GT) gm_start_kernel(GmDev G, double tol, int m
     G.ictl[GM_FIN] = 0;
     G.ictl[GM_MAXIT] = max_iter;
     G.ictl[GM_NCOLS] = 0;
+    G.ictl[GM_ARR0] = G.ictl[GM_ARR1] = G.ictl[GM_ARR2] = 0;
     G.ictl[GM_STATUS] = nrm == 0.0 ? 1 + 2 /* zero rhs */ : 0;
     G.beta[0] = nrm;
   }
@@ -338,14 +339,19 @@ __device__ void reduce_rows(const GmDev& G, const double* __restrict__ partial, 
   }
 }
 
-// rows of a dots pass: mode 0 the main pass (2 (k+1) rows), mode 1 the
-// finalising pass (k+2 rows, only when it ran)
-__global__ void __launch_bounds__(256) gm_reduce_kernel(GmDev G, const double* __restrict__ partial, int nblk,
-                                                        int mode) {
-  if (G.ictl[GM_STATUS] != 0 || (mode == 1 && G.ictl[GM_FIN] == 0)) return;
-  __shared__ double s[256];
-  const int k = G.ictl[GM_K];
-  reduce_rows(G, partial, nblk, mode == 0 ? 2 * (k + 1) : k + 2, s);
+// True in every thread of the block that arrives last at counter `word` of a
+// multi-block kernel (every block calls it once, after its global writes); that
+// block then sees what the others wrote and resets the counter.
+__device__ bool last_block(const GmDev& G, int word) {
+  __shared__ int s_last;
+  __threadfence();  // this block's writes before its arrival
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = atomicAdd(&G.ictl[word], 1) == (int)gridDim.x - 1;
+  __syncthreads();
+  if (!s_last) return false;
+  __threadfence();  // the other blocks' writes after their arrival
+  if (threadIdx.x == 0) G.ictl[word] = 0;
+  return true;
 }
 
 // end of an iteration: the next one works on column k + 1
@@ -363,7 +369,13 @@ __global__ void __launch_bounds__(KRT) gm_update_kernel(GmDev G, TQ* __restrict_
                                                         double* __restrict__ wbuf, size_t n, int mode,
                                                         double* __restrict__ partial) {
   const int status = G.ictl[GM_STATUS];
-  if (status != 0 || (mode == 1 && G.ictl[GM_FIN] == 0)) return;
+  if (status != 0) return;
+  if (mode == 1 && G.ictl[GM_FIN] == 0) {
+    // no finalisation: the iteration ends here (no block of this launch reads
+    // the words end_iteration changes beyond FIN, which stays 0)
+    if (blockIdx.x == 0 && threadIdx.x == 0) end_iteration(G);
+    return;
+  }
   const int k = G.ictl[GM_K];
   const int m = G.ictl[GM_M], mt = G.ictl[GM_MT];
   TQ* qt = mode == 0 ? (m > 0 ? Q + (size_t)k * ldq : nullptr) : Q + (size_t)(k + 1) * ldq;
@@ -430,17 +442,29 @@ __global__ void __launch_bounds__(KRT) gm_update_kernel(GmDev G, TQ* __restrict_
     if (lane == 0) red[wv] = acc;
     __syncthreads();
     if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+  } else if (last_block(G, GM_ARR2)) {
+    // the finalised vector is written: the next iteration works on column k + 1
+    if (threadIdx.x == 0) end_iteration(G);
   }
 }
 
-// ---- control kernels (one workgroup) --------------------------------------------
+// ---- control steps (one workgroup of GT threads) ----------------------------------
+// Step 1 is a kernel of its own; step 3 runs in the last block of the
+// reduction before it (a device-scope fence in every block of a launch costs
+// more than the launch it saves, so only the step that runs once per solve
+// is folded in).
+struct CtlShared {
+  double sh[8];
+  double col[GCAP], sCS[GCAP], sSN[GCAP];
+  int done;  // the solve has stopped
+};
+
 // after the main dots pass: finalise column k-1 (when the previous vector was
 // provisional), commit it to the least squares, and set up the update
-__global__ void __launch_bounds__(GT) gm_ctl1_kernel(GmDev G) {
-  if (G.ictl[GM_STATUS] != 0) return;
-  __shared__ double sh[8];
-  __shared__ double col[GCAP], sCS[GCAP], sSN[GCAP];
-  __shared__ int s_done;
+__device__ void ctl1_step(const GmDev& G, CtlShared& L) {
+  double* sh = L.sh;
+  double *col = L.col, *sCS = L.sCS, *sSN = L.sSN;
+  int& s_done = L.done;
   const int k = G.ictl[GM_K];
   const bool prov = G.ictl[GM_PROV] != 0;
   const int ncols = k + 1;
@@ -531,6 +555,21 @@ __global__ void __launch_bounds__(64) gm_hs_kernel(GmDev G) {
   }
 }
 
+// rows of the main dots pass: 2 (k+1)
+__global__ void __launch_bounds__(256) gm_reduce_kernel(GmDev G, const double* __restrict__ partial, int nblk) {
+  if (G.ictl[GM_STATUS] != 0) return;
+  __shared__ double s[256];
+  const int k = G.ictl[GM_K];
+  reduce_rows(G, partial, nblk, 2 * (k + 1), s);
+}
+
+// control step 1 (one workgroup)
+__global__ void __launch_bounds__(GT) gm_ctl1_kernel(GmDev G) {
+  if (G.ictl[GM_STATUS] != 0) return;
+  __shared__ CtlShared L;
+  ctl1_step(G, L);
+}
+
 // after the main update: normalisation, breakdown, predicted convergence
 // (one block; ||w||^2 from the update pass's block partials, fixed order)
 __global__ void __launch_bounds__(GT) gm_ctl2_kernel(GmDev G, const double* __restrict__ partial, int nblk) {
@@ -582,11 +621,10 @@ __global__ void gm_scale_kernel(GmDev G, const double* __restrict__ wbuf, TQ* __
 }
 
 // after the finalising dots pass
-__global__ void __launch_bounds__(GT) gm_ctl3_kernel(GmDev G) {
-  if (G.ictl[GM_STATUS] != 0 || G.ictl[GM_FIN] == 0) return;
-  __shared__ double sh[8];
-  __shared__ double col[GCAP], sCS[GCAP], sSN[GCAP];
-  __shared__ int s_done;
+__device__ void ctl3_step(const GmDev& G, CtlShared& L) {
+  double* sh = L.sh;
+  double *col = L.col, *sCS = L.sCS, *sSN = L.sSN;
+  int& s_done = L.done;
   const int k = G.ictl[GM_K];
   const double* s = G.red;  // k+2 values: s_0..s_k, alpha
   double ss = 0.0;
@@ -629,9 +667,15 @@ __global__ void __launch_bounds__(GT) gm_ctl3_kernel(GmDev G) {
   }
 }
 
-// end of an iteration (one thread)
-__global__ void gm_end_kernel(GmDev G) {
-  if (G.ictl[GM_STATUS] == 0 && threadIdx.x == 0) end_iteration(G);
+// the reduction of the finalising dots pass, then (last block) control step 3
+__global__ void __launch_bounds__(GT) gm_reduce_ctl3_kernel(GmDev G, const double* __restrict__ partial, int nblk) {
+  if (G.ictl[GM_STATUS] != 0 || G.ictl[GM_FIN] == 0) return;
+  __shared__ double s[GT];
+  __shared__ CtlShared L;
+  const int k = G.ictl[GM_K];
+  reduce_rows(G, partial, nblk, k + 2, s);
+  if (!last_block(G, GM_ARR1)) return;
+  ctl3_step(G, L);
 }
 
 // back substitution (gmres_solver.cpp:260-276) into G.y; one wavefront
@@ -713,7 +757,7 @@ static void gm_launch_basis_t(const GmDev& G, TQ* Q, size_t ldq, double* wbuf, s
   // main dots, reduce, control 1, H s, update, norm reduce + control 2, scale
   hipLaunchKernelGGL((gm_dots_kernel<MAP, TQ>), dim3(nblk, DOT_CG), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 0, partial,
                      nblk);
-  hipLaunchKernelGGL(gm_reduce_kernel, dim3(256), dim3(256), 0, st, G, partial, nblk, 0);
+  hipLaunchKernelGGL(gm_reduce_kernel, dim3(256), dim3(256), 0, st, G, partial, nblk);
   hipLaunchKernelGGL(gm_ctl1_kernel, dim3(1), dim3(GT), 0, st, G);
   hipLaunchKernelGGL(gm_hs_kernel, dim3(G.M + 1), dim3(64), 0, st, G);
   hipLaunchKernelGGL((gm_update_kernel<MAP, TQ>), dim3(nblk), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 0, partial);
@@ -722,10 +766,8 @@ static void gm_launch_basis_t(const GmDev& G, TQ* Q, size_t ldq, double* wbuf, s
   // finalisation of the new vector when convergence is predicted
   hipLaunchKernelGGL((gm_dots_kernel<MAP, TQ>), dim3(nblk, DOT_CG), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 1, partial,
                      nblk);
-  hipLaunchKernelGGL(gm_reduce_kernel, dim3(256), dim3(256), 0, st, G, partial, nblk, 1);
-  hipLaunchKernelGGL(gm_ctl3_kernel, dim3(1), dim3(GT), 0, st, G);
+  hipLaunchKernelGGL(gm_reduce_ctl3_kernel, dim3(256), dim3(GT), 0, st, G, partial, nblk);
   hipLaunchKernelGGL((gm_update_kernel<MAP, TQ>), dim3(nblk), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 1, partial);
-  hipLaunchKernelGGL(gm_end_kernel, dim3(1), dim3(64), 0, st, G);
 }
 
 // 16-byte accesses on long vectors (two FP64 or four FP32 words), single

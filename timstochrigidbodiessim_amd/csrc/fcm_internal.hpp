@@ -309,7 +309,12 @@ hipError_t rhs_assemble(const double* Vrand, const double* Vshear, const double*
 // ---------------------------------------------------------------------------
 // device-resident GMRES control (fcm_gmres.hip)
 // ---------------------------------------------------------------------------
-enum { GM_K = 0, GM_PROV = 1, GM_STATUS = 2, GM_NCOLS = 3, GM_FIN = 4, GM_MAXIT = 5, GM_M = 6, GM_MT = 7, GM_HS = 8, GM_NINT = 9 };
+// GM_ARR0/1/2: arrival counters of the multi-block kernels whose last block
+// runs a control step (zero between launches; each last block resets its own)
+enum {
+  GM_K = 0, GM_PROV = 1, GM_STATUS = 2, GM_NCOLS = 3, GM_FIN = 4, GM_MAXIT = 5, GM_M = 6, GM_MT = 7, GM_HS = 8,
+  GM_ARR0 = 9, GM_ARR1 = 10, GM_ARR2 = 11, GM_NINT = 12
+};
 
 struct GmDev {
   int* ictl = nullptr;     // GM_* words; status 0 = running, 1 + FCM_GMRES_* when done
