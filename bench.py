@@ -94,11 +94,12 @@ def measured_traffic(kernel, grid, nb):
 
 def cpu_baseline(args, ref, x, q, F6, fp64_iters, Minv, N):
     """Times the CPU oracle (OpenMP over x slabs) on a bounded sample of the
-    same workload and sums a solve from the measured parts: one fluctuating
-    stress field, one saddle-point mat-vec and one P^-1 at full size per
-    iteration, and the modified Gram-Schmidt step of each iteration
-    (gmres_solver.cpp:198-210), measured against 1, it/2 and it basis vectors
-    and summed over the GPU FP64 iteration count `it` (linear in j)."""
+    same workload (about 10 s of CPU work) and sums a solve from the measured
+    parts: one fluctuating stress field (median of 3 seeds), one saddle-point
+    mat-vec and one P^-1 at full size per iteration (medians of 9), and the
+    modified Gram-Schmidt step of each iteration (gmres_solver.cpp:198-210),
+    measured against 1, it/2 and it basis vectors and summed over the GPU FP64
+    iteration count `it` (linear in j)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
@@ -108,18 +109,23 @@ def cpu_baseline(args, ref, x, q, F6, fp64_iters, Minv, N):
     Y = po.fill_blob_pos(oc, ref, x, q)
     o.set_up_gaussians(Y)
     b = po.Bodies(ref, x, q, Minv, N)
-    t0 = time.perf_counter()
-    o.fill_stress(1)
-    o.random_field()
-    o.random_interp()
-    t_rand = time.perf_counter() - t0
+    def timed(fn, reps):
+        ts = []
+        for r in range(reps):
+            t0 = time.perf_counter()
+            fn(r)
+            ts.append(time.perf_counter() - t0)
+        return statistics.median(ts)
+
+    def field(r):
+        o.fill_stress(1 + r)
+        o.random_field()
+        o.random_interp()
+
+    t_rand = timed(field, 3)
     v = np.random.default_rng(0).standard_normal(b.n)
-    t0 = time.perf_counter()
-    po.saddle_matvec(o, b, v)
-    t_mv = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    po.apply_Pinv(b, v)
-    t_pinv = time.perf_counter() - t0
+    t_mv = timed(lambda r: po.saddle_matvec(o, b, v), 9)
+    t_pinv = timed(lambda r: po.apply_Pinv(b, v), 9)
     o.close()
     it = max(2, fp64_iters)
     js = (1, it // 2, it)
@@ -133,8 +139,8 @@ def cpu_baseline(args, ref, x, q, F6, fp64_iters, Minv, N):
         "cores": threads,
         "kind": "port",
         "sample": (f"oracle (C restatement, FP64, {threads} OpenMP threads) at {n}^3 with {Y.size // 3} blobs, "
-                   f"a solve summed from measured parts: 1 fluctuating field {t_rand:.2f} s + {it} iterations x "
-                   f"(saddle mat-vec {t_mv:.2f} s + P^-1 {t_pinv * 1e3:.1f} ms) + modified Gram-Schmidt "
+                   f"a solve summed from measured parts (medians): 1 fluctuating field {t_rand:.2f} s + {it} "
+                   f"iterations x (saddle mat-vec {t_mv:.2f} s + P^-1 {t_pinv * 1e3:.1f} ms) + modified Gram-Schmidt "
                    f"(gmres_solver.cpp:198-210) measured at j = {js[0]}, {js[1]}, {js[2]} basis vectors "
                    f"({', '.join(f'{t * 1e3:.1f}' for t in t_orth)} ms) and summed over j = 1..{it}: "
                    f"{mgs:.2f} s; total {t_solve:.1f} s"),

@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(GT) gm_start_kernel(GmDev G, double tol, int m
     G.ictl[GM_FIN] = 0;
     G.ictl[GM_MAXIT] = max_iter;
     G.ictl[GM_NCOLS] = 0;
-    G.ictl[GM_ARR0] = G.ictl[GM_ARR1] = G.ictl[GM_ARR2] = 0;
+    G.ictl[GM_ARR_CTL3] = G.ictl[GM_ARR_FIN] = 0;
     G.ictl[GM_STATUS] = nrm == 0.0 ? 1 + 2 /* zero rhs */ : 0;
     G.beta[0] = nrm;
   }
@@ -442,7 +442,7 @@ __global__ void __launch_bounds__(KRT) gm_update_kernel(GmDev G, TQ* __restrict_
     if (lane == 0) red[wv] = acc;
     __syncthreads();
     if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
-  } else if (last_block(G, GM_ARR2)) {
+  } else if (last_block(G, GM_ARR_FIN)) {
     // the finalised vector is written: the next iteration works on column k + 1
     if (threadIdx.x == 0) end_iteration(G);
   }
@@ -674,7 +674,7 @@ __global__ void __launch_bounds__(GT) gm_reduce_ctl3_kernel(GmDev G, const doubl
   __shared__ CtlShared L;
   const int k = G.ictl[GM_K];
   reduce_rows(G, partial, nblk, k + 2, s);
-  if (!last_block(G, GM_ARR1)) return;
+  if (!last_block(G, GM_ARR_CTL3)) return;
   ctl3_step(G, L);
 }
 

@@ -309,11 +309,12 @@ hipError_t rhs_assemble(const double* Vrand, const double* Vshear, const double*
 // ---------------------------------------------------------------------------
 // device-resident GMRES control (fcm_gmres.hip)
 // ---------------------------------------------------------------------------
-// GM_ARR0/1/2: arrival counters of the multi-block kernels whose last block
-// runs a control step (zero between launches; each last block resets its own)
+// GM_ARR_CTL3, GM_ARR_FIN: arrival counters of the two multi-block kernels of
+// the finalising step whose last block runs control work (zero between
+// launches; each last block resets its own)
 enum {
   GM_K = 0, GM_PROV = 1, GM_STATUS = 2, GM_NCOLS = 3, GM_FIN = 4, GM_MAXIT = 5, GM_M = 6, GM_MT = 7, GM_HS = 8,
-  GM_ARR0 = 9, GM_ARR1 = 10, GM_ARR2 = 11, GM_NINT = 12
+  GM_ARR_CTL3 = 9, GM_ARR_FIN = 10, GM_NINT = 11
 };
 
 struct GmDev {
