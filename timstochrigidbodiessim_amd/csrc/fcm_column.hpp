@@ -19,13 +19,13 @@
 //
 //   z C2R + interpolation (col_interp_kernel): the k-space rows are brought
 //   back to real space in LDS (imaginary parts of kz = 0 and nz/2 ignored, as
-//   FFTW's halfcomplex format stores none), each thread keeps its z column of
-//   the patch in registers and, for every blob covering its z, forms
-//   gz sum_xy gx gy u.  The ng values of a blob are summed in a fixed order and
-//   the patch's partial sum goes to the blob's slot for this patch;
-//   col_reduce_kernel adds the (at most 5 x 3) slots of a blob in a fixed
-//   order and writes dx^3 J u - K V (fcm_fluid_solver.cpp:363-424 preceded by
-//   the z stage of fft_solver.cpp:38-42).
+//   FFTW's halfcomplex format stores none) and left there z-major; here one
+//   thread owns one or two entries of the list instead of a z, and forms
+//   gz sum_xy gx gy u over the entry's stencil in a fixed order.  The patch's
+//   partial sum goes to the blob's slot for this patch; col_reduce_kernel
+//   adds the (at most 5 x 3) slots of a blob in a fixed order and writes
+//   dx^3 J u - K V (fcm_fluid_solver.cpp:363-424 preceded by the z stage of
+//   fft_solver.cpp:38-42).
 //
 // Nothing is accumulated with atomics, so both are bitwise repeatable.  Per
 // product the grid is written once and read once in k space, instead of real
@@ -166,6 +166,46 @@ constexpr bool c2r_lane_pairs(const FftPlan& pl) {
   return true;
 }
 
+// Lane pairing of the R2C untangle.  The forward z transform leaves bin
+// k = r2c_out_index(tcol, s) in slot s of thread tcol; the untangle of k needs
+// Z[M - k], which is thread (TC - tcol)'s slot P - 1 - s, or for tcol = 0 the
+// thread's own slot r2c_self_slot(s).  Checked at compile time over every
+// thread; where it does not hold, the bins go through LDS.
+constexpr int r2c_out_index(const FftPlan& pl, int tcol, int s) {
+  const int rl = pl.r[pl.ns - 1];
+  return tcol + (s / rl) * pl.T + (s % rl) * (pl.N / rl);
+}
+constexpr int r2c_self_slot(const FftPlan& pl, int s) {
+  const int want = (pl.N - r2c_out_index(pl, 0, s)) % pl.N;
+  for (int q = 0; q < pl.P; ++q)
+    if (r2c_out_index(pl, 0, q) == want) return q;
+  return -1;
+}
+constexpr bool r2c_lane_pairs(const FftPlan& pl) {
+  if (!pl.ok || pl.T > 64 || 64 % pl.T != 0) return false;
+  for (int tcol = 1; tcol < pl.T; ++tcol)
+    for (int s = 0; s < pl.P; ++s)
+      if (r2c_out_index(pl, pl.T - tcol, pl.P - 1 - s) != pl.N - r2c_out_index(pl, tcol, s)) return false;
+  for (int s = 0; s < pl.P; ++s)
+    if (r2c_self_slot(pl, s) < 0) return false;
+  return true;
+}
+
+// waves per SIMD the interpolation kernel is compiled for: at nz = 512 a
+// workgroup is 8 waves, and the 140 registers FP32 takes unconstrained leave
+// one workgroup per CU
+constexpr int col_interp_waves(int nz, int tbytes) { return (nz >= 512 && tbytes == 4) ? 4 : 1; }
+
+// z-major row stride (words) for rr rows of tbytes-byte words: a 16-byte read
+// covers 4 of the 64 LDS banks, and 16 lanes reading rows at 16 consecutive z
+// hit 16 distinct groups when the stride in 16-byte units is odd
+constexpr int col_ldu(int rr, int tbytes) {
+  const int q = tbytes == 4 ? 8 : 4, r = tbytes == 4 ? 4 : 2;
+  int v = rr;
+  while (v % q != r) ++v;
+  return v;
+}
+
 // patch x extent per nz by default: FP64 rows of a 4 x 4 patch exceed the LDS at nz = 512
 constexpr int col_cx_for(int nz) { return nz <= 256 ? 4 : 2; }
 
@@ -206,35 +246,32 @@ struct ColumnShape {
   static constexpr int S_ENT = 19 * (int)sizeof(T) + 4;   // gx 4, gy 4, F 3, gz 8 | stencil-start z
   static constexpr int S_CAP = col_odd_le(BUDGET / S_ENT);
   static constexpr int S_LDS = col_max(FFT_BYTES, S_CAP * S_ENT);
-  static constexpr int I_ENT = 40 * (int)sizeof(T) + 8;   // gx 4, gy 4, gz 8, slots 3 x 8 | z, partial slot
-  // FP64 interpolation at 256 threads fits 168 VGPRs (three waves per SIMD), so
-  // its staging is trimmed to let three workgroups share a CU's LDS
-  static constexpr int I_BUDGET = (sizeof(T) == 8 && NT == 256) ? col_max(col_min(BUDGET, 49664), FFT_BYTES) : BUDGET;
-  static constexpr int I_CAP = col_odd_le(I_BUDGET / I_ENT);
-  static constexpr int I_LDS = col_max(FFT_BYTES, I_CAP * I_ENT);
-  // a shape the kernels can be built for: a z plan that tiles the threads, FFT
-  // columns that do not overlap, room to stage 15 entries, and the LDS of a CU
-  static constexpr bool ok = PLAN_OK && LS >= NCB * RR && S_CAP >= 15 && I_CAP >= 15 &&
-                             S_LDS + (M + M / 2 + 1) * 2 * (int)sizeof(T) <= 163840 &&
-                             I_LDS + M * 2 * (int)sizeof(T) <= 163840;
+  // interpolation: the rows of a round kept z-major, E_LDU words per z, a row
+  // stride whose 16-byte reads by 16 lanes at neighbouring z fall on distinct
+  // banks; E_KR list entries per thread per pass over the transform
+  static constexpr int E_LDU = col_ldu(RR, (int)sizeof(T));
+  static constexpr int E_KR = 2;
+  static constexpr int E_LDS = col_max(FFT_BYTES, NZ * E_LDU * (int)sizeof(T));
+  // a shape the kernels can be built for: a z plan that tiles the threads in
+  // whole y columns of rows, FFT columns that do not overlap, room to stage 15
+  // entries, and the LDS of a CU
+  static constexpr bool ok = PLAN_OK && RR % COL_CY == 0 && LS >= NCB * RR && S_CAP >= 15 &&
+                             S_LDS + 2 * M * 2 * (int)sizeof(T) <= 163840 &&
+                             E_LDS + 2 * M * 2 * (int)sizeof(T) <= 163840;
 };
 
 // the range [a1, b1) u [a2, b2) of the patch list holds exactly the blobs
-// whose stencil covers z (stencil start in [z - ng + 1, z] modulo nz)
+// whose stencil covers z (stencil start in [z - ng + 1, z] modulo nz).  Written
+// with selects: as branches the compiler kept the four words in scratch.
 template <int NZ>
 __device__ __forceinline__ void col_ranges(const int* __restrict__ zs, int z, int ng, int& a1, int& b1, int& a2, int& b2) {
-  a1 = b1 = a2 = b2 = 0;
-  if (z >= NZ) return;
   const int lo = z - ng + 1;
-  if (lo >= 0) {
-    a1 = zs[lo];
-    b1 = zs[z + 1];
-  } else {
-    a1 = zs[NZ + lo];
-    b1 = zs[NZ];
-    a2 = zs[0];
-    b2 = zs[z + 1];
-  }
+  const bool in = z < NZ, wrap = lo < 0;
+  const int za = wrap ? NZ + lo : lo, zb = wrap ? NZ : z + 1;
+  a1 = in ? zs[za] : 0;
+  b1 = in ? zs[zb] : 0;
+  a2 = in && wrap ? zs[0] : 0;
+  b2 = in && wrap ? zs[z + 1] : 0;
 }
 
 // x and y weights of a blob over the patch (zero outside the stencil)
@@ -259,7 +296,10 @@ __device__ __forceinline__ void col_xy_weights(const T* ws, int ng, int rx, int 
 // f S_CAP + i), so that the lanes of a wavefront, which read neighbouring
 // entries of the list, read neighbouring banks.  Then, per round of RR rows:
 // the rows go row-major through LDS into registers, the FFT runs with a
-// padded scratch stride, and the untangled k-space rows are stored.
+// padded scratch stride, and the untangled k-space rows are stored.  In FP32
+// the untangle takes each bin's partner Z[M - k] from the lane that holds it
+// (no LDS round trip; spreading 0.125 -> 0.117 ms at 256^3); in FP64 the extra
+// registers would spill and the bins go through LDS.
 // ---------------------------------------------------------------------------
 template <typename T, int NZ, int CX, int PM>
 __global__ void __launch_bounds__(col_threads(NZ))
@@ -271,7 +311,7 @@ __global__ void __launch_bounds__(col_threads(NZ))
   constexpr int NT = SH::NT, M = SH::M, P = SH::P, TC = SH::TC, RR = SH::RR, NROW = SH::NROW, NRND = SH::NRND;
   constexpr int CY = COL_CY, CAP = SH::S_CAP, LS = SH::LS, NCB = SH::NCB;
   __shared__ __align__(16) unsigned char smem[SH::S_LDS];
-  __shared__ V s_tw[M], s_twz[M / 2 + 1];  // untangle twiddles W^k, k <= M/2
+  __shared__ V s_tw[M], s_twz[M];  // untangle twiddles W^k, k < M
   if (fp.gate && *fp.gate) return;
   const GridGeom& g = A.g;
   const int npatch = A.npx * A.npy;
@@ -286,7 +326,7 @@ __global__ void __launch_bounds__(col_threads(NZ))
   const int ng = A.ng, omin = 1 - ng / 2;
   for (int i = t; i < M; i += NT) {
     s_tw[i] = tw[i];
-    if (i <= M / 2) s_twz[i] = twz[i];
+    s_twz[i] = twz[i];
   }
   __syncthreads();
   // this patch's z prefix table, read from HBM (a few words per thread)
@@ -370,6 +410,9 @@ __global__ void __launch_bounds__(col_threads(NZ))
   V* area = reinterpret_cast<V*>(smem);
   T* rowsT = reinterpret_cast<T*>(smem);
   const int rloc = t / TC, tcol = t - (t / TC) * TC;
+  constexpr FftPlan FPL = make_plan(M, PM);
+  constexpr bool OPAIRS = sizeof(T) == 4 && TC > 1 && r2c_lane_pairs(FPL);
+  const int partner = (t & 63) - tcol + ((TC - tcol) & (TC - 1));  // lane of thread TC - tcol of this row
 #pragma unroll
   for (int qg = 0; qg < NRND; qg += NCB) {
     __syncthreads();  // staging (first group) or the previous group's scratch is free
@@ -386,25 +429,55 @@ __global__ void __launch_bounds__(col_threads(NZ))
       for (int s = 0; s < P; ++s) v[c][s] = area[SH::row0(rloc + c * RR) + fft_in_index<M, false, PM>(tcol, s)];
     __syncthreads();
     fft_columns<T, M, -1, false, PM, NCB, SH::SSH, SH::SB>(v, area, LS, rloc, RR, tcol, s_tw);
+    if constexpr (OPAIRS) {
+      // untangle straight from registers: bin k's partner Z[M - k] is held by
+      // the lane of thread TC - tcol (slot P - 1 - s), or by thread 0 itself
 #pragma unroll
-    for (int c = 0; c < NCB; ++c)
-#pragma unroll
-      for (int s = 0; s < P; ++s) area[SH::slot(fft_out_index<M, false, PM>(tcol, s), rloc + c * RR)] = v[c][s];
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < NCB; ++c) {
-      const int row = (qg + c) * RR + rloc;
-      if (row < NROW) {
+      for (int c = 0; c < NCB; ++c) {
+        const int row = (qg + c) * RR + rloc;
+        const bool valid = row < NROW;
         const int cc = row / (CX * CY), rem = row - (row / (CX * CY)) * (CX * CY);
         const int ii = rem / CY, jj = rem - (rem / CY) * CY;
-        V* out = fp.p[cc] + ((size_t)(x0 + ii) * g.ny + (y0 + jj)) * g.cpitch;
-        // untangle, one bin pair (k, M - k) per step
-        const int zc = rloc + c * RR;
-        for (int k = tcol; k <= M / 2; k += TC) {
-          V xk, xmk;
-          r2c_untangle_pair<T>(area[SH::slot(k, zc)], area[SH::slot(k == 0 ? 0 : M - k, zc)], s_twz[k], xk, xmk);
-          out[k] = xk;
-          if (M - k != k) out[M - k] = xmk;
+        V* out = fp.p[valid ? cc : 0] + ((size_t)(x0 + (valid ? ii : 0)) * g.ny + (y0 + (valid ? jj : 0))) * g.cpitch;
+#pragma unroll
+        for (int s = 0; s < P; ++s) {
+          const int k = fft_out_index<M, false, PM>(tcol, s);
+          const V sh = mk<T>(__shfl(v[c][P - 1 - s].x, partner), __shfl(v[c][P - 1 - s].y, partner));
+          const int self = r2c_self_slot(FPL, s);  // a constant once unrolled
+          const V pz = tcol == 0 ? v[c][self >= 0 ? self : 0] : sh;
+          // X[k] = e + W^k o with e = (Z[k] + conj Z[M-k]) / 2, o = (Z[k] - conj Z[M-k]) / 2i
+          const V zm = cconj(pz);
+          const V e = cscale(cadd(v[c][s], zm), T(0.5));
+          const V d = csub(v[c][s], zm);
+          const V o = mk<T>(T(0.5) * d.y, T(-0.5) * d.x);
+          const V tt = cmul(s_twz[k], o);
+          if (valid) {
+            out[k] = cadd(e, tt);
+            if (k == 0) out[M] = cconj(csub(e, tt));
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < NCB; ++c)
+#pragma unroll
+        for (int s = 0; s < P; ++s) area[SH::slot(fft_out_index<M, false, PM>(tcol, s), rloc + c * RR)] = v[c][s];
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < NCB; ++c) {
+        const int row = (qg + c) * RR + rloc;
+        if (row < NROW) {
+          const int cc = row / (CX * CY), rem = row - (row / (CX * CY)) * (CX * CY);
+          const int ii = rem / CY, jj = rem - (rem / CY) * CY;
+          V* out = fp.p[cc] + ((size_t)(x0 + ii) * g.ny + (y0 + jj)) * g.cpitch;
+          // untangle, one bin pair (k, M - k) per step
+          const int zc = rloc + c * RR;
+          for (int k = tcol; k <= M / 2; k += TC) {
+            V xk, xmk;
+            r2c_untangle_pair<T>(area[SH::slot(k, zc)], area[SH::slot(k == 0 ? 0 : M - k, zc)], s_twz[k], xk, xmk);
+            out[k] = xk;
+            if (M - k != k) out[M - k] = xmk;
+          }
         }
       }
     }
@@ -412,25 +485,36 @@ __global__ void __launch_bounds__(col_threads(NZ))
 }
 
 // ---------------------------------------------------------------------------
-// inverse z C2R + interpolation partial sums
+// inverse z C2R + interpolation partial sums, one thread per list entry
 //
-// Per group of rounds: each thread loads the k-space words its FFT points
-// need (X[k] and X[M-k]) from HBM, the rows are transformed through the padded
-// LDS scratch and read back row-major into this thread's z column
-// (registers).  Then the blob lists are staged in planes as for spreading,
-// and each (blob, stencil z) value lands in its own LDS slot, so the ng
-// values of a blob are added in a fixed order.
+// The entries of the patch list are taken E_KR per thread, their x, y and z
+// weights held in registers (loaded at the start, so the first transform
+// hides their latency).  Per round, the RR rows' k-space words are read
+// straight from HBM (each C2R point k needs X[k] and X[M - k]; the second
+// word comes from the partner lane where the plan allows), transformed
+// through the padded LDS scratch and left in LDS z-major (row r of node z at
+// z E_LDU + r).  Each thread then reads, for every stencil z of each of its
+// entries, the round's rows at that z (16-byte reads) and adds
+// gz sum_x gx sum_y gy u to the entry's three sums, while the next round's
+// k-space words are in flight.  Every lane does the same work whatever the z
+// density of the patch (one thread per z ran each wavefront as long as its
+// busiest lane: 24 entry steps against a mean of 10 at 256^3), each entry's
+// sum runs over its stencil in a fixed order, and no per-(entry, z) slots are
+// staged (they filled the LDS and split a patch's list into two passes).  A
+// patch with more than E_KR NT entries repeats the transform per pass.
+// FP32 at 256^3: 0.138 -> 0.113 ms; FP64 0.282 -> 0.240 ms.
 // ---------------------------------------------------------------------------
 template <typename T, int NZ, int CX, int PM>
-__global__ void __launch_bounds__(col_threads(NZ))
+__global__ void __launch_bounds__(col_threads(NZ), col_interp_waves(NZ, (int)sizeof(T)))
     col_interp_kernel(ColArgs A, FieldPtrs<T> fp, const c2<T>* __restrict__ tw, const c2<T>* __restrict__ twz,
                       T* __restrict__ part) {
   using SH = ColumnShape<T, NZ, CX, PM>;
   static_assert(SH::ok, "column shape");
   using V = c2<T>;
   constexpr int NT = SH::NT, M = SH::M, P = SH::P, TC = SH::TC, RR = SH::RR, NROW = SH::NROW, NRND = SH::NRND;
-  constexpr int CY = COL_CY, CAP = SH::I_CAP, LS = SH::LS, NCB = SH::NCB;
-  __shared__ __align__(16) unsigned char smem[SH::I_LDS];
+  constexpr int CY = COL_CY, LS = SH::LS, LDU = SH::E_LDU, KR = SH::E_KR;
+  static_assert(RR % CY == 0, "rows of a round come in whole y columns");
+  __shared__ __align__(16) unsigned char smem[SH::E_LDS];
   __shared__ V s_tw[M], s_twz[M];  // FFT twiddles, C2R twiddles W^k
   if (fp.gate && *fp.gate) return;
   const GridGeom& g = A.g;
@@ -440,156 +524,132 @@ __global__ void __launch_bounds__(col_threads(NZ))
   const int px = p / A.npy, py = p - (p / A.npy) * A.npy;
   const int x0 = CX * px, y0 = CY * py;
   const int t = threadIdx.x;
-  const int ng = A.ng, omin = 1 - ng / 2;
+  const int ng = A.ng;
   for (int i = t; i < M; i += NT) {
     s_tw[i] = tw[i];
     s_twz[i] = twz[i];
   }
-  __syncthreads();
-
+  __syncthreads();  // the C2R twiddles are read before the first transform's barriers
   V* area = reinterpret_cast<V*>(smem);
-  const T* rowsT = reinterpret_cast<const T*>(smem);
+  T* uc = reinterpret_cast<T*>(smem);
   const int rloc = t / TC, tcol = t - (t / TC) * TC;
-  T u[NROW];  // row (c CX + ii) CY + jj at this thread's z
+  constexpr FftPlan IPL = reverse_plan(make_plan(M, PM));
+  constexpr bool PAIRS = c2r_lane_pairs(IPL);
+  const int partner = (t & 63) - tcol + ((TC - tcol) & (TC - 1));  // lane of thread TC - tcol of this row
+  const int* zs = A.zst + (size_t)p * (NZ + 1);
+  const int L0 = zs[0], L1 = zs[NZ];
+  const T* W = col_w<T>(A.w64, A.w32);
+  // C2R input of round qg's rows, straight from HBM: each point k needs X[k]
+  // and X[M - k] (imaginary parts of k = 0 and M ignored)
+  V xk[P], xn, xm_nopair[PAIRS ? 1 : P];
+  auto load_round = [&](int qg) {
+    const int row = qg * RR + rloc;
+    const bool valid = row < NROW;
+    const int cc = row / (CX * CY), rem = row - (row / (CX * CY)) * (CX * CY);
+    const int ii = rem / CY, jj = rem - (rem / CY) * CY;
+    const V* in = fp.p[valid ? cc : 0] + ((size_t)(x0 + (valid ? ii : 0)) * g.ny + (y0 + (valid ? jj : 0))) * g.cpitch;
 #pragma unroll
-  for (int qg = 0; qg < NRND; qg += NCB) {
-    // C2R input of this thread's FFT points, straight from HBM: each point k
-    // needs X[k] and X[M - k] (imaginary parts of k = 0 and M ignored); the
-    // second word comes from the partner lane where the plan allows
-    constexpr FftPlan IPL = reverse_plan(make_plan(M, PM));
-    constexpr bool PAIRS = c2r_lane_pairs(IPL);
-    const int partner = (t & 63) - tcol + ((TC - tcol) & (TC - 1));  // lane of thread TC - tcol of this row
-    V v[NCB][P];
+    for (int s = 0; s < P; ++s) xk[s] = valid ? in[fft_in_index<M, true, PM>(tcol, s)] : mk<T>(T(0), T(0));
+    xn = mk<T>(T(0), T(0));
+    if (PAIRS && valid && tcol == 0) xn = in[M];
+    if constexpr (!PAIRS) {
 #pragma unroll
-    for (int c = 0; c < NCB; ++c) {
-      const int row = (qg + c) * RR + rloc;
-      const bool valid = row < NROW;
-      const int cc = row / (CX * CY), rem = row - (row / (CX * CY)) * (CX * CY);
-      const int ii = rem / CY, jj = rem - (rem / CY) * CY;
-      const V* in = fp.p[valid ? cc : 0] +
-                    ((size_t)(x0 + (valid ? ii : 0)) * g.ny + (y0 + (valid ? jj : 0))) * g.cpitch;
-      V xk[P];
+      for (int s = 0; s < P; ++s) xm_nopair[s] = valid ? in[M - fft_in_index<M, true, PM>(tcol, s)] : mk<T>(T(0), T(0));
+    }
+  };
+  for (int base = L0; base < L1; base += KR * NT) {
+    T gx[KR][4], gy[KR][4], gz[KR][8], acc[KR][3];
+    int z0[KR], oid[KR];
+    bool val[KR];
 #pragma unroll
-      for (int s = 0; s < P; ++s) xk[s] = valid ? in[fft_in_index<M, true, PM>(tcol, s)] : mk<T>(T(0), T(0));
-      V xn = mk<T>(T(0), T(0));  // X[M], the partner of k = 0 (thread 0 of the row)
-      if (PAIRS && valid && tcol == 0) xn = in[M];
+    for (int k = 0; k < KR; ++k) {
+      const int i = base + k * NT + t;
+      val[k] = i < L1;
+      const int4 e = val[k] ? A.ent[i] : make_int4(0, 0, 8 | (8 << 5), 0);
+      const T* ws = W + (size_t)e.x * 3 * ng;
+      col_xy_weights<T, CX>(ws, ng, col_rx(e.z), col_ry(e.z), gx[k], gy[k]);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) gz[k][q] = q < ng ? ws[2 * ng + q] : T(0);
+      z0[k] = col_sz(e.z);
+      oid[k] = e.w;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) acc[k][c] = T(0);
+    }
+#pragma unroll
+    for (int qg = 0; qg < NRND; ++qg) {
+      if (qg == 0) load_round(0);
+      V v[1][P];
 #pragma unroll
       for (int s = 0; s < P; ++s) {
         const int k = fft_in_index<M, true, PM>(tcol, s);
         V xm;
         if constexpr (PAIRS) {
           const V sh = mk<T>(__shfl(xk[P - 1 - s].x, partner), __shfl(xk[P - 1 - s].y, partner));
-          const int self = c2r_self_slot(IPL, s);  // a constant once unrolled
+          const int self = c2r_self_slot(IPL, s);
           const V own = self >= 0 ? xk[self >= 0 ? self : 0] : xn;
           xm = tcol == 0 ? own : sh;
         } else {
-          xm = valid ? in[M - k] : mk<T>(T(0), T(0));
+          xm = xm_nopair[s];
         }
         V xs = xk[s];
         if (k == 0) { xs.y = T(0); xm.y = T(0); }
         const V sum = cadd(xs, cconj(xm));
         const V dif = csub(xs, cconj(xm));
-        const V rot = cmulc(dif, s_twz[k]);      // W^{-k} (X[k] - conj X[M-k])
-        v[c][s] = cadd(sum, mk<T>(-rot.y, rot.x));  // + i * rot
+        const V rot = cmulc(dif, s_twz[k]);
+        v[0][s] = cadd(sum, mk<T>(-rot.y, rot.x));
       }
-    }
-    __syncthreads();  // the previous group's rows have been read (and the twiddles are in LDS)
-    fft_columns<T, M, +1, true, PM, NCB, SH::SSH, SH::SB>(v, area, LS, rloc, RR, tcol, s_tw);
-    __syncthreads();
+      __syncthreads();  // the previous round's rows have been read
+      fft_columns<T, M, +1, true, PM, 1, SH::SSH, SH::SB>(v, area, LS, rloc, RR, tcol, s_tw);
+      // fft_columns returns after the barrier that follows its last LDS reads
 #pragma unroll
-    for (int c = 0; c < NCB; ++c)
-#pragma unroll
-      for (int s = 0; s < P; ++s) area[SH::row0(rloc + c * RR) + fft_out_index<M, true, PM>(tcol, s)] = v[c][s];
-    __syncthreads();
-    if (t < NZ) {
-#pragma unroll
-      for (int r = 0; r < NCB * RR; ++r)
-        if (qg * RR + r < NROW) u[qg * RR + r] = rowsT[2 * SH::row0(r) + t];
-    }
-  }
-  // this patch's z prefix table, read from HBM (a few words per thread)
-  const int* zs = A.zst + (size_t)p * (NZ + 1);
-  const int L0 = zs[0], L1 = zs[NZ];
-  int a1, b1, a2, b2;
-  col_ranges<NZ>(zs, t, ng, a1, b1, a2, b2);
-  T* sw = reinterpret_cast<T*>(smem);  // planes: gx 0-3, gy 4-7, gz 8-15, slot (c, o) 16 + 8 c + o
-  int* ssz = reinterpret_cast<int*>(sw + 40 * CAP);
-  int* sout = ssz + CAP;  // partial-sum slot of the blob for this patch
-  const T* W = col_w<T>(A.w64, A.w32);
-  for (int c0 = L0; c0 < L1; c0 += CAP) {
-    const int cnt = min(CAP, L1 - c0);
-    __syncthreads();
-    for (int i0 = 0; i0 < cnt; i0 += 2 * NT) {
-      int4 e[2];
-      T gx[2][4], gy[2][4], gz[2][8];
-#pragma unroll
-      for (int uu = 0; uu < 2; ++uu) {
-        const int i = i0 + t + uu * NT;
-        e[uu] = i < cnt ? A.ent[c0 + i] : make_int4(0, 0, 8 | (8 << 5), 0);
+      for (int s = 0; s < P; ++s) {
+        const int m = fft_out_index<M, true, PM>(tcol, s);
+        uc[(2 * m) * LDU + rloc] = v[0][s].x;
+        uc[(2 * m + 1) * LDU + rloc] = v[0][s].y;
       }
+      __syncthreads();
+      // the next round's input is loaded while this round's rows are used
+      if (qg + 1 < NRND) load_round(qg + 1);
 #pragma unroll
-      for (int uu = 0; uu < 2; ++uu) {
-        const T* ws = W + (size_t)e[uu].x * 3 * ng;
-        col_xy_weights<T, CX>(ws, ng, col_rx(e[uu].z), col_ry(e[uu].z), gx[uu], gy[uu]);
+      for (int k = 0; k < KR; ++k) {
+        if (!val[k]) continue;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) gz[uu][k] = k < ng ? ws[2 * ng + k] : T(0);
-      }
+        for (int o = 0; o < 8; ++o) {
+          if (o >= ng) break;
+          int z = z0[k] + o;
+          if (z >= NZ) z -= NZ;
+          const T* ur = uc + z * LDU;
+          T a[3] = {T(0), T(0), T(0)};
 #pragma unroll
-      for (int uu = 0; uu < 2; ++uu) {
-        const int i = i0 + t + uu * NT;
-        if (i < cnt) {
+          for (int g4 = 0; g4 < RR / CY; ++g4) {
+            const int R = qg * RR + g4 * CY;  // a constant once unrolled
+            if (R < NROW) {
+              const int c = R / (CX * CY), ii = (R % (CX * CY)) / CY;
+              T u4[CY];
 #pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            sw[k * CAP + i] = gx[uu][k];
-            sw[(4 + k) * CAP + i] = gy[uu][k];
+              for (int j = 0; j < CY; ++j) u4[j] = ur[g4 * CY + j];
+              T b = gy[k][0] * u4[0];
+#pragma unroll
+              for (int j = 1; j < CY; ++j) b += gy[k][j] * u4[j];
+              a[c] += gx[k][ii] * b;
+            }
           }
+          // components with rows in this round (constants once unrolled)
+          const int clo = (qg * RR) / (CX * CY), chi = col_min(NROW - 1, qg * RR + RR - 1) / (CX * CY);
 #pragma unroll
-          for (int k = 0; k < 8; ++k) sw[(8 + k) * CAP + i] = gz[uu][k];
-          ssz[i] = col_sz(e[uu].z);
-          sout[i] = e[uu].w;
+          for (int c = 0; c < 3; ++c)
+            if (c >= clo && c <= chi) acc[k][c] += gz[k][o] * a[c];
         }
       }
     }
-    __syncthreads();
-    const int e1a = max(a1, c0), e1b = min(b1, c0 + cnt);
-    const int e2a = max(a2, c0), e2b = min(b2, c0 + cnt);
-    const int n1 = max(0, e1b - e1a), n2 = max(0, e2b - e2a);
-    for (int k = 0; k < n1 + n2; ++k) {
-      const int i = (k < n1 ? e1a + k : e2a + (k - n1)) - c0;
-      int o = t - ssz[i];
-      if (o < 0) o += NZ;
-      const T w = sw[(8 + o) * CAP + i];
-      T gx[CX], gy[CY];
 #pragma unroll
-      for (int q = 0; q < CX; ++q) gx[q] = sw[q * CAP + i];
-#pragma unroll
-      for (int q = 0; q < CY; ++q) gy[q] = sw[(4 + q) * CAP + i];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        T a = T(0);
-#pragma unroll
-        for (int ii = 0; ii < CX; ++ii) {
-          T b = T(0);
-#pragma unroll
-          for (int jj = 0; jj < CY; ++jj) b += gy[jj] * u[(c * CX + ii) * CY + jj];
-          a += gx[ii] * b;
-        }
-        sw[(16 + 8 * c + o) * CAP + i] = w * a;
+    for (int k = 0; k < KR; ++k)
+      if (val[k]) {
+        T* out = part + (size_t)oid[k] * 3;
+        out[0] = acc[k][0];
+        out[1] = acc[k][1];
+        out[2] = acc[k][2];
       }
-    }
-    __syncthreads();
-    for (int i = t; i < cnt; i += NT) {
-      T s0 = T(0), s1 = T(0), s2 = T(0);
-      for (int o = 0; o < ng; ++o) {
-        s0 += sw[(16 + o) * CAP + i];
-        s1 += sw[(24 + o) * CAP + i];
-        s2 += sw[(32 + o) * CAP + i];
-      }
-      T* out = part + (size_t)sout[i] * 3;
-      out[0] = s0;
-      out[1] = s1;
-      out[2] = s2;
-    }
   }
 }
 
