@@ -120,7 +120,7 @@ Sizes sizes_of(const fcm_config* c) {
     // the column buffers are sized for the narrower of the two patch widths
     if (col_patch_cx(g2, bb2, c->ng) > 0)
       t += (3 * (size_t)col_groups(g2, 2) + 1) * 4 + col_list_capacity(nb) * 20 + col_zst_words(g2, 2) * 4 +
-           (size_t)nb * COL_SLOTS_MAX * 3 * 8;
+           (size_t)nb * COL_SLOTS_MAX * 3 * 8 + (size_t)nb * COL_WIN * 12 * (c->slip_channel ? 2 : 1);
   }
   const int n3b = 3 * c->max_blobs_per_body;
   t += (size_t)(3 * n3b + 2 * n3b * n3b + 36) * 8 + (size_t)c->max_bodies * (3 + 4 + 9 + 6) * 8 + (size_t)nb * 3 * 8;
@@ -507,6 +507,15 @@ int fcm_create(const fcm_config* cfg, fcm_handle* out) {
     if (e == hipSuccess) h->bd.col_list = dalloc<int>(h, col_list_capacity(nb), e);
     if (e == hipSuccess) h->bd.col_ent = dalloc<int4>(h, col_list_capacity(nb), e);
     if (e == hipSuccess) h->bd.col_zst = dalloc<int>(h, col_zst_words(h->g, 2), e);
+    if (e == hipSuccess) h->bd.cwin64 = dalloc<double>(h, (size_t)nb * COL_WIN, e);
+    if (e == hipSuccess) h->bd.cwin32 = dalloc<float>(h, (size_t)nb * COL_WIN, e);
+    if (cfg->slip_channel) {
+      if (e == hipSuccess) h->bd.icwin64 = dalloc<double>(h, (size_t)nb * COL_WIN, e);
+      if (e == hipSuccess) h->bd.icwin32 = dalloc<float>(h, (size_t)nb * COL_WIN, e);
+    } else {
+      h->bd.icwin64 = h->bd.cwin64;
+      h->bd.icwin32 = h->bd.cwin32;
+    }
     if (e == hipSuccess) h->bd.col_part = dalloc<double>(h, (size_t)nb * COL_SLOTS_MAX * 3, e);
   } else {
     h->bd.col_fused[0] = h->bd.col_fused[1] = 0;

@@ -93,10 +93,46 @@ __global__ void __launch_bounds__(64) col_list_kernel(ColListArgs A, int pass) {
       const int sxw = col_wrapi(r.x + omin, g.nx), syw = col_wrapi(r.y + omin, g.ny);
       const int rx = col_wrap_half(sxw - x0, g.nx), ry = col_wrap_half(syw - y0, g.ny);
       const int dst = col_wrapi(px - sxw / cx, A.npx) * 3 + col_wrapi(py - syw / COL_CY, A.npy);
-      A.ent[e] = make_int4(sl, r.w, (rx + 8) | ((ry + 8) << 5) | (col_wrapi(r.z + omin, g.nz) << 10),
+      const int dxw = col_wrapi(px - sxw / cx, A.npx), dyw = col_wrapi(py - syw / COL_CY, A.npy);
+      A.ent[e] = make_int4(sl, r.w,
+                           (rx + 8) | ((ry + 8) << 5) | (col_wrapi(r.z + omin, g.nz) << 10) | (dxw << 20) | (dyw << 23),
                            dst * A.nb + sl);
     }
   }
+}
+
+// The weight windows of the column kernels (COL_WIN words per sorted blob).
+// A blob whose stencil starts at x node sx touches the patches floor(sx / cx)
+// + d, d = 0, 1, ...; in patch d its weights for the patch's columns ii are
+// g[cx d - sx mod cx + ii] (zero outside 0..ng-1), stored at words cx d + ii.
+// The same for y with COL_CY (words 12 + 4 d + jj), and the ng z weights at
+// 24..31.  A list entry then reads its weights as three aligned vectors of one
+// 128-byte line (FP32) instead of gathering them word by word.
+template <typename T>
+__device__ void col_window_one(const T* __restrict__ ws, int ng, int cx, int rx0, int ry0, T* __restrict__ out) {
+  for (int q = 0; q < 12; ++q) {
+    const int d = q / cx, ii = q - (q / cx) * cx;
+    const int o = cx * d - rx0 + ii;
+    out[q] = (o >= 0 && o < ng) ? ws[o] : T(0);
+  }
+  for (int q = 0; q < 12; ++q) {
+    const int d = q / COL_CY, jj = q - (q / COL_CY) * COL_CY;
+    const int o = COL_CY * d - ry0 + jj;
+    out[12 + q] = (o >= 0 && o < ng) ? ws[ng + o] : T(0);
+  }
+  for (int q = 0; q < 8; ++q) out[24 + q] = q < ng ? ws[2 * ng + q] : T(0);
+}
+
+__global__ void __launch_bounds__(256) col_window_kernel(GridGeom g, int ng, int cx, int nb, const int4* __restrict__ rec,
+                                                          const double* __restrict__ w64, const float* __restrict__ w32,
+                                                          double* __restrict__ o64, float* __restrict__ o32) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= nb) return;
+  const int omin = 1 - ng / 2;
+  const int4 r = rec[s];
+  const int rx0 = col_wrapi(r.x + omin, g.nx) % cx, ry0 = col_wrapi(r.y + omin, g.ny) % COL_CY;
+  col_window_one<double>(w64 + (size_t)s * 3 * ng, ng, cx, rx0, ry0, o64 + (size_t)s * COL_WIN);
+  col_window_one<float>(w32 + (size_t)s * 3 * ng, ng, cx, rx0, ry0, o32 + (size_t)s * COL_WIN);
 }
 
 static bool col_nz_supported(int nz) {
@@ -133,7 +169,7 @@ int col_groups(const GridGeom& g, int cx) {
 
 hipError_t build_col_lists(const GridGeom& g, const BlobBins& bb, BlobData& bd, hipStream_t st) {
   const int cx = bd.col_cx;
-  if (cx <= 0 || !bd.col_list || !bd.col_ent) return hipSuccess;
+  if (cx <= 0 || !bd.col_list || !bd.col_ent || !bd.cwin64) return hipSuccess;
   ColListArgs A{g, bb, bd.ng, cx, g.nx / cx, g.ny / COL_CY, bd.nb, bd.bin_start, bd.rec, bd.col_count, bd.col_start,
                 bd.col_list, bd.col_ent, bd.col_zst};
   const int npatch = A.npx * A.npy;
@@ -141,6 +177,15 @@ hipError_t build_col_lists(const GridGeom& g, const BlobBins& bb, BlobData& bd, 
   hipError_t e = exclusive_scan(bd.col_count, bd.col_start, bd.col_cursor, col_groups(g, cx), st);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(col_list_kernel, dim3(npatch), dim3(64), 0, st, A, 1);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (bd.nb > 0) {
+    const dim3 grid((bd.nb + 255) / 256);
+    hipLaunchKernelGGL(col_window_kernel, grid, dim3(256), 0, st, g, bd.ng, cx, bd.nb, bd.rec, bd.w64, bd.w32,
+                       bd.cwin64, bd.cwin32);
+    if (bd.icwin64 != bd.cwin64)
+      hipLaunchKernelGGL(col_window_kernel, grid, dim3(256), 0, st, g, bd.ng, cx, bd.nb, bd.rec, bd.iw64, bd.iw32,
+                         bd.icwin64, bd.icwin32);
+  }
   return hipGetLastError();
 }
 

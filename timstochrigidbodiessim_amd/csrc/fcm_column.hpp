@@ -61,18 +61,56 @@ struct ColArgs {
   int ng, npx, npy;
   const int4* ent;  // entry records of the patch lists (col_entry)
   const int* zst;
-  const double* w64;
+  const double* w64;  // weight windows (COL_WIN words per sorted blob)
   const float* w32;
 };
 
 // One entry of a patch list: (sorted blob slot, blob id, packed offsets, index
 // of the blob's partial-sum slot for this patch: slot-major, dst nb + blob, so
 // that the reduction reads consecutive blobs).  The packed word holds the
-// stencil start relative to the patch in x and y (+8, 5 bits each) and the
-// wrapped stencil start in z (bits 10 and up).
+// stencil start relative to the patch in x and y (+8, 5 bits each), the
+// wrapped stencil start in z (bits 10-19) and which of the blob's patches this
+// is in x (bits 20-22) and y (bits 23-24), the index of its weight windows.
 __device__ __forceinline__ int col_rx(int pk) { return (pk & 31) - 8; }
 __device__ __forceinline__ int col_ry(int pk) { return ((pk >> 5) & 31) - 8; }
-__device__ __forceinline__ int col_sz(int pk) { return pk >> 10; }
+__device__ __forceinline__ int col_sz(int pk) { return (pk >> 10) & 1023; }
+__device__ __forceinline__ int col_dx(int pk) { return (pk >> 20) & 7; }
+__device__ __forceinline__ int col_dy(int pk) { return (pk >> 23) & 3; }
+
+// N consecutive words from a 16-byte aligned address (8-byte aligned for two
+// FP32 words), as 16-byte loads
+template <typename T, int N>
+__device__ __forceinline__ void col_vload(const T* __restrict__ p, T* o) {
+  if constexpr (sizeof(T) == 4 && N % 4 == 0) {
+#pragma unroll
+    for (int i = 0; i < N / 4; ++i) {
+      const float4 v = reinterpret_cast<const float4*>(p)[i];
+      o[4 * i] = v.x; o[4 * i + 1] = v.y; o[4 * i + 2] = v.z; o[4 * i + 3] = v.w;
+    }
+  } else if constexpr (sizeof(T) == 4 && N == 2) {
+    const float2 v = *reinterpret_cast<const float2*>(p);
+    o[0] = v.x; o[1] = v.y;
+  } else {
+    static_assert(sizeof(T) == 8 && N % 2 == 0, "vector load shape");
+#pragma unroll
+    for (int i = 0; i < N / 2; ++i) {
+      const double2 v = reinterpret_cast<const double2*>(p)[i];
+      o[2 * i] = v.x; o[2 * i + 1] = v.y;
+    }
+  }
+}
+
+// x, y and z weights of a list entry over its patch, from the blob's windows
+// (zero outside the stencil)
+template <typename T, int CX>
+__device__ __forceinline__ void col_win_load(const T* __restrict__ win, int slot, int pk, T* gx, T* gy, T* gz) {
+  const T* wn = win + (size_t)slot * COL_WIN;
+  col_vload<T, CX>(wn + CX * col_dx(pk), gx);
+#pragma unroll
+  for (int ii = CX; ii < 4; ++ii) gx[ii] = T(0);
+  col_vload<T, 4>(wn + 12 + 4 * col_dy(pk), gy);
+  col_vload<T, 8>(wn + 24, gz);
+}
 
 // ---------------------------------------------------------------------------
 // launch shapes
@@ -274,21 +312,6 @@ __device__ __forceinline__ void col_ranges(const int* __restrict__ zs, int z, in
   b2 = in && wrap ? zs[z + 1] : 0;
 }
 
-// x and y weights of a blob over the patch (zero outside the stencil)
-template <typename T, int CX>
-__device__ __forceinline__ void col_xy_weights(const T* ws, int ng, int rx, int ry, T* gx, T* gy) {
-#pragma unroll
-  for (int ii = 0; ii < 4; ++ii) {
-    const int o = ii - rx;
-    gx[ii] = (ii < CX && o >= 0 && o < ng) ? ws[o] : T(0);
-  }
-#pragma unroll
-  for (int jj = 0; jj < 4; ++jj) {
-    const int o = jj - ry;
-    gy[jj] = (o >= 0 && o < ng) ? ws[ng + o] : T(0);
-  }
-}
-
 // ---------------------------------------------------------------------------
 // spreading + forward z R2C
 //
@@ -323,7 +346,7 @@ __global__ void __launch_bounds__(col_threads(NZ))
   // they are read, so a patch wholly above ny/2 has nothing to write
   if (g.channel && y0 > g.ny / 2) return;
   const int t = threadIdx.x;
-  const int ng = A.ng, omin = 1 - ng / 2;
+  const int ng = A.ng;
   for (int i = t; i < M; i += NT) {
     s_tw[i] = tw[i];
     s_twz[i] = twz[i];
@@ -356,10 +379,7 @@ __global__ void __launch_bounds__(col_threads(NZ))
       }
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
-        const T* ws = W + (size_t)e[u].x * 3 * ng;
-        col_xy_weights<T, CX>(ws, ng, col_rx(e[u].z), col_ry(e[u].z), gx[u], gy[u]);
-#pragma unroll
-        for (int k = 0; k < 8; ++k) gz[u][k] = k < ng ? ws[2 * ng + k] : T(0);
+        col_win_load<T, CX>(W, e[u].x, e[u].z, gx[u], gy[u], gz[u]);
 #pragma unroll
         for (int c = 0; c < 3; ++c) f[u][c] = T(F[3 * (size_t)e[u].y + c]);
       }
@@ -566,10 +586,7 @@ __global__ void __launch_bounds__(col_threads(NZ), col_interp_waves(NZ, (int)siz
       const int i = base + k * NT + t;
       val[k] = i < L1;
       const int4 e = val[k] ? A.ent[i] : make_int4(0, 0, 8 | (8 << 5), 0);
-      const T* ws = W + (size_t)e.x * 3 * ng;
-      col_xy_weights<T, CX>(ws, ng, col_rx(e.z), col_ry(e.z), gx[k], gy[k]);
-#pragma unroll
-      for (int q = 0; q < 8; ++q) gz[k][q] = q < ng ? ws[2 * ng + q] : T(0);
+      col_win_load<T, CX>(W, e.x, e.z, gx[k], gy[k], gz[k]);
       z0[k] = col_sz(e.z);
       oid[k] = e.w;
 #pragma unroll
@@ -701,8 +718,8 @@ __global__ void __launch_bounds__(256) col_reduce_kernel(GridGeom g, int ng, int
 namespace {
 
 ColArgs col_args(const GridGeom& g, const BlobData& bd, bool interp) {
-  return ColArgs{g,      bd.ng, g.nx / bd.col_cx, g.ny / COL_CY, bd.col_ent, bd.col_zst, interp ? bd.iw64 : bd.w64,
-                 interp ? bd.iw32 : bd.w32};
+  return ColArgs{g,          bd.ng,      g.nx / bd.col_cx, g.ny / COL_CY, bd.col_ent, bd.col_zst,
+                 interp ? bd.icwin64 : bd.cwin64, interp ? bd.icwin32 : bd.cwin32};
 }
 
 // points per thread of the z pass: the tuned request (8 or 16) or the default

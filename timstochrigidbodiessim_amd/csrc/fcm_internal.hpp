@@ -107,6 +107,13 @@ struct BlobData {
   int* col_list = nullptr;     // sorted blob slots, COL_SLOTS_MAX per blob at most
   int4* col_ent = nullptr;     // entry records in list order (slot, id, packed offsets, partial slot)
   int* col_zst = nullptr;      // npatch x (nz + 1): first entry with stencil-start z >= k
+  // weight windows of the column kernels, COL_WIN words per sorted blob: for
+  // each patch a blob's stencil can touch in x (y), the patch's cx (4) x (y)
+  // weights, then the ng z weights, zero-padded (built from w / iw per patch width)
+  double* cwin64 = nullptr;
+  float* cwin32 = nullptr;
+  double* icwin64 = nullptr;  // interpolation: cwin themselves, or in a slip channel from iw
+  float* icwin32 = nullptr;
   void* col_part = nullptr;    // interpolation partial sums: nb x COL_SLOTS_MAX x 3 (T)
   int col_fused[2] = {1, 1};   // per precision (0 FP64, 1 FP32): use the column path in the mobility product
 };
@@ -164,6 +171,7 @@ hipError_t interp(const GridGeom& g, const BlobBins& bb, const BlobData& bd, con
 // ---------------------------------------------------------------------------
 constexpr int COL_CY = 4;          // patch y extent
 constexpr int COL_SLOTS_MAX = 15;  // partial sums per blob: up to 5 x 3 patches
+constexpr int COL_WIN = 32;        // window words per blob: x windows 0-11, y windows 12-23, z 24-31
 // patch x extent usable on this grid for both precisions (0: none)
 int col_patch_cx(const GridGeom& g, const BlobBins& bb, int ng);
 // the column kernels exist for word size tbytes at this nz, patch width cx and
