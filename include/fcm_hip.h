@@ -377,8 +377,9 @@ int fcm_clear_tuning_cache(void);
  * inverse, y inverse, z inverse, interpolation(+K), K^T, and the reduction of
  * the interpolation partial sums.  On the column path the spread slot holds
  * the fused spread + z forward kernel, the interpolation slot the fused z
- * inverse + interpolation kernel, and the two z slots are 0; on the separate
- * path the last slot is 0. */
+ * inverse + interpolation kernel (each timed over 4 back-to-back launches and
+ * divided by 4, so that one launch's dispatch is not counted), and the two z
+ * slots are 0; on the separate path the last slot is 0. */
 int fcm_profile_matvec(fcm_handle h, int precision, int reps, double* ms_per_pass);
 /* A plain streaming copy of `bytes` (16-byte words) between two scratch buffers,
  * `reps` times, on the handle's stream; the reference point for HBM byte counters

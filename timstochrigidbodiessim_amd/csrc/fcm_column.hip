@@ -210,6 +210,6 @@ bool col_shape_valid(int tbytes, int nz, int cx, int pm) { return col_shape_pm(t
 template hipError_t spread_zfwd<double>(const GridGeom&, const BlobData&, const double*, const FieldPtrs<double>&,
                                         const TwiddleSet<double>&, hipStream_t, bool);
 template hipError_t zinv_interp<double>(const GridGeom&, const BlobData&, const FieldPtrs<double>&,
-                                        const TwiddleSet<double>&, double*, const KTerm&, hipStream_t, hipEvent_t);
+                                        const TwiddleSet<double>&, double*, const KTerm&, hipStream_t, hipEvent_t, int);
 
 }  // namespace fcm

@@ -803,21 +803,23 @@ hipError_t spread_zfwd(const GridGeom& g, const BlobData& bd, const double* F, c
 
 template <typename T>
 hipError_t zinv_interp(const GridGeom& g, const BlobData& bd, const FieldPtrs<T>& fp, const TwiddleSet<T>& tw,
-                       double* V, const KTerm& kt, hipStream_t st, hipEvent_t between) {
+                       double* V, const KTerm& kt, hipStream_t st, hipEvent_t between, int interp_reps) {
   if (bd.col_cx <= 0) return hipErrorInvalidValue;
   if (bd.nb == 0) return hipSuccess;
   const ColArgs A = col_args(g, bd, true);
   T* part = static_cast<T*>(bd.col_part);
   hipError_t e = hipErrorInvalidValue;
-  switch (g.nz) {
+  for (int r = 0; r < interp_reps; ++r) {
+    switch (g.nz) {
 #define FCM_COL_INTERP(v) \
   case v: e = col_launch<T, v>(true, A, bd, nullptr, fp, tw, part, st); break;
-    FCM_COL_NZ(FCM_COL_INTERP)
+      FCM_COL_NZ(FCM_COL_INTERP)
 #undef FCM_COL_INTERP
-    default:
-      break;
+      default:
+        break;
+    }
+    if (e != hipSuccess) return e;
   }
-  if (e != hipSuccess) return e;
   if (between && (e = hipEventRecord(between, st)) != hipSuccess) return e;
   const dim3 grid((bd.nb + 255) / 256);
   if (bd.col_cx == 4)

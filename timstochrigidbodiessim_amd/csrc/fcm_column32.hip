@@ -8,6 +8,6 @@ namespace fcm {
 template hipError_t spread_zfwd<float>(const GridGeom&, const BlobData&, const double*, const FieldPtrs<float>&,
                                        const TwiddleSet<float>&, hipStream_t, bool);
 template hipError_t zinv_interp<float>(const GridGeom&, const BlobData&, const FieldPtrs<float>&,
-                                       const TwiddleSet<float>&, double*, const KTerm&, hipStream_t, hipEvent_t);
+                                       const TwiddleSet<float>&, double*, const KTerm&, hipStream_t, hipEvent_t, int);
 
 }  // namespace fcm

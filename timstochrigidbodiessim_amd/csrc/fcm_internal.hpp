@@ -187,10 +187,11 @@ template <typename T>
 hipError_t spread_zfwd(const GridGeom& g, const BlobData& bd, const double* F, const FieldPtrs<T>& fp,
                        const TwiddleSet<T>& tw, hipStream_t st, bool images = true);
 // inverse z pass + interpolation partial sums, then the fixed-order reduction
-// into V (with the K term)
+// into V (with the K term).  For timing, `between` is recorded between the two
+// and the first kernel is launched `interp_reps` times (it is idempotent).
 template <typename T>
 hipError_t zinv_interp(const GridGeom& g, const BlobData& bd, const FieldPtrs<T>& fp, const TwiddleSet<T>& tw,
-                       double* V, const KTerm& kt, hipStream_t st, hipEvent_t between = nullptr);
+                       double* V, const KTerm& kt, hipStream_t st, hipEvent_t between = nullptr, int interp_reps = 1);
 
 // ---------------------------------------------------------------------------
 // rigid bodies, preconditioner, Krylov kernels
