@@ -288,8 +288,12 @@ struct ColumnShape {
   // stride whose 16-byte reads by 16 lanes at neighbouring z fall on distinct
   // banks; E_KR list entries per thread per pass over the transform
   static constexpr int E_LDU = col_ldu(RR, (int)sizeof(T));
+  // row z starts at z E_LDU + ((z >> 4) & 1) E_PAD: without the shift, the
+  // transform's writes of rows z and z + 16 fell on the same banks
+  static constexpr int E_PAD = E_LDU - RR;
+  static __device__ __forceinline__ int urow(int z) { return z * E_LDU + ((z >> 4) & 1) * E_PAD; }
   static constexpr int E_KR = 2;
-  static constexpr int E_LDS = col_max(FFT_BYTES, NZ * E_LDU * (int)sizeof(T));
+  static constexpr int E_LDS = col_max(FFT_BYTES, (NZ * E_LDU + E_PAD) * (int)sizeof(T));
   // a shape the kernels can be built for: a z plan that tiles the threads in
   // whole y columns of rows, FFT columns that do not overlap, room to stage 15
   // entries, and the LDS of a CU
@@ -513,7 +517,7 @@ __global__ void __launch_bounds__(col_threads(NZ))
 // straight from HBM (each C2R point k needs X[k] and X[M - k]; the second
 // word comes from the partner lane where the plan allows), transformed
 // through the padded LDS scratch and left in LDS z-major (row r of node z at
-// z E_LDU + r).  Each thread then reads, for every stencil z of each of its
+// urow(z) + r, rows z and z + 16 offset by the padding).  Each thread then reads, for every stencil z of each of its
 // entries, the round's rows at that z (16-byte reads) and adds
 // gz sum_x gx sum_y gy u to the entry's three sums, while the next round's
 // k-space words are in flight.  Every lane does the same work whatever the z
@@ -532,7 +536,7 @@ __global__ void __launch_bounds__(col_threads(NZ), col_interp_waves(NZ, (int)siz
   static_assert(SH::ok, "column shape");
   using V = c2<T>;
   constexpr int NT = SH::NT, M = SH::M, P = SH::P, TC = SH::TC, RR = SH::RR, NROW = SH::NROW, NRND = SH::NRND;
-  constexpr int CY = COL_CY, LS = SH::LS, LDU = SH::E_LDU, KR = SH::E_KR;
+  constexpr int CY = COL_CY, LS = SH::LS, KR = SH::E_KR;
   static_assert(RR % CY == 0, "rows of a round come in whole y columns");
   __shared__ __align__(16) unsigned char smem[SH::E_LDS];
   __shared__ V s_tw[M], s_twz[M];  // FFT twiddles, C2R twiddles W^k
@@ -621,8 +625,8 @@ __global__ void __launch_bounds__(col_threads(NZ), col_interp_waves(NZ, (int)siz
 #pragma unroll
       for (int s = 0; s < P; ++s) {
         const int m = fft_out_index<M, true, PM>(tcol, s);
-        uc[(2 * m) * LDU + rloc] = v[0][s].x;
-        uc[(2 * m + 1) * LDU + rloc] = v[0][s].y;
+        uc[SH::urow(2 * m) + rloc] = v[0][s].x;
+        uc[SH::urow(2 * m + 1) + rloc] = v[0][s].y;
       }
       __syncthreads();
       // the next round's input is loaded while this round's rows are used
@@ -635,7 +639,7 @@ __global__ void __launch_bounds__(col_threads(NZ), col_interp_waves(NZ, (int)siz
           if (o >= ng) break;
           int z = z0[k] + o;
           if (z >= NZ) z -= NZ;
-          const T* ur = uc + z * LDU;
+          const T* ur = uc + SH::urow(z);
           T a[3] = {T(0), T(0), T(0)};
 #pragma unroll
           for (int g4 = 0; g4 < RR / CY; ++g4) {
