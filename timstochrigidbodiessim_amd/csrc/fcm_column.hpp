@@ -100,6 +100,14 @@ __device__ __forceinline__ void col_vload(const T* __restrict__ p, T* o) {
   }
 }
 
+template <typename T, int N>
+__device__ __forceinline__ void col_vstore(T* p, const T* v) {
+  static_assert(sizeof(T) == 4 && N % 4 == 0, "16-byte stores of FP32 words");
+#pragma unroll
+  for (int i = 0; i < N / 4; ++i)
+    reinterpret_cast<float4*>(p)[i] = make_float4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+}
+
 // x, y and z weights of a list entry over its patch, from the blob's windows
 // (zero outside the stencil)
 template <typename T, int CX>
@@ -281,7 +289,13 @@ struct ColumnShape {
   static __device__ __forceinline__ int row0(int r) { return r * RS; }
   // staging of the blob lists: budget per workgroup, then planes of CAP entries
   static constexpr int BUDGET = col_max(STAGE, FFT_BYTES);
-  static constexpr int S_ENT = 19 * (int)sizeof(T) + 4;   // gx 4, gy 4, F 3, gz 8 | stencil-start z
+  // staged entry: gx 4, gy 4, F 3, stencil-start z, gz 8.  In FP32 one 80-byte
+  // record per entry (four 16-byte reads per (entry, z) step, and records 16
+  // entries apart are the first to share banks); in FP64 planes of single words
+  // (entry i of field f at f S_CAP + i; 160-byte records shared banks every
+  // 8 entries and were 3% slower)
+  static constexpr bool S_AOS = sizeof(T) == 4;
+  static constexpr int S_ENT = 20 * (int)sizeof(T) - (S_AOS ? 0 : (int)sizeof(T) - 4);
   static constexpr int S_CAP = col_odd_le(BUDGET / S_ENT);
   static constexpr int S_LDS = col_max(FFT_BYTES, S_CAP * S_ENT);
   // interpolation: the rows of a round kept z-major, E_LDU words per z, a row
@@ -362,8 +376,8 @@ __global__ void __launch_bounds__(col_threads(NZ))
   int a1, b1, a2, b2;
   col_ranges<NZ>(zs, t, ng, a1, b1, a2, b2);
 
-  T* sw = reinterpret_cast<T*>(smem);  // planes: gx 0-3, gy 4-7, F 8-10, gz 11-18
-  int* ssz = reinterpret_cast<int*>(sw + 19 * CAP);
+  T* sw = reinterpret_cast<T*>(smem);  // records (gx 0-3, gy 4-7, F 8-10, z 11, gz 12-19) or planes (gz 11-18)
+  int* ssz = reinterpret_cast<int*>(sw + 19 * CAP);  // planes: stencil-start z
   const T* W = col_w<T>(A.w64, A.w32);
   T acc[NROW];  // row (c CX + ii) CY + jj of this thread's z
 #pragma unroll
@@ -391,16 +405,25 @@ __global__ void __launch_bounds__(col_threads(NZ))
       for (int u = 0; u < 2; ++u) {
         const int i = i0 + t + u * NT;
         if (i < cnt) {
+          if constexpr (SH::S_AOS) {
+            T* r = sw + (size_t)i * 20;
+            const T fz[4] = {f[u][0], f[u][1], f[u][2], __int_as_float(col_sz(e[u].z))};
+            col_vstore<T, 4>(r, gx[u]);
+            col_vstore<T, 4>(r + 4, gy[u]);
+            col_vstore<T, 4>(r + 8, fz);
+            col_vstore<T, 8>(r + 12, gz[u]);
+          } else {
 #pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            sw[k * CAP + i] = gx[u][k];
-            sw[(4 + k) * CAP + i] = gy[u][k];
+            for (int k = 0; k < 4; ++k) {
+              sw[k * CAP + i] = gx[u][k];
+              sw[(4 + k) * CAP + i] = gy[u][k];
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sw[(8 + c) * CAP + i] = f[u][c];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) sw[(11 + k) * CAP + i] = gz[u][k];
+            ssz[i] = col_sz(e[u].z);
           }
-#pragma unroll
-          for (int c = 0; c < 3; ++c) sw[(8 + c) * CAP + i] = f[u][c];
-#pragma unroll
-          for (int k = 0; k < 8; ++k) sw[(11 + k) * CAP + i] = gz[u][k];
-          ssz[i] = col_sz(e[u].z);
         }
       }
     }
@@ -410,15 +433,27 @@ __global__ void __launch_bounds__(col_threads(NZ))
     const int n1 = max(0, e1b - e1a), n2 = max(0, e2b - e2a);
     for (int k = 0; k < n1 + n2; ++k) {
       const int i = (k < n1 ? e1a + k : e2a + (k - n1)) - c0;
-      int o = t - ssz[i];
-      if (o < 0) o += NZ;
-      const T w = sw[(11 + o) * CAP + i];
-      T gx[CX], gy[CY];
+      T gx[4], gy[4], fz[4], w;
+      if constexpr (SH::S_AOS) {
+        const T* r = sw + (size_t)i * 20;
+        col_vload<T, 4>(r + 8, fz);
+        int o = t - __float_as_int(fz[3]);
+        if (o < 0) o += NZ;
+        w = r[12 + o];
+        col_vload<T, 4>(r, gx);
+        col_vload<T, 4>(r + 4, gy);
+      } else {
+        int o = t - ssz[i];
+        if (o < 0) o += NZ;
+        w = sw[(11 + o) * CAP + i];
 #pragma unroll
-      for (int q = 0; q < CX; ++q) gx[q] = sw[q * CAP + i];
+        for (int q = 0; q < CX; ++q) gx[q] = sw[q * CAP + i];
 #pragma unroll
-      for (int q = 0; q < CY; ++q) gy[q] = sw[(4 + q) * CAP + i];
-      const T wf0 = w * sw[8 * CAP + i], wf1 = w * sw[9 * CAP + i], wf2 = w * sw[10 * CAP + i];
+        for (int q = 0; q < CY; ++q) gy[q] = sw[(4 + q) * CAP + i];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) fz[c] = sw[(8 + c) * CAP + i];
+      }
+      const T wf0 = w * fz[0], wf1 = w * fz[1], wf2 = w * fz[2];
 #pragma unroll
       for (int jj = 0; jj < CY; ++jj) {
         const T wy0 = wf0 * gy[jj], wy1 = wf1 * gy[jj], wy2 = wf2 * gy[jj];
