@@ -433,6 +433,57 @@ __device__ __forceinline__ double wave_sum(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// Partner exchange of the reduce-scatter below: lane l receives lane l ^ X
+// within its row of 16.  X = 1, 2: quad permutations; X = 4, 8: a row shift
+// left (lane l reads l + X) into the banks of 4 lanes whose bit X is clear,
+// then a row shift right (l - X) into the others.
+template <int X>
+__device__ __forceinline__ int dpp_xor_row(int v) {
+  if constexpr (X == 1) return __builtin_amdgcn_update_dpp(v, v, 0xb1, 0xf, 0xf, false);
+  else if constexpr (X == 2) return __builtin_amdgcn_update_dpp(v, v, 0x4e, 0xf, 0xf, false);
+  else if constexpr (X == 4) {
+    const int r = __builtin_amdgcn_update_dpp(v, v, 0x104, 0xf, 0x5, false);
+    return __builtin_amdgcn_update_dpp(r, v, 0x114, 0xf, 0xa, false);
+  } else {
+    static_assert(X == 8, "partner distance");
+    const int r = __builtin_amdgcn_update_dpp(v, v, 0x108, 0xf, 0x3, false);
+    return __builtin_amdgcn_update_dpp(r, v, 0x118, 0xf, 0xc, false);
+  }
+}
+template <int X>
+__device__ __forceinline__ double dpp_xor_row(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_xor_row<X>((int)(b & 0xffffffffLL));
+  const int hi = dpp_xor_row<X>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// one halving step: the lane keeps the half of v selected by its bit X and
+// adds the partner's copy of that half
+template <int X, int H>
+__device__ __forceinline__ void rs_step(const double* v, double* out, bool upper) {
+#pragma unroll
+  for (int i = 0; i < H; ++i) {
+    const double mine = upper ? v[H + i] : v[i];
+    const double send = upper ? v[i] : v[H + i];
+    out[i] = mine + dpp_xor_row<X>(send);
+  }
+}
+
+// Reduce-scatter of 16 values over each row of 16 lanes: returns the sum over
+// the row of value rs16_index(lane).  Four halving steps (15 exchanges) where
+// 16 wavefront sums would take 96; the addition order is fixed.
+__device__ __forceinline__ int rs16_index(int lane) {
+  return ((lane & 1) << 3) | ((lane & 2) << 1) | ((lane & 4) >> 1) | ((lane & 8) >> 3);
+}
+__device__ __forceinline__ double row_reduce_scatter16(const double (&v)[16], int lane) {
+  double s8[8], s4[4], s2[2], s1[1];
+  rs_step<1, 8>(v, s8, (lane & 1) != 0);
+  rs_step<2, 4>(s8, s4, (lane & 2) != 0);
+  rs_step<4, 2>(s4, s2, (lane & 4) != 0);
+  rs_step<8, 1>(s2, s1, (lane & 8) != 0);
+  return s1[0];
+}
+
 __device__ __forceinline__ double warp_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -262,7 +262,7 @@ __global__ void __launch_bounds__(KRT) gm_dots_kernel(GmDev G, const TQ* __restr
   if ((int)blockIdx.y * 8 >= ncols) return;
   const TQ* v1 = mode == 0 ? (prov ? Q + (size_t)k * ldq : nullptr) : Q + (size_t)(k + 1) * ldq;
   const double* v2 = mode == 0 ? wbuf : nullptr;
-  __shared__ double red[4 * 16];
+  __shared__ double red[16 * 16];  // [row of 16 lanes][value]
   double a[KRP], b[KRP];
   if (v1) {
     gm_load<MAP>(v1, n, a);
@@ -299,22 +299,19 @@ __global__ void __launch_bounds__(KRT) gm_dots_kernel(GmDev G, const TQ* __restr
         acc[8 + c] += qv[c][qq] * b[qq];
       }
     }
-    if (v1) {
-#pragma unroll
-      for (int c = 0; c < 8; ++c) acc[c] = wave_sum(acc[c]);
-    }
-    if (v2) {
-#pragma unroll
-      for (int c = 0; c < 8; ++c) acc[8 + c] = wave_sum(acc[8 + c]);
-    }
-    if (lane == 0)
-#pragma unroll
-      for (int c = 0; c < 16; ++c) red[wv * 16 + c] = acc[c];
+    // each row of 16 lanes reduce-scatters the 16 sums; the 16 row sums of
+    // every value are then added in a fixed order
+    red[(wv * 4 + (lane >> 4)) * 16 + rs16_index(lane)] = row_reduce_scatter16(acc, lane);
     __syncthreads();
     if (threadIdx.x < 16) {
       const int c = threadIdx.x & 7, which = threadIdx.x >> 3;
       const bool have = which == 0 ? v1 != nullptr : v2 != nullptr;
-      const double v = (red[threadIdx.x] + red[16 + threadIdx.x]) + (red[32 + threadIdx.x] + red[48 + threadIdx.x]);
+      double q[4];
+#pragma unroll
+      for (int w = 0; w < 4; ++w)
+        q[w] = (red[(4 * w) * 16 + threadIdx.x] + red[(4 * w + 1) * 16 + threadIdx.x]) +
+               (red[(4 * w + 2) * 16 + threadIdx.x] + red[(4 * w + 3) * 16 + threadIdx.x]);
+      const double v = (q[0] + q[1]) + (q[2] + q[3]);
       if (have && c0 + c < ncols) partial[((size_t)which * ncols + c0 + c) * nblk + blockIdx.x] = v;
     }
     __syncthreads();
