@@ -120,7 +120,8 @@ Sizes sizes_of(const fcm_config* c) {
     // the column buffers are sized for the narrower of the two patch widths
     if (col_patch_cx(g2, bb2, c->ng) > 0)
       t += (3 * (size_t)col_groups(g2, 2) + 1) * 4 + col_list_capacity(nb) * 20 + col_zst_words(g2, 2) * 4 +
-           (size_t)nb * COL_SLOTS_MAX * 3 * 8 + (size_t)nb * COL_WIN * 12 * (c->slip_channel ? 2 : 1);
+           (size_t)nb * COL_SLOTS_MAX * 3 * 8 + (size_t)nb * COL_WIN * 12 * (c->slip_channel ? 2 : 1) +
+           col_list_capacity(nb) * COL_EWIN * 12;
   }
   const int n3b = 3 * c->max_blobs_per_body;
   t += (size_t)(3 * n3b + 2 * n3b * n3b + 36) * 8 + (size_t)c->max_bodies * (3 + 4 + 9 + 6) * 8 + (size_t)nb * 3 * 8;
@@ -521,6 +522,8 @@ int fcm_create(const fcm_config* cfg, fcm_handle* out) {
       h->bd.icwin64 = h->bd.cwin64;
       h->bd.icwin32 = h->bd.cwin32;
     }
+    if (e == hipSuccess) h->bd.ewin64 = dalloc<double>(h, col_list_capacity(nb) * COL_EWIN, e);
+    if (e == hipSuccess) h->bd.ewin32 = dalloc<float>(h, col_list_capacity(nb) * COL_EWIN, e);
     if (e == hipSuccess) h->bd.col_part = dalloc<double>(h, (size_t)nb * COL_SLOTS_MAX * 3, e);
   } else {
     h->bd.col_fused[0] = h->bd.col_fused[1] = 0;

@@ -135,6 +135,28 @@ __global__ void __launch_bounds__(256) col_window_kernel(GridGeom g, int ng, int
   col_window_one<float>(w32 + (size_t)s * 3 * ng, ng, cx, rx0, ry0, o32 + (size_t)s * COL_WIN);
 }
 
+// The spreading weights of list entry i (COL_EWIN words), cut from its blob's
+// window: the x weights of its patch (words cx dx.., zero past cx), the y
+// weights (12 + 4 dy..) and the z weights; the values col_win_load reads.
+template <typename T>
+__device__ void col_ewin_one(const T* __restrict__ wn, int cx, int pk, T* __restrict__ out) {
+  const int dx = col_dx(pk), dy = col_dy(pk);
+  for (int q = 0; q < 4; ++q) out[q] = q < cx ? wn[cx * dx + q] : T(0);
+  for (int q = 0; q < 4; ++q) out[4 + q] = wn[12 + 4 * dy + q];
+  for (int q = 0; q < 8; ++q) out[8 + q] = wn[24 + q];
+}
+
+__global__ void __launch_bounds__(256) col_ewin_kernel(int cx, const int* __restrict__ total,
+                                                        const int4* __restrict__ ent, const double* __restrict__ w64,
+                                                        const float* __restrict__ w32, double* __restrict__ o64,
+                                                        float* __restrict__ o32) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= *total) return;
+  const int4 e = ent[i];
+  col_ewin_one<double>(w64 + (size_t)e.x * COL_WIN, cx, e.z, o64 + (size_t)i * COL_EWIN);
+  col_ewin_one<float>(w32 + (size_t)e.x * COL_WIN, cx, e.z, o32 + (size_t)i * COL_EWIN);
+}
+
 static bool col_nz_supported(int nz) {
   switch (nz) {
 #define FCM_COL_CASE(v) case v:
@@ -169,7 +191,7 @@ int col_groups(const GridGeom& g, int cx) {
 
 hipError_t build_col_lists(const GridGeom& g, const BlobBins& bb, BlobData& bd, hipStream_t st) {
   const int cx = bd.col_cx;
-  if (cx <= 0 || !bd.col_list || !bd.col_ent || !bd.cwin64) return hipSuccess;
+  if (cx <= 0 || !bd.col_list || !bd.col_ent || !bd.cwin64 || !bd.ewin64) return hipSuccess;
   ColListArgs A{g, bb, bd.ng, cx, g.nx / cx, g.ny / COL_CY, bd.nb, bd.bin_start, bd.rec, bd.col_count, bd.col_start,
                 bd.col_list, bd.col_ent, bd.col_zst};
   const int npatch = A.npx * A.npy;
@@ -185,6 +207,10 @@ hipError_t build_col_lists(const GridGeom& g, const BlobBins& bb, BlobData& bd, 
     if (bd.icwin64 != bd.cwin64)
       hipLaunchKernelGGL(col_window_kernel, grid, dim3(256), 0, st, g, bd.ng, cx, bd.nb, bd.rec, bd.iw64, bd.iw32,
                          bd.icwin64, bd.icwin32);
+    // every entry a list can hold (the count of this geometry is read on the device)
+    const size_t cap = col_list_capacity(bd.nb);
+    hipLaunchKernelGGL(col_ewin_kernel, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, st, cx,
+                       bd.col_start + col_groups(g, cx), bd.col_ent, bd.cwin64, bd.cwin32, bd.ewin64, bd.ewin32);
   }
   return hipGetLastError();
 }

@@ -63,6 +63,8 @@ struct ColArgs {
   const int* zst;
   const double* w64;  // weight windows (COL_WIN words per sorted blob)
   const float* w32;
+  const double* ew64;  // spreading: weights per list entry (COL_EWIN words)
+  const float* ew32;
 };
 
 // One entry of a patch list: (sorted blob slot, blob id, packed offsets, index
@@ -378,7 +380,7 @@ __global__ void __launch_bounds__(col_threads(NZ))
 
   T* sw = reinterpret_cast<T*>(smem);  // records (gx 0-3, gy 4-7, F 8-10, z 11, gz 12-19) or planes (gz 11-18)
   int* ssz = reinterpret_cast<int*>(sw + 19 * CAP);  // planes: stencil-start z
-  const T* W = col_w<T>(A.w64, A.w32);
+  const T* EW = col_w<T>(A.ew64, A.ew32);
   T acc[NROW];  // row (c CX + ii) CY + jj of this thread's z
 #pragma unroll
   for (int r = 0; r < NROW; ++r) acc[r] = T(0);
@@ -397,7 +399,11 @@ __global__ void __launch_bounds__(col_threads(NZ))
       }
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
-        col_win_load<T, CX>(W, e[u].x, e[u].z, gx[u], gy[u], gz[u]);
+        const int i = i0 + t + u * NT;
+        const T* ew = EW + (size_t)(c0 + (i < cnt ? i : 0)) * COL_EWIN;
+        col_vload<T, 4>(ew, gx[u]);
+        col_vload<T, 4>(ew + 4, gy[u]);
+        col_vload<T, 8>(ew + 8, gz[u]);
 #pragma unroll
         for (int c = 0; c < 3; ++c) f[u][c] = T(F[3 * (size_t)e[u].y + c]);
       }
@@ -758,7 +764,8 @@ namespace {
 
 ColArgs col_args(const GridGeom& g, const BlobData& bd, bool interp) {
   return ColArgs{g,          bd.ng,      g.nx / bd.col_cx, g.ny / COL_CY, bd.col_ent, bd.col_zst,
-                 interp ? bd.icwin64 : bd.cwin64, interp ? bd.icwin32 : bd.cwin32};
+                 interp ? bd.icwin64 : bd.cwin64, interp ? bd.icwin32 : bd.cwin32,
+                 interp ? nullptr : bd.ewin64, interp ? nullptr : bd.ewin32};
 }
 
 // points per thread of the z pass: the tuned request (8 or 16) or the default

@@ -114,6 +114,11 @@ struct BlobData {
   float* cwin32 = nullptr;
   double* icwin64 = nullptr;  // interpolation: cwin themselves, or in a slip channel from iw
   float* icwin32 = nullptr;
+  // spreading weights of every list entry, COL_EWIN words in list order (gx 4,
+  // gy 4, gz 8: the entry's patch cut out of cwin), so that spreading streams
+  // them with the entries instead of gathering them through the blob
+  double* ewin64 = nullptr;
+  float* ewin32 = nullptr;
   void* col_part = nullptr;    // interpolation partial sums: nb x COL_SLOTS_MAX x 3 (T)
   int col_fused[2] = {1, 1};   // per precision (0 FP64, 1 FP32): use the column path in the mobility product
 };
@@ -172,6 +177,7 @@ hipError_t interp(const GridGeom& g, const BlobBins& bb, const BlobData& bd, con
 constexpr int COL_CY = 4;          // patch y extent
 constexpr int COL_SLOTS_MAX = 15;  // partial sums per blob: up to 5 x 3 patches
 constexpr int COL_WIN = 32;        // window words per blob: x windows 0-11, y windows 12-23, z 24-31
+constexpr int COL_EWIN = 16;       // spreading weight words per list entry: x 0-3, y 4-7, z 8-15
 // patch x extent usable on this grid for both precisions (0: none)
 int col_patch_cx(const GridGeom& g, const BlobBins& bb, int ng);
 // the column kernels exist for word size tbytes at this nz, patch width cx and
