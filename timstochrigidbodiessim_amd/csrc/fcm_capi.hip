@@ -108,7 +108,7 @@ Sizes sizes_of(const fcm_config* c) {
   t += 3 * s.field64 + s.twbytes64;
   if (c->fp32) t += 3 * s.field32 + s.twbytes32;
   if (c->random_field) t += 6 * s.field64;
-  t += (size_t)nb * (3 * 8 + 4 + 4 + 16 + 3 * c->ng * 12 + 8 * 4) + (3 * nbins + 2) * 4;
+  t += (size_t)nb * (3 * 8 + 4 + 4 + 16 + 3 * c->ng * 12 + 8 * 4 + 16) + (3 * nbins + 2) * 4;
   if (c->slip_channel) t += (size_t)nb * 3 * c->ng * 12;  // truncated interpolation weights
   {
     GridGeom g2 = make_geom(c);
@@ -120,8 +120,8 @@ Sizes sizes_of(const fcm_config* c) {
     // the column buffers are sized for the narrower of the two patch widths
     if (col_patch_cx(g2, bb2, c->ng) > 0)
       t += (3 * (size_t)col_groups(g2, 2) + 1) * 4 + col_list_capacity(nb) * 20 + col_zst_words(g2, 2) * 4 +
-           (size_t)nb * COL_SLOTS_MAX * 3 * 8 + (size_t)nb * COL_WIN * 12 * (c->slip_channel ? 2 : 1) +
-           col_list_capacity(nb) * COL_EWIN * 12;
+           (size_t)nb * COL_SLOTS_MAX * 3 * 8 + (size_t)nb * COL_WIN * 8 * (c->slip_channel ? 2 : 1) +
+           col_list_capacity(nb) * COL_EWIN * 8;
   }
   const int n3b = 3 * c->max_blobs_per_body;
   t += (size_t)(3 * n3b + 2 * n3b * n3b + 36) * 8 + (size_t)c->max_bodies * (3 + 4 + 9 + 6) * 8 + (size_t)nb * 3 * 8;
@@ -489,6 +489,7 @@ int fcm_create(const fcm_config* cfg, fcm_handle* out) {
   if (e == hipSuccess) h->bd.rec = dalloc<int4>(h, nb, e);
   if (e == hipSuccess) h->bd.w64 = dalloc<double>(h, 3 * (size_t)nb * cfg->ng, e);
   if (e == hipSuccess) h->bd.w32 = dalloc<float>(h, 3 * (size_t)nb * cfg->ng, e);
+  if (e == hipSuccess) h->bd.woff = dalloc<float4>(h, (size_t)nb, e);
   if (cfg->slip_channel) {
     if (e == hipSuccess) h->bd.iw64 = dalloc<double>(h, 3 * (size_t)nb * cfg->ng, e);
     if (e == hipSuccess) h->bd.iw32 = dalloc<float>(h, 3 * (size_t)nb * cfg->ng, e);
@@ -514,16 +515,12 @@ int fcm_create(const fcm_config* cfg, fcm_handle* out) {
     if (e == hipSuccess) h->bd.col_ent = dalloc<int4>(h, col_list_capacity(nb), e);
     if (e == hipSuccess) h->bd.col_zst = dalloc<int>(h, col_zst_words(h->g, 2), e);
     if (e == hipSuccess) h->bd.cwin64 = dalloc<double>(h, (size_t)nb * COL_WIN, e);
-    if (e == hipSuccess) h->bd.cwin32 = dalloc<float>(h, (size_t)nb * COL_WIN, e);
     if (cfg->slip_channel) {
       if (e == hipSuccess) h->bd.icwin64 = dalloc<double>(h, (size_t)nb * COL_WIN, e);
-      if (e == hipSuccess) h->bd.icwin32 = dalloc<float>(h, (size_t)nb * COL_WIN, e);
     } else {
       h->bd.icwin64 = h->bd.cwin64;
-      h->bd.icwin32 = h->bd.cwin32;
     }
     if (e == hipSuccess) h->bd.ewin64 = dalloc<double>(h, col_list_capacity(nb) * COL_EWIN, e);
-    if (e == hipSuccess) h->bd.ewin32 = dalloc<float>(h, col_list_capacity(nb) * COL_EWIN, e);
     if (e == hipSuccess) h->bd.col_part = dalloc<double>(h, (size_t)nb * COL_SLOTS_MAX * 3, e);
   } else {
     h->bd.col_fused[0] = h->bd.col_fused[1] = 0;

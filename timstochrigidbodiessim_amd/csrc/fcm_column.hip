@@ -101,60 +101,54 @@ __global__ void __launch_bounds__(64) col_list_kernel(ColListArgs A, int pass) {
   }
 }
 
-// The weight windows of the column kernels (COL_WIN words per sorted blob).
+// The FP64 weight windows of the column kernels (COL_WIN words per sorted
+// blob; the FP32 kernels evaluate their weights from the stencil offsets).
 // A blob whose stencil starts at x node sx touches the patches floor(sx / cx)
 // + d, d = 0, 1, ...; in patch d its weights for the patch's columns ii are
 // g[cx d - sx mod cx + ii] (zero outside 0..ng-1), stored at words cx d + ii.
 // The same for y with COL_CY (words 12 + 4 d + jj), and the ng z weights at
 // 24..31.  A list entry then reads its weights as three aligned vectors of one
-// 128-byte line (FP32) instead of gathering them word by word.
-template <typename T>
-__device__ void col_window_one(const T* __restrict__ ws, int ng, int cx, int rx0, int ry0, T* __restrict__ out) {
+// 256-byte window instead of gathering them word by word.
+__device__ void col_window_one(const double* __restrict__ ws, int ng, int cx, int rx0, int ry0,
+                               double* __restrict__ out) {
   for (int q = 0; q < 12; ++q) {
     const int d = q / cx, ii = q - (q / cx) * cx;
     const int o = cx * d - rx0 + ii;
-    out[q] = (o >= 0 && o < ng) ? ws[o] : T(0);
+    out[q] = (o >= 0 && o < ng) ? ws[o] : 0.0;
   }
   for (int q = 0; q < 12; ++q) {
     const int d = q / COL_CY, jj = q - (q / COL_CY) * COL_CY;
     const int o = COL_CY * d - ry0 + jj;
-    out[12 + q] = (o >= 0 && o < ng) ? ws[ng + o] : T(0);
+    out[12 + q] = (o >= 0 && o < ng) ? ws[ng + o] : 0.0;
   }
-  for (int q = 0; q < 8; ++q) out[24 + q] = q < ng ? ws[2 * ng + q] : T(0);
+  for (int q = 0; q < 8; ++q) out[24 + q] = q < ng ? ws[2 * ng + q] : 0.0;
 }
 
 __global__ void __launch_bounds__(256) col_window_kernel(GridGeom g, int ng, int cx, int nb, const int4* __restrict__ rec,
-                                                          const double* __restrict__ w64, const float* __restrict__ w32,
-                                                          double* __restrict__ o64, float* __restrict__ o32) {
+                                                          const double* __restrict__ w64, double* __restrict__ o64) {
   const int s = blockIdx.x * 256 + threadIdx.x;
   if (s >= nb) return;
   const int omin = 1 - ng / 2;
   const int4 r = rec[s];
   const int rx0 = col_wrapi(r.x + omin, g.nx) % cx, ry0 = col_wrapi(r.y + omin, g.ny) % COL_CY;
-  col_window_one<double>(w64 + (size_t)s * 3 * ng, ng, cx, rx0, ry0, o64 + (size_t)s * COL_WIN);
-  col_window_one<float>(w32 + (size_t)s * 3 * ng, ng, cx, rx0, ry0, o32 + (size_t)s * COL_WIN);
+  col_window_one(w64 + (size_t)s * 3 * ng, ng, cx, rx0, ry0, o64 + (size_t)s * COL_WIN);
 }
 
-// The spreading weights of list entry i (COL_EWIN words), cut from its blob's
-// window: the x weights of its patch (words cx dx.., zero past cx), the y
-// weights (12 + 4 dy..) and the z weights; the values col_win_load reads.
-template <typename T>
-__device__ void col_ewin_one(const T* __restrict__ wn, int cx, int pk, T* __restrict__ out) {
-  const int dx = col_dx(pk), dy = col_dy(pk);
-  for (int q = 0; q < 4; ++q) out[q] = q < cx ? wn[cx * dx + q] : T(0);
-  for (int q = 0; q < 4; ++q) out[4 + q] = wn[12 + 4 * dy + q];
-  for (int q = 0; q < 8; ++q) out[8 + q] = wn[24 + q];
-}
-
+// The FP64 spreading weights of list entry i (COL_EWIN words), cut from its
+// blob's window: the x weights of its patch (words cx dx.., zero past cx), the
+// y weights (12 + 4 dy..) and the z weights; the values col_win_load reads.
 __global__ void __launch_bounds__(256) col_ewin_kernel(int cx, const int* __restrict__ total,
                                                         const int4* __restrict__ ent, const double* __restrict__ w64,
-                                                        const float* __restrict__ w32, double* __restrict__ o64,
-                                                        float* __restrict__ o32) {
+                                                        double* __restrict__ o64) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= *total) return;
   const int4 e = ent[i];
-  col_ewin_one<double>(w64 + (size_t)e.x * COL_WIN, cx, e.z, o64 + (size_t)i * COL_EWIN);
-  col_ewin_one<float>(w32 + (size_t)e.x * COL_WIN, cx, e.z, o32 + (size_t)i * COL_EWIN);
+  const double* wn = w64 + (size_t)e.x * COL_WIN;
+  double* out = o64 + (size_t)i * COL_EWIN;
+  const int dx = col_dx(e.z), dy = col_dy(e.z);
+  for (int q = 0; q < 4; ++q) out[q] = q < cx ? wn[cx * dx + q] : 0.0;
+  for (int q = 0; q < 4; ++q) out[4 + q] = wn[12 + 4 * dy + q];
+  for (int q = 0; q < 8; ++q) out[8 + q] = wn[24 + q];
 }
 
 static bool col_nz_supported(int nz) {
@@ -202,15 +196,14 @@ hipError_t build_col_lists(const GridGeom& g, const BlobBins& bb, BlobData& bd, 
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if (bd.nb > 0) {
     const dim3 grid((bd.nb + 255) / 256);
-    hipLaunchKernelGGL(col_window_kernel, grid, dim3(256), 0, st, g, bd.ng, cx, bd.nb, bd.rec, bd.w64, bd.w32,
-                       bd.cwin64, bd.cwin32);
+    hipLaunchKernelGGL(col_window_kernel, grid, dim3(256), 0, st, g, bd.ng, cx, bd.nb, bd.rec, bd.w64, bd.cwin64);
     if (bd.icwin64 != bd.cwin64)
-      hipLaunchKernelGGL(col_window_kernel, grid, dim3(256), 0, st, g, bd.ng, cx, bd.nb, bd.rec, bd.iw64, bd.iw32,
-                         bd.icwin64, bd.icwin32);
+      hipLaunchKernelGGL(col_window_kernel, grid, dim3(256), 0, st, g, bd.ng, cx, bd.nb, bd.rec, bd.iw64,
+                         bd.icwin64);
     // every entry a list can hold (the count of this geometry is read on the device)
     const size_t cap = col_list_capacity(bd.nb);
     hipLaunchKernelGGL(col_ewin_kernel, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, st, cx,
-                       bd.col_start + col_groups(g, cx), bd.col_ent, bd.cwin64, bd.cwin32, bd.ewin64, bd.ewin32);
+                       bd.col_start + col_groups(g, cx), bd.col_ent, bd.cwin64, bd.ewin64);
   }
   return hipGetLastError();
 }

@@ -52,19 +52,15 @@ __device__ __forceinline__ int col_xcd(int b, int n) {
   return (b % 8) * per + b / 8;
 }
 
-template <typename T> __device__ __forceinline__ const T* col_w(const double* w64, const float* w32);
-template <> __device__ __forceinline__ const double* col_w<double>(const double* w64, const float*) { return w64; }
-template <> __device__ __forceinline__ const float* col_w<float>(const double*, const float* w32) { return w32; }
-
 struct ColArgs {
   GridGeom g;
   int ng, npx, npy;
   const int4* ent;  // entry records of the patch lists (col_entry)
   const int* zst;
-  const double* w64;  // weight windows (COL_WIN words per sorted blob)
-  const float* w32;
-  const double* ew64;  // spreading: weights per list entry (COL_EWIN words)
-  const float* ew32;
+  const double* w64;   // FP64 weight windows (COL_WIN words per sorted blob)
+  const double* ew64;  // FP64 spreading: weights per list entry (COL_EWIN words)
+  const float4* woff;  // FP32: offsets of stencil node 0 per sorted blob
+  float wa, wc, wdx;   // FP32 weight wa exp2(wc xx^2), node spacing
 };
 
 // One entry of a patch list: (sorted blob slot, blob id, packed offsets, index
@@ -110,8 +106,8 @@ __device__ __forceinline__ void col_vstore(T* p, const T* v) {
     reinterpret_cast<float4*>(p)[i] = make_float4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
 }
 
-// x, y and z weights of a list entry over its patch, from the blob's windows
-// (zero outside the stencil)
+// x, y and z weights of a list entry over its patch, from the blob's FP64
+// windows (zero outside the stencil)
 template <typename T, int CX>
 __device__ __forceinline__ void col_win_load(const T* __restrict__ win, int slot, int pk, T* gx, T* gy, T* gz) {
   const T* wn = win + (size_t)slot * COL_WIN;
@@ -120,6 +116,35 @@ __device__ __forceinline__ void col_win_load(const T* __restrict__ win, int slot
   for (int ii = CX; ii < 4; ++ii) gx[ii] = T(0);
   col_vload<T, 4>(wn + 12 + 4 * col_dy(pk), gy);
   col_vload<T, 8>(wn + 24, gz);
+}
+
+// FP32 weights of one list entry for its patch, evaluated from the blob's
+// stencil offsets: x for the patch's CX columns (stencil node ii - rx), y for
+// its 4 rows (node jj - ry; in a slip channel interpolation drops the rows at
+// and above the upper wall, as the FP64 interpolation weights do), z for the
+// ng nodes.  The same expression as the FP64 weights, in FP32 from the offset
+// rounded to FP32 (relative error a few 1e-7).
+template <int CX>
+__device__ __forceinline__ void col_win_eval(const ColArgs& A, int slot, int pk, bool wall, int y0, float* gx,
+                                             float* gy, float* gz) {
+  const float4 o = A.woff[slot];
+  const int rx = col_rx(pk), ry = col_ry(pk), ng = A.ng;
+  auto w = [&](float x0, int k) {
+    const float xx = fmaf((float)k, A.wdx, x0);
+    return A.wa * exp2f(A.wc * (xx * xx));
+  };
+#pragma unroll
+  for (int ii = 0; ii < 4; ++ii) {
+    const int k = ii - rx;
+    gx[ii] = (ii < CX && k >= 0 && k < ng) ? w(o.x, k) : 0.f;
+  }
+#pragma unroll
+  for (int jj = 0; jj < COL_CY; ++jj) {
+    const int k = jj - ry;
+    gy[jj] = (k >= 0 && k < ng && !(wall && y0 + jj >= A.g.ny / 2)) ? w(o.y, k) : 0.f;
+  }
+#pragma unroll
+  for (int q = 0; q < 8; ++q) gz[q] = q < ng ? w(o.z, q) : 0.f;
 }
 
 // ---------------------------------------------------------------------------
@@ -380,7 +405,7 @@ __global__ void __launch_bounds__(col_threads(NZ))
 
   T* sw = reinterpret_cast<T*>(smem);  // records (gx 0-3, gy 4-7, F 8-10, z 11, gz 12-19) or planes (gz 11-18)
   int* ssz = reinterpret_cast<int*>(sw + 19 * CAP);  // planes: stencil-start z
-  const T* EW = col_w<T>(A.ew64, A.ew32);
+  const double* EW = A.ew64;
   T acc[NROW];  // row (c CX + ii) CY + jj of this thread's z
 #pragma unroll
   for (int r = 0; r < NROW; ++r) acc[r] = T(0);
@@ -399,11 +424,16 @@ __global__ void __launch_bounds__(col_threads(NZ))
       }
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
-        const int i = i0 + t + u * NT;
-        const T* ew = EW + (size_t)(c0 + (i < cnt ? i : 0)) * COL_EWIN;
-        col_vload<T, 4>(ew, gx[u]);
-        col_vload<T, 4>(ew + 4, gy[u]);
-        col_vload<T, 8>(ew + 8, gz[u]);
+        if constexpr (sizeof(T) == 8) {
+          // the entry's own copy of its weights, read in list order
+          const int i = i0 + t + u * NT;
+          const T* ew = EW + (size_t)(c0 + (i < cnt ? i : 0)) * COL_EWIN;
+          col_vload<T, 4>(ew, gx[u]);
+          col_vload<T, 4>(ew + 4, gy[u]);
+          col_vload<T, 8>(ew + 8, gz[u]);
+        } else {
+          col_win_eval<CX>(A, e[u].x, e[u].z, false, y0, gx[u], gy[u], gz[u]);
+        }
 #pragma unroll
         for (int c = 0; c < 3; ++c) f[u][c] = T(F[3 * (size_t)e[u].y + c]);
       }
@@ -603,7 +633,7 @@ __global__ void __launch_bounds__(col_threads(NZ), col_interp_waves(NZ, (int)siz
   const int partner = (t & 63) - tcol + ((TC - tcol) & (TC - 1));  // lane of thread TC - tcol of this row
   const int* zs = A.zst + (size_t)p * (NZ + 1);
   const int L0 = zs[0], L1 = zs[NZ];
-  const T* W = col_w<T>(A.w64, A.w32);
+  const double* W = A.w64;
   // C2R input of round qg's rows, straight from HBM: each point k needs X[k]
   // and X[M - k] (imaginary parts of k = 0 and M ignored)
   V xk[P], xn, xm_nopair[PAIRS ? 1 : P];
@@ -631,7 +661,8 @@ __global__ void __launch_bounds__(col_threads(NZ), col_interp_waves(NZ, (int)siz
       const int i = base + k * NT + t;
       val[k] = i < L1;
       const int4 e = val[k] ? A.ent[i] : make_int4(0, 0, 8 | (8 << 5), 0);
-      col_win_load<T, CX>(W, e.x, e.z, gx[k], gy[k], gz[k]);
+      if constexpr (sizeof(T) == 8) col_win_load<double, CX>(W, e.x, e.z, gx[k], gy[k], gz[k]);
+      else col_win_eval<CX>(A, e.x, e.z, g.channel != 0, y0, gx[k], gy[k], gz[k]);
       z0[k] = col_sz(e.z);
       oid[k] = e.w;
 #pragma unroll
@@ -764,8 +795,8 @@ namespace {
 
 ColArgs col_args(const GridGeom& g, const BlobData& bd, bool interp) {
   return ColArgs{g,          bd.ng,      g.nx / bd.col_cx, g.ny / COL_CY, bd.col_ent, bd.col_zst,
-                 interp ? bd.icwin64 : bd.cwin64, interp ? bd.icwin32 : bd.cwin32,
-                 interp ? nullptr : bd.ewin64, interp ? nullptr : bd.ewin32};
+                 interp ? bd.icwin64 : bd.cwin64, interp ? nullptr : bd.ewin64,
+                 bd.woff, bd.wa32, bd.wc32, (float)g.dx};
 }
 
 // points per thread of the z pass: the tuned request (8 or 16) or the default

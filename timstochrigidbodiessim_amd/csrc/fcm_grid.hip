@@ -91,7 +91,8 @@ __global__ void bin_sort_kernel(const int* start, int nbins, int* sorted) {
 }
 
 __global__ void record_kernel(const double* __restrict__ Y, const int* __restrict__ sorted, int nb, GridGeom g,
-                              int ng, double a, int4* rec, double* w64, float* w32, double* iw64, float* iw32) {
+                              int ng, double a, int4* rec, double* w64, float* w32, double* iw64, float* iw32,
+                              float4* woff) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nb) return;
   const int id = sorted[s];
@@ -100,10 +101,12 @@ __global__ void record_kernel(const double* __restrict__ Y, const int* __restric
   const double anorm2 = 2.0 * a * a / FCM_PI;
   const int n[3] = {g.nx, g.ny, g.nz};
   int wa[3];
+  float off[3];
   for (int d = 0; d < 3; ++d) {
     const double y = Y[3 * id + d];
     const int xc = (int)(y / g.dx);
     wa[d] = wrapi(xc, n[d]);
+    off[d] = (float)((xc - ngdh + 1) * g.dx - y);
     for (int i = 0; i < ng; ++i) {
       const int xg = xc - ngdh + (i + 1);
       const double xx = xg * g.dx - y;
@@ -120,6 +123,7 @@ __global__ void record_kernel(const double* __restrict__ Y, const int* __restric
     }
   }
   rec[s] = make_int4(wa[0], wa[1], wa[2], id);
+  if (woff) woff[s] = make_float4(off[0], off[1], off[2], 0.f);
 }
 
 hipError_t build_tile_lists(const GridGeom& g, const BlobBins& bb, BlobData& bd, hipStream_t st);
@@ -146,6 +150,9 @@ hipError_t fill_ints(int* p, int n, int v, hipStream_t st) {
 
 hipError_t setup_gaussians(const GridGeom& g, const BlobBins& bb, BlobData& bd, double a, hipStream_t st) {
   const int nb = bd.nb, nbins = bb.nbins();
+  // anorm exp(-xx^2 / anorm2) = wa exp2(wc xx^2), anorm = 1 / (sqrt(2) a), anorm2 = 2 a^2 / pi
+  bd.wa32 = (float)(1.0 / (sqrt(2.0) * a));
+  bd.wc32 = (float)(-FCM_PI / (2.0 * a * a) / log(2.0));
   hipLaunchKernelGGL(fill_int_kernel, dim3((nbins + 255) / 256), dim3(256), 0, st, bd.bin_count, nbins, 0);
   if (nb > 0)
     hipLaunchKernelGGL(bin_kernel, dim3((nb + 255) / 256), dim3(256), 0, st, bd.Y, nb, g, bb, bd.bin_of, bd.bin_count);
@@ -155,7 +162,7 @@ hipError_t setup_gaussians(const GridGeom& g, const BlobBins& bb, BlobData& bd, 
                        bd.bin_cursor, bd.sorted);
     hipLaunchKernelGGL(bin_sort_kernel, dim3((nbins + 255) / 256), dim3(256), 0, st, bd.bin_start, nbins, bd.sorted);
     hipLaunchKernelGGL(record_kernel, dim3((nb + 255) / 256), dim3(256), 0, st, bd.Y, bd.sorted, nb, g, bd.ng, a,
-                       bd.rec, bd.w64, bd.w32, bd.iw64, bd.iw32);
+                       bd.rec, bd.w64, bd.w32, bd.iw64, bd.iw32, bd.woff);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;

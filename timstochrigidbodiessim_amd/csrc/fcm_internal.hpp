@@ -83,6 +83,11 @@ struct BlobData {
   int4* anc = nullptr;        // nb, sorted order: unwrapped anchors
   double* w64 = nullptr;      // nb x 3 x ng, sorted order
   float* w32 = nullptr;       // nb x 3 x ng, sorted order
+  // FP32 column kernels: the weights are evaluated from the offset of stencil
+  // node 0 (node position - blob position, x y z; sorted order) as
+  // wa32 exp2(wc32 xx^2), xx = offset + k dx
+  float4* woff = nullptr;
+  float wa32 = 0.f, wc32 = 0.f;
   // interpolation weights: w64/w32 themselves, or in a slip channel a copy whose
   // y weights are zero on rows >= ny/2 (truncated Gaussians)
   double* iw64 = nullptr;
@@ -107,18 +112,16 @@ struct BlobData {
   int* col_list = nullptr;     // sorted blob slots, COL_SLOTS_MAX per blob at most
   int4* col_ent = nullptr;     // entry records in list order (slot, id, packed offsets, partial slot)
   int* col_zst = nullptr;      // npatch x (nz + 1): first entry with stencil-start z >= k
-  // weight windows of the column kernels, COL_WIN words per sorted blob: for
-  // each patch a blob's stencil can touch in x (y), the patch's cx (4) x (y)
-  // weights, then the ng z weights, zero-padded (built from w / iw per patch width)
+  // FP64 weight windows of the column kernels, COL_WIN words per sorted blob:
+  // for each patch a blob's stencil can touch in x (y), the patch's cx (4) x (y)
+  // weights, then the ng z weights, zero-padded (built from w64 / iw64 per
+  // patch width; the FP32 kernels evaluate their weights from woff instead)
   double* cwin64 = nullptr;
-  float* cwin32 = nullptr;
-  double* icwin64 = nullptr;  // interpolation: cwin themselves, or in a slip channel from iw
-  float* icwin32 = nullptr;
-  // spreading weights of every list entry, COL_EWIN words in list order (gx 4,
-  // gy 4, gz 8: the entry's patch cut out of cwin), so that spreading streams
-  // them with the entries instead of gathering them through the blob
+  double* icwin64 = nullptr;  // interpolation: cwin64 itself, or in a slip channel from iw64
+  // FP64 spreading weights of every list entry, COL_EWIN words in list order
+  // (gx 4, gy 4, gz 8: the entry's patch cut out of cwin64), so that spreading
+  // streams them with the entries instead of gathering them through the blob
   double* ewin64 = nullptr;
-  float* ewin32 = nullptr;
   void* col_part = nullptr;    // interpolation partial sums: nb x COL_SLOTS_MAX x 3 (T)
   int col_fused[2] = {1, 1};   // per precision (0 FP64, 1 FP32): use the column path in the mobility product
 };
@@ -177,7 +180,7 @@ hipError_t interp(const GridGeom& g, const BlobBins& bb, const BlobData& bd, con
 constexpr int COL_CY = 4;          // patch y extent
 constexpr int COL_SLOTS_MAX = 15;  // partial sums per blob: up to 5 x 3 patches
 constexpr int COL_WIN = 32;        // window words per blob: x windows 0-11, y windows 12-23, z 24-31
-constexpr int COL_EWIN = 16;       // spreading weight words per list entry: x 0-3, y 4-7, z 8-15
+constexpr int COL_EWIN = 16;       // FP64 spreading weights per list entry: x 0-3, y 4-7, z 8-15
 // patch x extent usable on this grid for both precisions (0: none)
 int col_patch_cx(const GridGeom& g, const BlobBins& bb, int ng);
 // the column kernels exist for word size tbytes at this nz, patch width cx and
