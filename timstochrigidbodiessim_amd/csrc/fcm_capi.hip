@@ -273,7 +273,7 @@ int saddle_dev(fcm_handle h, const double* in, double* out, int precision, hipSt
 // steps queued after convergence do nothing.
 int launch_arnoldi_step(fcm_handle h, int precision, hipStream_t st) {
   const size_t n = h->nsys(), n3 = h->n3();
-  const int* gate = h->gm.ictl + GM_STATUS;
+  const int* gate = h->gm.ictl + GM_GATE;
   BodyData b = h->body();
   b.gate = gate;
   b.kcol = h->gm.ictl + GM_K;

@@ -207,6 +207,7 @@ __device__ __forceinline__ void givens_store(const GmDev& G, int k, const double
 
 __device__ __forceinline__ void set_done(const GmDev& G, int status, int ncols, double rel) {
   G.ictl[GM_STATUS] = 1 + status;
+  G.ictl[GM_GATE] = 1;
   G.ictl[GM_NCOLS] = ncols;
   G.dctl[4] = rel;
 }
@@ -228,8 +229,9 @@ __global__ void __launch_bounds__(GT) gm_start_kernel(GmDev G, double tol, int m
     G.ictl[GM_FIN] = 0;
     G.ictl[GM_MAXIT] = max_iter;
     G.ictl[GM_NCOLS] = 0;
-    G.ictl[GM_ARR_CTL3] = G.ictl[GM_ARR_FIN] = 0;
+    G.ictl[GM_ARR_FIN] = 0;
     G.ictl[GM_STATUS] = nrm == 0.0 ? 1 + 2 /* zero rhs */ : 0;
+    G.ictl[GM_GATE] = nrm == 0.0 ? 1 : 0;
     G.beta[0] = nrm;
   }
 }
@@ -244,18 +246,18 @@ __global__ void gm_q0_kernel(GmDev G, const double* __restrict__ b, TQ* __restri
 }
 
 // ---- basis passes ------------------------------------------------------------
-// mode 0 (main): columns 0..k against v1 = Q col k (if provisional) and v2 = w
-// mode 1 (finalise): columns 0..k+1 against v1 = Q col k+1 (only if fin)
+// main step: columns 0..k against v1 = Q col k (if provisional) and v2 = w
+// finalising step (FIN = 1): columns 0..k+1 against v1 = Q col k+1
 // Grid (row blocks, DOT_CG): each block takes every DOT_CG-th group of 8
 // columns, so that enough waves stream the basis at once.
 static constexpr int DOT_CG = 4;
 
 template <int MAP, typename TQ>
 __global__ void __launch_bounds__(KRT) gm_dots_kernel(GmDev G, const TQ* __restrict__ Q, size_t ldq,
-                                                      const double* __restrict__ wbuf, size_t n, int mode,
+                                                      const double* __restrict__ wbuf, size_t n,
                                                       double* __restrict__ partial, int nblk) {
-  const int status = G.ictl[GM_STATUS];
-  if (status != 0 || (mode == 1 && G.ictl[GM_FIN] == 0)) return;
+  if (G.ictl[GM_STATUS] != 0) return;
+  const int mode = G.ictl[GM_FIN] == 1 ? 1 : 0;
   const int k = G.ictl[GM_K];
   const bool prov = G.ictl[GM_PROV] != 0;
   const int ncols = mode == 0 ? k + 1 : k + 2;
@@ -351,28 +353,27 @@ __device__ bool last_block(const GmDev& G, int word) {
   return true;
 }
 
-// end of an iteration: the next one works on column k + 1
-__device__ __forceinline__ void end_iteration(const GmDev& G) {
-  G.ictl[GM_PROV] = G.ictl[GM_FIN] ? 0 : 1;
-  G.ictl[GM_FIN] = 0;
+// end of an iteration: the next one works on column k + 1, provisional after
+// a main step, final after a finalising step (whose mat-vec gate reopens)
+__device__ __forceinline__ void end_iteration(const GmDev& G, bool finalised) {
+  G.ictl[GM_PROV] = finalised ? 0 : 1;
   G.ictl[GM_K] += 1;
+  if (finalised) {
+    G.ictl[GM_FIN] = 2;
+    G.ictl[GM_GATE] = 0;
+  }
 }
 
-// mode 0: q = (Q col k - sum_{i<m} s_i Q_i) / nu written back (if m >= 0 and
-// provisional), w <- w - sum_{i<mt} t_i Q_i - pk q, partial ||w||^2.
-// mode 1: Q col k+1 <- (Q col k+1 - sum_{i<=k} s_i Q_i) / nu (if fin).
+// main step: q = (Q col k - sum_{i<m} s_i Q_i) / nu written back (if m >= 0
+// and provisional), w <- w - sum_{i<mt} t_i Q_i - pk q, partial ||w||^2.
+// finalising step: Q col k+1 <- (Q col k+1 - sum_{i<=k} s_i Q_i) / nu, and the
+// last block ends the iteration.
 template <int MAP, typename TQ>
 __global__ void __launch_bounds__(KRT) gm_update_kernel(GmDev G, TQ* __restrict__ Q, size_t ldq,
-                                                        double* __restrict__ wbuf, size_t n, int mode,
+                                                        double* __restrict__ wbuf, size_t n,
                                                         double* __restrict__ partial) {
-  const int status = G.ictl[GM_STATUS];
-  if (status != 0) return;
-  if (mode == 1 && G.ictl[GM_FIN] == 0) {
-    // no finalisation: the iteration ends here (no block of this launch reads
-    // the words end_iteration changes beyond FIN, which stays 0)
-    if (blockIdx.x == 0 && threadIdx.x == 0) end_iteration(G);
-    return;
-  }
+  if (G.ictl[GM_STATUS] != 0) return;
+  const int mode = G.ictl[GM_FIN] == 1 ? 1 : 0;
   const int k = G.ictl[GM_K];
   const int m = G.ictl[GM_M], mt = G.ictl[GM_MT];
   TQ* qt = mode == 0 ? (m > 0 ? Q + (size_t)k * ldq : nullptr) : Q + (size_t)(k + 1) * ldq;
@@ -441,15 +442,19 @@ __global__ void __launch_bounds__(KRT) gm_update_kernel(GmDev G, TQ* __restrict_
     if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
   } else if (last_block(G, GM_ARR_FIN)) {
     // the finalised vector is written: the next iteration works on column k + 1
-    if (threadIdx.x == 0) end_iteration(G);
+    if (threadIdx.x == 0) end_iteration(G, true);
   }
 }
 
 // ---- control steps (one workgroup of GT threads) ----------------------------------
-// Step 1 is a kernel of its own; step 3 runs in the last block of the
-// reduction before it (a device-scope fence in every block of a launch costs
-// more than the launch it saves, so only the step that runs once per solve
-// is folded in).
+// A step of the Arnoldi sequence is: the mat-vec (gated), projections,
+// reduction, control step 1 (or 3), H s, update, control step 2, scaling.
+// When control step 2 predicts convergence, the next step is a finalising
+// step: its mat-vec kernels return at once (GM_GATE), and its projections,
+// reduction, control and update kernels finalise the new vector (control
+// step 3 in place of step 1); control step 2 and the scaling return at once.
+// So the finalisation, needed about once per solve, costs no launches in the
+// other steps.
 struct CtlShared {
   double sh[8];
   double col[GCAP], sCS[GCAP], sSN[GCAP];
@@ -536,7 +541,7 @@ __device__ void ctl1_step(const GmDev& G, CtlShared& L) {
 // A P^-1 q_k on the final basis, corrected for the provisional input.  One
 // wavefront per row of H (rows are contiguous); grid = max_krylov + 1 blocks.
 __global__ void __launch_bounds__(64) gm_hs_kernel(GmDev G) {
-  if (G.ictl[GM_STATUS] != 0 || G.ictl[GM_HS] == 0) return;
+  if (G.ictl[GM_STATUS] != 0 || G.ictl[GM_HS] == 0 || G.ictl[GM_FIN] == 1) return;
   const int k = G.ictl[GM_K];
   const int l = blockIdx.x;
   if (l > k) return;
@@ -552,25 +557,35 @@ __global__ void __launch_bounds__(64) gm_hs_kernel(GmDev G) {
   }
 }
 
-// rows of the main dots pass: 2 (k+1)
+// rows of the projections: 2 (k+1) in a main step, k + 2 in a finalising one
 __global__ void __launch_bounds__(256) gm_reduce_kernel(GmDev G, const double* __restrict__ partial, int nblk) {
   if (G.ictl[GM_STATUS] != 0) return;
   __shared__ double s[256];
   const int k = G.ictl[GM_K];
-  reduce_rows(G, partial, nblk, 2 * (k + 1), s);
+  reduce_rows(G, partial, nblk, G.ictl[GM_FIN] == 1 ? k + 2 : 2 * (k + 1), s);
 }
 
-// control step 1 (one workgroup)
+__device__ void ctl3_step(const GmDev& G, CtlShared& L);
+
+// control step 1, or step 3 in a finalising step (one workgroup); the first
+// main step after a finalising one clears its mark
 __global__ void __launch_bounds__(GT) gm_ctl1_kernel(GmDev G) {
   if (G.ictl[GM_STATUS] != 0) return;
   __shared__ CtlShared L;
+  const int fin = G.ictl[GM_FIN];
+  if (fin == 1) {
+    ctl3_step(G, L);
+    return;
+  }
+  __syncthreads();  // every thread has read the mark
+  if (fin == 2 && threadIdx.x == 0) G.ictl[GM_FIN] = 0;
   ctl1_step(G, L);
 }
 
 // after the main update: normalisation, breakdown, predicted convergence
 // (one block; ||w||^2 from the update pass's block partials, fixed order)
 __global__ void __launch_bounds__(GT) gm_ctl2_kernel(GmDev G, const double* __restrict__ partial, int nblk) {
-  if (G.ictl[GM_STATUS] != 0) return;
+  if (G.ictl[GM_STATUS] != 0 || G.ictl[GM_FIN] != 0) return;
   __shared__ double col[GCAP], sCS[GCAP], sSN[GCAP];
   __shared__ int s_brk;
   const int k = G.ictl[GM_K];
@@ -599,25 +614,33 @@ __global__ void __launch_bounds__(GT) gm_ctl2_kernel(GmDev G, const double* __re
       G.dctl[5] = 1.0 / etar;
       col[k + 1] = eta;
       const double est = givens_sweep(G, k, col, sCS, sSN, false);
-      G.ictl[GM_FIN] = (est < G.dctl[0] || k + 1 == G.ictl[GM_MAXIT]) ? 1 : 0;
+      if (est < G.dctl[0] || k + 1 == G.ictl[GM_MAXIT]) {
+        // the next step finalises the new vector (its mat-vec is gated off)
+        G.ictl[GM_FIN] = 1;
+        G.ictl[GM_GATE] = 1;
+      } else {
+        end_iteration(G, false);
+      }
     }
   }
   __syncthreads();
   if (s_brk) givens_store(G, k, col);
 }
 
-// Q col k+1 = w / ||w||
+// Q col k+1 = w / ||w|| (control step 2 has already advanced K unless the
+// next step finalises)
 template <typename TQ>
 __global__ void gm_scale_kernel(GmDev G, const double* __restrict__ wbuf, TQ* __restrict__ Q, size_t ldq, size_t n) {
-  if (G.ictl[GM_STATUS] != 0) return;
-  const int k = G.ictl[GM_K];
+  const int fin = G.ictl[GM_FIN];
+  if (G.ictl[GM_STATUS] != 0 || fin == 2) return;
+  const int col = fin == 1 ? G.ictl[GM_K] + 1 : G.ictl[GM_K];
   const double s = G.dctl[5];
-  TQ* q = Q + (size_t)(k + 1) * ldq;
+  TQ* q = Q + (size_t)col * ldq;
   for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (size_t)gridDim.x * blockDim.x)
     q[r] = (TQ)(wbuf[r] * s);
 }
 
-// after the finalising dots pass
+// control step 3, after the projections of a finalising step
 __device__ void ctl3_step(const GmDev& G, CtlShared& L) {
   double* sh = L.sh;
   double *col = L.col, *sCS = L.sCS, *sSN = L.sSN;
@@ -662,17 +685,6 @@ __device__ void ctl3_step(const GmDev& G, CtlShared& L) {
     G.ictl[GM_M] = k + 1;
     G.ictl[GM_MT] = 0;
   }
-}
-
-// the reduction of the finalising dots pass, then (last block) control step 3
-__global__ void __launch_bounds__(GT) gm_reduce_ctl3_kernel(GmDev G, const double* __restrict__ partial, int nblk) {
-  if (G.ictl[GM_STATUS] != 0 || G.ictl[GM_FIN] == 0) return;
-  __shared__ double s[GT];
-  __shared__ CtlShared L;
-  const int k = G.ictl[GM_K];
-  reduce_rows(G, partial, nblk, k + 2, s);
-  if (!last_block(G, GM_ARR_CTL3)) return;
-  ctl3_step(G, L);
 }
 
 // back substitution (gmres_solver.cpp:260-276) into G.y; one wavefront
@@ -751,20 +763,16 @@ hipError_t gm_launch_start(const GmDev& G, double tol, int max_iter, const doubl
 template <int MAP, typename TQ>
 static void gm_launch_basis_t(const GmDev& G, TQ* Q, size_t ldq, double* wbuf, size_t n, double* partial, int nblk,
                               hipStream_t st) {
-  // main dots, reduce, control 1, H s, update, norm reduce + control 2, scale
-  hipLaunchKernelGGL((gm_dots_kernel<MAP, TQ>), dim3(nblk, DOT_CG), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 0, partial,
+  // projections, reduce, control 1 (3), H s, update, norm reduce + control 2,
+  // scale; each kernel reads from GM_FIN whether the step is a finalising one
+  hipLaunchKernelGGL((gm_dots_kernel<MAP, TQ>), dim3(nblk, DOT_CG), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, partial,
                      nblk);
   hipLaunchKernelGGL(gm_reduce_kernel, dim3(256), dim3(256), 0, st, G, partial, nblk);
   hipLaunchKernelGGL(gm_ctl1_kernel, dim3(1), dim3(GT), 0, st, G);
   hipLaunchKernelGGL(gm_hs_kernel, dim3(G.M + 1), dim3(64), 0, st, G);
-  hipLaunchKernelGGL((gm_update_kernel<MAP, TQ>), dim3(nblk), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 0, partial);
+  hipLaunchKernelGGL((gm_update_kernel<MAP, TQ>), dim3(nblk), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, partial);
   hipLaunchKernelGGL(gm_ctl2_kernel, dim3(1), dim3(GT), 0, st, G, partial, nblk);
   hipLaunchKernelGGL(gm_scale_kernel<TQ>, dim3(1024), dim3(256), 0, st, G, wbuf, Q, ldq, n);
-  // finalisation of the new vector when convergence is predicted
-  hipLaunchKernelGGL((gm_dots_kernel<MAP, TQ>), dim3(nblk, DOT_CG), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 1, partial,
-                     nblk);
-  hipLaunchKernelGGL(gm_reduce_ctl3_kernel, dim3(256), dim3(GT), 0, st, G, partial, nblk);
-  hipLaunchKernelGGL((gm_update_kernel<MAP, TQ>), dim3(nblk), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, 1, partial);
 }
 
 // 16-byte accesses on long vectors (two FP64 or four FP32 words), single

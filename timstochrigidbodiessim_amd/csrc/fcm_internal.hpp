@@ -327,12 +327,15 @@ hipError_t rhs_assemble(const double* Vrand, const double* Vshear, const double*
 // ---------------------------------------------------------------------------
 // device-resident GMRES control (fcm_gmres.hip)
 // ---------------------------------------------------------------------------
-// GM_ARR_CTL3, GM_ARR_FIN: arrival counters of the two multi-block kernels of
-// the finalising step whose last block runs control work (zero between
-// launches; each last block resets its own)
+// GM_FIN: 0 while iterating, 1 from the step that predicts convergence until
+// the next step (a finalising step) has finalised the new vector, 2 for the
+// rest of that step.  GM_GATE: nonzero when the mat-vec kernels of a step must
+// not run (the solve has stopped, or the step is a finalising step).
+// GM_ARR_FIN: arrival counter of the finalising update, whose last block ends
+// the finalising step (zero between launches; the last block resets it).
 enum {
   GM_K = 0, GM_PROV = 1, GM_STATUS = 2, GM_NCOLS = 3, GM_FIN = 4, GM_MAXIT = 5, GM_M = 6, GM_MT = 7, GM_HS = 8,
-  GM_ARR_CTL3 = 9, GM_ARR_FIN = 10, GM_NINT = 11
+  GM_GATE = 9, GM_ARR_FIN = 10, GM_NINT = 11
 };
 
 struct GmDev {
