@@ -364,8 +364,26 @@ __device__ __forceinline__ void end_iteration(const GmDev& G, bool finalised) {
   }
 }
 
+// hnew_l = (proj_l - sum_i H(l,i) s_i) / nu: the projection of A P^-1 q_k on
+// q_l of the final basis, corrected for the provisional input (one wavefront,
+// H rows are contiguous)
+__device__ void hs_row(const GmDev& G, int l, int k, int lane) {
+  const double* s = G.red;
+  const double* t = G.red + (k + 1);
+  const double* row = Hp(G, l, 0);
+  double c = 0.0;
+  for (int i = (l > 0 ? l - 1 : 0) + lane; i < k; i += 64) c += row[i] * s[i];
+  c = wave_sum(c);
+  if (lane == 0) {
+    const double proj = l < k ? t[l] : G.coef[2 * k + 1];
+    G.hnew[l] = (proj - c) / G.dctl[3];
+  }
+}
+
 // main step: q = (Q col k - sum_{i<m} s_i Q_i) / nu written back (if m >= 0
-// and provisional), w <- w - sum_{i<mt} t_i Q_i - pk q, partial ||w||^2.
+// and provisional), w <- w - sum_{i<mt} t_i Q_i - pk q, partial ||w||^2; after
+// a provisional input the wavefronts also form hnew_0..k (one row each, for
+// control step 2).
 // finalising step: Q col k+1 <- (Q col k+1 - sum_{i<=k} s_i Q_i) / nu, and the
 // last block ends the iteration.
 template <int MAP, typename TQ>
@@ -376,6 +394,11 @@ __global__ void __launch_bounds__(KRT) gm_update_kernel(GmDev G, TQ* __restrict_
   const int mode = G.ictl[GM_FIN] == 1 ? 1 : 0;
   const int k = G.ictl[GM_K];
   const int m = G.ictl[GM_M], mt = G.ictl[GM_MT];
+  if (mode == 0 && G.ictl[GM_HS] != 0) {
+    constexpr int WPB = KRT / 64;
+    for (int l = (int)blockIdx.x * WPB + (int)(threadIdx.x >> 6); l <= k; l += (int)gridDim.x * WPB)
+      hs_row(G, l, k, threadIdx.x & 63);
+  }
   TQ* qt = mode == 0 ? (m > 0 ? Q + (size_t)k * ldq : nullptr) : Q + (size_t)(k + 1) * ldq;
   double* w = mode == 0 ? wbuf : nullptr;
   __shared__ double cs[2048 + 4];
@@ -448,7 +471,8 @@ __global__ void __launch_bounds__(KRT) gm_update_kernel(GmDev G, TQ* __restrict_
 
 // ---- control steps (one workgroup of GT threads) ----------------------------------
 // A step of the Arnoldi sequence is: the mat-vec (gated), projections,
-// reduction, control step 1 (or 3), H s, update, control step 2, scaling.
+// reduction, control step 1 (or 3), update (which also forms H s), control
+// step 2, scaling.
 // When control step 2 predicts convergence, the next step is a finalising
 // step: its mat-vec kernels return at once (GM_GATE), and its projections,
 // reduction, control and update kernels finalise the new vector (control
@@ -507,7 +531,7 @@ __device__ void ctl1_step(const GmDev& G, CtlShared& L) {
         s_done = 1;
       }
       G.dctl[3] = nu;
-      // the projections of A P^-1 q_k on q_0..q_k are finished by gm_hs_kernel
+      // the projections of A P^-1 q_k on q_0..q_k are finished by the update (hs_row)
       G.ictl[GM_HS] = 1;
       G.ictl[GM_M] = k;
       G.ictl[GM_MT] = k;
@@ -534,26 +558,6 @@ __device__ void ctl1_step(const GmDev& G, CtlShared& L) {
       G.ictl[GM_HS] = 0;
       G.dctl[3] = 1.0;
     }
-  }
-}
-
-// hnew_l = (proj_l - sum_i H(l,i) s_i) / nu for l = 0..k: the projections of
-// A P^-1 q_k on the final basis, corrected for the provisional input.  One
-// wavefront per row of H (rows are contiguous); grid = max_krylov + 1 blocks.
-__global__ void __launch_bounds__(64) gm_hs_kernel(GmDev G) {
-  if (G.ictl[GM_STATUS] != 0 || G.ictl[GM_HS] == 0 || G.ictl[GM_FIN] == 1) return;
-  const int k = G.ictl[GM_K];
-  const int l = blockIdx.x;
-  if (l > k) return;
-  const double* s = G.red;
-  const double* t = G.red + (k + 1);
-  const double* row = Hp(G, l, 0);
-  double c = 0.0;
-  for (int i = (l > 0 ? l - 1 : 0) + (int)threadIdx.x; i < k; i += 64) c += row[i] * s[i];
-  c = wave_sum(c);
-  if (threadIdx.x == 0) {
-    const double proj = l < k ? t[l] : G.coef[2 * k + 1];
-    G.hnew[l] = (proj - c) / G.dctl[3];
   }
 }
 
@@ -763,13 +767,13 @@ hipError_t gm_launch_start(const GmDev& G, double tol, int max_iter, const doubl
 template <int MAP, typename TQ>
 static void gm_launch_basis_t(const GmDev& G, TQ* Q, size_t ldq, double* wbuf, size_t n, double* partial, int nblk,
                               hipStream_t st) {
-  // projections, reduce, control 1 (3), H s, update, norm reduce + control 2,
-  // scale; each kernel reads from GM_FIN whether the step is a finalising one
+  // projections, reduce, control 1 (3), update (with H s), norm reduce +
+  // control 2, scale; each kernel reads from GM_FIN whether the step is a
+  // finalising one
   hipLaunchKernelGGL((gm_dots_kernel<MAP, TQ>), dim3(nblk, DOT_CG), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, partial,
                      nblk);
   hipLaunchKernelGGL(gm_reduce_kernel, dim3(256), dim3(256), 0, st, G, partial, nblk);
   hipLaunchKernelGGL(gm_ctl1_kernel, dim3(1), dim3(GT), 0, st, G);
-  hipLaunchKernelGGL(gm_hs_kernel, dim3(G.M + 1), dim3(64), 0, st, G);
   hipLaunchKernelGGL((gm_update_kernel<MAP, TQ>), dim3(nblk), dim3(KRT), 0, st, G, Q, ldq, wbuf, n, partial);
   hipLaunchKernelGGL(gm_ctl2_kernel, dim3(1), dim3(GT), 0, st, G, partial, nblk);
   hipLaunchKernelGGL(gm_scale_kernel<TQ>, dim3(1024), dim3(256), 0, st, G, wbuf, Q, ldq, n);
