@@ -224,8 +224,9 @@ template <typename T, int S> struct Dft<T, 16, S> {
                      T(0.7071067811865475244008443621048490), T(0.3826834323650897717284599840303989)};
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
+      // W16^0 = 1 and W16^4 = S i take no multiplications
       const c2<T> w = mk<T>(cs[k], T(S) * sn[k]);
-      const c2<T> t = (k == 0) ? o[0] : cmul(o[k], w);
+      const c2<T> t = (k == 0) ? o[0] : (k == 4) ? cmul_si<S>(o[4]) : cmul(o[k], w);
       a[k] = cadd(e[k], t);
       a[k + 8] = csub(e[k], t);
     }
