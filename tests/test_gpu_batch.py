@@ -158,3 +158,32 @@ def test_batch_mixes_channel_and_periodic_instances():
         assert np.array_equal(s.solution(), sol)
     for s, _ in insts:
         s.close()
+
+
+@pytest.mark.parametrize("steps_per_round", [1, 2, 5])
+def test_round_length_does_not_change_results(steps_per_round):
+    """The step that finalises a converging inner solve is a step of its own
+    (its mat-vec gated off), so it can fall anywhere within a round of graph
+    steps, or open one: every round length gives bitwise the solve alone,
+    including an iteration cap reached by a finalising step."""
+    say("round length", steps_per_round)
+    cases = [(32, 20, 6, fa.FCM_PREC_FP64, 200), (48, 40, 8, fa.FCM_PREC_FP32, 200), (32, 30, 7, fa.FCM_PREC_FP64, 9)]
+    insts = [(make_instance(51 + i, n, NB, nbpb), prec, cap) for i, (n, NB, nbpb, prec, cap) in enumerate(cases)]
+    alone = []
+    for i, (s, prec, cap) in enumerate(insts):
+        s.mobility_prepare(200 + i)
+        alone.append(s.solve(None, tol=1e-8, max_iter=cap, precision=prec))
+    assert alone[2][1]["status"] == "max_iter" and alone[2][1]["iterations"] == 9
+    with fa.FcmBatch(max_jobs=4, worker_streams=0, steps_per_round=steps_per_round) as b:
+        for i, (s, prec, cap) in enumerate(insts):
+            s.mobility_prepare(200 + i)
+            b.submit(s, tol=1e-8, max_iter=cap, precision=prec)
+        b.run()
+        reps = {id(s): r for s, r in b.take()}
+    for (s, prec, cap), (sol, rep) in zip(insts, alone):
+        brep = reps[id(s)]
+        assert (brep["status"], brep["iterations"], brep["refinements"]) == (
+            rep["status"], rep["iterations"], rep["refinements"]), (brep, rep)
+        assert np.array_equal(s.solution(), sol)
+    for s, _, _ in insts:
+        s.close()
