@@ -257,14 +257,18 @@ __global__ void __launch_bounds__(128) pinv_kernel(BodyData b, const TI* __restr
 // a contiguous read) and ref_N are staged in LDS once.  Every sum
 // runs over k in the order of pinv_kernel, so the results are the same bit for
 // bit.  Used when 6 <= 3 n_b <= 72 (the threads of a body's rows also compute
-// its six body rows, and the LDS stays under 64 KiB).
-template <typename TI>
+// its six body rows, and the LDS stays under 64 KiB).  N3 = 3 n_b when it is
+// compiled in (the products then unroll with constant LDS offsets), 0 for any.
+// Each thread loads the column and the row of its body's rotation that its
+// component c needs, instead of all nine words, so that no register array is
+// indexed at run time (that had cost a select per word and use).
+template <typename TI, int N3>
 __global__ void __launch_bounds__(256) pinv_pack_kernel(BodyData b, const TI* __restrict__ in,
                                                         double* __restrict__ out, double* __restrict__ wKT) {
   extern __shared__ double sm[];
   if (b.gate && *b.gate) return;
   if (b.kcol) in += (size_t)(*b.kcol) * b.ldq;
-  const int n3 = 3 * b.nbpb;
+  const int n3 = N3 > 0 ? N3 : 3 * b.nbpb;
   const int BP = 256 / n3;
   double* MT = sm;               // n3 x n3, MT[k n3 + i] = Minv[i n3 + k]
   double* NR = MT + n3 * n3;     // 36
@@ -291,13 +295,18 @@ __global__ void __launch_bounds__(256) pinv_pack_kernel(BodyData b, const TI* __
   for (int base = blockIdx.x * BP; base < b.NB; base += gridDim.x * BP) {
     const int n = base + q;
     const bool live = q < BP && n < b.NB;
-    // every global load of the pass first: the body's rotation (the same 9
-    // words for the threads of a body), this row's blob vector, the body
-    // vector (rows 0..5) and the blob offset R r_m (for -K^T of the output)
-    double R[9], v3[3] = {0.0, 0.0, 0.0}, w3[3] = {0.0, 0.0, 0.0}, d3[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int r = 0; r < 9; ++r) R[r] = live ? b.R[9 * (size_t)n + r] : 0.0;
+    // every global load of the pass first: column c and row c of the body's
+    // rotation, this row's blob vector, the body vector (rows 0..5) and
+    // components c1, c2 of the blob offset R r_m (for -K^T of the output)
+    double Rc[3] = {0.0, 0.0, 0.0}, Rr[3] = {0.0, 0.0, 0.0}, v3[3] = {0.0, 0.0, 0.0}, w3[3] = {0.0, 0.0, 0.0};
+    double dc1 = 0.0, dc2 = 0.0;
     if (live) {
+      const double* Rn = b.R + 9 * (size_t)n;
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        Rc[r] = Rn[3 * r + c];
+        Rr[r] = Rn[3 * c + r];
+      }
       const TI* v = in + (size_t)n3 * n + 3 * m;
       v3[0] = v[0]; v3[1] = v[1]; v3[2] = v[2];
       if (i < 6) {
@@ -306,10 +315,11 @@ __global__ void __launch_bounds__(256) pinv_pack_kernel(BodyData b, const TI* __
       }
       if (wKT) {
         const double* d = b.dpos + 3 * ((size_t)n * b.nbpb + m);
-        d3[0] = d[0]; d3[1] = d[1]; d3[2] = d[2];
+        dc1 = d[c1];
+        dc2 = d[c2i];
       }
     }
-    if (live) bpq[i] = R[c] * v3[0] + R[3 + c] * v3[1] + R[6 + c] * v3[2];  // (R^T v)_c
+    if (live) bpq[i] = Rc[0] * v3[0] + Rc[1] * v3[1] + Rc[2] * v3[2];  // (R^T v)_c
     __syncthreads();
     if (live) {
       double s = 0.0;
@@ -319,8 +329,8 @@ __global__ void __launch_bounds__(256) pinv_pack_kernel(BodyData b, const TI* __
     }
     __syncthreads();
     if (live && i < 6) {
-      const int cc = i % 3;
-      const double rv = R[cc] * w3[0] + R[3 + cc] * w3[1] + R[6 + cc] * w3[2];
+      const int cc = c;  // i % 3
+      const double rv = Rc[0] * w3[0] + Rc[1] * w3[1] + Rc[2] * w3[2];
       double k = 0.0;
       if (i < 3) {
 #pragma unroll 4
@@ -350,21 +360,20 @@ __global__ void __launch_bounds__(256) pinv_pack_kernel(BodyData b, const TI* __
     __syncthreads();
     if (live) {
       const double* o = ooq + 3 * m;
-      const double v = R[3 * c] * o[0] + R[3 * c + 1] * o[1] + R[3 * c + 2] * o[2];
+      const double v = Rr[0] * o[0] + Rr[1] * o[1] + Rr[2] * o[2];
       out[(size_t)n3 * n + i] = v;
       ggq[i] = v;
     }
     if (live && i < 6) {
-      const int cc = i % 3;
-      const double* v = vvq + (i < 3 ? 0 : 3);
-      out[nlam + 6 * (size_t)n + i] = R[3 * cc] * v[0] + R[3 * cc + 1] * v[1] + R[3 * cc + 2] * v[2];
+      const double* v = vvq + (i < 3 ? 0 : 3);  // row c = i % 3 of R
+      out[nlam + 6 * (size_t)n + i] = Rr[0] * v[0] + Rr[1] * v[1] + Rr[2] * v[2];
     }
     if (wKT) {
       __syncthreads();
       // torque term of this row: (d_m x lambda_m)_c, from the blob's three outputs
       if (live) {
         const double* l = ggq + 3 * m;
-        ttq[i] = d3[c1] * l[c2i] - d3[c2i] * l[c1];
+        ttq[i] = dc1 * l[c2i] - dc2 * l[c1];
       }
       __syncthreads();
       if (live && i < 6) {
@@ -393,8 +402,13 @@ static hipError_t apply_pinv_t(const BodyData& b, const TI* in, double* out, dou
     const int BP = 256 / n3;
     const size_t lds = sizeof(double) * ((size_t)n3 * n3 + 36 + n3 + (size_t)BP * (4 * n3 + 12));
     const int groups = (b.NB + BP - 1) / BP;
-    hipLaunchKernelGGL(pinv_pack_kernel<TI>, dim3(groups < 2048 ? groups : 2048), dim3(256), lds, st, b, in, out,
-                       wKT);
+    const dim3 grid(groups < 2048 ? groups : 2048);
+    switch (n3) {
+      case 18: hipLaunchKernelGGL((pinv_pack_kernel<TI, 18>), grid, dim3(256), lds, st, b, in, out, wKT); break;
+      case 30: hipLaunchKernelGGL((pinv_pack_kernel<TI, 30>), grid, dim3(256), lds, st, b, in, out, wKT); break;
+      case 36: hipLaunchKernelGGL((pinv_pack_kernel<TI, 36>), grid, dim3(256), lds, st, b, in, out, wKT); break;
+      default: hipLaunchKernelGGL((pinv_pack_kernel<TI, 0>), grid, dim3(256), lds, st, b, in, out, wKT); break;
+    }
     return hipGetLastError();
   }
   const size_t lds = sizeof(double) * (5 * (size_t)n3 + 21);
