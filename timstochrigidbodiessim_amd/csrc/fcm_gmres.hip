@@ -497,27 +497,31 @@ __device__ void ctl1_step(const GmDev& G, CtlShared& L) {
   const double* s = G.red;           // Q.v1, k+1 values (s_0..s_{k-1}, alpha)
   const double* t = G.red + ncols;   // Q.v2, k+1 values
   if (prov) {
+    // every load first (one round trip): the projections, the provisional
+    // column k-1 of H (finalised here with the reorthogonalisation
+    // coefficients s) and the rotations; the update's coefficients are the
+    // projections themselves
+    const double eta = G.dctl[2];
+    const double alpha = s[k], tau = t[k];
     double ss = 0.0, st = 0.0;
     for (int i = threadIdx.x; i < k; i += GT) {
-      ss += s[i] * s[i];
-      st += s[i] * t[i];
-    }
-    ss = block_sum(ss, sh);
-    st = block_sum(st, sh);
-    const double alpha = s[k], tau = t[k];
-    const double nu = sqrt(fmax(alpha - ss, 0.0));
-    const double eta = G.dctl[2];
-    // finalise column k-1 of H
-    for (int i = threadIdx.x; i < k; i += GT) {
-      const double h = G.hprov[i] + eta * s[i];
+      const double si = s[i], ti = t[i];
+      ss += si * si;
+      st += si * ti;
+      const double h = G.hprov[i] + eta * si;
       *Hp(G, i, k - 1) = h;
       col[i] = h;
+      G.coef[i] = si;
+      G.coef[k + i] = ti;
     }
+    givens_stage(G, k - 1, sCS, sSN);
+    ss = block_sum(ss, sh);
+    st = block_sum(st, sh);
+    const double nu = sqrt(fmax(alpha - ss, 0.0));
     if (threadIdx.x == 0) {
       *Hp(G, k, k - 1) = eta * nu;
       col[k] = eta * nu;
     }
-    givens_stage(G, k - 1, sCS, sSN);
     __syncthreads();
     if (threadIdx.x == 0) {
       const double rel = givens_sweep(G, k - 1, col, sCS, sSN, true);
@@ -540,11 +544,6 @@ __device__ void ctl1_step(const GmDev& G, CtlShared& L) {
     }
     __syncthreads();
     givens_store(G, k - 1, col);
-    if (s_done) return;
-    for (int l = threadIdx.x; l < k; l += GT) {
-      G.coef[l] = s[l];
-      G.coef[k + l] = t[l];
-    }
   } else {
     for (int l = threadIdx.x; l <= k; l += GT) {
       G.hnew[l] = t[l];
@@ -590,19 +589,29 @@ __global__ void __launch_bounds__(GT) gm_ctl1_kernel(GmDev G) {
 // (one block; ||w||^2 from the update pass's block partials, fixed order)
 __global__ void __launch_bounds__(GT) gm_ctl2_kernel(GmDev G, const double* __restrict__ partial, int nblk) {
   if (G.ictl[GM_STATUS] != 0 || G.ictl[GM_FIN] != 0) return;
-  __shared__ double col[GCAP], sCS[GCAP], sSN[GCAP];
+  __shared__ double col[GCAP], sCS[GCAP], sSN[GCAP], s[GT];
   __shared__ int s_brk;
   const int k = G.ictl[GM_K];
-  reduce_rows(G, partial, nblk, 1, col);
-  const double etar = sqrt(G.red[0]);
-  const double eta = etar / G.dctl[3];
+  const double nu = G.dctl[3];
+  // every load first (one round trip): the update pass's block partials of
+  // ||w||^2 (summed as reduce_rows does), the new column and the rotations
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += GT) a += partial[i];
   for (int l = threadIdx.x; l <= k; l += GT) {
     const double h = G.hnew[l];
     G.hprov[l] = h;
     col[l] = h;
   }
   givens_stage(G, k, sCS, sSN);
+  s[threadIdx.x] = a;
   __syncthreads();
+  for (int off = GT / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) G.red[0] = s[0];
+  const double etar = sqrt(s[0]);
+  const double eta = etar / nu;
   if (threadIdx.x == 0) {
     G.dctl[2] = eta;
     s_brk = !(etar > 0.0);
