@@ -208,13 +208,14 @@ def test_ref_matrices_and_Pinv_match_oracle():
         assert rel(s.saddle_matvec(v), po.saddle_matvec(o, b, v)) < FP64_TOL
 
 
-@pytest.mark.parametrize("nbpb,NB", [(1, 40), (2, 30), (3, 25), (5, 12), (12, 300), (13, 20), (16, 9), (21, 4),
-                                     (22, 6), (24, 5), (30, 3)])
+@pytest.mark.parametrize("nbpb,NB", [(1, 40), (2, 30), (3, 25), (5, 12), (6, 50), (10, 30), (12, 300), (13, 20),
+                                     (16, 9), (21, 4), (22, 6), (24, 5), (30, 3)])
 def test_Pinv_every_body_size_matches_oracle(nbpb, NB):
     """P^-1 on bodies of 1 to 30 spheres: the packed kernel (2 to 24 spheres,
     whose row threads also form the six body rows; 300 bodies of 12 take more
-    than one pass of a workgroup) and the one-workgroup-per-body kernel (1
-    sphere, whose three rows could not, and more than 24)."""
+    than one pass of a workgroup; 6, 10 and 12 spheres have the row count
+    compiled in) and the one-workgroup-per-body kernel (1 sphere, whose three
+    rows could not, and more than 24)."""
     say("Pinv nbpb", nbpb, NB)
     n = 48
     gc, oc = pair_cfg(n, max_blobs=NB * nbpb, max_bodies=NB, max_blobs_per_body=nbpb)
