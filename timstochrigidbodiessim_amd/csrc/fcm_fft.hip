@@ -223,7 +223,7 @@ __global__ void __launch_bounds__(256) cols_kernel(FieldPtrs<T> fp, size_t plane
   using V = c2<T>;
   using SH = ColShape<T, N, PM>;
   constexpr FftPlan PL = make_plan(N, PM);
-  constexpr int P = PL.P, TC = SH::TC, C = SH::C;
+  constexpr int P = PL.P, C = SH::C;
   constexpr bool REV = (S > 0);
   __shared__ V lds[N * C];
   __shared__ V s_tw[N];
@@ -315,7 +315,7 @@ __global__ void __launch_bounds__(XTMAX<T>::value) xproj_kernel(FieldPtrs<T> fp,
   using V = c2<T>;
   using SH = ColShape<T, N, PM, XTMAX<T>::value>;
   constexpr FftPlan PL = make_plan(N, PM);
-  constexpr int P = PL.P, TC = SH::TC, C = SH::C;
+  constexpr int P = PL.P, C = SH::C;
   __shared__ V lds[N * C];
   __shared__ V s_tw[N];
   const int t = threadIdx.x;
@@ -380,7 +380,7 @@ __global__ void __launch_bounds__(XTMAX<T>::value) xstress_kernel(FieldPtrs<T> f
   constexpr int PM = XPM<T>::value;
   using SH = ColShape<T, N, PM, XTMAX<T>::value>;
   constexpr FftPlan PL = make_plan(N, PM);
-  constexpr int P = PL.P, TC = SH::TC, C = SH::C;
+  constexpr int P = PL.P, C = SH::C;
   __shared__ V lds[N * C];
   const int t = threadIdx.x;
   const int col = t % C, tcol = t / C;

@@ -1218,7 +1218,6 @@ __global__ void __launch_bounds__(256) interp_slab_kernel(InterpArgs A, FieldPtr
   constexpr int NBOX = HX * HY * HZ;
   constexpr int NYB = TY / 8, NZB = TZ / 8;  // y and z bins per tile
   constexpr int NR = 2 * NYB;                // bin ranges: (x bin, y bin) pairs
-  constexpr int ROWS = HX * HY;
   static_assert(HZ <= 32, "z rows are loaded by 32 lanes");
   __shared__ T box[NBOX];
   __shared__ double vres[256][3];
